@@ -1,0 +1,162 @@
+/*
+ * thea_gpu.h -- C ABI of the MI355X BVH-query and mesh-sampling library (libthea_gpu.so).
+ *
+ * This is the drop-in boundary for Thea's BvhN / MeshBvh / MeshSampler query path. Thea exposes
+ * that path as C++ templates (Algorithms/BvhN.hpp, Algorithms/MeshBvh.hpp,
+ * Algorithms/MeshSampler.hpp); each entry point below names the Thea interface it replaces.
+ * include/thea_gpu_bvh.hpp wraps these calls in classes that keep Thea's method names.
+ *
+ * Conventions
+ *  - Every function returns THG_OK (0) or a negative THG_E* status; thg_last_error() returns a
+ *    thread-local message for the last failure on the calling thread. Nothing calls exit().
+ *  - "Not found" is encoded per query as index -1 and time/distance -1, as in Thea
+ *    (BvhN.hpp:1062 closestElement returns -1; BvhN.hpp:1315 ray miss is time -1, index -1).
+ *  - `mem` says where the caller's query/result pointers live: THG_MEM_DEVICE (device memory,
+ *    the call is asynchronous on `stream`) or THG_MEM_HOST (host memory, the call stages
+ *    through library-owned device buffers and returns when results are in host memory).
+ *  - `stream` is a hipStream_t passed as void*; NULL is the null stream.
+ *  - Handles are immutable after creation; queries on one handle may run concurrently on
+ *    different streams. Destroying a handle while work on it is in flight is undefined.
+ *  - Indices returned are the caller's triangle (or point) indices, in input order, as Thea's
+ *    MeshBvh returns mesh triangle order (Algorithms/MeshTriangles.hpp:328-339, BvhN.hpp:720-725).
+ *  - Exact ties (bit-identical float distance or hit time) resolve to the LOWEST index.
+ */
+#ifndef THEA_GPU_H
+#define THEA_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define THG_OK 0
+#define THG_E_INVALID (-1)     /* bad argument */
+#define THG_E_OOM (-2)         /* device allocation failed */
+#define THG_E_DEVICE (-3)      /* HIP runtime error */
+#define THG_E_BUDGET (-4)      /* request exceeds a memory budget */
+#define THG_E_UNSUPPORTED (-5) /* valid request this build does not support */
+#define THG_E_CANCELLED (-6)   /* request cancelled or timed out (serving loop) */
+
+#define THG_MEM_DEVICE 0
+#define THG_MEM_HOST 1
+
+typedef struct thg_ctx_s *thg_ctx;
+typedef struct thg_mesh_s *thg_mesh;
+typedef struct thg_bvh_s *thg_bvh;
+
+/* BVH build options. Replaces the `init(method, max_depth, max_elems_in_leaf, save_memory, ...)`
+ * arguments of BvhN::init (Algorithms/BvhN.hpp:705-706) and MeshBvh::init
+ * (Algorithms/MeshBvh.hpp:104-111). The tree is a GPU LBVH, not Thea's kd-tree; results are
+ * defined by the element arithmetic and the lowest-index tie-break, not by tree shape. */
+typedef struct thg_build_opts {
+  int32_t leaf_size; /* max elements per leaf, 1..16; 0 = default (4) */
+  int32_t reserved[7];
+} thg_build_opts;
+
+/* Library status text for the calling thread's last failed call (never NULL). */
+const char *thg_last_error(void);
+
+/* Version string of the library. */
+const char *thg_version(void);
+
+/* Context on one HIP device (the device index as HIP numbers it). One per process per device is
+ * typical. Thea has no analogue (its queries run on the calling CPU thread). */
+int thg_ctx_create(int32_t device, thg_ctx *out);
+int thg_ctx_destroy(thg_ctx ctx);
+
+/* Upload a triangle mesh: `verts` = nv float3 (x,y,z interleaved), `tris` = nt int32 triplets of
+ * vertex indices. Replaces MeshBvh::add / addTriangles and MeshTriangles::add
+ * (Algorithms/MeshBvh.hpp:55-67, Algorithms/MeshTriangles.hpp:348-422). Vertices and indices are
+ * copied; the caller's arrays may be freed afterwards. */
+int thg_mesh_create(thg_ctx ctx, const float *verts, int64_t nv, const int32_t *tris, int64_t nt, int32_t mem,
+                    thg_mesh *out);
+int thg_mesh_destroy(thg_mesh mesh);
+int64_t thg_mesh_num_triangles(thg_mesh mesh);
+/* Total surface area (double sum of per-triangle float areas, MeshSampler.hpp:147-149). */
+int thg_mesh_total_area(thg_mesh mesh, double *area);
+
+/* Build a BVH over the triangles of `mesh` on `stream` (synchronous). Replaces BvhN::init /
+ * initTree / createKdTree (Algorithms/BvhN.hpp:705-794, :1366-1525) and MeshBvh::init
+ * (Algorithms/MeshBvh.hpp:104-111). The BVH keeps a reference to the mesh; destroy the BVH first. */
+int thg_bvh_build_mesh(thg_mesh mesh, const thg_build_opts *opts, void *stream, thg_bvh *out);
+
+/* Build a BVH over n points (float3). Replaces BvhN<Vector3, 3>::init over points (the point BVH
+ * of TestBvh.cpp:159-163, PointSet3.hpp:71 SampleBvh). Points are copied. */
+int thg_bvh_build_points(thg_ctx ctx, const float *xyz, int64_t n, int32_t mem, const thg_build_opts *opts,
+                         void *stream, thg_bvh *out);
+int thg_bvh_destroy(thg_bvh bvh);
+/* Statistics of a built BVH: out[0] nodes, [1] leaves, [2] max depth, [3] elements,
+ * [4] device bytes held by the BVH, [5] leaf size. */
+int thg_bvh_info(thg_bvh bvh, int64_t out[6]);
+
+/* Closest element and closest point for n query points `q` (float3).
+ * Replaces BvhN::closestElement<MetricL2>(query, dist_bound, compat, &dist, &closest_point)
+ * (Algorithms/BvhN.hpp:1050-1063) and BvhN::closestPair (:1082-1114).
+ *  dist_bound: only elements at distance < dist_bound are considered; negative = unbounded.
+ *  idx[i]: element index or -1. cp (nullable): closest point on that element (float3).
+ *  dist (nullable): distance sqrt(d^2) as float (Thea returns sqrt((double)d^2), MetricL2.hpp:97;
+ *  the two agree after rounding to float). d2 (nullable): the exact float squared distance. */
+int thg_closest_point(thg_bvh bvh, const float *q, int64_t n, double dist_bound, int32_t *idx, float *cp, float *dist,
+                      float *d2, int32_t mem, void *stream);
+
+/* k nearest elements per query, sorted by (distance, index) ascending; rows shorter than k are
+ * padded with index -1 and distance -1. Replaces BvhN::kClosestPairs<MetricL2>
+ * (Algorithms/BvhN.hpp:1139-1146) with a BoundedSortedArray of capacity k.
+ *  k: 1..32. exclude_self: skip element i for query i (queries are the BVH's own points; the
+ *  self-exclusion PointSet3 applies with a filter, Algorithms/PointSet3.hpp:576-583).
+ *  idx: n*k int32. dist (nullable): n*k float distances. d2 (nullable): n*k float squared. */
+int thg_k_closest(thg_bvh bvh, const float *q, int64_t n, int32_t k, double dist_bound, int32_t exclude_self,
+                  int32_t *idx, float *dist, float *d2, int32_t mem, void *stream);
+
+/* First hit of n rays (origins, directions: float3; directions need not be unit length).
+ * Replaces BvhN::rayStructureIntersection<RayIntersectionTester> (Algorithms/BvhN.hpp:1291-1316),
+ * rayIntersectionTime (:1274-1289) and rayIntersects (:1266-1269).
+ *  max_time: only hits with t < max_time count (Thea's structure traversal accepts a hit only if it
+ *  is strictly earlier than the running bound, which starts at max_time, BvhN.hpp:2247-2258);
+ *  negative = unbounded.
+ *  idx[i]: triangle index or -1. t[i] (nullable): hit time or -1. normal (nullable): face normal
+ *  of the hit triangle (float3, Triangle3.hpp:542-552) or 0 on a miss. */
+int thg_ray_first_hit(thg_bvh bvh, const float *origins, const float *dirs, int64_t n, float max_time, int32_t *idx,
+                      float *t, float *normal, int32_t mem, void *stream);
+
+/* Any hit: hit[i] = 1 if some triangle is hit with 0 <= t (< max_time when max_time >= 0),
+ * else 0. Replaces BvhN::rayIntersects (Algorithms/BvhN.hpp:1266-1269). */
+int thg_ray_any_hit(thg_bvh bvh, const float *origins, const float *dirs, int64_t n, float max_time, uint8_t *hit,
+                    int32_t mem, void *stream);
+
+/* Area-weighted random samples on the mesh surface, exact count. Replaces
+ * MeshSampler::sampleEvenlyByArea(n, positions, face_normals, triangles, CountMode::EXACT)
+ * (Algorithms/MeshSampler.hpp:100-204) and TriangleN::randomPoint (Triangle3.hpp:179-195).
+ * Randomness: sample i uses Philox4x32-10 with counter (i, stream_id) and key `seed`; its first
+ * three 32-bit words drive the triangle pick and the two barycentric draws, each mapped to [0,1]
+ * as Thea's Random::uniform01 does (Random.hpp:126-131). The same stream is reproducible on the
+ * host. `n_out` receives the number of samples written (0 for an empty or zero-area mesh).
+ *  pos: n float3. normal (nullable): n float3 face normals. tri (nullable): n int32. */
+int thg_sample_area(thg_mesh mesh, int64_t n, uint64_t seed, uint64_t stream_id, float *pos, float *normal,
+                    int32_t *tri, int64_t *n_out, int32_t mem, void *stream);
+
+/* Kernel timing (diagnostics; Thea's analogue is Stopwatch, Stopwatch.hpp:41-76). While enabled on
+ * a context, each query or sampling call records HIP events around its main kernel on the call's
+ * stream. thg_ctx_kernel_time synchronises those events and returns the summed kernel time (ms) and
+ * launch count for `kernel` ("closest", "knn", "ray", "any_hit", "sample", "order") since the last
+ * reset; reset != 0 clears the record. */
+int thg_ctx_set_timing(thg_ctx ctx, int32_t enable);
+int thg_ctx_kernel_time(thg_ctx ctx, const char *kernel, double *total_ms, int64_t *launches, int32_t reset);
+
+/* Many meshes, one call: build a BVH for each of `nm` meshes and answer `nq[m]` closest-point
+ * queries for mesh m, with one persistent launch for all builds' per-element work and one for all
+ * queries. Thea has no analogue (each BvhN is built and queried by its own caller). Inputs are
+ * concatenated: verts (sum nv float3), tris (sum nt int32 triplets, indices local to their mesh),
+ * queries (sum nq float3); nv/nt/nq are per-mesh counts (host arrays). Results use local triangle
+ * indices. leaf_size 0 = autotuned per size class. All pointers are device pointers. */
+int thg_batch_closest(thg_ctx ctx, int32_t nm, const int64_t *nv, const int64_t *nt, const int64_t *nq,
+                      const float *verts, const int32_t *tris, const float *queries, int32_t leaf_size,
+                      int32_t *idx, float *dist, void *stream, double timings_ms[2]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* THEA_GPU_H */

@@ -1,0 +1,1147 @@
+// TEST INFRASTRUCTURE ONLY -- the oracle.
+//
+// A from-scratch CPU restatement of Thea's BVH-query and mesh-sampling path, used to check the HIP
+// library and, in bench.py's cpu_baseline leg, as the timed CPU baseline. Only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and only as the checker.
+// The product library (thea_amd/csrc) never links or calls anything here.
+//
+// Every function cites the Thea source it restates (paths relative to Code/Source/ of sidch/Thea).
+// Thea itself cannot be compiled in this environment (it requires Eigen >= 3.4, which is absent),
+// so 3-vector reductions follow Eigen 3.4's published reduction order for fixed size 3
+// (redux_novec_unroller splits [0,3) into [0,1) + [1,3), i.e. x0 + (x1 + x2)), and
+// normalized() divides each component by sqrt(squaredNorm()). Those Eigen bits are "parity
+// unpinned" against real Thea; everything else is pinned by the tests listed in
+// tests/test_oracle.py (closed forms, brute force, FP64, known-answer vectors).
+//
+// Build: see oracle/Makefile (g++ -O2 -ffp-contract=off, Thea's own FP environment:
+// Code/Build/Common/BuildSetup.cmake:96-111 -- no fast-math, no -march, IEEE single precision).
+
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <limits>
+#include <map>
+#include <thread>
+#include <utility>
+#include <vector>
+
+namespace {
+
+//==================================================================================================
+// Eigen-order float 3-vectors (MatVec.hpp:52-66 typedefs; Eigen 3.4 reduction order)
+//==================================================================================================
+
+struct V3 {
+  float x, y, z;
+  float operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+  float &operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
+};
+
+inline V3 mk(float a, float b, float c) { V3 v; v.x = a; v.y = b; v.z = c; return v; }
+inline V3 add(V3 a, V3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline V3 sub(V3 a, V3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline V3 mul(float s, V3 a) { return mk(s * a.x, s * a.y, s * a.z); }
+inline V3 vdiv(V3 a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
+// Eigen fixed-size-3 sum: x0 + (x1 + x2).
+inline float dot(V3 a, V3 b) { return a.x * b.x + (a.y * b.y + a.z * b.z); }
+inline float sqn(V3 a) { return dot(a, a); }
+inline float norm(V3 a) { return std::sqrt(sqn(a)); }
+// Eigen cross(): (a1 b2 - a2 b1, a2 b0 - a0 b2, a0 b1 - a1 b0).
+inline V3 cross(V3 a, V3 b) {
+  return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+inline V3 normalized(V3 a) {
+  float n = sqn(a);
+  if (n > 0) return vdiv(a, std::sqrt(n));
+  return a;
+}
+inline V3 cmin(V3 a, V3 b) { return mk(std::min(a.x, b.x), std::min(a.y, b.y), std::min(a.z, b.z)); }
+inline V3 cmax(V3 a, V3 b) { return mk(std::max(a.x, b.x), std::max(a.y, b.y), std::max(a.z, b.z)); }
+// Eigen's cwiseMin/Max evaluate std::min(a, b) per coefficient (scalar_min_op).
+
+// MatrixUtil.hpp:84-115 maxAxis / maxAbsAxis: Eigen maxCoeff(&i) keeps the FIRST maximum.
+inline int max_axis(V3 v) {
+  int k = 0; float m = v.x;
+  if (v.y > m) { m = v.y; k = 1; }
+  if (v.z > m) { k = 2; }
+  return k;
+}
+inline int max_abs_axis(V3 v) { return max_axis(mk(std::fabs(v.x), std::fabs(v.y), std::fabs(v.z))); }
+
+//==================================================================================================
+// Triangle3 (Triangle3.hpp:143-155 update, :278-304 closestPoint, :1310-1462 helpers)
+//==================================================================================================
+
+struct Tri {
+  V3 v[3];
+  V3 n;          // plane normal (Hyperplane3.hpp:34-41)
+  float d;       // plane distance n . v0
+  int axis;      // primary axis (maxAbsAxis of normal)
+  V3 centroid, e01, e02;
+  float area;
+};
+
+void tri_update(Tri &t) {
+  V3 v0 = t.v[0], v1 = t.v[1], v2 = t.v[2];
+  t.n = normalized(cross(sub(v1, v0), sub(v2, v0)));
+  t.d = dot(t.n, v0);
+  t.axis = max_abs_axis(t.n);
+  V3 s = add(add(v0, v1), v2);
+  t.centroid = vdiv(s, 3.0f);
+  t.e01 = sub(v1, v0);
+  t.e02 = sub(v2, v0);
+  t.area = 0.5f * std::fabs(dot(t.n, cross(t.e01, t.e02)));
+}
+
+// Triangle3.hpp:1414-1462 isPointInsideTriangle
+bool inside_tri(V3 v0, V3 v1, V3 v2, int primary_axis, V3 p) {
+  int i, j;
+  switch (primary_axis) {
+    case 1: i = 2; j = 0; break;
+    case 2: i = 0; j = 1; break;
+    default: i = 1; j = 2;
+  }
+  auto area2 = [&](V3 d, V3 e, V3 f) { return (e[i] - d[i]) * (f[j] - d[j]) - (f[i] - d[i]) * (e[j] - d[j]); };
+  float area = area2(v0, v1, v2);
+  if (area == 0) return v0.x == p.x && v0.y == p.y && v0.z == p.z;
+  float inv_area = 1.0f / area;
+  float b0 = area2(p, v1, v2) * inv_area;
+  if (b0 < 0 || b0 > 1) return false;
+  float b1 = area2(v0, p, v2) * inv_area;
+  if (b1 < 0 || b1 > 1) return false;
+  float b2 = 1.0f - b0 - b1;
+  return b2 >= 0 && b2 <= 1;
+}
+
+// Triangle3.hpp:1310-1338 closestPointOnLineSegment
+V3 closest_on_segment(V3 v0, V3 v1, V3 dir, float len, V3 p) {
+  V3 c = sub(p, v0);
+  float t = dot(dir, c);
+  if (t <= 0) return v0;
+  if (t >= len) return v1;
+  return add(v0, mul(t, dir));
+}
+
+// Triangle3.hpp:1340-1412 closestPointOnTrianglePerimeter
+V3 closest_on_perimeter(V3 v0, V3 v1, V3 v2, V3 p) {
+  V3 v[3] = {v0, v1, v2};
+  V3 dir[3] = {sub(v1, v0), sub(v2, v1), sub(v0, v2)};
+  float len[3];
+  for (int i = 0; i < 3; ++i) {
+    len[i] = norm(dir[i]);
+    dir[i] = vdiv(dir[i], len[i]);
+  }
+  static const int next[3] = {1, 2, 0};
+  V3 r[3];
+  float d[3];
+  for (int i = 0; i < 3; ++i) {
+    r[i] = closest_on_segment(v[i], v[next[i]], dir[i], len[i], p);
+    d[i] = sqn(sub(r[i], p));
+  }
+  int e;
+  if (d[0] < d[1]) e = (d[0] < d[2]) ? 0 : 2;
+  else e = (d[1] < d[2]) ? 1 : 2;
+  return r[e];
+}
+
+// Triangle3.hpp:294-304 closestPoint + MetricL2.hpp:195-216 closestPoints (d^2 = |cp - p|^2)
+float tri_closest(const Tri &t, V3 p, V3 &cp) {
+  // HyperplaneN.hpp:145-160: proj = p - (p.n - d) n
+  float sd = dot(p, t.n) - t.d;
+  V3 proj = sub(p, mul(sd, t.n));
+  if (inside_tri(t.v[0], t.v[1], t.v[2], t.axis, proj)) cp = proj;
+  else cp = closest_on_perimeter(t.v[0], t.v[1], t.v[2], p);
+  return sqn(sub(cp, p));
+}
+
+// Triangle3.hpp:1464-1524 rayTriangleIntersectionTime (Eberly, Wild Magic 5.3)
+float tri_ray_time(V3 o, V3 dir, V3 v0, V3 e01, V3 e02) {
+  const float EPS = FLT_EPSILON;
+  V3 diff = sub(o, v0);
+  V3 normal = cross(e01, e02);
+  float DdN = dot(dir, normal);
+  int sign;
+  if (DdN > EPS) sign = 1;
+  else if (DdN < -EPS) { sign = -1; DdN = -DdN; }
+  else return -1;
+  float DdQxE2 = (float)sign * dot(dir, cross(diff, e02));
+  if (DdQxE2 >= 0) {
+    float DdE1xQ = (float)sign * dot(dir, cross(e01, diff));
+    if (DdE1xQ >= 0) {
+      if (DdQxE2 + DdE1xQ <= DdN) {
+        float QdN = (float)(-sign) * dot(diff, normal);
+        if (QdN >= 0) return QdN / DdN;
+      }
+    }
+  }
+  return -1;
+}
+
+// Triangle3.hpp:542-552 rayIntersection(ray, max_time)
+float tri_ray_isec(const Tri &t, V3 o, V3 dir, float max_time) {
+  float tt = tri_ray_time(o, dir, t.v[0], t.e01, t.e02);
+  if (tt >= 0 && (max_time < 0 || tt <= max_time)) return tt;
+  return -1;
+}
+
+// BvhN.hpp:176-181 distanceLessThan: negatives are +infinity.
+template <typename U, typename W> inline bool dist_less(U lhs, W rhs) { return lhs >= 0 && (rhs < 0 || lhs < rhs); }
+
+//==================================================================================================
+// Axis-aligned boxes (AxisAlignedBoxN.hpp)
+//==================================================================================================
+
+struct Box {
+  V3 lo, hi;
+  bool null_box = true;
+  void merge(const Box &b) {  // :252-272
+    if (b.null_box) return;
+    if (null_box) { lo = b.lo; hi = b.hi; null_box = false; }
+    else { lo = cmin(lo, b.lo); hi = cmax(hi, b.hi); }
+  }
+  V3 center() const { return mul(0.5f, add(hi, lo)); }  // :275
+  V3 extent() const { return sub(hi, lo); }             // :289
+  void scale_centered(float s) {                         // :316-324
+    if (null_box) return;
+    V3 c = center();
+    lo = add(mul(s, sub(lo, c)), c);
+    hi = add(mul(s, sub(hi, c)), c);
+  }
+  // :429-442 squaredDistance(AAB) with the degenerate query box [p, p].
+  float sqdist_point_box(V3 p) const {
+    V3 vmax = cmin(hi, p);
+    V3 vmin = cmax(lo, p);
+    V3 vd = cmax(sub(vmin, vmax), mk(0, 0, 0));
+    return sqn(vd);
+  }
+  // :448-456 squaredDistance(point)
+  float sqdist_point(V3 p) const {
+    V3 vmax = cmin(p, hi);
+    V3 vmin = cmax(p, lo);
+    return sqn(sub(vmin, vmax));
+  }
+  // :589-657 rayIntersection (Woo); returns hit time or -1.
+  float ray_time(V3 o, V3 dir, float max_time) const {
+    if (null_box) return -1;
+    if (max_time >= 0) {
+      float ray_sqlen = sqn(dir);
+      float origin_sqdist = sqdist_point(o);
+      if (origin_sqdist > max_time * max_time * ray_sqlen) return -1;
+    }
+    float max_t[3] = {-1, -1, -1};
+    float location[3];
+    bool inside = true;
+    for (int i = 0; i < 3; ++i) {
+      if (o[i] < lo[i]) {
+        location[i] = lo[i]; inside = false;
+        if (fuzzy_ne0(dir[i])) max_t[i] = (lo[i] - o[i]) / dir[i];
+      } else if (o[i] > hi[i]) {
+        location[i] = hi[i]; inside = false;
+        if (fuzzy_ne0(dir[i])) max_t[i] = (hi[i] - o[i]) / dir[i];
+      }
+    }
+    if (inside) return 0;
+    int w = 0;
+    for (int i = 1; i < 3; ++i) if (max_t[i] > max_t[w]) w = i;
+    if (max_t[w] < -0.5f) return -1;
+    for (int i = 0; i < 3; ++i) {
+      if (i != w) {
+        location[i] = o[i] + max_t[w] * dir[i];
+        if (location[i] < lo[i] || location[i] > hi[i]) return -1;
+      }
+    }
+    (void)location;
+    return max_t[w];
+  }
+  // Math.hpp:127-154 fuzzyNe(a, 0) with eps(a, 0)
+  static bool fuzzy_ne0(float a) {
+    float larger = std::fabs(a);
+    float tol = larger > 1 ? larger * FLT_EPSILON : FLT_EPSILON;
+    bool eq = (a == 0.0f) || std::fabs(a) <= tol;
+    return !eq;
+  }
+};
+
+inline Box tri_bounds(const Tri &t) {  // Triangle3.hpp:215-219
+  Box b;
+  b.lo = cmin(t.v[0], cmin(t.v[1], t.v[2]));
+  b.hi = cmax(t.v[0], cmax(t.v[1], t.v[2]));
+  b.null_box = false;
+  return b;
+}
+inline Box point_bounds(V3 p) { Box b; b.lo = p; b.hi = p; b.null_box = false; return b; }
+
+//==================================================================================================
+// kd-tree (BvhN.hpp:705-794 init, :1366-1440 initTree, :1452-1525 createKdTree, :2458 pad 1.05)
+//==================================================================================================
+
+struct Node {
+  Box bounds;
+  int depth = 0;
+  int child[2] = {-1, -1};
+  int64_t first = 0, count = 0;  // range into `order` (elements of this node)
+  bool leaf() const { return child[0] < 0 && child[1] < 0; }
+};
+
+// Math.cpp:23-36 treeDepth
+int64_t tree_depth(int64_t n, int leaf, float mcf) {
+  if (n <= 0) return 0;
+  double log_inv = -std::log((double)mcf);
+  int64_t d = (int64_t)std::ceil(std::log(n / (double)leaf) / log_inv);
+  return d < 0 ? 0 : d;
+}
+
+struct KdTree {
+  bool points = false;
+  std::vector<Tri> tris;     // element storage (triangles)
+  std::vector<V3> pts;       // element storage (points)
+  std::vector<Box> ebox;     // element bounds
+  std::vector<V3> ecenter;   // element centres (BoundedTraitsN: triangle centroid / point)
+  std::vector<int64_t> order;
+  std::vector<Node> nodes;
+  int leaf_size = 5;
+  int64_t max_depth = 0;
+  float pad = 1.05f;
+  float scale = 0;  // largest |coordinate| over all elements (for the canonical pruning slack)
+
+  int64_t size() const { return points ? (int64_t)pts.size() : (int64_t)tris.size(); }
+
+  void build(int leaf) {
+    int64_t n = size();
+    leaf_size = leaf <= 0 ? 5 : leaf;
+    max_depth = tree_depth(n, leaf_size, 0.5f);
+    nodes.clear();
+    order.resize(n);
+    if (n <= 0) return;
+    Node root;
+    root.depth = 0;
+    root.first = 0; root.count = n;
+    for (int64_t i = 0; i < n; ++i) { order[i] = i; root.bounds.merge(ebox[i]); }
+    for (int k = 0; k < 3; ++k) scale = std::max(scale, std::max(std::fabs(root.bounds.lo[k]), std::fabs(root.bounds.hi[k])));
+    nodes.push_back(root);
+    // Iterative replacement of the recursion; children are created in the same order and each
+    // element range is partitioned exactly as createKdTree does (nth_element on the range).
+    split(0);
+    nodes[0].bounds.scale_centered(pad);
+  }
+
+  void split(int ni) {
+    Node nd = nodes[ni];
+    if (nd.depth >= max_depth || nd.count <= leaf_size) return;
+    int coord = max_axis(nd.bounds.extent());
+    int64_t *b = order.data() + nd.first;
+    size_t mid = (size_t)nd.count / 2;
+    // BvhN.hpp:1320-1335 ObjectLess compares element centres along `coord`.
+    std::nth_element(b, b + mid, b + nd.count, [&](int64_t a, int64_t c) {
+      return ecenter[a][coord] < ecenter[c][coord];
+    });
+    for (int i = 0; i < 2; ++i) {
+      Node ch;
+      ch.depth = nd.depth + 1;
+      int64_t eb = (i == 0) ? 0 : nd.count - (int64_t)mid;
+      int64_t ee = (i == 0) ? nd.count - (int64_t)mid : nd.count;
+      ch.first = nd.first + eb;
+      ch.count = ee - eb;
+      for (int64_t k = ch.first; k < ch.first + ch.count; ++k) ch.bounds.merge(ebox[order[k]]);
+      int ci = (int)nodes.size();
+      nodes.push_back(ch);
+      nodes[ni].child[i] = ci;
+      split(ci);
+      nodes[ci].bounds.scale_centered(pad);
+    }
+  }
+};
+
+struct Stats { int64_t box = 0, tri = 0, node = 0; };
+
+// ---- closest element --------------------------------------------------------------------------
+// Thea order (mode 1): BvhN.hpp:1082-1114 closestPair, :441-471 ClosestPairFunctor,
+// :1866-1893 processNeighbors, :2036-2089 processNeighborsLeaf. Prune when lb >= best,
+// replace when strictly smaller, first found wins ties; a NaN distance is kept like Thea keeps it.
+// Canonical (mode 0): identical arithmetic; result is the lexicographic minimum of (d^2, index)
+// over elements with d^2 < bound^2; NaN distances never win; prune only when lb > best.
+// Canonical pruning treats the query as the box [q - s, q + s], s = 2^-16 max(|q|, mesh scale), so a
+// box lower bound never exceeds the float distance of a contained triangle despite rounding in the
+// triangle arithmetic (Thea instead relies on its x1.05 node padding, BvhN.hpp:2458).
+inline float canon_slack(const KdTree *t, V3 q) {
+  float m = std::max(t->scale, std::max(std::fabs(q.x), std::max(std::fabs(q.y), std::fabs(q.z))));
+  return m * (1.0f / 65536.0f);
+}
+inline float lb_canon(const Box &b, V3 q, float s) {
+  V3 qlo = mk(q.x - s, q.y - s, q.z - s), qhi = mk(q.x + s, q.y + s, q.z + s);
+  V3 vmax = cmin(b.hi, qhi), vmin = cmax(b.lo, qlo);
+  return sqn(cmax(sub(vmin, vmax), mk(0, 0, 0)));
+}
+
+struct ClosestQuery {
+  const KdTree *t;
+  V3 q;
+  double bound2;  // < 0: none
+  int mode;
+  int64_t exclude;  // element index to skip (points, exclude-self), or -1
+  // result
+  int64_t best_idx = -1;
+  double best_mad = -1;
+  V3 best_cp{};
+  Stats st;
+
+  float elem_dist(int64_t e, V3 &cp) {
+    if (t->points) { cp = t->pts[e]; return sqn(sub(cp, q)); }
+    return tri_closest(t->tris[e], q, cp);
+  }
+
+  bool allows_thea(double mad) const { return mad < 0 || best_mad < 0 || mad < best_mad; }
+
+  void leaf(const Node &nd) {
+    for (int64_t k = nd.first; k < nd.first + nd.count; ++k) {
+      int64_t e = t->order[k];
+      if (e == exclude) continue;
+      V3 cp;
+      st.tri++;
+      double mad = (double)elem_dist(e, cp);
+      if (mode == 1) {
+        if (allows_thea(mad)) { if (best_mad < 0 || mad < best_mad) { best_mad = mad; best_idx = e; best_cp = cp; } }
+      } else {
+        if (!(mad == mad)) continue;                    // NaN never wins
+        if (bound2 >= 0 && !(mad < bound2)) continue;   // strict bound
+        if (best_idx < 0 || mad < best_mad || (mad == best_mad && e < best_idx)) {
+          best_mad = mad; best_idx = e; best_cp = cp;
+        }
+      }
+    }
+  }
+
+  void visit(int ni) {
+    const Node &nd = t->nodes[ni];
+    st.node++;
+    if (nd.leaf()) { leaf(nd); return; }
+    // ChildDistanceArray: children inserted in order 0, 1; a later equal one goes after.
+    double lb[2]; int c[2];
+    for (int i = 0; i < 2; ++i) {
+      st.box++;
+      const Box &cb = t->nodes[nd.child[i]].bounds;
+      lb[i] = mode == 1 ? (double)cb.sqdist_point_box(q) : (double)lb_canon(cb, q, canon_slack(t, q));
+    }
+    if (dist_less(lb[1], lb[0])) { c[0] = nd.child[1]; c[1] = nd.child[0]; std::swap(lb[0], lb[1]); }
+    else { c[0] = nd.child[0]; c[1] = nd.child[1]; }
+    for (int i = 0; i < 2; ++i) {
+      if (mode == 1) { if (allows_thea(lb[i])) visit(c[i]); }
+      else {
+        double thr = best_idx >= 0 ? best_mad : bound2;
+        if (thr < 0 || !(lb[i] > thr)) visit(c[i]);
+      }
+    }
+  }
+
+  void run() {
+    if (t->nodes.empty()) return;
+    if (bound2 >= 0) {
+      const Box &rb = t->nodes[0].bounds;
+      double lb = mode == 1 ? (double)rb.sqdist_point_box(q) : (double)lb_canon(rb, q, canon_slack(t, q));
+      if (lb >= 0 && lb > bound2) return;
+    }
+    if (mode == 1) best_mad = bound2 >= 0 ? bound2 : -1;  // ClosestPairFunctor pre-initialisation
+    visit(0);
+    if (mode == 1 && best_idx < 0) best_mad = -1;
+  }
+};
+
+// ---- k nearest --------------------------------------------------------------------------------
+// Thea order (mode 1): BvhN.hpp:1139-1146 kClosestPairs, :475-500 KClosestPairsFunctor,
+// BoundedSortedArray.hpp:185-265 insert/isInsertable/upperBound, ordering by
+// NeighborPair::operator< (ProximityQueryStructureN.hpp:117-123).
+// Canonical (mode 0): the k lexicographically smallest (d^2, index) with d^2 < bound^2.
+struct NP { int64_t qi, ti; double mad; };
+inline bool np_less(const NP &a, const NP &b) {
+  return a.mad < b.mad || (a.mad == b.mad && (a.qi < b.qi || (a.qi == b.qi && a.ti == b.ti)));
+}
+
+struct KnnQuery {
+  const KdTree *t;
+  V3 q;
+  int k;
+  double bound2;
+  int mode;
+  int64_t exclude;
+  std::vector<NP> arr;  // sorted
+  Stats st;
+
+  bool insertable_thea(const NP &p) const { return (int)arr.size() < k || np_less(p, arr.back()); }
+  bool contains(int64_t ti) const { for (auto &a : arr) if (a.ti == ti) return true; return false; }
+  void insert_thea(const NP &p) {
+    if (arr.empty()) { arr.push_back(p); return; }
+    if (!insertable_thea(p)) return;
+    size_t first = 0, count = arr.size();
+    while (count > 0) {  // upperBound
+      size_t step = count >> 1, mid = first + step;
+      if (!np_less(p, arr[mid])) { first = mid + 1; count -= step + 1; } else count = step;
+    }
+    arr.insert(arr.begin() + first, p);
+    if ((int)arr.size() > k) arr.pop_back();
+  }
+  bool canon_less(double m1, int64_t i1, double m2, int64_t i2) const { return m1 < m2 || (m1 == m2 && i1 < i2); }
+  void insert_canon(double mad, int64_t e) {
+    if ((int)arr.size() == k && !canon_less(mad, e, arr.back().mad, arr.back().ti)) return;
+    size_t pos = arr.size();
+    while (pos > 0 && canon_less(mad, e, arr[pos - 1].mad, arr[pos - 1].ti)) --pos;
+    arr.insert(arr.begin() + pos, NP{0, e, mad});
+    if ((int)arr.size() > k) arr.pop_back();
+  }
+
+  float elem_dist(int64_t e) {
+    if (t->points) return sqn(sub(t->pts[e], q));
+    V3 cp; return tri_closest(t->tris[e], q, cp);
+  }
+
+  void leaf(const Node &nd) {
+    for (int64_t kk = nd.first; kk < nd.first + nd.count; ++kk) {
+      int64_t e = t->order[kk];
+      if (mode == 1) {
+        // allows(pair with unknown distance): not already present
+        if (contains(e)) continue;
+        if (e == exclude) continue;  // compatibility check
+        st.tri++;
+        double mad = (double)elem_dist(e);
+        NP probe{-1, -1, mad};
+        if (mad < 0 || insertable_thea(probe)) {
+          NP p{0, e, mad};
+          if ((bound2 < 0 || mad < bound2) && insertable_thea(p)) insert_thea(p);
+        }
+      } else {
+        if (e == exclude) continue;
+        st.tri++;
+        double mad = (double)elem_dist(e);
+        if (!(mad == mad)) continue;
+        if (bound2 >= 0 && !(mad < bound2)) continue;
+        insert_canon(mad, e);
+      }
+    }
+  }
+
+  void visit(int ni) {
+    const Node &nd = t->nodes[ni];
+    st.node++;
+    if (nd.leaf()) { leaf(nd); return; }
+    double lb[2]; int c[2];
+    for (int i = 0; i < 2; ++i) {
+      st.box++;
+      const Box &cb = t->nodes[nd.child[i]].bounds;
+      lb[i] = mode == 1 ? (double)cb.sqdist_point_box(q) : (double)lb_canon(cb, q, canon_slack(t, q));
+    }
+    if (dist_less(lb[1], lb[0])) { c[0] = nd.child[1]; c[1] = nd.child[0]; std::swap(lb[0], lb[1]); }
+    else { c[0] = nd.child[0]; c[1] = nd.child[1]; }
+    for (int i = 0; i < 2; ++i) {
+      if (mode == 1) {
+        NP probe{-1, -1, lb[i]};
+        if (lb[i] < 0 || insertable_thea(probe)) visit(c[i]);
+      } else {
+        bool go = true;
+        if ((int)arr.size() == k && lb[i] > arr.back().mad) go = false;
+        if (bound2 >= 0 && lb[i] > bound2) go = false;
+        if (go) visit(c[i]);
+      }
+    }
+  }
+
+  void run() {
+    if (t->nodes.empty() || k <= 0) return;
+    double lb = (double)t->nodes[0].bounds.sqdist_point_box(q);
+    if (mode == 1) { NP probe{-1, -1, lb}; if (!(lb < 0 || insertable_thea(probe))) return; }
+    visit(0);
+  }
+};
+
+// ---- ray first hit ------------------------------------------------------------------------------
+// Thea order (mode 1): BvhN.hpp:1291-1316 rayStructureIntersection and :2241-2291 recursion,
+// Woo box test with fuzzy zero test. Canonical (mode 0): minimum (t, index) over triangles with
+// t >= 0 and t <= max_time (when max_time >= 0), traversal pruned by a conservative slab test on
+// boxes widened by `slack`.
+struct RayQuery {
+  const KdTree *t;
+  V3 o, d;
+  float max_time;
+  int mode;
+  float slack;
+  float best_t = -1;
+  int64_t best_idx = -1;
+  Stats st;
+
+  // Returns (found, time, idx) Thea style.
+  bool thea_visit(int ni, float mt, float &bt, int64_t &bi) {
+    const Node &nd = t->nodes[ni];
+    st.node++;
+    if (nd.leaf()) {
+      float best = mt; bool found = false; int64_t idx = -1;
+      for (int64_t k = nd.first; k < nd.first + nd.count; ++k) {
+        int64_t e = t->order[k];
+        st.tri++;
+        float tt = tri_ray_isec(t->tris[e], o, d, best);
+        if (dist_less(tt, best)) { best = tt; idx = e; found = true; }
+      }
+      if (found) { bt = best; bi = idx; }
+      return found;
+    }
+    float ct[2]; int c[2];
+    for (int i = 0; i < 2; ++i) { st.box++; ct[i] = t->nodes[nd.child[i]].bounds.ray_time(o, d, mt); }
+    if (dist_less(ct[1], ct[0])) { c[0] = nd.child[1]; c[1] = nd.child[0]; std::swap(ct[0], ct[1]); }
+    else { c[0] = nd.child[0]; c[1] = nd.child[1]; }
+    float best = mt; bool found = false; int64_t idx = -1;
+    for (int i = 0; i < 2; ++i) {
+      if (dist_less(ct[i], best)) {
+        float ht; int64_t hi;
+        if (thea_visit(c[i], best, ht, hi) && dist_less(ht, best)) { best = ht; idx = hi; found = true; }
+      }
+    }
+    if (found) { bt = best; bi = idx; }
+    return found;
+  }
+
+  // Conservative slab entry time (double precision, widened box); +inf when missed.
+  double slab(const Box &b) const {
+    double t0 = 0, t1 = std::numeric_limits<double>::infinity();
+    for (int i = 0; i < 3; ++i) {
+      double lo = (double)b.lo[i] - slack, hi = (double)b.hi[i] + slack;
+      double oi = o[i], di = d[i];
+      if (di == 0) { if (oi < lo || oi > hi) return std::numeric_limits<double>::infinity(); continue; }
+      double a = (lo - oi) / di, c = (hi - oi) / di;
+      if (a > c) std::swap(a, c);
+      t0 = std::max(t0, a); t1 = std::min(t1, c);
+    }
+    if (t0 > t1 * (1 + 1e-9) + 1e-12) return std::numeric_limits<double>::infinity();
+    return t0;
+  }
+
+  void canon_visit(int ni) {
+    const Node &nd = t->nodes[ni];
+    st.node++;
+    if (nd.leaf()) {
+      for (int64_t k = nd.first; k < nd.first + nd.count; ++k) {
+        int64_t e = t->order[k];
+        st.tri++;
+        float tt = tri_ray_isec(t->tris[e], o, d, max_time);
+        if (tt < 0) continue;
+        if (max_time >= 0 && !(tt < max_time)) continue;  // structure level is strict (BvhN.hpp:2257)
+        if (best_idx < 0 || tt < best_t || (tt == best_t && e < best_idx)) { best_t = tt; best_idx = e; }
+      }
+      return;
+    }
+    double ct[2]; int c[2];
+    for (int i = 0; i < 2; ++i) { st.box++; ct[i] = slab(t->nodes[nd.child[i]].bounds); }
+    if (ct[1] < ct[0]) { c[0] = nd.child[1]; c[1] = nd.child[0]; std::swap(ct[0], ct[1]); }
+    else { c[0] = nd.child[0]; c[1] = nd.child[1]; }
+    for (int i = 0; i < 2; ++i) {
+      if (ct[i] == std::numeric_limits<double>::infinity()) continue;
+      double lim = best_idx >= 0 ? (double)best_t : (max_time >= 0 ? (double)max_time : -1);
+      if (lim >= 0 && ct[i] > lim * (1 + 1e-6) + 1e-30) continue;
+      canon_visit(c[i]);
+    }
+  }
+
+  void run() {
+    if (t->nodes.empty()) return;
+    if (mode == 1) {
+      if (t->nodes[0].bounds.ray_time(o, d, max_time) > -0.5f) {
+        float bt; int64_t bi;
+        if (thea_visit(0, max_time, bt, bi)) { best_t = bt; best_idx = bi; }
+      }
+    } else {
+      canon_visit(0);
+    }
+  }
+};
+
+template <typename F> void parallel_for(int64_t n, int nthreads, F f) {
+  if (nthreads <= 1 || n < 2 * nthreads) { f(0, n); return; }
+  std::vector<std::thread> th;
+  int64_t chunk = (n + nthreads - 1) / nthreads;
+  for (int i = 0; i < nthreads; ++i) {
+    int64_t b = i * chunk, e = std::min(n, b + chunk);
+    if (b >= e) break;
+    th.emplace_back([=]() { f(b, e); });
+  }
+  for (auto &x : th) x.join();
+}
+
+inline V3 ld3(const float *p) { return mk(p[0], p[1], p[2]); }
+
+void gather_tris(const float *verts, const int32_t *tris, int64_t nt, std::vector<Tri> &out) {
+  out.resize(nt);
+  for (int64_t i = 0; i < nt; ++i) {
+    for (int k = 0; k < 3; ++k) out[i].v[k] = ld3(verts + 3 * (int64_t)tris[3 * i + k]);
+    tri_update(out[i]);
+  }
+}
+
+//==================================================================================================
+// FP64 reference (closed-form region-based closest point; Ericson, Real-Time Collision Detection,
+// 5.1.5) -- used only to state the error of the float path against exact-ish geometry.
+//==================================================================================================
+struct D3 { double x, y, z; };
+inline D3 dsub(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline D3 dadd(D3 a, D3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline D3 dmul(double s, D3 a) { return {s * a.x, s * a.y, s * a.z}; }
+inline double ddot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline D3 dcross(D3 a, D3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+
+D3 closest_pt_tri_f64(D3 p, D3 a, D3 b, D3 c) {
+  D3 ab = dsub(b, a), ac = dsub(c, a), ap = dsub(p, a);
+  double d1 = ddot(ab, ap), d2 = ddot(ac, ap);
+  if (d1 <= 0 && d2 <= 0) return a;
+  D3 bp = dsub(p, b);
+  double d3 = ddot(ab, bp), d4 = ddot(ac, bp);
+  if (d3 >= 0 && d4 <= d3) return b;
+  double vc = d1 * d4 - d3 * d2;
+  if (vc <= 0 && d1 >= 0 && d3 <= 0) { double v = d1 / (d1 - d3); return dadd(a, dmul(v, ab)); }
+  D3 cp = dsub(p, c);
+  double d5 = ddot(ab, cp), d6 = ddot(ac, cp);
+  if (d6 >= 0 && d5 <= d6) return c;
+  double vb = d5 * d2 - d1 * d6;
+  if (vb <= 0 && d2 >= 0 && d6 <= 0) { double w = d2 / (d2 - d6); return dadd(a, dmul(w, ac)); }
+  double va = d3 * d6 - d5 * d4;
+  if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
+    double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    return dadd(b, dmul(w, dsub(c, b)));
+  }
+  double denom = 1.0 / (va + vb + vc);
+  if (!std::isfinite(denom)) {  // degenerate: best of the three edges
+    D3 best = a; double bd = ddot(dsub(p, a), dsub(p, a));
+    D3 e[3][2] = {{a, b}, {b, c}, {c, a}};
+    for (auto &s : e) {
+      D3 dd = dsub(s[1], s[0]); double L = ddot(dd, dd);
+      double tt = L > 0 ? std::min(1.0, std::max(0.0, ddot(dsub(p, s[0]), dd) / L)) : 0;
+      D3 q = dadd(s[0], dmul(tt, dd)); double qd = ddot(dsub(p, q), dsub(p, q));
+      if (qd < bd) { bd = qd; best = q; }
+    }
+    return best;
+  }
+  double v = vb * denom, w = vc * denom;
+  return dadd(a, dadd(dmul(v, ab), dmul(w, ac)));
+}
+
+// Möller-Trumbore in FP64 (no parallel threshold beyond exact zero).
+double ray_tri_f64(D3 o, D3 d, D3 a, D3 b, D3 c) {
+  D3 e1 = dsub(b, a), e2 = dsub(c, a);
+  D3 pv = dcross(d, e2);
+  double det = ddot(e1, pv);
+  if (det == 0) return -1;
+  double inv = 1.0 / det;
+  D3 tv = dsub(o, a);
+  double u = ddot(tv, pv) * inv;
+  if (u < 0 || u > 1) return -1;
+  D3 qv = dcross(tv, e1);
+  double v = ddot(d, qv) * inv;
+  if (v < 0 || u + v > 1) return -1;
+  double t = ddot(e2, qv) * inv;
+  return t >= 0 ? t : -1;
+}
+
+//==================================================================================================
+// RNGs
+//==================================================================================================
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3", SC'11).
+void philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+  uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+  uint32_t k0 = key_in[0], k1 = key_in[1];
+  for (int r = 0; r < 10; ++r) {
+    uint64_t p0 = (uint64_t)M0 * c0, p1 = (uint64_t)M1 * c2;
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    uint32_t n0 = hi1 ^ c1 ^ k0, n1 = lo1, n2 = hi0 ^ c3 ^ k1, n3 = lo0;
+    c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    if (r < 9) { k0 += W0; k1 += W1; }
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// Random.hpp:126-131 uniform01: (float)bits * (1.0f / (float)0xFFFFFFFF) == bits * 2^-32 in float.
+inline float u01(uint32_t bits) {
+  static const float NORM = 1.0f / (float)0xFFFFFFFFUL;
+  return (float)bits * NORM;
+}
+
+// Random.cpp:49-132: Thea's MT19937 variant (its generate() skips state index N - M).
+struct TheaMT {
+  enum { N = 624, M = 397, R = 31, U = 11, S = 7, T = 15, L = 18 };
+  static const uint32_t A = 0x9908B0DFu, B = 0x9D2C5680u, C = 0xEFC60000u;
+  uint32_t state[N];
+  int32_t index;
+  explicit TheaMT(uint32_t seed) {
+    state[0] = seed;
+    for (index = 1; index < (int32_t)N; ++index)
+      state[index] = 1812433253u * (state[index - 1] ^ (state[index - 1] >> 30)) + (uint32_t)index;
+  }
+  void generate() {
+    const uint32_t LOWER = (1u << R) - 1, UPPER = 0xFFFFFFFFu << R;
+    const uint32_t mag01[2] = {0u, A};
+    for (unsigned i = 0; i < N - M; ++i) {
+      uint32_t x = (state[i] & UPPER) | (state[i + 1] & LOWER);
+      state[i] = state[i + M] ^ (x >> 1) ^ mag01[x & 1];
+    }
+    for (unsigned i = N - M + 1; i < N - 1; ++i) {
+      uint32_t x = (state[i] & UPPER) | (state[i + 1] & LOWER);
+      state[i] = state[i + (M - N)] ^ (x >> 1) ^ mag01[x & 1];
+    }
+    uint32_t y = (state[N - 1] & UPPER) | (state[0] & LOWER);
+    state[N - 1] = state[M - 1] ^ (y >> 1) ^ mag01[y & 1];
+    index = 0;
+  }
+  uint32_t bits() {
+    int32_t li = index;
+    if ((uint32_t)li >= (uint32_t)N) { generate(); li = 0; }
+    ++index;
+    uint32_t r = state[li];
+    r ^= r >> U;
+    r ^= (r << S) & B;
+    r ^= (r << T) & C;
+    r ^= r >> L;
+    return r;
+  }
+};
+
+}  // namespace
+
+//==================================================================================================
+// C interface (ctypes)
+//==================================================================================================
+extern "C" {
+
+// Per-triangle cached properties (Triangle3.hpp:143-155): out[13*i + ...] =
+// n.xyz, d, axis, centroid.xyz, area, e01.x, e01.y, e01.z, (unused)
+void or_tri_props(const float *verts, const int32_t *tris, int64_t nt, float *out) {
+  std::vector<Tri> t;
+  gather_tris(verts, tris, nt, t);
+  for (int64_t i = 0; i < nt; ++i) {
+    float *o = out + 13 * i;
+    o[0] = t[i].n.x; o[1] = t[i].n.y; o[2] = t[i].n.z; o[3] = t[i].d; o[4] = (float)t[i].axis;
+    o[5] = t[i].centroid.x; o[6] = t[i].centroid.y; o[7] = t[i].centroid.z; o[8] = t[i].area;
+    o[9] = t[i].e01.x; o[10] = t[i].e01.y; o[11] = t[i].e01.z; o[12] = 0;
+  }
+}
+
+// Single triangle closest point (Thea float arithmetic). Returns d^2.
+float or_tri_closest(const float *v9, const float *q, float *cp) {
+  Tri t;
+  t.v[0] = ld3(v9); t.v[1] = ld3(v9 + 3); t.v[2] = ld3(v9 + 6);
+  tri_update(t);
+  V3 c;
+  float d2 = tri_closest(t, ld3(q), c);
+  cp[0] = c.x; cp[1] = c.y; cp[2] = c.z;
+  return d2;
+}
+
+float or_tri_ray(const float *v9, const float *o, const float *d, float max_time) {
+  Tri t;
+  t.v[0] = ld3(v9); t.v[1] = ld3(v9 + 3); t.v[2] = ld3(v9 + 6);
+  tri_update(t);
+  return tri_ray_isec(t, ld3(o), ld3(d), max_time);
+}
+
+void *or_kd_build_mesh(const float *verts, const int32_t *tris, int64_t nt, int leaf) {
+  KdTree *t = new KdTree();
+  t->points = false;
+  gather_tris(verts, tris, nt, t->tris);
+  t->ebox.resize(nt); t->ecenter.resize(nt);
+  for (int64_t i = 0; i < nt; ++i) { t->ebox[i] = tri_bounds(t->tris[i]); t->ecenter[i] = t->tris[i].centroid; }
+  t->build(leaf);
+  return t;
+}
+
+void *or_kd_build_points(const float *xyz, int64_t n, int leaf) {
+  KdTree *t = new KdTree();
+  t->points = true;
+  t->pts.resize(n); t->ebox.resize(n); t->ecenter.resize(n);
+  for (int64_t i = 0; i < n; ++i) { t->pts[i] = ld3(xyz + 3 * i); t->ebox[i] = point_bounds(t->pts[i]); t->ecenter[i] = t->pts[i]; }
+  t->build(leaf);
+  return t;
+}
+
+void or_kd_free(void *h) { delete (KdTree *)h; }
+
+int64_t or_kd_num_nodes(void *h) { return (int64_t)((KdTree *)h)->nodes.size(); }
+
+// Node dump: per node lo.xyz, hi.xyz (float), and child0, child1, first, count (int64)
+void or_kd_nodes(void *h, float *boxes, int64_t *links) {
+  KdTree *t = (KdTree *)h;
+  for (size_t i = 0; i < t->nodes.size(); ++i) {
+    const Node &n = t->nodes[i];
+    boxes[6 * i + 0] = n.bounds.lo.x; boxes[6 * i + 1] = n.bounds.lo.y; boxes[6 * i + 2] = n.bounds.lo.z;
+    boxes[6 * i + 3] = n.bounds.hi.x; boxes[6 * i + 4] = n.bounds.hi.y; boxes[6 * i + 5] = n.bounds.hi.z;
+    links[4 * i + 0] = n.child[0]; links[4 * i + 1] = n.child[1]; links[4 * i + 2] = n.first; links[4 * i + 3] = n.count;
+  }
+}
+void or_kd_order(void *h, int64_t *order) {
+  KdTree *t = (KdTree *)h;
+  std::copy(t->order.begin(), t->order.end(), order);
+}
+
+// Closest element per query. bound < 0: none. mode 0 canonical, 1 Thea order.
+// stats (nullable): [sum box tests, sum element tests, sum nodes visited].
+void or_kd_closest(void *h, const float *q, int64_t nq, double dist_bound, int mode, int exclude_self, int nthreads,
+                   int32_t *idx, float *cp, float *d2, int64_t *stats) {
+  KdTree *t = (KdTree *)h;
+  std::atomic<int64_t> sb(0), st(0), sn(0);
+  double bound2 = dist_bound >= 0 ? dist_bound * dist_bound : -1;
+  parallel_for(nq, nthreads, [&](int64_t b, int64_t e) {
+    Stats loc;
+    for (int64_t i = b; i < e; ++i) {
+      ClosestQuery cq{t, ld3(q + 3 * i), bound2, mode, exclude_self ? i : -1};
+      cq.run();
+      idx[i] = (int32_t)cq.best_idx;
+      if (cp) { cp[3 * i] = cq.best_cp.x; cp[3 * i + 1] = cq.best_cp.y; cp[3 * i + 2] = cq.best_cp.z; }
+      if (d2) d2[i] = cq.best_idx >= 0 ? (float)cq.best_mad : -1.0f;
+      loc.box += cq.st.box; loc.tri += cq.st.tri; loc.node += cq.st.node;
+    }
+    sb += loc.box; st += loc.tri; sn += loc.node;
+  });
+  if (stats) { stats[0] = sb; stats[1] = st; stats[2] = sn; }
+}
+
+void or_kd_knn(void *h, const float *q, int64_t nq, int k, double dist_bound, int mode, int exclude_self, int nthreads,
+               int32_t *idx, float *d2, int64_t *stats) {
+  KdTree *t = (KdTree *)h;
+  std::atomic<int64_t> sb(0), st(0), sn(0);
+  double bound2 = dist_bound >= 0 ? dist_bound * dist_bound : -1;
+  parallel_for(nq, nthreads, [&](int64_t b, int64_t e) {
+    Stats loc;
+    for (int64_t i = b; i < e; ++i) {
+      KnnQuery kq{t, ld3(q + 3 * i), k, bound2, mode, exclude_self ? i : -1, {}, {}};
+      kq.run();
+      for (int j = 0; j < k; ++j) {
+        if (j < (int)kq.arr.size()) { idx[i * k + j] = (int32_t)kq.arr[j].ti; d2[i * k + j] = (float)kq.arr[j].mad; }
+        else { idx[i * k + j] = -1; d2[i * k + j] = -1; }
+      }
+      loc.box += kq.st.box; loc.tri += kq.st.tri; loc.node += kq.st.node;
+    }
+    sb += loc.box; st += loc.tri; sn += loc.node;
+  });
+  if (stats) { stats[0] = sb; stats[1] = st; stats[2] = sn; }
+}
+
+void or_kd_ray(void *h, const float *orig, const float *dir, int64_t n, float max_time, int mode, float slack,
+               int nthreads, int32_t *idx, float *tout, int64_t *stats) {
+  KdTree *t = (KdTree *)h;
+  std::atomic<int64_t> sb(0), st(0), sn(0);
+  parallel_for(n, nthreads, [&](int64_t b, int64_t e) {
+    Stats loc;
+    for (int64_t i = b; i < e; ++i) {
+      RayQuery rq{t, ld3(orig + 3 * i), ld3(dir + 3 * i), max_time, mode, slack};
+      rq.run();
+      idx[i] = (int32_t)rq.best_idx;
+      tout[i] = rq.best_idx >= 0 ? rq.best_t : -1.0f;
+      loc.box += rq.st.box; loc.tri += rq.st.tri; loc.node += rq.st.node;
+    }
+    sb += loc.box; st += loc.tri; sn += loc.node;
+  });
+  if (stats) { stats[0] = sb; stats[1] = st; stats[2] = sn; }
+}
+
+// Brute force, canonical (lexicographic (d^2, index), NaN never wins, strict bound).
+void or_brute_closest(const float *verts, const int32_t *tris, int64_t nt, const float *q, int64_t nq, double dist_bound,
+                      int nthreads, int32_t *idx, float *cp, float *d2) {
+  std::vector<Tri> t;
+  gather_tris(verts, tris, nt, t);
+  double bound2 = dist_bound >= 0 ? dist_bound * dist_bound : -1;
+  parallel_for(nq, nthreads, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) {
+      V3 p = ld3(q + 3 * i), bc{};
+      int64_t bi = -1; float bd = 0;
+      for (int64_t j = 0; j < nt; ++j) {
+        V3 c;
+        float d = tri_closest(t[j], p, c);
+        if (!(d == d)) continue;
+        if (bound2 >= 0 && !((double)d < bound2)) continue;
+        if (bi < 0 || d < bd) { bi = j; bd = d; bc = c; }
+      }
+      idx[i] = (int32_t)bi;
+      if (cp) { cp[3 * i] = bc.x; cp[3 * i + 1] = bc.y; cp[3 * i + 2] = bc.z; }
+      if (d2) d2[i] = bi >= 0 ? bd : -1.0f;
+    }
+  });
+}
+
+// FP64 brute force closest distance (Ericson) -> exact-ish distance and its argmin.
+void or_brute_closest_f64(const float *verts, const int32_t *tris, int64_t nt, const float *q, int64_t nq, int nthreads,
+                          int32_t *idx, double *dist) {
+  parallel_for(nq, nthreads, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) {
+      D3 p{q[3 * i], q[3 * i + 1], q[3 * i + 2]};
+      int64_t bi = -1; double bd = 0;
+      for (int64_t j = 0; j < nt; ++j) {
+        const float *a = verts + 3 * (int64_t)tris[3 * j], *bb = verts + 3 * (int64_t)tris[3 * j + 1],
+                    *c = verts + 3 * (int64_t)tris[3 * j + 2];
+        D3 cp = closest_pt_tri_f64(p, {a[0], a[1], a[2]}, {bb[0], bb[1], bb[2]}, {c[0], c[1], c[2]});
+        D3 dd = dsub(cp, p);
+        double d = ddot(dd, dd);
+        if (bi < 0 || d < bd) { bi = j; bd = d; }
+      }
+      idx[i] = (int32_t)bi;
+      dist[i] = bi >= 0 ? std::sqrt(bd) : -1;
+    }
+  });
+}
+
+// FP64 distance from each query to one given triangle per query (for error statements).
+void or_dist_f64(const float *verts, const int32_t *tris, const float *q, const int32_t *which, int64_t nq, double *dist) {
+  for (int64_t i = 0; i < nq; ++i) {
+    if (which[i] < 0) { dist[i] = -1; continue; }
+    int64_t j = which[i];
+    const float *a = verts + 3 * (int64_t)tris[3 * j], *bb = verts + 3 * (int64_t)tris[3 * j + 1],
+                *c = verts + 3 * (int64_t)tris[3 * j + 2];
+    D3 p{q[3 * i], q[3 * i + 1], q[3 * i + 2]};
+    D3 cp = closest_pt_tri_f64(p, {a[0], a[1], a[2]}, {bb[0], bb[1], bb[2]}, {c[0], c[1], c[2]});
+    D3 dd = dsub(cp, p);
+    dist[i] = std::sqrt(ddot(dd, dd));
+  }
+}
+
+void or_brute_knn(const float *pts, int64_t n, const float *q, int64_t nq, int k, double dist_bound, int exclude_self,
+                  int nthreads, int32_t *idx, float *d2) {
+  double bound2 = dist_bound >= 0 ? dist_bound * dist_bound : -1;
+  parallel_for(nq, nthreads, [&](int64_t b, int64_t e) {
+    std::vector<std::pair<float, int64_t>> c;
+    for (int64_t i = b; i < e; ++i) {
+      c.clear();
+      V3 p = ld3(q + 3 * i);
+      for (int64_t j = 0; j < n; ++j) {
+        if (exclude_self && j == i) continue;
+        float d = sqn(sub(ld3(pts + 3 * j), p));
+        if (!(d == d)) continue;
+        if (bound2 >= 0 && !((double)d < bound2)) continue;
+        c.emplace_back(d, j);
+      }
+      size_t kk = std::min<size_t>(k, c.size());
+      std::partial_sort(c.begin(), c.begin() + kk, c.end());
+      for (int j = 0; j < k; ++j) {
+        if (j < (int)kk) { idx[i * k + j] = (int32_t)c[j].second; d2[i * k + j] = c[j].first; }
+        else { idx[i * k + j] = -1; d2[i * k + j] = -1; }
+      }
+    }
+  });
+}
+
+void or_brute_ray(const float *verts, const int32_t *tris, int64_t nt, const float *orig, const float *dir, int64_t n,
+                  float max_time, int nthreads, int32_t *idx, float *tout) {
+  std::vector<Tri> t;
+  gather_tris(verts, tris, nt, t);
+  parallel_for(n, nthreads, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) {
+      V3 o = ld3(orig + 3 * i), d = ld3(dir + 3 * i);
+      int64_t bi = -1; float bt = -1;
+      for (int64_t j = 0; j < nt; ++j) {
+        float tt = tri_ray_isec(t[j], o, d, max_time);
+        if (tt < 0) continue;
+        if (max_time >= 0 && !(tt < max_time)) continue;  // structure level is strict (BvhN.hpp:2257)
+        if (bi < 0 || tt < bt) { bi = j; bt = tt; }
+      }
+      idx[i] = (int32_t)bi;
+      tout[i] = bt;
+    }
+  });
+}
+
+// FP64 hit time of ray i against triangle which[i] (Möller-Trumbore in double).
+void or_ray_f64(const float *verts, const int32_t *tris, const float *orig, const float *dir, const int32_t *which,
+                int64_t n, double *tout) {
+  for (int64_t i = 0; i < n; ++i) {
+    if (which[i] < 0) { tout[i] = -1; continue; }
+    int64_t j = which[i];
+    const float *a = verts + 3 * (int64_t)tris[3 * j], *b = verts + 3 * (int64_t)tris[3 * j + 1],
+                *c = verts + 3 * (int64_t)tris[3 * j + 2];
+    tout[i] = ray_tri_f64({orig[3 * i], orig[3 * i + 1], orig[3 * i + 2]}, {dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]},
+                          {a[0], a[1], a[2]}, {b[0], b[1], b[2]}, {c[0], c[1], c[2]});
+  }
+}
+
+void or_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out) { philox4x32_10(ctr, key, out); }
+
+float or_u01(uint32_t bits) { return u01(bits); }
+
+void or_thea_mt(uint32_t seed, int64_t n, uint32_t *out) {
+  TheaMT mt(seed);
+  for (int64_t i = 0; i < n; ++i) out[i] = mt.bits();
+}
+void or_thea_mt_u01(uint32_t seed, int64_t n, float *out) {
+  TheaMT mt(seed);
+  for (int64_t i = 0; i < n; ++i) out[i] = u01(mt.bits());
+}
+
+// Area-weighted sampling, EXACT count mode (MeshSampler.hpp:100-204 + Triangle3.hpp:179-195),
+// with Thea's three uniform01 draws per sample taken from Philox4x32-10:
+//   counter = (sample index lo, sample index hi, stream lo, stream hi), key = (seed lo, seed hi);
+//   words 0, 1, 2 -> triangle pick, s, t.
+// Returns the number of samples (0 for an empty or zero-area mesh).
+int64_t or_sample_area(const float *verts, const int32_t *tris, int64_t nt, int64_t n, uint64_t seed, uint64_t stream,
+                       float *pos, float *nrm, int32_t *tri_out, double *cdf_out) {
+  if (n <= 0 || nt <= 0) return 0;
+  std::vector<Tri> t;
+  gather_tris(verts, tris, nt, t);
+  double total = 0;
+  for (int64_t i = 0; i < nt; ++i) total += t[i].area;
+  if (total <= 0) return 0;
+  std::vector<double> cum(nt);
+  cum[0] = t[0].area;
+  for (int64_t i = 1; i < nt; ++i) cum[i] = cum[i - 1] + t[i].area;
+  if (cdf_out) std::copy(cum.begin(), cum.end(), cdf_out);
+  uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  for (int64_t i = 0; i < n; ++i) {
+    uint32_t ctr[4] = {(uint32_t)i, (uint32_t)((uint64_t)i >> 32), (uint32_t)stream, (uint32_t)(stream >> 32)};
+    uint32_t w[4];
+    philox4x32_10(ctr, key, w);
+    double tt = (double)u01(w[0]) * total;
+    int64_t ti = std::min((int64_t)(std::lower_bound(cum.begin(), cum.end(), tt) - cum.begin()), nt - 1);
+    const Tri &tr = t[ti];
+    float s = u01(w[1]), u = u01(w[2]);
+    if (s + u > 1) { u = 1 - u; s = 1 - s; }
+    V3 p = add(add(mul(s, tr.e01), mul(u, tr.e02)), tr.v[0]);
+    pos[3 * i] = p.x; pos[3 * i + 1] = p.y; pos[3 * i + 2] = p.z;
+    if (nrm) { nrm[3 * i] = tr.n.x; nrm[3 * i + 1] = tr.n.y; nrm[3 * i + 2] = tr.n.z; }
+    if (tri_out) tri_out[i] = (int32_t)ti;
+  }
+  return n;
+}
+
+// GeodesicSphere3.cpp:28-134, restated: icosahedron, recursive 4-split with a shared midpoint map.
+// Returns the vertex count; writes up to the given capacities.
+int64_t or_geodesic_sphere(int depth, float *verts, int64_t vcap, int32_t *tris, int64_t tcap, int64_t *ntris) {
+  const float X = 0.525731112119133606f, Z = 0.850650808352039932f;
+  std::vector<V3> v = {mk(-X, 0, Z), mk(X, 0, Z), mk(-X, 0, -Z), mk(X, 0, -Z), mk(0, Z, X), mk(0, Z, -X),
+                       mk(0, -Z, X), mk(0, -Z, -X), mk(Z, X, 0), mk(-Z, X, 0), mk(Z, -X, 0), mk(-Z, -X, 0)};
+  static const int T[20][3] = {{1, 4, 0}, {4, 9, 0}, {4, 5, 9}, {8, 5, 4}, {1, 8, 4}, {1, 10, 8}, {10, 3, 8},
+                               {8, 3, 5}, {3, 2, 5}, {3, 7, 2}, {3, 10, 7}, {10, 6, 7}, {6, 11, 7}, {6, 0, 11},
+                               {6, 1, 0}, {10, 1, 6}, {11, 0, 9}, {2, 11, 9}, {5, 2, 9}, {11, 2, 7}};
+  std::vector<int32_t> out;
+  std::map<std::pair<int64_t, int64_t>, int64_t> mids;
+  auto midpoint = [&](int64_t i, int64_t j) -> int64_t {
+    auto key = i < j ? std::make_pair(i, j) : std::make_pair(j, i);
+    auto it = mids.find(key);
+    if (it != mids.end()) return it->second;
+    V3 p = normalized(add(v[i], v[j]));
+    mids[key] = (int64_t)v.size();
+    v.push_back(p);
+    return (int64_t)v.size() - 1;
+  };
+  std::function<void(int, int64_t, int64_t, int64_t)> sub4 = [&](int d, int64_t i0, int64_t i1, int64_t i2) {
+    if (d <= 0) { out.push_back((int32_t)i0); out.push_back((int32_t)i1); out.push_back((int32_t)i2); return; }
+    int64_t m01 = midpoint(i0, i1);
+    int64_t m12 = midpoint(i1, i2);
+    int64_t m20 = midpoint(i2, i0);
+    sub4(d - 1, i0, m01, m20);
+    sub4(d - 1, i1, m12, m01);
+    sub4(d - 1, i2, m20, m12);
+    sub4(d - 1, m01, m12, m20);
+  };
+  for (int i = 0; i < 20; ++i) sub4(depth, T[i][0], T[i][1], T[i][2]);
+  for (int64_t i = 0; i < (int64_t)v.size() && i < vcap; ++i) { verts[3 * i] = v[i].x; verts[3 * i + 1] = v[i].y; verts[3 * i + 2] = v[i].z; }
+  int64_t nt = (int64_t)out.size() / 3;
+  for (int64_t i = 0; i < 3 * nt && i < 3 * tcap; ++i) tris[i] = out[i];
+  if (ntris) *ntris = nt;
+  return (int64_t)v.size();
+}
+
+}  // extern "C"

@@ -1,0 +1,203 @@
+"""GPU parity: closest element / closest point through the C ABI against the oracle.
+
+Bar: indices, squared distances and closest points bit-exact against the oracle's canonical
+restatement of Thea's float arithmetic (lowest index on exact ties); the reported distance equals
+float(sqrt(double(d^2))), which is Thea's sqrt((double)d^2) rounded to float; the error of the float
+path against an FP64 closed form is printed for each case.
+"""
+import numpy as np
+import pytest
+
+from conftest import random_mesh
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_closest(verts, tris, q, leaf=4, dist_bound=-1.0):
+    import thea_amd
+    m = thea_amd.Mesh(verts, tris)
+    b = thea_amd.MeshBvh(m, leaf_size=leaf)
+    idx, dist, cp, d2 = b.closest(q, dist_bound=dist_bound, want_d2=True)
+    return idx, dist, cp, d2, b.info
+
+
+def _assert_same(idx, cp, d2, ridx, rcp, rd2, label):
+    bad = np.nonzero(idx != ridx)[0]
+    assert len(bad) == 0, f"{label}: {len(bad)} index mismatches, first {bad[:5]}: gpu {idx[bad[:5]]} ref {ridx[bad[:5]]}"
+    hit = ridx >= 0
+    assert np.array_equal(d2[hit].view(np.uint32), rd2[hit].view(np.uint32)), f"{label}: d2 bits differ"
+    assert np.array_equal(cp[hit].view(np.uint32), rcp[hit].view(np.uint32)), f"{label}: closest point bits differ"
+
+
+def _dist_rule(dist, d2, idx):
+    hit = idx >= 0
+    want = np.sqrt(d2[hit].astype(np.float64)).astype(np.float32)
+    assert np.array_equal(dist[hit], want)
+    assert np.all(dist[~hit] == -1)
+
+
+def test_c1_sphere_full(oracle):
+    from thea_amd import workloads as W
+    v, t, q = W.sphere_c1()
+    print(f"case C1 sphere: {len(t)} tris x {len(q)} queries", flush=True)
+    idx, dist, cp, d2, info = _gpu_closest(v, t, q)
+    kd = oracle.KdTree(v, t)
+    ridx, rcp, rd2, _ = kd.closest(q, mode=0)
+    _assert_same(idx, cp, d2, ridx, rcp, rd2, "C1")
+    _dist_rule(dist, d2, idx)
+    # brute force on a subset pins the kd-tree oracle as well
+    bidx, bcp, bd2 = oracle.brute_closest(v, t, q[:2000])
+    _assert_same(idx[:2000], cp[:2000], d2[:2000], bidx, bcp, bd2, "C1 brute")
+    f64 = oracle.dist_f64(v, t, q, idx)
+    err = np.abs(dist.astype(np.float64) - f64)
+    print(f"C1 max |dist - fp64| = {err.max():.3e} (rel {np.max(err / np.maximum(f64, 1e-30)):.3e}); bvh {info}")
+    assert err.max() < 1e-5
+
+
+def test_c3_torus_subset(oracle):
+    from thea_amd import workloads as W
+    v, t, q, _, _ = W.torus_c3(n_queries=200_000, n_rays=0)
+    print(f"case C3 torus subset: {len(t)} tris x {len(q)} queries", flush=True)
+    idx, dist, cp, d2, info = _gpu_closest(v, t, q)
+    kd = oracle.KdTree(v, t)
+    ridx, rcp, rd2, st = kd.closest(q, mode=0)
+    _assert_same(idx, cp, d2, ridx, rcp, rd2, "C3")
+    _dist_rule(dist, d2, idx)
+    f64 = oracle.dist_f64(v, t, q[:20000], idx[:20000])
+    err = np.abs(dist[:20000].astype(np.float64) - f64)
+    print(f"C3 max |dist - fp64| = {err.max():.3e}; bvh {info}; thea V_box={st[0]/len(q):.1f} V_tri={st[1]/len(q):.1f}")
+
+
+@pytest.mark.parametrize("leaf", [1, 2, 4, 8, 16])
+def test_leaf_sizes_identical(oracle, leaf):
+    g = np.random.default_rng(7)
+    v, t = random_mesh(g, 3000)
+    q = g.uniform(-1.2, 1.2, size=(20000, 3)).astype(np.float32)
+    print(f"case leaf={leaf}", flush=True)
+    idx, dist, cp, d2, _ = _gpu_closest(v, t, q, leaf=leaf)
+    ridx, rcp, rd2 = oracle.brute_closest(v, t, q)
+    _assert_same(idx, cp, d2, ridx, rcp, rd2, f"leaf {leaf}")
+
+
+def test_exact_ties_lowest_index(oracle):
+    # Duplicate every triangle several times and query exactly at shared vertices: every candidate
+    # has a bit-identical distance, and the lowest index must win.
+    from thea_amd import workloads as W
+    v, t = W.geodesic_sphere(3)
+    t4 = np.concatenate([t[::-1], t, t[::7]]).astype(np.int32)
+    q = np.concatenate([v, v * np.float32(1.5), v * np.float32(0.5)]).astype(np.float32)
+    print("case ties", flush=True)
+    idx, dist, cp, d2, _ = _gpu_closest(v, t4, q)
+    ridx, rcp, rd2 = oracle.brute_closest(v, t4, q)
+    _assert_same(idx, cp, d2, ridx, rcp, rd2, "ties")
+    kd = oracle.KdTree(v, t4)
+    tidx, _, td2, _ = kd.closest(q, mode=1)
+    assert np.array_equal(td2, d2)  # Thea's traversal order finds the same distance
+    print(f"ties: {np.sum(tidx != idx)} of {len(q)} queries differ from Thea traversal order only by an exact tie")
+
+
+def test_degenerate_triangles(oracle):
+    g = np.random.default_rng(11)
+    v, t = random_mesh(g, 500)
+    # zero-area triangles: repeated vertex, collinear, all three equal
+    v = v.copy()
+    v[3 * 10 + 1] = v[3 * 10]
+    v[3 * 20 + 2] = v[3 * 20]
+    v[3 * 30 + 2] = 2 * v[3 * 30 + 1] - v[3 * 30]
+    v[3 * 40 + 1] = v[3 * 40]
+    v[3 * 40 + 2] = v[3 * 40]
+    q = np.concatenate([g.uniform(-1.2, 1.2, size=(5000, 3)), v[[30, 31, 40, 60, 61]]]).astype(np.float32)
+    print("case degenerate", flush=True)
+    idx, dist, cp, d2, _ = _gpu_closest(v, t, q)
+    ridx, rcp, rd2 = oracle.brute_closest(v, t, q)
+    _assert_same(idx, cp, d2, ridx, rcp, rd2, "degenerate")
+
+
+@pytest.mark.parametrize("bound", [0.0, 0.05, 0.1, 0.3, 1e-3])
+def test_dist_bound(oracle, bound):
+    g = np.random.default_rng(5)
+    v, t = random_mesh(g, 2000)
+    q = g.uniform(-1.2, 1.2, size=(20000, 3)).astype(np.float32)
+    print(f"case bound={bound}", flush=True)
+    idx, dist, cp, d2, _ = _gpu_closest(v, t, q, dist_bound=bound)
+    ridx, rcp, rd2 = oracle.brute_closest(v, t, q, dist_bound=bound)
+    _assert_same(idx, cp, d2, ridx, rcp, rd2, f"bound {bound}")
+    assert np.all(dist[idx >= 0] <= np.float32(bound) * (1 + 1e-6))
+
+
+def test_empty_and_tiny(oracle):
+    import thea_amd
+    g = np.random.default_rng(3)
+    print("case empty/tiny", flush=True)
+    # empty query batch
+    v, t = random_mesh(g, 10)
+    idx, dist, cp, d2, _ = _gpu_closest(v, t, np.zeros((0, 3), np.float32))
+    assert idx.shape == (0,)
+    # empty mesh
+    m = thea_amd.Mesh(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.int32))
+    b = thea_amd.MeshBvh(m)
+    q = g.uniform(-1, 1, size=(100, 3)).astype(np.float32)
+    idx, dist, cp = b.closestPoint(q)
+    assert np.all(idx == -1) and np.all(dist == -1)
+    # single triangle, and fewer triangles than a leaf
+    for nt in (1, 2, 3, 5):
+        v, t = random_mesh(g, nt)
+        idx, dist, cp, d2, _ = _gpu_closest(v, t, q)
+        ridx, rcp, rd2 = oracle.brute_closest(v, t, q)
+        _assert_same(idx, cp, d2, ridx, rcp, rd2, f"nt={nt}")
+
+
+def test_far_queries(oracle):
+    g = np.random.default_rng(9)
+    v, t = random_mesh(g, 1000)
+    q = (g.normal(size=(5000, 3)) * 1e3).astype(np.float32)
+    print("case far queries", flush=True)
+    idx, dist, cp, d2, _ = _gpu_closest(v, t, q)
+    ridx, rcp, rd2 = oracle.brute_closest(v, t, q)
+    _assert_same(idx, cp, d2, ridx, rcp, rd2, "far")
+
+
+def test_host_and_device_pointers_agree():
+    import torch
+    import thea_amd
+    from thea_amd import workloads as W
+    v, t, q = W.sphere_c1(n_queries=50_000)
+    print("case host vs device pointers", flush=True)
+    m = thea_amd.Mesh(v, t)
+    b = thea_amd.MeshBvh(m)
+    hidx, hdist, hcp = b.closestPoint(q)
+    qd = torch.from_numpy(q).cuda()
+    didx, ddist, dcp = b.closestPoint(qd)
+    torch.cuda.synchronize()
+    assert np.array_equal(hidx, didx.cpu().numpy())
+    assert np.array_equal(hdist, ddist.cpu().numpy())
+    assert np.array_equal(hcp, dcp.cpu().numpy())
+    # device-resident mesh
+    m2 = thea_amd.Mesh(torch.from_numpy(v).cuda(), torch.from_numpy(t).cuda())
+    b2 = thea_amd.MeshBvh(m2)
+    assert np.array_equal(b2.closestElement(q), hidx)
+
+
+def test_c3_full_size_properties():
+    import torch
+    import thea_amd
+    from thea_amd import workloads as W
+    v, t, q, _, _ = W.torus_c3(n_rays=0)
+    print(f"case C3 full size: {len(t)} tris x {len(q)} queries", flush=True)
+    m = thea_amd.Mesh(v, t)
+    b = thea_amd.MeshBvh(m)
+    qd = torch.from_numpy(q).cuda()
+    idx, dist, cp = b.closestPoint(qd)
+    torch.cuda.synchronize()
+    idx, dist, cp = idx.cpu().numpy(), dist.cpu().numpy(), cp.cpu().numpy()
+    assert idx.min() >= 0 and idx.max() < len(t)
+    # closest point lies on the reported triangle's box and at the reported distance
+    tri = v[t[idx]]
+    lo, hi = tri.min(1), tri.max(1)
+    tol = 1e-5
+    assert np.all(cp >= lo - tol) and np.all(cp <= hi + tol)
+    dd = np.sqrt(((cp.astype(np.float64) - q) ** 2).sum(1))
+    assert np.max(np.abs(dd - dist)) < 1e-5
+    # idempotence: the closest point of a closest point is itself, at distance ~0
+    i2, d2_, cp2 = b.closestPoint(cp[:100000])
+    assert np.max(d2_) < 1e-5

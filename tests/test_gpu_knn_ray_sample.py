@@ -1,0 +1,225 @@
+"""GPU parity: k nearest, ray first/any hit and area-weighted sampling against the oracle.
+
+Bar: bit-exact indices and float values against the oracle (canonical (value, index) ordering);
+rays also report their error against an FP64 Möller-Trumbore hit time.
+"""
+import numpy as np
+import pytest
+
+from conftest import random_mesh
+
+pytestmark = pytest.mark.gpu
+
+
+# --------------------------------------------------------------------------------------------------
+# k nearest
+# --------------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("k", [1, 3, 8, 16, 32])
+def test_knn_points_vs_brute(oracle, k):
+    import thea_amd
+    g = np.random.default_rng(100 + k)
+    pts = g.uniform(-1, 1, size=(20000, 3)).astype(np.float32)
+    q = g.uniform(-1.1, 1.1, size=(5000, 3)).astype(np.float32)
+    print(f"case knn points k={k}", flush=True)
+    b = thea_amd.PointBvh(pts)
+    idx, dist, d2 = b.kClosestElements(q, k, want_d2=True)
+    ridx, rd2 = oracle.brute_knn(pts, q, k)
+    assert np.array_equal(idx, ridx)
+    assert np.array_equal(d2.view(np.uint32), rd2.view(np.uint32))
+
+
+def test_knn_exclude_self_and_duplicates(oracle):
+    import thea_amd
+    g = np.random.default_rng(42)
+    pts = g.uniform(-1, 1, size=(8000, 3)).astype(np.float32)
+    pts = np.concatenate([pts, pts[:2000]])  # exact duplicates -> exact ties
+    print("case knn exclude_self", flush=True)
+    b = thea_amd.PointBvh(pts)
+    idx, dist, d2 = b.kClosestElements(pts, 16, exclude_self=True, want_d2=True)
+    ridx, rd2 = oracle.brute_knn(pts, pts, 16, exclude_self=True)
+    assert np.array_equal(idx, ridx)
+    assert np.array_equal(d2.view(np.uint32), rd2.view(np.uint32))
+    assert not np.any(idx == np.arange(len(pts))[:, None])
+
+
+@pytest.mark.parametrize("bound", [0.02, 0.05])
+def test_knn_bound_short_rows(oracle, bound):
+    import thea_amd
+    g = np.random.default_rng(8)
+    pts = g.uniform(-1, 1, size=(10000, 3)).astype(np.float32)
+    q = g.uniform(-1, 1, size=(3000, 3)).astype(np.float32)
+    print(f"case knn bound {bound}", flush=True)
+    b = thea_amd.PointBvh(pts)
+    idx, dist, d2 = b.kClosestElements(q, 8, dist_bound=bound, want_d2=True)
+    ridx, rd2 = oracle.brute_knn(pts, q, 8, dist_bound=bound)
+    assert np.array_equal(idx, ridx)
+    assert np.array_equal(d2.view(np.uint32), rd2.view(np.uint32))
+    assert np.all(dist[idx < 0] == -1)
+
+
+def test_knn_on_triangles(oracle):
+    import thea_amd
+    g = np.random.default_rng(12)
+    v, t = random_mesh(g, 1500)
+    q = g.uniform(-1.2, 1.2, size=(3000, 3)).astype(np.float32)
+    print("case knn triangles", flush=True)
+    b = thea_amd.MeshBvh(thea_amd.Mesh(v, t))
+    idx, dist, d2 = b.kClosestElements(q, 5, want_d2=True)
+    kd = oracle.KdTree(v, t)
+    ridx, rd2, _ = kd.knn(q, 5, mode=0)
+    assert np.array_equal(idx, ridx)
+    assert np.array_equal(d2.view(np.uint32), rd2.view(np.uint32))
+    # row 0 of k-NN equals the closest element
+    assert np.array_equal(idx[:, 0], b.closestElement(q))
+
+
+def test_c2_knn_graph(oracle):
+    """C2: 64 ellipsoids, 1M area samples (GPU sampler), 16-NN graph without self."""
+    import torch
+    import thea_amd
+    from thea_amd import workloads as W
+    v, t = W.ellipsoids_c2()
+    print(f"case C2: {len(t)} tris, 1M samples, k=16", flush=True)
+    m = thea_amd.Mesh(v, t)
+    pos, nrm, tri = thea_amd.MeshSampler(m).sampleEvenlyByArea(1_000_000, seed=2, stream_id=W.STREAM_SAMPLES)
+    opos, onrm, otri = oracle.sample_area(v, t, 1_000_000, 2, W.STREAM_SAMPLES)
+    assert np.array_equal(tri, otri)
+    assert np.array_equal(pos.view(np.uint32), opos.view(np.uint32))
+    assert np.array_equal(nrm.view(np.uint32), onrm.view(np.uint32))
+    b = thea_amd.PointBvh(torch.from_numpy(pos).cuda())
+    idx, dist = b.kClosestElements(torch.from_numpy(pos).cuda(), 16, exclude_self=True)
+    torch.cuda.synchronize()
+    idx = idx.cpu().numpy()
+    sub = np.arange(0, len(pos), 500)
+    # exclude_self needs query index == element index: brute-force 17-NN of the subset, minus self
+    ridx2, rd22 = oracle.brute_knn(pos, pos[sub], 17)
+    # drop self from the 17-NN rows (self has d2 == 0 and the lowest index among exact duplicates
+    # only when it is the first; remove the query's own index wherever it is)
+    rows = []
+    for r, qi in enumerate(sub):
+        row = [x for x in ridx2[r] if x != qi][:16]
+        rows.append(row)
+    assert np.array_equal(idx[sub], np.array(rows, np.int32))
+
+
+# --------------------------------------------------------------------------------------------------
+# Rays
+# --------------------------------------------------------------------------------------------------
+def _ray_case(oracle, v, t, o, d, max_time=-1.0, label=""):
+    import thea_amd
+    b = thea_amd.MeshBvh(thea_amd.Mesh(v, t))
+    idx, tt, nrm = b.rayStructureIntersection(o, d, max_time=max_time)
+    ridx, rt = oracle.brute_ray(v, t, o, d, max_time=max_time)
+    bad = np.nonzero(idx != ridx)[0]
+    assert len(bad) == 0, f"{label}: {len(bad)} hit-id mismatches, first {bad[:5]}: {idx[bad[:5]]} vs {ridx[bad[:5]]}"
+    assert np.array_equal(tt.view(np.uint32), rt.view(np.uint32)), f"{label}: hit time bits differ"
+    hit = b.rayIntersects(o, d, max_time=max_time)
+    assert np.array_equal(hit.astype(bool), idx >= 0), f"{label}: any-hit disagrees with first-hit"
+    return b, idx, tt, nrm
+
+
+def test_rays_random_mesh(oracle):
+    g = np.random.default_rng(21)
+    v, t = random_mesh(g, 3000, size=0.08)
+    o = g.uniform(-1.3, 1.3, size=(20000, 3)).astype(np.float32)
+    d = g.normal(size=(20000, 3)).astype(np.float32)
+    print("case rays random", flush=True)
+    b, idx, tt, nrm = _ray_case(oracle, v, t, o, d, label="random")
+    h = idx >= 0
+    t64 = oracle.ray_f64(v, t, o, d, idx)
+    err = np.abs(tt[h] - t64[h]) / np.maximum(t64[h], 1e-6)
+    print(f"rays: {h.mean():.3f} hit; max rel |t - t64| = {err.max():.3e}")
+    props = oracle.tri_props(v, t)
+    assert np.array_equal(nrm[h], props[idx[h], 0:3])
+
+
+@pytest.mark.parametrize("max_time", [0.0, 0.1, 0.5])
+def test_rays_max_time(oracle, max_time):
+    g = np.random.default_rng(22)
+    v, t = random_mesh(g, 2000, size=0.08)
+    o = g.uniform(-1.3, 1.3, size=(10000, 3)).astype(np.float32)
+    d = g.normal(size=(10000, 3)).astype(np.float32)
+    print(f"case rays max_time={max_time}", flush=True)
+    _ray_case(oracle, v, t, o, d, max_time=max_time, label=f"max_time {max_time}")
+
+
+def test_rays_axis_aligned_and_grazing(oracle):
+    from thea_amd import workloads as W
+    v, t = W.uv_sphere(30, 60)
+    v = v.astype(np.float32)
+    g = np.random.default_rng(23)
+    o = g.uniform(-2, 2, size=(6000, 3)).astype(np.float32)
+    d = np.zeros((6000, 3), np.float32)
+    ax = g.integers(0, 3, size=6000)
+    d[np.arange(6000), ax] = np.where(g.random(6000) < 0.5, -1.0, 1.0)
+    o[:100] = v[t[:100, 0]]  # rays starting exactly on vertices
+    print("case rays axis-aligned", flush=True)
+    _ray_case(oracle, v, t, o, d, label="axis")
+
+
+def test_c3_rays_subset(oracle):
+    from thea_amd import workloads as W
+    v, t, _, o, d = W.torus_c3(n_queries=0, n_rays=200_000)
+    print(f"case C3 rays: {len(t)} tris x {len(o)} rays", flush=True)
+    import thea_amd
+    b = thea_amd.MeshBvh(thea_amd.Mesh(v, t))
+    idx, tt, _ = b.rayStructureIntersection(o, d)
+    kd = oracle.KdTree(v, t)
+    ridx, rt, st = kd.ray(o, d, mode=0)
+    assert np.array_equal(idx, ridx)
+    assert np.array_equal(tt.view(np.uint32), rt.view(np.uint32))
+    bidx, bt = oracle.brute_ray(v, t, o[:300], d[:300])
+    assert np.array_equal(idx[:300], bidx)
+    tidx, tt2, _ = kd.ray(o, d, mode=1)
+    same_t = np.array_equal(tt2[tidx >= 0], tt[tidx >= 0])
+    print(f"C3 rays: hit rate {np.mean(idx >= 0):.3f}; Thea-order ids differ on {np.sum(tidx != idx)} rays; "
+          f"times equal where Thea hits: {same_t}; V_box={st[0]/len(o):.1f} V_tri={st[1]/len(o):.1f}")
+
+
+# --------------------------------------------------------------------------------------------------
+# Sampling
+# --------------------------------------------------------------------------------------------------
+def test_sampling_matches_oracle(oracle):
+    import thea_amd
+    from thea_amd import workloads as W
+    v, t, _ = W.sphere_c1(n_queries=0)
+    print("case sampling C1 mesh", flush=True)
+    m = thea_amd.Mesh(v, t)
+    s = thea_amd.MeshSampler(m)
+    for seed, stream in [(0, 0), (1, 7), (2**40 + 3, 2**33 + 1)]:
+        pos, nrm, tri = s.sampleEvenlyByArea(200_000, seed=seed, stream_id=stream)
+        opos, onrm, otri, cdf = oracle.sample_area(v, t, 200_000, seed, stream, want_cdf=True)
+        assert np.array_equal(tri, otri)
+        assert np.array_equal(pos.view(np.uint32), opos.view(np.uint32))
+        assert np.array_equal(nrm.view(np.uint32), onrm.view(np.uint32))
+    assert m.total_area() == cdf[-1]
+
+
+def test_sampling_empty_and_zero_area():
+    import thea_amd
+    print("case sampling empty", flush=True)
+    m = thea_amd.Mesh(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.int32))
+    pos, nrm, tri = thea_amd.MeshSampler(m).sampleEvenlyByArea(100)
+    assert len(pos) == 0
+    v = np.array([[0, 0, 0], [1, 1, 1], [2, 2, 2]], np.float32)  # collinear: zero area
+    m = thea_amd.Mesh(v, np.array([[0, 1, 2]], np.int32))
+    pos, nrm, tri = thea_amd.MeshSampler(m).sampleEvenlyByArea(100)
+    assert len(pos) == 0
+
+
+def test_sampling_sequential_cdf_path(oracle):
+    """Areas spanning > 53 bits force the sequential CDF; results must still match the oracle."""
+    import thea_amd
+    g = np.random.default_rng(31)
+    v, t = random_mesh(g, 4000, size=0.1)
+    v = v.reshape(-1, 3, 3)
+    v[::3] *= np.float32(1e-9)  # tiny triangles (areas ~1e-20) next to ordinary ones
+    v[1::3] = v[1::3] * np.float32(1e3)
+    v = v.reshape(-1, 3).astype(np.float32)
+    print("case sampling sequential cdf", flush=True)
+    m = thea_amd.Mesh(v, t)
+    pos, nrm, tri = thea_amd.MeshSampler(m).sampleEvenlyByArea(100_000, seed=5)
+    opos, onrm, otri, cdf = oracle.sample_area(v, t, 100_000, 5, 0, want_cdf=True)
+    assert np.array_equal(tri, otri)
+    assert np.array_equal(pos.view(np.uint32), opos.view(np.uint32))
+    assert m.total_area() == cdf[-1]

@@ -1,0 +1,8 @@
+"""thea_amd: MI355X-native BVH queries and mesh sampling with Thea's interface.
+
+The compute path is libthea_gpu.so (HIP kernels for gfx950 behind the C ABI in
+include/thea_gpu.h). This package is the Python host side: ctypes bindings (_capi), Thea-named
+classes (bvh), and synthetic workload generators (workloads).
+"""
+from .bvh import Context, Mesh, MeshBvh, MeshSampler, PointBvh, default_context, version  # noqa: F401
+from ._capi import TheaGpuError  # noqa: F401

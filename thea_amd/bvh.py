@@ -1,0 +1,294 @@
+"""Python mirror of Thea's BVH query and mesh sampling interface over libthea_gpu's C ABI.
+
+Method names follow Thea (Algorithms/BvhN.hpp, Algorithms/MeshBvh.hpp, Algorithms/MeshSampler.hpp)
+and the north-star spellings (closestElement, closestPoint, kClosestElements, rayIntersection,
+sampleUniformRandom). Every method takes a batch of queries. Arrays may be numpy arrays (results
+come back as numpy, the call is synchronous) or torch tensors on the GPU (results are torch tensors
+on the same device, computed asynchronously on torch's current stream).
+
+Return conventions follow Thea: a query with no result has index -1 and distance/time -1
+(BvhN.hpp:1062, :1315). Exact ties resolve to the lowest element index.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _capi as capi
+
+
+def _is_torch(x):
+    return type(x).__module__.startswith("torch")
+
+
+def _torch():
+    import torch  # noqa: WPS433 (imported only when torch tensors are used)
+    return torch
+
+
+class _Args:
+    """Marshals one call's arrays: host (numpy) or device (torch) pointers, plus result arrays."""
+
+    def __init__(self, device_mode, torch_device=None):
+        self.device_mode = device_mode
+        self.torch_device = torch_device
+        self.keep = []
+
+    @property
+    def mem(self):
+        return capi.THG_MEM_DEVICE if self.device_mode else capi.THG_MEM_HOST
+
+    @property
+    def stream(self):
+        if not self.device_mode:
+            return None
+        torch = _torch()
+        return C.c_void_p(torch.cuda.current_stream(self.torch_device).cuda_stream)
+
+    def inp(self, a, dtype, cols=None):
+        if self.device_mode:
+            torch = _torch()
+            tdt = {np.float32: torch.float32, np.int32: torch.int32}[dtype]
+            if not _is_torch(a):
+                a = torch.as_tensor(np.asarray(a), device=self.torch_device)
+            t = a.to(device=self.torch_device, dtype=tdt).contiguous()
+            if cols:
+                t = t.reshape(-1, cols)
+            self.keep.append(t)
+            return t, C.c_void_p(t.data_ptr()), t.shape[0]
+        arr = np.ascontiguousarray(np.asarray(a), dtype=dtype)
+        if cols:
+            arr = arr.reshape(-1, cols)
+        self.keep.append(arr)
+        return arr, arr.ctypes.data_as(C.c_void_p), arr.shape[0]
+
+    def out(self, shape, dtype, want=True):
+        if not want:
+            return None, None
+        if self.device_mode:
+            torch = _torch()
+            tdt = {np.float32: torch.float32, np.int32: torch.int32, np.uint8: torch.uint8}[dtype]
+            t = torch.empty(shape, dtype=tdt, device=self.torch_device)
+            return t, C.c_void_p(t.data_ptr()) if t.numel() else None
+        arr = np.empty(shape, dtype=dtype)
+        return arr, arr.ctypes.data_as(C.c_void_p) if arr.size else None
+
+
+class Context:
+    """A device context (one per GPU)."""
+
+    def __init__(self, device=0):
+        self.device = device
+        h = C.c_void_p()
+        capi.check(capi.lib().thg_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            capi.lib().thg_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_timing(self, enable=True):
+        """Record HIP events around every query/sampling kernel launched through this context."""
+        capi.check(capi.lib().thg_ctx_set_timing(self.h, 1 if enable else 0))
+
+    def kernel_time(self, kernel, reset=True):
+        """(summed milliseconds, launch count) of `kernel` since the last reset."""
+        ms, cnt = C.c_double(), C.c_int64()
+        capi.check(capi.lib().thg_ctx_kernel_time(self.h, kernel.encode(), C.byref(ms), C.byref(cnt),
+                                                  1 if reset else 0))
+        return ms.value, cnt.value
+
+    def torch_device(self):
+        return _torch().device("cuda", self.device)
+
+
+_default_ctx = {}
+
+
+def default_context(device=0):
+    if device not in _default_ctx:
+        _default_ctx[device] = Context(device)
+    return _default_ctx[device]
+
+
+class Mesh:
+    """Triangle mesh resident in device memory (MeshBvh::add / MeshTriangles, MeshBvh.hpp:55-67)."""
+
+    def __init__(self, vertices, triangles, ctx=None):
+        self.ctx = ctx or default_context()
+        dev = _is_torch(vertices) and vertices.is_cuda
+        a = _Args(dev, self.ctx.torch_device() if dev else None)
+        _, vp, nv = a.inp(vertices, np.float32, 3)
+        _, tp, nt = a.inp(triangles, np.int32, 3)
+        h = C.c_void_p()
+        capi.check(capi.lib().thg_mesh_create(self.ctx.h, vp, nv, tp, nt, a.mem, C.byref(h)))
+        if dev:
+            _torch().cuda.synchronize(self.ctx.device)
+        self.h = h
+        self.num_vertices, self.num_triangles = int(nv), int(nt)
+
+    def close(self):
+        if getattr(self, "h", None):
+            capi.lib().thg_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def total_area(self):
+        a = C.c_double()
+        capi.check(capi.lib().thg_mesh_total_area(self.h, C.byref(a)))
+        return a.value
+
+
+class _BvhBase:
+    points = False
+
+    def _finish_build(self, h):
+        self.h = h
+        info = (C.c_int64 * 6)()
+        capi.check(capi.lib().thg_bvh_info(self.h, info))
+        self.info = {"nodes": info[0], "leaves": info[1], "max_depth": info[2], "elements": info[3],
+                     "device_bytes": info[4], "leaf_size": info[5]}
+
+    def close(self):
+        if getattr(self, "h", None):
+            capi.lib().thg_bvh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _args(self, x):
+        dev = _is_torch(x) and x.is_cuda
+        return _Args(dev, self.ctx.torch_device() if dev else None)
+
+    # -- closest element (BvhN::closestElement / closestPair, BvhN.hpp:1050-1114) -----------------
+    def closest(self, queries, dist_bound=-1.0, want_point=True, want_dist=True, want_d2=False):
+        """Returns (idx, dist, closest_point, d2) for a batch; unrequested outputs are None."""
+        a = self._args(queries)
+        _, qp, n = a.inp(queries, np.float32, 3)
+        idx, ip = a.out((n,), np.int32)
+        dist, dp = a.out((n,), np.float32, want_dist)
+        cp, cpp = a.out((n, 3), np.float32, want_point)
+        d2, d2p = a.out((n,), np.float32, want_d2)
+        capi.check(capi.lib().thg_closest_point(self.h, qp, n, float(dist_bound), ip, cpp, dp, d2p, a.mem, a.stream))
+        return idx, dist, cp, d2
+
+    def closestElement(self, queries, dist_bound=-1.0):
+        """Index of the nearest element per query (-1 if none within dist_bound)."""
+        return self.closest(queries, dist_bound, want_point=False, want_dist=False)[0]
+
+    def closestElementWithDistance(self, queries, dist_bound=-1.0):
+        idx, dist, _, _ = self.closest(queries, dist_bound, want_point=False)
+        return idx, dist
+
+    def closestPoint(self, queries, dist_bound=-1.0):
+        """(index, distance, closest point) per query: closestElement(..., &dist, &closest_point)."""
+        idx, dist, cp, _ = self.closest(queries, dist_bound)
+        return idx, dist, cp
+
+    # -- k nearest (BvhN::kClosestPairs, BvhN.hpp:1139-1146) ---------------------------------------
+    def kClosestElements(self, queries, k, dist_bound=-1.0, exclude_self=False, want_d2=False):
+        """(idx[n,k], dist[n,k]) sorted by (distance, index); missing entries are -1."""
+        a = self._args(queries)
+        _, qp, n = a.inp(queries, np.float32, 3)
+        idx, ip = a.out((n, k), np.int32)
+        dist, dp = a.out((n, k), np.float32)
+        d2, d2p = a.out((n, k), np.float32, want_d2)
+        capi.check(capi.lib().thg_k_closest(self.h, qp, n, int(k), float(dist_bound), int(bool(exclude_self)), ip, dp,
+                                            d2p, a.mem, a.stream))
+        return (idx, dist, d2) if want_d2 else (idx, dist)
+
+    kClosestPairs = kClosestElements
+
+
+class MeshBvh(_BvhBase):
+    """BVH over a triangle mesh (Thea's MeshBvh<Mesh>, Algorithms/MeshBvh.hpp:31-165)."""
+
+    def __init__(self, mesh, leaf_size=4):
+        self.mesh = mesh
+        self.ctx = mesh.ctx
+        opts = capi.BuildOpts(leaf_size=leaf_size)
+        h = C.c_void_p()
+        capi.check(capi.lib().thg_bvh_build_mesh(mesh.h, C.byref(opts), None, C.byref(h)))
+        self._finish_build(h)
+
+    # -- rays (BvhN::rayStructureIntersection / rayIntersectionTime / rayIntersects) --------------
+    def rayStructureIntersection(self, origins, directions, max_time=-1.0, want_normal=True):
+        """(idx, t, normal) of the first hit per ray; misses give idx -1, t -1."""
+        a = self._args(origins)
+        _, op, n = a.inp(origins, np.float32, 3)
+        _, dp_, n2 = a.inp(directions, np.float32, 3)
+        if n != n2:
+            raise ValueError("origins and directions differ in length")
+        idx, ip = a.out((n,), np.int32)
+        t, tp = a.out((n,), np.float32)
+        nrm, np_ = a.out((n, 3), np.float32, want_normal)
+        capi.check(capi.lib().thg_ray_first_hit(self.h, op, dp_, n, float(max_time), ip, tp, np_, a.mem, a.stream))
+        return idx, t, nrm
+
+    rayIntersection = rayStructureIntersection
+
+    def rayIntersectionTime(self, origins, directions, max_time=-1.0):
+        return self.rayStructureIntersection(origins, directions, max_time, want_normal=False)[1]
+
+    def rayIntersects(self, origins, directions, max_time=-1.0):
+        a = self._args(origins)
+        _, op, n = a.inp(origins, np.float32, 3)
+        _, dp_, _ = a.inp(directions, np.float32, 3)
+        hit, hp = a.out((n,), np.uint8)
+        capi.check(capi.lib().thg_ray_any_hit(self.h, op, dp_, n, float(max_time), hp, a.mem, a.stream))
+        return hit
+
+
+class PointBvh(_BvhBase):
+    """BVH over points (Thea's BvhN<Vector3, 3> over a point set)."""
+
+    points = True
+
+    def __init__(self, points, leaf_size=4, ctx=None):
+        self.ctx = ctx or default_context()
+        a = self._args(points)
+        _, pp, n = a.inp(points, np.float32, 3)
+        opts = capi.BuildOpts(leaf_size=leaf_size)
+        h = C.c_void_p()
+        capi.check(capi.lib().thg_bvh_build_points(self.ctx.h, pp, n, a.mem, C.byref(opts), a.stream, C.byref(h)))
+        self._finish_build(h)
+
+
+class MeshSampler:
+    """Area-weighted sampling (Thea's MeshSampler::sampleEvenlyByArea, CountMode::EXACT)."""
+
+    def __init__(self, mesh):
+        self.mesh = mesh
+
+    def sampleEvenlyByArea(self, n, seed=0, stream_id=0, device=False):
+        """(positions[n,3], face_normals[n,3], triangle_indices[n]); fewer rows for an empty mesh."""
+        a = _Args(device, self.mesh.ctx.torch_device() if device else None)
+        pos, pp = a.out((n, 3), np.float32)
+        nrm, np_ = a.out((n, 3), np.float32)
+        tri, tp = a.out((n,), np.int32)
+        got = C.c_int64()
+        capi.check(capi.lib().thg_sample_area(self.mesh.h, int(n), C.c_uint64(seed), C.c_uint64(stream_id), pp, np_,
+                                              tp, C.byref(got), a.mem, a.stream))
+        m = got.value
+        return pos[:m], nrm[:m], tri[:m]
+
+    sampleUniformRandom = sampleEvenlyByArea
+
+
+def version():
+    return capi.lib().thg_version().decode()

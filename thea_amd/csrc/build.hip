@@ -1,0 +1,430 @@
+// LBVH construction on the GPU (replaces Thea's kd-tree build, Algorithms/BvhN.hpp:1366-1525).
+//
+//  1. k_elem_prep     element boxes and centres (triangle: Thea's centroid formula (v0+v1+v2)/3,
+//                     Triangle3.hpp:148; point: the point), scene bounds by ordered-int atomics.
+//  2. k_morton        30-bit Morton code of each centre in the centre bounds.
+//  3. radix sort      (code, element) pairs, stable, so equal codes keep element order.
+//  4. k_karras        binary radix tree over the sorted codes (Karras, "Maximizing Parallelism in
+//                     the Construction of BVHs, Octrees, and k-d Trees", HPG 2012), duplicate codes
+//                     disambiguated by position.
+//  5. k_refit         bottom-up boxes; the second child to arrive at a node (atomic counter) merges.
+//  6. k_collapse/emit subtrees of <= leaf_size elements become leaves; remaining nodes are
+//                     compacted into 64-byte BNodes holding both children's boxes.
+//  7. k_records       element records in leaf order (TriRec with Thea's per-triangle constants).
+// Boxes are exact float min/max of element coordinates; the query kernels add their own rounding
+// slack, so no padding is applied here.
+#include "kernels_common.hpp"
+#include "tri_math.hpp"
+
+namespace thg {
+
+constexpr int B_BLOCK = 256;
+
+struct BoundsAcc {  // ordered-int encoded: elem lo xyz, elem hi xyz, centre lo xyz, centre hi xyz
+  int v[12];
+};
+
+__device__ __forceinline__ float3 ld_vert(const float *__restrict__ verts, int64_t i) {
+  return f3(verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]);
+}
+
+__global__ void __launch_bounds__(B_BLOCK) k_elem_prep(const float *__restrict__ verts, const int32_t *__restrict__ tris,
+                                                       int64_t n, int points, float4 *__restrict__ center,
+                                                       float4 *__restrict__ blo, float4 *__restrict__ bhi,
+                                                       BoundsAcc *__restrict__ acc) {
+  float mn[6] = {INFINITY, INFINITY, INFINITY, INFINITY, INFINITY, INFINITY};
+  float mx[6] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  for (int64_t i = (int64_t)blockIdx.x * B_BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * B_BLOCK) {
+    float3 lo, hi, c;
+    if (points) {
+      c = ld_vert(verts, i);
+      lo = c;
+      hi = c;
+    } else {
+      float3 v0 = ld_vert(verts, tris[3 * i]), v1 = ld_vert(verts, tris[3 * i + 1]), v2 = ld_vert(verts, tris[3 * i + 2]);
+      lo = f3(fminf(v0.x, fminf(v1.x, v2.x)), fminf(v0.y, fminf(v1.y, v2.y)), fminf(v0.z, fminf(v1.z, v2.z)));
+      hi = f3(fmaxf(v0.x, fmaxf(v1.x, v2.x)), fmaxf(v0.y, fmaxf(v1.y, v2.y)), fmaxf(v0.z, fmaxf(v1.z, v2.z)));
+      float3 s = add3(add3(v0, v1), v2);
+      c = f3(s.x / 3.0f, s.y / 3.0f, s.z / 3.0f);
+    }
+    center[i] = make_float4(c.x, c.y, c.z, 0.0f);
+    blo[i] = make_float4(lo.x, lo.y, lo.z, 0.0f);
+    bhi[i] = make_float4(hi.x, hi.y, hi.z, 0.0f);
+    mn[0] = fminf(mn[0], lo.x); mn[1] = fminf(mn[1], lo.y); mn[2] = fminf(mn[2], lo.z);
+    mx[0] = fmaxf(mx[0], hi.x); mx[1] = fmaxf(mx[1], hi.y); mx[2] = fmaxf(mx[2], hi.z);
+    mn[3] = fminf(mn[3], c.x); mn[4] = fminf(mn[4], c.y); mn[5] = fminf(mn[5], c.z);
+    mx[3] = fmaxf(mx[3], c.x); mx[4] = fmaxf(mx[4], c.y); mx[5] = fmaxf(mx[5], c.z);
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    float a = wave_min(mn[k]), b = wave_max(mx[k]);
+    if (lane_id() == 0) {
+      int slot_lo = k < 3 ? k : 6 + (k - 3);
+      int slot_hi = k < 3 ? 3 + k : 9 + (k - 3);
+      atomicMin(&acc->v[slot_lo], float_to_ord(a));
+      atomicMax(&acc->v[slot_hi], float_to_ord(b));
+    }
+  }
+}
+
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {
+  v &= 0x3FF;
+  v = (v | (v << 16)) & 0x030000FF;
+  v = (v | (v << 8)) & 0x0300F00F;
+  v = (v | (v << 4)) & 0x030C30C3;
+  v = (v | (v << 2)) & 0x09249249;
+  return v;
+}
+
+__device__ __forceinline__ uint32_t morton30(float3 c, float3 lo, float3 inv) {
+  float x = (c.x - lo.x) * inv.x, y = (c.y - lo.y) * inv.y, z = (c.z - lo.z) * inv.z;
+  uint32_t ix = (uint32_t)fminf(fmaxf(x * 1024.0f, 0.0f), 1023.0f);
+  uint32_t iy = (uint32_t)fminf(fmaxf(y * 1024.0f, 0.0f), 1023.0f);
+  uint32_t iz = (uint32_t)fminf(fmaxf(z * 1024.0f, 0.0f), 1023.0f);
+  return (spread10(ix) << 2) | (spread10(iy) << 1) | spread10(iz);
+}
+
+__global__ void __launch_bounds__(B_BLOCK) k_morton(const float4 *__restrict__ center, int64_t n, float3 lo, float3 inv,
+                                                    uint32_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+  int64_t i = (int64_t)blockIdx.x * B_BLOCK + threadIdx.x;
+  if (i >= n) return;
+  float4 c = center[i];
+  keys[i] = morton30(f3(c.x, c.y, c.z), lo, inv);
+  vals[i] = (uint32_t)i;
+}
+
+// Common prefix length of sorted (code, position) keys i and j; -1 outside [0, n).
+__device__ __forceinline__ int delta(const uint32_t *__restrict__ k, int64_t n, int64_t i, int64_t j) {
+  if (j < 0 || j >= n) return -1;
+  uint32_t a = k[i], b = k[j];
+  if (a == b) return 32 + __clz((uint32_t)i ^ (uint32_t)j);
+  return __clz(a ^ b);
+}
+
+// Internal node i of n - 1: children (leaf if flag bit 31 set), range, parent links.
+__global__ void __launch_bounds__(B_BLOCK) k_karras(const uint32_t *__restrict__ k, int64_t n, int32_t *__restrict__ left,
+                                                    int32_t *__restrict__ right, int32_t *__restrict__ rfirst,
+                                                    int32_t *__restrict__ rlast, int32_t *__restrict__ node_parent,
+                                                    int32_t *__restrict__ leaf_parent) {
+  int64_t i = (int64_t)blockIdx.x * B_BLOCK + threadIdx.x;
+  if (i >= n - 1) return;
+  int d = (delta(k, n, i, i + 1) - delta(k, n, i, i - 1)) >= 0 ? 1 : -1;
+  int dmin = delta(k, n, i, i - d);
+  int64_t lmax = 2;
+  while (delta(k, n, i, i + lmax * d) > dmin) lmax *= 2;
+  int64_t l = 0;
+  for (int64_t t = lmax / 2; t >= 1; t /= 2)
+    if (delta(k, n, i, i + (l + t) * d) > dmin) l += t;
+  int64_t j = i + l * d;
+  int dnode = delta(k, n, i, j);
+  int64_t s = 0;
+  int64_t div = 2;
+  int64_t t;
+  do {
+    t = (l + div - 1) / div;
+    if (t >= 1 && delta(k, n, i, i + (s + t) * d) > dnode) s += t;
+    div *= 2;
+  } while (t > 1);
+  int64_t g = i + s * d + (d < 0 ? -1 : 0);
+  int64_t first = i < j ? i : j, last = i < j ? j : i;
+  int32_t lc, rc;
+  if (first == g) { lc = (int32_t)(g | 0x80000000u); leaf_parent[g] = (int32_t)i; }
+  else { lc = (int32_t)g; node_parent[g] = (int32_t)i; }
+  if (last == g + 1) { rc = (int32_t)((g + 1) | 0x80000000u); leaf_parent[g + 1] = (int32_t)i; }
+  else { rc = (int32_t)(g + 1); node_parent[g + 1] = (int32_t)i; }
+  left[i] = lc;
+  right[i] = rc;
+  rfirst[i] = (int32_t)first;
+  rlast[i] = (int32_t)last;
+  if (i == 0) node_parent[0] = -1;
+}
+
+__device__ __forceinline__ float ld_coherent(const float *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_coherent(float *p, float v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Bottom-up refit over the Karras tree. Node boxes: nbox[6 i + 0..5] = lo xyz, hi xyz.
+// `sorted` maps sorted position -> element; element boxes come from blo/bhi.
+__global__ void __launch_bounds__(B_BLOCK) k_refit(int64_t n, const uint32_t *__restrict__ sorted,
+                                                   const float4 *__restrict__ blo, const float4 *__restrict__ bhi,
+                                                   const int32_t *__restrict__ left, const int32_t *__restrict__ right,
+                                                   const int32_t *__restrict__ node_parent,
+                                                   const int32_t *__restrict__ leaf_parent, uint32_t *__restrict__ flags,
+                                                   float *nbox) {
+  int64_t k = (int64_t)blockIdx.x * B_BLOCK + threadIdx.x;
+  if (k >= n) return;
+  int32_t cur = leaf_parent[k];
+  while (cur >= 0) {
+    __threadfence();
+    uint32_t old = atomicAdd(&flags[cur], 1u);
+    if (old == 0) return;  // first child to arrive; the sibling's thread continues
+    __threadfence();
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    int32_t ch[2] = {left[cur], right[cur]};
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      if (ch[c] < 0) {
+        uint32_t e = sorted[ch[c] & 0x7fffffff];
+        float4 a = blo[e], b = bhi[e];
+        lo[0] = fminf(lo[0], a.x); lo[1] = fminf(lo[1], a.y); lo[2] = fminf(lo[2], a.z);
+        hi[0] = fmaxf(hi[0], b.x); hi[1] = fmaxf(hi[1], b.y); hi[2] = fmaxf(hi[2], b.z);
+      } else {
+        const float *q = nbox + 6 * (int64_t)ch[c];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          lo[a] = fminf(lo[a], ld_coherent(q + a));
+          hi[a] = fmaxf(hi[a], ld_coherent(q + 3 + a));
+        }
+      }
+    }
+    float *o = nbox + 6 * (int64_t)cur;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { st_coherent(o + a, lo[a]); st_coherent(o + 3 + a, hi[a]); }
+    cur = node_parent[cur];
+  }
+}
+
+// keep[i] = 1 if internal node i stays an internal node of the collapsed tree (range > leaf_size).
+__global__ void __launch_bounds__(B_BLOCK) k_keep(int64_t n, const int32_t *__restrict__ rfirst,
+                                                  const int32_t *__restrict__ rlast, int leaf_size,
+                                                  uint32_t *__restrict__ keep) {
+  int64_t i = (int64_t)blockIdx.x * B_BLOCK + threadIdx.x;
+  if (i >= n - 1) return;
+  keep[i] = (rlast[i] - rfirst[i] + 1) > leaf_size ? 1u : 0u;
+}
+
+__device__ __forceinline__ void child_box(int32_t c, const uint32_t *sorted, const float4 *blo, const float4 *bhi,
+                                          const float *nbox, float lo[3], float hi[3]) {
+  if (c < 0) {
+    uint32_t e = sorted[c & 0x7fffffff];
+    float4 a = blo[e], b = bhi[e];
+    lo[0] = a.x; lo[1] = a.y; lo[2] = a.z; hi[0] = b.x; hi[1] = b.y; hi[2] = b.z;
+  } else {
+    const float *q = nbox + 6 * (int64_t)c;
+    lo[0] = q[0]; lo[1] = q[1]; lo[2] = q[2]; hi[0] = q[3]; hi[1] = q[4]; hi[2] = q[5];
+  }
+}
+
+// Emit the collapsed tree. `fidx` = exclusive prefix of keep (final node index of kept nodes).
+__global__ void __launch_bounds__(B_BLOCK) k_emit(int64_t n, int leaf_size, const uint32_t *__restrict__ keep,
+                                                  const uint32_t *__restrict__ fidx, const int32_t *__restrict__ left,
+                                                  const int32_t *__restrict__ right, const int32_t *__restrict__ rfirst,
+                                                  const int32_t *__restrict__ rlast, const uint32_t *__restrict__ sorted,
+                                                  const float4 *__restrict__ blo, const float4 *__restrict__ bhi,
+                                                  const float *__restrict__ nbox, BNode *__restrict__ out,
+                                                  uint32_t *__restrict__ leaf_count) {
+  int64_t i = (int64_t)blockIdx.x * B_BLOCK + threadIdx.x;
+  if (i >= n - 1 || !keep[i]) return;
+  int32_t ch[2] = {left[i], right[i]};
+  float lo[2][3], hi[2][3];
+  int32_t ref[2], cnt[2];
+  uint32_t leaves = 0;
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    child_box(ch[c], sorted, blo, bhi, nbox, lo[c], hi[c]);
+    if (ch[c] < 0) {
+      ref[c] = ~(ch[c] & 0x7fffffff);
+      cnt[c] = 1;
+      ++leaves;
+    } else {
+      int32_t sz = rlast[ch[c]] - rfirst[ch[c]] + 1;
+      if (sz <= leaf_size) { ref[c] = ~rfirst[ch[c]]; cnt[c] = sz; ++leaves; }
+      else { ref[c] = (int32_t)fidx[ch[c]]; cnt[c] = 0; }
+    }
+  }
+  BNode nd;
+  nd.a = make_float4(lo[0][0], lo[0][1], lo[0][2], hi[0][0]);
+  nd.b = make_float4(hi[0][1], hi[0][2], lo[1][0], lo[1][1]);
+  nd.c = make_float4(lo[1][2], hi[1][0], hi[1][1], hi[1][2]);
+  nd.d = make_int4(ref[0], ref[1], cnt[0], cnt[1]);
+  out[fidx[i]] = nd;
+  if (leaves) atomicAdd(leaf_count, leaves);
+}
+
+// Depth (number of kept ancestors) of every element's leaf; max via atomicMax.
+__global__ void __launch_bounds__(B_BLOCK) k_depth(int64_t n, const int32_t *__restrict__ leaf_parent,
+                                                   const int32_t *__restrict__ node_parent,
+                                                   const uint32_t *__restrict__ keep, int32_t *__restrict__ max_depth) {
+  int64_t k = (int64_t)blockIdx.x * B_BLOCK + threadIdx.x;
+  int d = 0;
+  if (k < n)
+    for (int32_t cur = leaf_parent[k]; cur >= 0; cur = node_parent[cur]) d += keep[cur] ? 1 : 0;
+  int m = (int)wave_max((float)d);  // every lane of the wave takes part in the reduction
+  if (lane_id() == 0) atomicMax(max_depth, (int)m);
+}
+
+__global__ void __launch_bounds__(B_BLOCK) k_tri_records(const float *__restrict__ verts, const int32_t *__restrict__ tris,
+                                                         const uint32_t *__restrict__ sorted, int64_t n,
+                                                         TriRec *__restrict__ out) {
+  int64_t k = (int64_t)blockIdx.x * B_BLOCK + threadIdx.x;
+  if (k >= n) return;
+  uint32_t e = sorted[k];
+  float3 v0 = ld_vert(verts, tris[3 * (int64_t)e]), v1 = ld_vert(verts, tris[3 * (int64_t)e + 1]),
+         v2 = ld_vert(verts, tris[3 * (int64_t)e + 2]);
+  out[k] = make_trirec(v0, v1, v2, (int32_t)e);
+}
+
+__global__ void __launch_bounds__(B_BLOCK) k_point_records(const float *__restrict__ xyz, const uint32_t *__restrict__ sorted,
+                                                           int64_t n, PointRec *__restrict__ out) {
+  int64_t k = (int64_t)blockIdx.x * B_BLOCK + threadIdx.x;
+  if (k >= n) return;
+  uint32_t e = sorted[k];
+  PointRec r;
+  r.p = make_float4(xyz[3 * (int64_t)e], xyz[3 * (int64_t)e + 1], xyz[3 * (int64_t)e + 2], __int_as_float((int)e));
+  out[k] = r;
+}
+
+// Single-leaf tree for n <= leaf_size: child 0 holds every element, child 1 is an empty leaf.
+__global__ void k_single_leaf(int64_t n, float3 lo, float3 hi, BNode *out) {
+  BNode nd;
+  nd.a = make_float4(lo.x, lo.y, lo.z, hi.x);
+  nd.b = make_float4(hi.y, hi.z, INFINITY, INFINITY);
+  nd.c = make_float4(INFINITY, -INFINITY, -INFINITY, -INFINITY);
+  nd.d = make_int4(~0, ~0, (int)n, 0);
+  out[0] = nd;
+}
+
+__global__ void k_init_bounds(BoundsAcc *acc) {
+  int t = threadIdx.x;
+  if (t < 12) {
+    bool is_lo = (t < 3) || (t >= 6 && t < 9);
+    acc->v[t] = is_lo ? float_to_ord(INFINITY) : float_to_ord(-INFINITY);
+  }
+}
+
+// Exclusive scan of n uint32 into out (total in out[n]); uses the radix scan kernel's approach.
+__global__ void __launch_bounds__(1024) k_excl_scan_u32(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
+                                                        int64_t n) {
+  __shared__ uint32_t part[1024 / WAVE];
+  __shared__ uint32_t carry;
+  int t = threadIdx.x;
+  if (t == 0) carry = 0;
+  __syncthreads();
+  for (int64_t base = 0; base < n; base += 1024) {
+    int64_t i = base + t;
+    uint32_t v = i < n ? in[i] : 0;
+    uint32_t inc = wave_incl_scan(v);
+    int w = t / WAVE, l = t % WAVE;
+    if (l == WAVE - 1) part[w] = inc;
+    __syncthreads();
+    if (w == 0) {
+      uint32_t pv = l < 1024 / WAVE ? part[l] : 0;
+      uint32_t pinc = wave_incl_scan(pv);
+      if (l < 1024 / WAVE) part[l] = pinc - pv;
+    }
+    __syncthreads();
+    uint32_t excl = carry + part[w] + inc - v;
+    if (i < n) out[i] = excl;
+    __syncthreads();
+    if (t == 1023) carry = excl + v;
+    __syncthreads();
+  }
+  if (t == 0) out[n] = carry;
+}
+
+// Host driver shared by mesh and point builds.
+void build_bvh(Bvh &b, const float *verts, const int32_t *tris, int64_t n, bool points, int leaf_size, hipStream_t s) {
+  b.points = points;
+  b.n = n;
+  b.leaf_size = leaf_size;
+  if (n <= 0) {
+    b.num_nodes = 0;
+    b.num_leaves = 0;
+    b.max_depth = 0;
+    return;
+  }
+  DevBuf center(n * sizeof(float4)), blo(n * sizeof(float4)), bhi(n * sizeof(float4)), acc(sizeof(BoundsAcc));
+  hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, s, acc.as<BoundsAcc>());
+  launch_check("k_init_bounds");
+  uint32_t grid = std::min<uint32_t>(ceil_div(n, B_BLOCK), 4096);
+  hipLaunchKernelGGL(k_elem_prep, dim3(grid), dim3(B_BLOCK), 0, s, verts, tris, n, points ? 1 : 0,
+                     center.as<float4>(), blo.as<float4>(), bhi.as<float4>(), acc.as<BoundsAcc>());
+  launch_check("k_elem_prep");
+  BoundsAcc h;
+  THG_HIP(hipMemcpyAsync(&h, acc.p, sizeof(h), hipMemcpyDeviceToHost, s));
+  THG_HIP(hipStreamSynchronize(s));
+  float f[12];
+  for (int k = 0; k < 12; ++k) f[k] = ord_to_float(h.v[k]);
+  b.lo = f3(f[0], f[1], f[2]);
+  b.hi = f3(f[3], f[4], f[5]);
+  b.scale = 0;
+  for (int k = 0; k < 6; ++k) b.scale = std::max(b.scale, std::fabs(f[k]));
+  require(std::isfinite(b.scale), "element coordinates must be finite");
+
+  size_t rec_bytes = points ? sizeof(PointRec) : sizeof(TriRec);
+  b.elems.alloc(n * rec_bytes);
+
+  DevBuf keys(n * 4), vals(n * 4), tk(n * 4), tv(n * 4), hist(radix_hist_words(n) * 4);
+  float3 clo = f3(f[6], f[7], f[8]);
+  float3 cext = f3(f[9] - f[6], f[10] - f[7], f[11] - f[8]);
+  float3 inv = f3(cext.x > 0 ? 1.0f / cext.x : 0.0f, cext.y > 0 ? 1.0f / cext.y : 0.0f, cext.z > 0 ? 1.0f / cext.z : 0.0f);
+  hipLaunchKernelGGL(k_morton, dim3(ceil_div(n, B_BLOCK)), dim3(B_BLOCK), 0, s, center.as<float4>(), n, clo, inv,
+                     keys.as<uint32_t>(), vals.as<uint32_t>());
+  launch_check("k_morton");
+  radix_sort_pairs(keys.as<uint32_t>(), vals.as<uint32_t>(), tk.as<uint32_t>(), tv.as<uint32_t>(), hist.as<uint32_t>(), n,
+                   0, 32, s);
+  const uint32_t *sorted = vals.as<uint32_t>();
+
+  if (points)
+    hipLaunchKernelGGL(k_point_records, dim3(ceil_div(n, B_BLOCK)), dim3(B_BLOCK), 0, s, verts, sorted, n,
+                       b.elems.as<PointRec>());
+  else
+    hipLaunchKernelGGL(k_tri_records, dim3(ceil_div(n, B_BLOCK)), dim3(B_BLOCK), 0, s, verts, tris, sorted, n,
+                       b.elems.as<TriRec>());
+  launch_check("k_records");
+
+  if (n <= leaf_size) {
+    b.nodes.alloc(sizeof(BNode));
+    hipLaunchKernelGGL(k_single_leaf, dim3(1), dim3(1), 0, s, n, b.lo, b.hi, b.nodes.as<BNode>());
+    launch_check("k_single_leaf");
+    b.num_nodes = 1;
+    b.num_leaves = 1;
+    b.max_depth = 1;
+    THG_HIP(hipStreamSynchronize(s));
+    return;
+  }
+
+  int64_t ni = n - 1;
+  DevBuf left(ni * 4), right(ni * 4), rfirst(ni * 4), rlast(ni * 4), nparent(ni * 4), lparent(n * 4);
+  DevBuf flags(ni * 4), nbox(ni * 6 * sizeof(float)), keep(ni * 4), fidx((ni + 1) * 4), counters(16);
+  THG_HIP(hipMemsetAsync(flags.p, 0, ni * 4, s));
+  THG_HIP(hipMemsetAsync(counters.p, 0, 16, s));
+  hipLaunchKernelGGL(k_karras, dim3(ceil_div(ni, B_BLOCK)), dim3(B_BLOCK), 0, s, keys.as<uint32_t>(), n,
+                     left.as<int32_t>(), right.as<int32_t>(), rfirst.as<int32_t>(), rlast.as<int32_t>(),
+                     nparent.as<int32_t>(), lparent.as<int32_t>());
+  launch_check("k_karras");
+  hipLaunchKernelGGL(k_refit, dim3(ceil_div(n, B_BLOCK)), dim3(B_BLOCK), 0, s, n, sorted, blo.as<float4>(),
+                     bhi.as<float4>(), left.as<int32_t>(), right.as<int32_t>(), nparent.as<int32_t>(),
+                     lparent.as<int32_t>(), flags.as<uint32_t>(), nbox.as<float>());
+  launch_check("k_refit");
+  hipLaunchKernelGGL(k_keep, dim3(ceil_div(ni, B_BLOCK)), dim3(B_BLOCK), 0, s, n, rfirst.as<int32_t>(),
+                     rlast.as<int32_t>(), leaf_size, keep.as<uint32_t>());
+  launch_check("k_keep");
+  hipLaunchKernelGGL(k_excl_scan_u32, dim3(1), dim3(1024), 0, s, keep.as<uint32_t>(), fidx.as<uint32_t>(), ni);
+  launch_check("k_excl_scan_u32");
+  uint32_t kept = 0;
+  THG_HIP(hipMemcpyAsync(&kept, fidx.as<uint32_t>() + ni, 4, hipMemcpyDeviceToHost, s));
+  THG_HIP(hipStreamSynchronize(s));
+  b.nodes.alloc((size_t)kept * sizeof(BNode));
+  uint32_t *leaf_count = counters.as<uint32_t>();
+  int32_t *maxd = reinterpret_cast<int32_t *>(counters.as<uint32_t>() + 1);
+  hipLaunchKernelGGL(k_emit, dim3(ceil_div(ni, B_BLOCK)), dim3(B_BLOCK), 0, s, n, leaf_size, keep.as<uint32_t>(),
+                     fidx.as<uint32_t>(), left.as<int32_t>(), right.as<int32_t>(), rfirst.as<int32_t>(),
+                     rlast.as<int32_t>(), sorted, blo.as<float4>(), bhi.as<float4>(), nbox.as<float>(),
+                     b.nodes.as<BNode>(), leaf_count);
+  launch_check("k_emit");
+  hipLaunchKernelGGL(k_depth, dim3(ceil_div(n, B_BLOCK)), dim3(B_BLOCK), 0, s, n, lparent.as<int32_t>(),
+                     nparent.as<int32_t>(), keep.as<uint32_t>(), maxd);
+  launch_check("k_depth");
+  uint32_t cnt[2];
+  THG_HIP(hipMemcpyAsync(cnt, counters.p, 8, hipMemcpyDeviceToHost, s));
+  THG_HIP(hipStreamSynchronize(s));
+  b.num_nodes = kept;
+  b.num_leaves = cnt[0];
+  b.max_depth = (int32_t)cnt[1];
+}
+
+}  // namespace thg

@@ -1,0 +1,46 @@
+// Device data layout of libthea_gpu (shared by all kernels).
+//
+// HBM layout per BVH:
+//   BNode[num_nodes]  64 B each: both children's boxes + refs, one 64-B segment per visit.
+//   TriRec[n]        128 B each (one L2 line): a triangle with every per-triangle constant the
+//                     closest-point and ray tests need, in leaf order, so a leaf is a contiguous run.
+//   PointRec[n]       16 B each: point + original index, in leaf order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace thg {
+
+// Binary BVH node. Child k's box is [lo_k, hi_k]; ref_k >= 0 is an internal node index, ref_k < 0
+// is a leaf whose elements are [~ref_k, ~ref_k + cnt_k) in leaf order (cnt_k may be 0).
+struct __align__(64) BNode {
+  float4 a;  // lo0.x lo0.y lo0.z hi0.x
+  float4 b;  // hi0.y hi0.z lo1.x lo1.y
+  float4 c;  // lo1.z hi1.x hi1.y hi1.z
+  int4 d;    // ref0, ref1, cnt0, cnt1
+};
+static_assert(sizeof(BNode) == 64, "BNode must be 64 bytes");
+
+// Triangle record. v0/v1/v2 are the input vertices; n, pd are Thea's plane (Hyperplane3.hpp:34-41);
+// the in-plane 2-D area (`ar`) and its reciprocal (`inv`) are the constants of
+// isPointInsideTriangle (Triangle3.hpp:1436-1447); dir_k/len_k are the perimeter edge directions
+// and lengths of closestPointOnTrianglePerimeter (Triangle3.hpp:1401-1409). `meta` packs the
+// primary axis (bits 0-1) and `id` is the caller's triangle index.
+struct __align__(128) TriRec {
+  float4 p0;   // v0.xyz, n.x
+  float4 p1;   // v1.xyz, n.y
+  float4 p2;   // v2.xyz, n.z
+  float4 pl;   // pd, inv, ar, meta (int bits)
+  float4 e0;   // dir0.xyz, len0   (edge v0 -> v1)
+  float4 e1;   // dir1.xyz, len1   (edge v1 -> v2)
+  float4 e2;   // dir2.xyz, len2   (edge v2 -> v0)
+  int4 ex;     // id, 0, 0, 0
+};
+static_assert(sizeof(TriRec) == 128, "TriRec must be 128 bytes");
+
+struct __align__(16) PointRec {
+  float4 p;    // xyz, id (int bits)
+};
+
+}  // namespace thg

@@ -1,0 +1,516 @@
+// Query kernels: closest element / closest point, k nearest, ray first hit and any hit.
+//
+// One query per lane; each lane walks the BVH depth first with its own stack in LDS (node ref and
+// the lower bound it was pushed with), visiting the nearer child first. Queries are first put in
+// spatial (Morton) order so the 64 lanes of a wave walk nearby paths; blocks are remapped so each
+// XCD's L2 serves one contiguous spatial range. Results go to the caller's query index.
+//
+// Results are defined canonically: the lexicographic minimum of (float d^2, element index) for
+// closest queries (and the k smallest such pairs for k-NN), (float t, element index) for rays,
+// where d^2 and t come from Thea's own float arithmetic (tri_math.hpp). A node is skipped only when
+// its lower bound is strictly above the current best, and the lower bound treats the query as a
+// box of half-width 2^-16 max(|q|, scene scale), so rounding in the element arithmetic can never
+// make a box prune an element that would have won.
+#include "kernels_common.hpp"
+#include "query.hpp"
+#include "tri_math.hpp"
+
+namespace thg {
+
+constexpr int Q_BLOCK = 64;
+constexpr float SLACK_REL = 1.0f / 65536.0f;
+
+__device__ __forceinline__ float box_lb(float3 qlo, float3 qhi, float lx, float ly, float lz, float hx, float hy,
+                                        float hz) {
+  float dx = fmaxf(fmaxf(lx - qhi.x, qlo.x - hx), 0.0f);
+  float dy = fmaxf(fmaxf(ly - qhi.y, qlo.y - hy), 0.0f);
+  float dz = fmaxf(fmaxf(lz - qhi.z, qlo.z - hz), 0.0f);
+  return dx * dx + (dy * dy + dz * dz);
+}
+
+__device__ __forceinline__ bool better(float d, int32_t i, float bd, int32_t bi) {
+  return d < bd || (d == bd && i < bi);
+}
+
+__device__ __forceinline__ float3 load_q(const float *__restrict__ q, int64_t i) {
+  return f3(q[3 * i], q[3 * i + 1], q[3 * i + 2]);
+}
+
+__device__ __forceinline__ int64_t query_slot(uint32_t nb_real, int64_t n, const uint32_t *__restrict__ order,
+                                              bool &active) {
+  uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
+  int64_t t = (int64_t)lb * Q_BLOCK + threadIdx.x;
+  active = lb < nb_real && t < n;
+  if (!active) return 0;
+  return order ? (int64_t)order[t] : t;
+}
+
+//--------------------------------------------------------------------------------------------------
+// Closest element
+//--------------------------------------------------------------------------------------------------
+template <int STACK, bool POINTS>
+__global__ void __launch_bounds__(Q_BLOCK) k_closest(const BNode *__restrict__ nodes, const void *__restrict__ elems,
+                                                     const float *__restrict__ q, const uint32_t *__restrict__ order,
+                                                     int64_t n, uint32_t nb_real, float best_init, float scale,
+                                                     int exclude_self, int32_t *__restrict__ out_idx,
+                                                     float *__restrict__ out_cp, float *__restrict__ out_dist,
+                                                     float *__restrict__ out_d2) {
+  __shared__ int32_t s_ref[STACK * Q_BLOCK];
+  __shared__ float s_lb[STACK * Q_BLOCK];
+  bool active;
+  int64_t qi = query_slot(nb_real, n, order, active);
+  if (!active) return;
+  const int tid = threadIdx.x;
+  float3 p = load_q(q, qi);
+  float m = fmaxf(scale, fmaxf(fabsf(p.x), fmaxf(fabsf(p.y), fabsf(p.z))));
+  float sl = m * SLACK_REL;
+  float3 qlo = f3(p.x - sl, p.y - sl, p.z - sl), qhi = f3(p.x + sl, p.y + sl, p.z + sl);
+  float best = best_init;
+  int32_t bidx = 0x7fffffff;
+  float3 bcp = f3(0, 0, 0);
+  const int32_t self = exclude_self ? (int32_t)qi : -1;
+  int sp = 0;
+  int32_t node = 0;
+  const TriRec *__restrict__ tr = (const TriRec *)elems;
+  const PointRec *__restrict__ pr = (const PointRec *)elems;
+  while (true) {
+    const BNode nd = nodes[node];
+    float lb0 = box_lb(qlo, qhi, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y);
+    float lb1 = box_lb(qlo, qhi, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w);
+    int32_t r0 = nd.d.x, r1 = nd.d.y, c0 = nd.d.z, c1 = nd.d.w;
+    if (lb1 < lb0) {
+      float tl = lb0; lb0 = lb1; lb1 = tl;
+      int32_t tr_ = r0; r0 = r1; r1 = tr_;
+      int32_t tc = c0; c0 = c1; c1 = tc;
+    }
+    int32_t next = -1;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      float lb = c == 0 ? lb0 : lb1;
+      int32_t r = c == 0 ? r0 : r1;
+      int32_t cnt = c == 0 ? c0 : c1;
+      if (!(lb <= best)) continue;
+      if (r < 0) {
+        int32_t f = ~r;
+        for (int32_t k = f; k < f + cnt; ++k) {
+          float d;
+          float3 cp;
+          int32_t id;
+          if (POINTS) {
+            float4 e = pr[k].p;
+            id = __float_as_int(e.w);
+            cp = f3(e.x, e.y, e.z);
+            d = sqn3(sub3(cp, p));
+          } else {
+            const TriRec *t = tr + k;
+            id = t->ex.x;
+            d = tri_closest(t->p0, t->p1, t->p2, t->pl, t->e0, t->e1, t->e2, p, cp);
+          }
+          if (id != self && better(d, id, best, bidx)) {
+            best = d;
+            bidx = id;
+            bcp = cp;
+          }
+        }
+      } else if (next < 0) {
+        next = r;
+      } else {
+        s_ref[sp * Q_BLOCK + tid] = r;
+        s_lb[sp * Q_BLOCK + tid] = lb;
+        ++sp;
+      }
+    }
+    if (next >= 0) { node = next; continue; }
+    bool found = false;
+    while (sp > 0) {
+      --sp;
+      if (s_lb[sp * Q_BLOCK + tid] <= best) { node = s_ref[sp * Q_BLOCK + tid]; found = true; break; }
+    }
+    if (!found) break;
+  }
+  bool hit = bidx != 0x7fffffff;
+  out_idx[qi] = hit ? bidx : -1;
+  if (out_cp) {
+    out_cp[3 * qi] = hit ? bcp.x : 0.0f;
+    out_cp[3 * qi + 1] = hit ? bcp.y : 0.0f;
+    out_cp[3 * qi + 2] = hit ? bcp.z : 0.0f;
+  }
+  if (out_dist) out_dist[qi] = hit ? sqrtf(best) : -1.0f;
+  if (out_d2) out_d2[qi] = hit ? best : -1.0f;
+}
+
+//--------------------------------------------------------------------------------------------------
+// k nearest (sorted list of K (d^2, id) pairs per lane in registers)
+//--------------------------------------------------------------------------------------------------
+template <int K>
+__device__ __forceinline__ void knn_insert(float (&kd)[K], int32_t (&ki)[K], float d, int32_t id) {
+#pragma unroll
+  for (int j = K - 1; j >= 0; --j) {
+    bool prev = j > 0 ? better(d, id, kd[j > 0 ? j - 1 : 0], ki[j > 0 ? j - 1 : 0]) : false;
+    if (better(d, id, kd[j], ki[j])) {
+      kd[j] = prev ? kd[j > 0 ? j - 1 : 0] : d;
+      ki[j] = prev ? ki[j > 0 ? j - 1 : 0] : id;
+    }
+  }
+}
+
+template <int STACK, bool POINTS, int K>
+__global__ void __launch_bounds__(Q_BLOCK) k_knn(const BNode *__restrict__ nodes, const void *__restrict__ elems,
+                                                 const float *__restrict__ q, const uint32_t *__restrict__ order,
+                                                 int64_t n, uint32_t nb_real, int k, float best_init, float scale,
+                                                 int exclude_self, int32_t *__restrict__ out_idx,
+                                                 float *__restrict__ out_dist, float *__restrict__ out_d2) {
+  __shared__ int32_t s_ref[STACK * Q_BLOCK];
+  __shared__ float s_lb[STACK * Q_BLOCK];
+  bool active;
+  int64_t qi = query_slot(nb_real, n, order, active);
+  if (!active) return;
+  const int tid = threadIdx.x;
+  float3 p = load_q(q, qi);
+  float m = fmaxf(scale, fmaxf(fabsf(p.x), fmaxf(fabsf(p.y), fabsf(p.z))));
+  float sl = m * SLACK_REL;
+  float3 qlo = f3(p.x - sl, p.y - sl, p.z - sl), qhi = f3(p.x + sl, p.y + sl, p.z + sl);
+  float kd[K];
+  int32_t ki[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) { kd[j] = best_init; ki[j] = 0x7fffffff; }
+  // Only the first k slots are live; the worst live slot is the pruning bound.
+  const int32_t self = exclude_self ? (int32_t)qi : -1;
+  int sp = 0;
+  int32_t node = 0;
+  const TriRec *__restrict__ tr = (const TriRec *)elems;
+  const PointRec *__restrict__ pr = (const PointRec *)elems;
+  // The k-th best pair (slot k - 1): the pruning bound and the admission threshold.
+  auto kth = [&](float &bd, int32_t &bi) {
+    bd = kd[K - 1];
+    bi = ki[K - 1];
+#pragma unroll
+    for (int j = 0; j < K - 1; ++j)
+      if (j == k - 1) { bd = kd[j]; bi = ki[j]; }
+  };
+  auto bound = [&]() {
+    float bd;
+    int32_t bi;
+    kth(bd, bi);
+    return bd;
+  };
+  while (true) {
+    const BNode nd = nodes[node];
+    float lb0 = box_lb(qlo, qhi, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y);
+    float lb1 = box_lb(qlo, qhi, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w);
+    int32_t r0 = nd.d.x, r1 = nd.d.y, c0 = nd.d.z, c1 = nd.d.w;
+    if (lb1 < lb0) {
+      float tl = lb0; lb0 = lb1; lb1 = tl;
+      int32_t tr_ = r0; r0 = r1; r1 = tr_;
+      int32_t tc = c0; c0 = c1; c1 = tc;
+    }
+    int32_t next = -1;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      float lb = c == 0 ? lb0 : lb1;
+      int32_t r = c == 0 ? r0 : r1;
+      int32_t cnt = c == 0 ? c0 : c1;
+      if (!(lb <= bound())) continue;
+      if (r < 0) {
+        int32_t f = ~r;
+        for (int32_t kk = f; kk < f + cnt; ++kk) {
+          float d;
+          int32_t id;
+          if (POINTS) {
+            float4 e = pr[kk].p;
+            id = __float_as_int(e.w);
+            d = sqn3(sub3(f3(e.x, e.y, e.z), p));
+          } else {
+            const TriRec *t = tr + kk;
+            id = t->ex.x;
+            float3 cp;
+            d = tri_closest(t->p0, t->p1, t->p2, t->pl, t->e0, t->e1, t->e2, p, cp);
+          }
+          float bd;
+          int32_t bi;
+          kth(bd, bi);
+          if (id != self && better(d, id, bd, bi)) knn_insert<K>(kd, ki, d, id);
+        }
+      } else if (next < 0) {
+        next = r;
+      } else {
+        s_ref[sp * Q_BLOCK + tid] = r;
+        s_lb[sp * Q_BLOCK + tid] = lb;
+        ++sp;
+      }
+    }
+    if (next >= 0) { node = next; continue; }
+    bool found = false;
+    while (sp > 0) {
+      --sp;
+      if (s_lb[sp * Q_BLOCK + tid] <= bound()) { node = s_ref[sp * Q_BLOCK + tid]; found = true; break; }
+    }
+    if (!found) break;
+  }
+  for (int j = 0; j < K; ++j) {
+    if (j >= k) break;
+    bool hit = ki[j] != 0x7fffffff;
+    out_idx[qi * k + j] = hit ? ki[j] : -1;
+    if (out_dist) out_dist[qi * k + j] = hit ? sqrtf(kd[j]) : -1.0f;
+    if (out_d2) out_d2[qi * k + j] = hit ? kd[j] : -1.0f;
+  }
+}
+
+//--------------------------------------------------------------------------------------------------
+// Rays
+//--------------------------------------------------------------------------------------------------
+// Slab entry/exit of the box widened by `w` (per axis), using the reciprocal direction.
+__device__ __forceinline__ float2 slab(float3 o, float3 inv, float w, float lx, float ly, float lz, float hx, float hy,
+                                       float hz) {
+  float tx0 = (lx - w - o.x) * inv.x, tx1 = (hx + w - o.x) * inv.x;
+  float ty0 = (ly - w - o.y) * inv.y, ty1 = (hy + w - o.y) * inv.y;
+  float tz0 = (lz - w - o.z) * inv.z, tz1 = (hz + w - o.z) * inv.z;
+  float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+  float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+  return make_float2(tn, tf);
+}
+
+// ANY = true: stop at the first accepted hit (any-hit query).
+template <int STACK, bool ANY>
+__global__ void __launch_bounds__(Q_BLOCK) k_ray(const BNode *__restrict__ nodes, const TriRec *__restrict__ tr,
+                                                 const float *__restrict__ orig, const float *__restrict__ dirs,
+                                                 const uint32_t *__restrict__ order, int64_t n, uint32_t nb_real,
+                                                 float best_init, float max_time, float scale,
+                                                 int32_t *__restrict__ out_idx, float *__restrict__ out_t,
+                                                 float *__restrict__ out_n, uint8_t *__restrict__ out_any) {
+  __shared__ int32_t s_ref[STACK * Q_BLOCK];
+  __shared__ float s_t[STACK * Q_BLOCK];
+  bool active;
+  int64_t ri = query_slot(nb_real, n, order, active);
+  if (!active) return;
+  const int tid = threadIdx.x;
+  float3 o = load_q(orig, ri), d = load_q(dirs, ri);
+  float3 inv = f3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+  float m = fmaxf(scale, fmaxf(fabsf(o.x), fmaxf(fabsf(o.y), fabsf(o.z))));
+  float w = m * SLACK_REL;
+  float best = best_init;
+  int32_t bidx = 0x7fffffff;
+  int32_t bslot = -1;
+  int sp = 0;
+  int32_t node = 0;
+  // A box is entered when its (widened) entry time is within this relative margin of the best.
+  const float REL = 1.0f + 1.0f / 4096.0f;
+  bool done = false;
+  while (!done) {
+    const BNode nd = nodes[node];
+    float2 s0 = slab(o, inv, w, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y);
+    float2 s1 = slab(o, inv, w, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w);
+    bool h0 = s0.x <= s0.y * REL, h1 = s1.x <= s1.y * REL;
+    float t0 = h0 ? s0.x : INFINITY, t1 = h1 ? s1.x : INFINITY;
+    int32_t r0 = nd.d.x, r1 = nd.d.y, c0 = nd.d.z, c1 = nd.d.w;
+    if (t1 < t0) {
+      float tt = t0; t0 = t1; t1 = tt;
+      int32_t a = r0; r0 = r1; r1 = a;
+      int32_t b = c0; c0 = c1; c1 = b;
+    }
+    int32_t next = -1;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      float tn = c == 0 ? t0 : t1;
+      int32_t r = c == 0 ? r0 : r1;
+      int32_t cnt = c == 0 ? c0 : c1;
+      if (!(tn <= best * REL)) continue;
+      if (r < 0) {
+        int32_t f = ~r;
+        for (int32_t k = f; k < f + cnt; ++k) {
+          const TriRec *t = tr + k;
+          float4 a = t->p0, b = t->p1, cc = t->p2;
+          float th = tri_ray(f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), f3(cc.x, cc.y, cc.z), o, d, max_time);
+          if (th >= 0) {
+            int32_t id = t->ex.x;
+            if (better(th, id, best, bidx)) {
+              best = th;
+              bidx = id;
+              bslot = k;
+              if (ANY) { done = true; break; }
+            }
+          }
+        }
+        if (ANY && done) break;
+      } else if (next < 0) {
+        next = r;
+      } else {
+        s_ref[sp * Q_BLOCK + tid] = r;
+        s_t[sp * Q_BLOCK + tid] = tn;
+        ++sp;
+      }
+    }
+    if (done) break;
+    if (next >= 0) { node = next; continue; }
+    bool found = false;
+    while (sp > 0) {
+      --sp;
+      if (s_t[sp * Q_BLOCK + tid] <= best * REL) { node = s_ref[sp * Q_BLOCK + tid]; found = true; break; }
+    }
+    if (!found) break;
+  }
+  bool hit = bidx != 0x7fffffff;
+  if (ANY) {
+    out_any[ri] = hit ? 1 : 0;
+    return;
+  }
+  out_idx[ri] = hit ? bidx : -1;
+  if (out_t) out_t[ri] = hit ? best : -1.0f;
+  if (out_n) {
+    float nx = 0, ny = 0, nz = 0;
+    if (hit) { nx = tr[bslot].p0.w; ny = tr[bslot].p1.w; nz = tr[bslot].p2.w; }
+    out_n[3 * ri] = nx; out_n[3 * ri + 1] = ny; out_n[3 * ri + 2] = nz;
+  }
+}
+
+//--------------------------------------------------------------------------------------------------
+// Spatial ordering of a query batch
+//--------------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t spread10q(uint32_t v) {
+  v &= 0x3FF;
+  v = (v | (v << 16)) & 0x030000FF;
+  v = (v | (v << 8)) & 0x0300F00F;
+  v = (v | (v << 4)) & 0x030C30C3;
+  v = (v | (v << 2)) & 0x09249249;
+  return v;
+}
+
+__global__ void __launch_bounds__(256) k_query_keys(const float *__restrict__ q, int64_t n, float3 lo, float3 inv,
+                                                    uint32_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  float3 p = load_q(q, i);
+  float x = (p.x - lo.x) * inv.x, y = (p.y - lo.y) * inv.y, z = (p.z - lo.z) * inv.z;
+  // NaN coordinates map to cell 0 (fmaxf drops NaN).
+  uint32_t ix = (uint32_t)fminf(fmaxf(x * 1024.0f, 0.0f), 1023.0f);
+  uint32_t iy = (uint32_t)fminf(fmaxf(y * 1024.0f, 0.0f), 1023.0f);
+  uint32_t iz = (uint32_t)fminf(fmaxf(z * 1024.0f, 0.0f), 1023.0f);
+  keys[i] = (spread10q(ix) << 2) | (spread10q(iy) << 1) | spread10q(iz);
+  vals[i] = (uint32_t)i;
+}
+
+//--------------------------------------------------------------------------------------------------
+// Host launchers
+//--------------------------------------------------------------------------------------------------
+static int pick_stack(int depth) {
+  if (depth <= 16) return 16;
+  if (depth <= 32) return 32;
+  if (depth <= 64) return 64;
+  if (depth <= 128) return 128;
+  fail(THG_E_UNSUPPORTED, "BVH deeper than 128 levels");
+}
+
+void order_queries(const Bvh &b, const float *q, int64_t n, QueryScratch &ws, hipStream_t s) {
+  ws.order = nullptr;
+  if (n < ORDER_MIN_QUERIES) return;
+  KernelTimer kt(b.ctx, s, "order");
+  size_t hw = radix_hist_words(n);
+  ws.keys.ensure(n * 4, s);
+  ws.vals.ensure(n * 4, s);
+  ws.tk.ensure(n * 4, s);
+  ws.tv.ensure(n * 4, s);
+  ws.hist.ensure(hw * 4, s);
+  float3 ext = f3(b.hi.x - b.lo.x, b.hi.y - b.lo.y, b.hi.z - b.lo.z);
+  float3 inv = f3(ext.x > 0 ? 1.0f / ext.x : 0.0f, ext.y > 0 ? 1.0f / ext.y : 0.0f, ext.z > 0 ? 1.0f / ext.z : 0.0f);
+  hipLaunchKernelGGL(k_query_keys, dim3(ceil_div(n, 256)), dim3(256), 0, s, q, n, b.lo, inv, ws.keys.as<uint32_t>(),
+                     ws.vals.as<uint32_t>());
+  launch_check("k_query_keys");
+  // The top 16 of the 30 code bits give cells of ~1/32 of the scene per axis: enough coherence.
+  radix_sort_pairs(ws.keys.as<uint32_t>(), ws.vals.as<uint32_t>(), ws.tk.as<uint32_t>(), ws.tv.as<uint32_t>(),
+                   ws.hist.as<uint32_t>(), n, 14, 30, s);
+  ws.order = ws.vals.as<uint32_t>();
+  kt.stop();
+}
+
+static float prev_float(float x) { return std::nextafter(x, -INFINITY); }
+
+// Largest float strictly below the squared bound, or +inf when unbounded: with the lowest-index tie
+// rule, initialising the best to (this, INT_MAX) accepts exactly d^2 < dist_bound^2, as Thea's
+// double comparison does (BvhN.hpp:446-447, :459-466).
+float bound_to_best(double dist_bound) {
+  if (dist_bound < 0) return INFINITY;
+  double b2 = dist_bound * dist_bound;
+  float f = (float)b2;  // round to nearest
+  if ((double)f < b2) f = std::nextafter(f, INFINITY);  // smallest float >= b2
+  return prev_float(f);
+}
+
+void launch_closest(const Bvh &b, const float *q, int64_t n, double dist_bound, int exclude_self, int32_t *idx,
+                    float *cp, float *dist, float *d2, QueryScratch &ws, hipStream_t s) {
+  if (n <= 0) return;
+  order_queries(b, q, n, ws, s);
+  uint32_t nb = ceil_div(n, Q_BLOCK);
+  uint32_t grid = round_up8(nb);
+  float best = bound_to_best(dist_bound);
+  int st = pick_stack(b.max_depth);
+  KernelTimer kt(b.ctx, s, "closest");
+#define THG_CLOSEST(S, P)                                                                                            \
+  hipLaunchKernelGGL((k_closest<S, P>), dim3(grid), dim3(Q_BLOCK), 0, s, b.nodes.as<BNode>(), b.elems.p, q, ws.order, n, \
+                     nb, best, b.scale, exclude_self, idx, cp, dist, d2)
+  if (b.points) {
+    if (st == 16) THG_CLOSEST(16, true); else if (st == 32) THG_CLOSEST(32, true);
+    else if (st == 64) THG_CLOSEST(64, true); else THG_CLOSEST(128, true);
+  } else {
+    if (st == 16) THG_CLOSEST(16, false); else if (st == 32) THG_CLOSEST(32, false);
+    else if (st == 64) THG_CLOSEST(64, false); else THG_CLOSEST(128, false);
+  }
+#undef THG_CLOSEST
+  launch_check("k_closest");
+  kt.stop();
+}
+
+template <int K>
+static void launch_knn_k(const Bvh &b, int st, uint32_t grid, uint32_t nb, const float *q, int64_t n, int k, float best,
+                         int exclude_self, int32_t *idx, float *dist, float *d2, const uint32_t *order, hipStream_t s) {
+#define THG_KNN(S, P)                                                                                                \
+  hipLaunchKernelGGL((k_knn<S, P, K>), dim3(grid), dim3(Q_BLOCK), 0, s, b.nodes.as<BNode>(), b.elems.p, q, order, n, nb, \
+                     k, best, b.scale, exclude_self, idx, dist, d2)
+  if (b.points) {
+    if (st <= 32) THG_KNN(32, true); else if (st == 64) THG_KNN(64, true); else THG_KNN(128, true);
+  } else {
+    if (st <= 32) THG_KNN(32, false); else if (st == 64) THG_KNN(64, false); else THG_KNN(128, false);
+  }
+#undef THG_KNN
+}
+
+void launch_knn(const Bvh &b, const float *q, int64_t n, int k, double dist_bound, int exclude_self, int32_t *idx,
+                float *dist, float *d2, QueryScratch &ws, hipStream_t s) {
+  if (n <= 0) return;
+  order_queries(b, q, n, ws, s);
+  uint32_t nb = ceil_div(n, Q_BLOCK);
+  uint32_t grid = round_up8(nb);
+  float best = bound_to_best(dist_bound);
+  int st = pick_stack(b.max_depth);
+  KernelTimer kt(b.ctx, s, "knn");
+  if (k <= 4) launch_knn_k<4>(b, st, grid, nb, q, n, k, best, exclude_self, idx, dist, d2, ws.order, s);
+  else if (k <= 8) launch_knn_k<8>(b, st, grid, nb, q, n, k, best, exclude_self, idx, dist, d2, ws.order, s);
+  else if (k <= 16) launch_knn_k<16>(b, st, grid, nb, q, n, k, best, exclude_self, idx, dist, d2, ws.order, s);
+  else launch_knn_k<32>(b, st, grid, nb, q, n, k, best, exclude_self, idx, dist, d2, ws.order, s);
+  launch_check("k_knn");
+  kt.stop();
+}
+
+void launch_ray(const Bvh &b, const float *o, const float *d, int64_t n, float max_time, bool any, int32_t *idx,
+                float *t, float *nrm, uint8_t *hit, QueryScratch &ws, hipStream_t s) {
+  if (n <= 0) return;
+  order_queries(b, o, n, ws, s);
+  uint32_t nb = ceil_div(n, Q_BLOCK);
+  uint32_t grid = round_up8(nb);
+  // Thea accepts only hits strictly before max_time at the structure level (BvhN.hpp:2247-2258).
+  float best = max_time >= 0 ? prev_float(max_time) : INFINITY;
+  int st = pick_stack(b.max_depth);
+  KernelTimer kt(b.ctx, s, any ? "any_hit" : "ray");
+#define THG_RAY(S, A)                                                                                                \
+  hipLaunchKernelGGL((k_ray<S, A>), dim3(grid), dim3(Q_BLOCK), 0, s, b.nodes.as<BNode>(), b.elems.as<TriRec>(), o, d,   \
+                     ws.order, n, nb, best, max_time, b.scale, idx, t, nrm, hit)
+  if (any) {
+    if (st <= 32) THG_RAY(32, true); else if (st == 64) THG_RAY(64, true); else THG_RAY(128, true);
+  } else {
+    if (st <= 32) THG_RAY(32, false); else if (st == 64) THG_RAY(64, false); else THG_RAY(128, false);
+  }
+#undef THG_RAY
+  launch_check("k_ray");
+  kt.stop();
+}
+
+}  // namespace thg

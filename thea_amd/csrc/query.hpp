@@ -1,0 +1,32 @@
+// Host-side interface of the query kernels (query.hip) and sampler (sample.hip).
+#pragma once
+
+#include "common.hpp"
+
+namespace thg {
+
+// Batches smaller than this are processed in input order (sorting would cost more than it saves).
+constexpr int64_t ORDER_MIN_QUERIES = 16384;
+
+// Per-call temporaries of a query batch (spatial order of the queries).
+struct QueryScratch {
+  AsyncBuf keys, vals, tk, tv, hist;
+  const uint32_t *order = nullptr;
+};
+
+void build_bvh(Bvh &b, const float *verts, const int32_t *tris, int64_t n, bool points, int leaf_size, hipStream_t s);
+void order_queries(const Bvh &b, const float *q, int64_t n, QueryScratch &ws, hipStream_t s);
+float bound_to_best(double dist_bound);
+void launch_closest(const Bvh &b, const float *q, int64_t n, double dist_bound, int exclude_self, int32_t *idx,
+                    float *cp, float *dist, float *d2, QueryScratch &ws, hipStream_t s);
+void launch_knn(const Bvh &b, const float *q, int64_t n, int k, double dist_bound, int exclude_self, int32_t *idx,
+                float *dist, float *d2, QueryScratch &ws, hipStream_t s);
+void launch_ray(const Bvh &b, const float *o, const float *d, int64_t n, float max_time, bool any, int32_t *idx,
+                float *t, float *nrm, uint8_t *hit, QueryScratch &ws, hipStream_t s);
+
+// Sampler
+void mesh_cdf(Mesh &m, hipStream_t s);
+void launch_sample(Mesh &m, int64_t n, uint64_t seed, uint64_t stream_id, float *pos, float *nrm, int32_t *tri,
+                   hipStream_t s);
+
+}  // namespace thg

@@ -1,0 +1,188 @@
+// Per-element float arithmetic of Thea's query path, restated for HIP device code.
+//
+// Every expression keeps Thea's operation order so results are bit-identical to Thea's float
+// path compiled with Code/Build/Common/BuildSetup.cmake:96-111 (-O2, no fast-math, SSE2, no FMA):
+//  - no contraction of a*b+c into FMA (pragma below, and -ffp-contract=off on the command line);
+//  - IEEE-correct division and square root (HIP default, -fhip-fp32-correctly-rounded-divide-sqrt);
+//  - 3-vector reductions in Eigen 3.4's fixed-size order x0 + (x1 + x2) (Thea's Vector3 is Eigen);
+//  - Eigen normalized(): each component divided by sqrt(squaredNorm()), only if squaredNorm() > 0.
+// What is precomputed per triangle (plane, 2-D inside-test area and its reciprocal, perimeter edge
+// directions and lengths) is computed with exactly the operations Thea performs per query, so
+// caching changes no bits.
+#pragma once
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_types.hpp"
+
+namespace thg {
+
+#define THG_HD __host__ __device__ __forceinline__
+
+THG_HD float3 f3(float x, float y, float z) { float3 r; r.x = x; r.y = y; r.z = z; return r; }
+THG_HD float3 sub3(float3 a, float3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+THG_HD float3 add3(float3 a, float3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+THG_HD float3 scl3(float s, float3 a) { return f3(s * a.x, s * a.y, s * a.z); }
+THG_HD float dot3(float3 a, float3 b) { return a.x * b.x + (a.y * b.y + a.z * b.z); }
+THG_HD float sqn3(float3 a) { return dot3(a, a); }
+THG_HD float3 cross3(float3 a, float3 b) {
+  return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+THG_HD float comp(float3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+THG_HD float fabs_(float x) { return x < 0 ? -x : (x == 0 ? 0.0f : x); }
+
+// Hyperplane3.hpp:34-41 fromThreePoints normal, with Eigen normalized().
+THG_HD float3 plane_normal(float3 v0, float3 v1, float3 v2) {
+  float3 c = cross3(sub3(v1, v0), sub3(v2, v0));
+  float nn = sqn3(c);
+  if (nn > 0) {
+    float s = sqrtf(nn);
+    return f3(c.x / s, c.y / s, c.z / s);
+  }
+  return c;
+}
+
+// MatrixUtil.hpp:108-115 maxAbsAxis (first maximum wins, as Eigen maxCoeff).
+THG_HD int max_abs_axis(float3 n) {
+  float ax = fabsf(n.x), ay = fabsf(n.y), az = fabsf(n.z);
+  int k = 0;
+  float m = ax;
+  if (ay > m) { m = ay; k = 1; }
+  if (az > m) k = 2;
+  return k;
+}
+
+// Triangle3.hpp:154 area = 0.5 |n . (e01 x e02)|.
+THG_HD float tri_area(float3 v0, float3 v1, float3 v2) {
+  float3 n = plane_normal(v0, v1, v2);
+  float3 e01 = sub3(v1, v0), e02 = sub3(v2, v0);
+  return 0.5f * fabsf(dot3(n, cross3(e01, e02)));
+}
+
+// In-plane axes of isPointInsideTriangle (Triangle3.hpp:1426-1431).
+THG_HD void plane_axes(int axis, int &i, int &j) {
+  if (axis == 1) { i = 2; j = 0; }
+  else if (axis == 2) { i = 0; j = 1; }
+  else { i = 1; j = 2; }
+}
+
+THG_HD TriRec make_trirec(float3 v0, float3 v1, float3 v2, int32_t id) {
+  TriRec r;
+  float3 n = plane_normal(v0, v1, v2);
+  float pd = dot3(n, v0);
+  int axis = max_abs_axis(n);
+  int i, j;
+  plane_axes(axis, i, j);
+  // THEA_TRI3_AREA2(v0, v1, v2)
+  float ar = (comp(v1, i) - comp(v0, i)) * (comp(v2, j) - comp(v0, j)) -
+             (comp(v2, i) - comp(v0, i)) * (comp(v1, j) - comp(v0, j));
+  float inv = (ar == 0) ? 0.0f : 1.0f / ar;
+  float3 d0 = sub3(v1, v0), d1 = sub3(v2, v1), d2 = sub3(v0, v2);
+  float l0 = sqrtf(sqn3(d0)), l1 = sqrtf(sqn3(d1)), l2 = sqrtf(sqn3(d2));
+  d0 = f3(d0.x / l0, d0.y / l0, d0.z / l0);
+  d1 = f3(d1.x / l1, d1.y / l1, d1.z / l1);
+  d2 = f3(d2.x / l2, d2.y / l2, d2.z / l2);
+  r.p0 = make_float4(v0.x, v0.y, v0.z, n.x);
+  r.p1 = make_float4(v1.x, v1.y, v1.z, n.y);
+  r.p2 = make_float4(v2.x, v2.y, v2.z, n.z);
+  union { int i; float f; } m;
+  m.i = axis;
+  r.pl = make_float4(pd, inv, ar, m.f);
+  r.e0 = make_float4(d0.x, d0.y, d0.z, l0);
+  r.e1 = make_float4(d1.x, d1.y, d1.z, l1);
+  r.e2 = make_float4(d2.x, d2.y, d2.z, l2);
+  r.ex = make_int4(id, 0, 0, 0);
+  return r;
+}
+
+// Triangle3.hpp:1310-1338 closestPointOnLineSegment.
+THG_HD float3 seg_closest(float3 a, float3 b, float3 dir, float len, float3 p) {
+  float3 c = sub3(p, a);
+  float t = dot3(dir, c);
+  if (t <= 0) return a;
+  if (t >= len) return b;
+  return add3(a, scl3(t, dir));
+}
+
+// Thea's TriangleN::closestPoint (Triangle3.hpp:294-304) followed by MetricL2's
+// (cp - p).squaredNorm() (MetricL2.hpp:209-214). Returns d^2, writes the closest point.
+THG_HD float tri_closest(const float4 p0, const float4 p1, const float4 p2, const float4 pl, const float4 e0,
+                         const float4 e1, const float4 e2, float3 p, float3 &cp) {
+  float3 v0 = f3(p0.x, p0.y, p0.z), v1 = f3(p1.x, p1.y, p1.z), v2 = f3(p2.x, p2.y, p2.z);
+  float3 n = f3(p0.w, p1.w, p2.w);
+  // HyperplaneN.hpp:145-160: signedDistance = p.n - d; closestPoint = p - sd n.
+  float sd = dot3(p, n) - pl.x;
+  float3 proj = sub3(p, scl3(sd, n));
+  union { float f; int i; } m;
+  m.f = pl.w;
+  int i, j;
+  plane_axes(m.i & 3, i, j);
+  bool inside;
+  // isPointInsideTriangle (Triangle3.hpp:1414-1462)
+  if (pl.z == 0) {
+    inside = (v0.x == proj.x) && (v0.y == proj.y) && (v0.z == proj.z);
+  } else {
+    float pi = comp(proj, i), pj = comp(proj, j);
+    float v0i = comp(v0, i), v0j = comp(v0, j), v1i = comp(v1, i), v1j = comp(v1, j);
+    float v2i = comp(v2, i), v2j = comp(v2, j);
+    float b0 = ((v1i - pi) * (v2j - pj) - (v2i - pi) * (v1j - pj)) * pl.y;
+    inside = !(b0 < 0 || b0 > 1);
+    if (inside) {
+      float b1 = ((pi - v0i) * (v2j - v0j) - (v2i - v0i) * (pj - v0j)) * pl.y;
+      inside = !(b1 < 0 || b1 > 1);
+      if (inside) {
+        float b2 = 1.0f - b0 - b1;
+        inside = (b2 >= 0) && (b2 <= 1);
+      }
+    }
+  }
+  if (inside) {
+    cp = proj;
+  } else {
+    // closestPointOnTrianglePerimeter (Triangle3.hpp:1340-1412), cached directions/lengths.
+    float3 r0 = seg_closest(v0, v1, f3(e0.x, e0.y, e0.z), e0.w, p);
+    float3 r1 = seg_closest(v1, v2, f3(e1.x, e1.y, e1.z), e1.w, p);
+    float3 r2 = seg_closest(v2, v0, f3(e2.x, e2.y, e2.z), e2.w, p);
+    float d0 = sqn3(sub3(r0, p)), d1 = sqn3(sub3(r1, p)), d2 = sqn3(sub3(r2, p));
+    if (d0 < d1) cp = (d0 < d2) ? r0 : r2;
+    else cp = (d1 < d2) ? r1 : r2;
+  }
+  return sqn3(sub3(cp, p));
+}
+
+THG_HD float tri_closest(const TriRec &r, float3 p, float3 &cp) {
+  return tri_closest(r.p0, r.p1, r.p2, r.pl, r.e0, r.e1, r.e2, p, cp);
+}
+
+// Triangle3.hpp:1464-1524 rayTriangleIntersectionTime (Eberly) with edges e01 = v1 - v0 and
+// e02 = v2 - v0 (Triangle3.hpp:151-152), then TriangleN::rayIntersection's max_time rule
+// (Triangle3.hpp:542-552). Returns t >= 0 on a hit, -1 otherwise.
+THG_HD float tri_ray(float3 v0, float3 v1, float3 v2, float3 o, float3 dir, float max_time) {
+  const float EPS = 1.19209290e-07f;  // FLT_EPSILON, Math::eps<float>()
+  float3 e01 = sub3(v1, v0), e02 = sub3(v2, v0);
+  float3 diff = sub3(o, v0);
+  float3 normal = cross3(e01, e02);
+  float DdN = dot3(dir, normal);
+  float sign;
+  if (DdN > EPS) sign = 1.0f;
+  else if (DdN < -EPS) { sign = -1.0f; DdN = -DdN; }
+  else return -1.0f;
+  float DdQxE2 = sign * dot3(dir, cross3(diff, e02));
+  if (DdQxE2 >= 0) {
+    float DdE1xQ = sign * dot3(dir, cross3(e01, diff));
+    if (DdE1xQ >= 0) {
+      if (DdQxE2 + DdE1xQ <= DdN) {
+        float QdN = (-sign) * dot3(diff, normal);
+        if (QdN >= 0) {
+          float t = QdN / DdN;
+          if (t >= 0 && (max_time < 0 || t <= max_time)) return t;
+        }
+      }
+    }
+  }
+  return -1.0f;
+}
+
+}  // namespace thg
