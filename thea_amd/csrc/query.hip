@@ -140,6 +140,165 @@ __global__ void __launch_bounds__(Q_BLOCK) k_closest(const BNode *__restrict__ n
 }
 
 //--------------------------------------------------------------------------------------------------
+// Closest element, wave-cooperative ("packet") traversal
+//
+// The 64 lanes of a wave hold 64 spatially sorted queries and walk the tree together: a child is
+// entered if any lane's lower bound admits it, node and triangle loads are wave-uniform (scalar
+// loads), and every lane evaluates every admitted triangle, so there is no divergence in the walk.
+// The uniform traversal stack is spread over the wave's lanes: entry i (child ref and its box)
+// lives in lane i's registers; a push is one predicated register write, a pop is v_readlane.
+// Results are identical to the per-lane kernel: evaluating extra triangles never changes the
+// canonical minimum, and a subtree is skipped only if its bound exceeds every lane's best.
+//--------------------------------------------------------------------------------------------------
+constexpr int W_BLOCK = 256;
+constexpr int W_WAVES = W_BLOCK / WAVE;
+constexpr int W_STACK = WAVE;  // stack entries, one per lane
+
+__device__ __forceinline__ int32_t uni(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ float unif(float x) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x))); }
+__device__ __forceinline__ int32_t lane_read(int32_t x, int32_t l) { return __builtin_amdgcn_readlane(x, l); }
+__device__ __forceinline__ float lane_readf(float x, int32_t l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+
+template <bool POINTS>
+__global__ void __launch_bounds__(W_BLOCK) k_closest_wave(const BNode *__restrict__ nodes, const void *__restrict__ elems,
+                                                          const float *__restrict__ q, const uint32_t *__restrict__ order,
+                                                          int64_t n, uint32_t nb_real, float best_init, float scale,
+                                                          int exclude_self, int32_t *__restrict__ out_idx,
+                                                          float *__restrict__ out_cp, float *__restrict__ out_dist,
+                                                          float *__restrict__ out_d2) {
+  const int lane = lane_id();
+  const uint32_t lb_blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t t = ((int64_t)lb_blk * W_WAVES + (threadIdx.x / WAVE)) * WAVE + lane;
+  bool active = lb_blk < nb_real && t < n;
+  int64_t qi = 0;
+  float3 p = f3(0, 0, 0);
+  if (active) {
+    qi = order ? (int64_t)order[t] : t;
+    p = load_q(q, qi);
+    active = !(isnan(p.x) || isnan(p.y) || isnan(p.z));
+  }
+  const uint64_t amask = __ballot(active);
+  if (amask == 0) {
+    if (lb_blk < nb_real && t < n) {  // NaN queries: no result
+      out_idx[qi] = -1;
+      if (out_cp) { out_cp[3 * qi] = 0; out_cp[3 * qi + 1] = 0; out_cp[3 * qi + 2] = 0; }
+      if (out_dist) out_dist[qi] = -1.0f;
+      if (out_d2) out_d2[qi] = -1.0f;
+    }
+    return;
+  }
+  const int nact = __popcll(amask);
+  float m = fmaxf(scale, fmaxf(fabsf(p.x), fmaxf(fabsf(p.y), fabsf(p.z))));
+  float sl = m * SLACK_REL;
+  float3 qlo = f3(p.x - sl, p.y - sl, p.z - sl), qhi = f3(p.x + sl, p.y + sl, p.z + sl);
+  float best = active ? best_init : -1.0f;  // inactive lanes never admit anything
+  int32_t bidx = 0x7fffffff;
+  float3 bcp = f3(0, 0, 0);
+  const int32_t self = exclude_self ? (int32_t)qi : -1;
+  const TriRec *__restrict__ tr = (const TriRec *)elems;
+  const PointRec *__restrict__ pr = (const PointRec *)elems;
+  // lane-distributed stack slot
+  int32_t s_ref = 0;
+  float s_lx = 0, s_ly = 0, s_lz = 0, s_hx = 0, s_hy = 0, s_hz = 0;
+  int32_t sp = 0;
+  int32_t node = 0;
+  while (true) {
+    node = uni(node);
+    const BNode nd = nodes[node];
+    const float a0 = unif(nd.a.x), a1 = unif(nd.a.y), a2 = unif(nd.a.z), a3 = unif(nd.a.w);
+    const float b0 = unif(nd.b.x), b1 = unif(nd.b.y), b2 = unif(nd.b.z), b3 = unif(nd.b.w);
+    const float c0 = unif(nd.c.x), c1 = unif(nd.c.y), c2 = unif(nd.c.z), c3 = unif(nd.c.w);
+    const int32_t r0 = uni(nd.d.x), r1 = uni(nd.d.y), n0 = uni(nd.d.z), n1 = uni(nd.d.w);
+    float lb0 = box_lb(qlo, qhi, a0, a1, a2, a3, b0, b1);
+    float lb1 = box_lb(qlo, qhi, b2, b3, c0, c1, c2, c3);
+    uint64_t m0 = __ballot(lb0 <= best), m1 = __ballot(lb1 <= best);
+    bool first1 = 2 * __popcll(__ballot(lb1 < lb0) & amask) > nact;
+    int32_t next = -1;
+#pragma unroll
+    for (int cc = 0; cc < 2; ++cc) {
+      const int c = first1 ? 1 - cc : cc;
+      const uint64_t mc = c == 0 ? m0 : m1;
+      if (mc == 0) continue;
+      const int32_t r = c == 0 ? r0 : r1;
+      const int32_t cnt = c == 0 ? n0 : n1;
+      if (r < 0) {
+        const int32_t f = ~r;
+        for (int32_t k = f; k < f + cnt; ++k) {
+          float d;
+          float3 cp;
+          int32_t id;
+          if (POINTS) {
+            const float4 e = pr[k].p;
+            id = uni(__float_as_int(e.w));
+            cp = f3(unif(e.x), unif(e.y), unif(e.z));
+            d = sqn3(sub3(cp, p));
+          } else {
+            const TriRec *tk = tr + k;
+            const float4 p0 = tk->p0, p1 = tk->p1, p2 = tk->p2, pl = tk->pl, e0 = tk->e0, e1 = tk->e1, e2 = tk->e2;
+            id = uni(tk->ex.x);
+            d = tri_closest(make_float4(unif(p0.x), unif(p0.y), unif(p0.z), unif(p0.w)),
+                            make_float4(unif(p1.x), unif(p1.y), unif(p1.z), unif(p1.w)),
+                            make_float4(unif(p2.x), unif(p2.y), unif(p2.z), unif(p2.w)),
+                            make_float4(unif(pl.x), unif(pl.y), unif(pl.z), unif(pl.w)),
+                            make_float4(unif(e0.x), unif(e0.y), unif(e0.z), unif(e0.w)),
+                            make_float4(unif(e1.x), unif(e1.y), unif(e1.z), unif(e1.w)),
+                            make_float4(unif(e2.x), unif(e2.y), unif(e2.z), unif(e2.w)), p, cp);
+          }
+          if (id != self && better(d, id, best, bidx)) {
+            best = d;
+            bidx = id;
+            bcp = cp;
+          }
+        }
+      } else if (next < 0) {
+        next = r;
+      } else {
+        // push child c (box and ref) into lane `sp`
+        if (lane == sp) {
+          s_ref = r;
+          if (c == 0) { s_lx = a0; s_ly = a1; s_lz = a2; s_hx = a3; s_hy = b0; s_hz = b1; }
+          else { s_lx = b2; s_ly = b3; s_lz = c0; s_hx = c1; s_hy = c2; s_hz = c3; }
+        }
+        ++sp;
+      }
+    }
+    if (next >= 0) { node = next; continue; }
+    bool found = false;
+    while (sp > 0) {
+      --sp;
+      const float lx = lane_readf(s_lx, sp), ly = lane_readf(s_ly, sp), lz = lane_readf(s_lz, sp);
+      const float hx = lane_readf(s_hx, sp), hy = lane_readf(s_hy, sp), hz = lane_readf(s_hz, sp);
+      if (__ballot(box_lb(qlo, qhi, lx, ly, lz, hx, hy, hz) <= best)) {
+        node = lane_read(s_ref, sp);
+        found = true;
+        break;
+      }
+    }
+    if (!found) break;
+  }
+  if (!active) {
+    if (lb_blk < nb_real && t < n) {
+      out_idx[qi] = -1;
+      if (out_cp) { out_cp[3 * qi] = 0; out_cp[3 * qi + 1] = 0; out_cp[3 * qi + 2] = 0; }
+      if (out_dist) out_dist[qi] = -1.0f;
+      if (out_d2) out_d2[qi] = -1.0f;
+    }
+    return;
+  }
+  bool hit = bidx != 0x7fffffff;
+  out_idx[qi] = hit ? bidx : -1;
+  if (out_cp) {
+    out_cp[3 * qi] = hit ? bcp.x : 0.0f;
+    out_cp[3 * qi + 1] = hit ? bcp.y : 0.0f;
+    out_cp[3 * qi + 2] = hit ? bcp.z : 0.0f;
+  }
+  if (out_dist) out_dist[qi] = hit ? sqrtf(best) : -1.0f;
+  if (out_d2) out_d2[qi] = hit ? best : -1.0f;
+}
+
+//--------------------------------------------------------------------------------------------------
 // k nearest (sorted list of K (d^2, id) pairs per lane in registers)
 //--------------------------------------------------------------------------------------------------
 template <int K>
@@ -400,6 +559,14 @@ static int pick_stack(int depth) {
   fail(THG_E_UNSUPPORTED, "BVH deeper than 128 levels");
 }
 
+// Lowest Morton bit the query sort uses (THG_ORDER_BITS = number of top bits, default all 30).
+static int order_bits_lo() {
+  static const char *env = getenv("THG_ORDER_BITS");
+  int bits = env ? atoi(env) : 30;
+  bits = bits < 8 ? 8 : (bits > 30 ? 30 : bits);
+  return 30 - bits;
+}
+
 void order_queries(const Bvh &b, const float *q, int64_t n, QueryScratch &ws, hipStream_t s) {
   ws.order = nullptr;
   if (n < ORDER_MIN_QUERIES) return;
@@ -417,12 +584,20 @@ void order_queries(const Bvh &b, const float *q, int64_t n, QueryScratch &ws, hi
   launch_check("k_query_keys");
   // The top 16 of the 30 code bits give cells of ~1/32 of the scene per axis: enough coherence.
   radix_sort_pairs(ws.keys.as<uint32_t>(), ws.vals.as<uint32_t>(), ws.tk.as<uint32_t>(), ws.tv.as<uint32_t>(),
-                   ws.hist.as<uint32_t>(), n, 14, 30, s);
+                   ws.hist.as<uint32_t>(), n, order_bits_lo(), 30, s);
   ws.order = ws.vals.as<uint32_t>();
   kt.stop();
 }
 
 static float prev_float(float x) { return std::nextafter(x, -INFINITY); }
+
+// Traversal strategy for point-proximity queries: wave-cooperative unless THG_TRAVERSAL=lane, or the
+// tree is deeper than the lane-distributed stack.
+bool use_wave_traversal(const Bvh &b) {
+  static const char *env = getenv("THG_TRAVERSAL");
+  bool want = !(env && std::string(env) == "lane");
+  return want && b.max_depth < W_STACK;
+}
 
 // Largest float strictly below the squared bound, or +inf when unbounded: with the lowest-index tie
 // rule, initialising the best to (this, INT_MAX) accepts exactly d^2 < dist_bound^2, as Thea's
@@ -442,6 +617,20 @@ void launch_closest(const Bvh &b, const float *q, int64_t n, double dist_bound, 
   uint32_t nb = ceil_div(n, Q_BLOCK);
   uint32_t grid = round_up8(nb);
   float best = bound_to_best(dist_bound);
+  if (use_wave_traversal(b)) {
+    uint32_t wnb = ceil_div(n, W_BLOCK);
+    uint32_t wgrid = round_up8(wnb);
+    KernelTimer kt(b.ctx, s, "closest");
+    if (b.points)
+      hipLaunchKernelGGL((k_closest_wave<true>), dim3(wgrid), dim3(W_BLOCK), 0, s, b.nodes.as<BNode>(), b.elems.p, q,
+                         ws.order, n, wnb, best, b.scale, exclude_self, idx, cp, dist, d2);
+    else
+      hipLaunchKernelGGL((k_closest_wave<false>), dim3(wgrid), dim3(W_BLOCK), 0, s, b.nodes.as<BNode>(), b.elems.p, q,
+                         ws.order, n, wnb, best, b.scale, exclude_self, idx, cp, dist, d2);
+    launch_check("k_closest_wave");
+    kt.stop();
+    return;
+  }
   int st = pick_stack(b.max_depth);
   KernelTimer kt(b.ctx, s, "closest");
 #define THG_CLOSEST(S, P)                                                                                            \
