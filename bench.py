@@ -199,6 +199,15 @@ def main():
         "input_generation_s": gen_s,
     }
 
+    if wl in ("c3", "c1"):
+        st = (C.c_int64 * 6)()
+        capi.check(L.thg_closest_stats(bvh.h, P(qd), nq, stream, st))
+        waves = max(st[0], 1)
+        result["gpu_traversal"] = {
+            "per_wave": {"nodes": st[2] / waves, "box_tests": st[3] / waves, "elements_loaded": st[4] / waves,
+                         "elements_evaluated": st[5] / waves},
+            "queries_per_wave": st[1] / waves,
+            "note": "wave-cooperative traversal: every lane evaluates each element its wave evaluates"}
     if not args.no_cpu_baseline and wl in ("c3", "c1", "c3ray", "c2knn"):
         result["cpu_baseline"] = cpu_baseline(args, wl, v, t, q if wl in ("c3", "c1") else None,
                                               (ro, rd) if wl == "c3ray" else None, unit)

@@ -145,6 +145,12 @@ int thg_sample_area(thg_mesh mesh, int64_t n, uint64_t seed, uint64_t stream_id,
 int thg_ctx_set_timing(thg_ctx ctx, int32_t enable);
 int thg_ctx_kernel_time(thg_ctx ctx, const char *kernel, double *total_ms, int64_t *launches, int32_t reset);
 
+/* Traversal statistics of the closest-point kernel (diagnostics): runs the wave-cooperative
+ * traversal over n queries (device pointer) and returns out[0] waves, [1] queries,
+ * [2] nodes loaded, [3] child-box tests, [4] elements loaded, [5] elements evaluated exactly
+ * (sums over waves; every lane of a wave evaluates each element the wave evaluates). */
+int thg_closest_stats(thg_bvh bvh, const float *q, int64_t n, void *stream, int64_t out[6]);
+
 /* Many meshes, one call: build a BVH for each of `nm` meshes and answer `nq[m]` closest-point
  * queries for mesh m, with one persistent launch for all builds' per-element work and one for all
  * queries. Thea has no analogue (each BvhN is built and queried by its own caller). Inputs are

@@ -78,7 +78,7 @@ inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1)
 
 // Radix sort of (key, value) pairs over key bits [begin_bit, end_bit), stable, LSD with 8-bit
 // digits. keys/vals are sorted in place (using tmp_keys/tmp_vals as the ping-pong partner);
-// `hist` must hold 256 * ceil(n / 4096) + 1 uint32.
+// `hist` must hold radix_hist_words(n) uint32.
 void radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *tmp_keys, uint32_t *tmp_vals, uint32_t *hist,
                       int64_t n, int begin_bit, int end_bit, hipStream_t s);
 size_t radix_hist_words(int64_t n);

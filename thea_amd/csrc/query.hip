@@ -161,17 +161,29 @@ __device__ __forceinline__ float lane_readf(float x, int32_t l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
 }
 
-template <bool POINTS>
+// G lanes share one query (64 / G queries per wave): lane g of a group tests elements g, g + G, ...
+// of each admitted leaf; the group's best distance (a min over its lanes) drives pruning, and the
+// group's lexicographic minimum is reduced at the end. With G = 1 element records are wave-uniform
+// and come through scalar loads. PRE adds a per-element box test before the exact arithmetic: the
+// element is skipped by every lane whose box lower bound exceeds its best (never changes results:
+// the box bound never exceeds the element's float distance, see box_lb's slack).
+template <bool POINTS, int G, bool PRE, bool STATS = false>
 __global__ void __launch_bounds__(W_BLOCK) k_closest_wave(const BNode *__restrict__ nodes, const void *__restrict__ elems,
                                                           const float *__restrict__ q, const uint32_t *__restrict__ order,
                                                           int64_t n, uint32_t nb_real, float best_init, float scale,
                                                           int exclude_self, int32_t *__restrict__ out_idx,
                                                           float *__restrict__ out_cp, float *__restrict__ out_dist,
-                                                          float *__restrict__ out_d2) {
+                                                          float *__restrict__ out_d2,
+                                                          unsigned long long *__restrict__ stats = nullptr) {
+  // STATS: per wave, count nodes loaded, box tests, leaf elements loaded and elements evaluated.
+  uint32_t st_nodes = 0, st_boxes = 0, st_loaded = 0, st_tested = 0;
+  constexpr int QW = WAVE / G;  // queries per wave
   const int lane = lane_id();
+  const int sub = lane % G;
   const uint32_t lb_blk = xcd_remap(blockIdx.x, gridDim.x);
-  const int64_t t = ((int64_t)lb_blk * W_WAVES + (threadIdx.x / WAVE)) * WAVE + lane;
-  bool active = lb_blk < nb_real && t < n;
+  const int64_t t = ((int64_t)lb_blk * W_WAVES + (threadIdx.x / WAVE)) * QW + lane / G;
+  const bool inrange = lb_blk < nb_real && t < n;
+  bool active = inrange;
   int64_t qi = 0;
   float3 p = f3(0, 0, 0);
   if (active) {
@@ -180,32 +192,24 @@ __global__ void __launch_bounds__(W_BLOCK) k_closest_wave(const BNode *__restric
     active = !(isnan(p.x) || isnan(p.y) || isnan(p.z));
   }
   const uint64_t amask = __ballot(active);
-  if (amask == 0) {
-    if (lb_blk < nb_real && t < n) {  // NaN queries: no result
-      out_idx[qi] = -1;
-      if (out_cp) { out_cp[3 * qi] = 0; out_cp[3 * qi + 1] = 0; out_cp[3 * qi + 2] = 0; }
-      if (out_dist) out_dist[qi] = -1.0f;
-      if (out_d2) out_d2[qi] = -1.0f;
-    }
-    return;
-  }
   const int nact = __popcll(amask);
   float m = fmaxf(scale, fmaxf(fabsf(p.x), fmaxf(fabsf(p.y), fabsf(p.z))));
   float sl = m * SLACK_REL;
   float3 qlo = f3(p.x - sl, p.y - sl, p.z - sl), qhi = f3(p.x + sl, p.y + sl, p.z + sl);
-  float best = active ? best_init : -1.0f;  // inactive lanes never admit anything
+  float best = active ? best_init : -1.0f;  // group best (pruning); inactive lanes admit nothing
+  float own = best;                          // this lane's best over the elements it tested
   int32_t bidx = 0x7fffffff;
   float3 bcp = f3(0, 0, 0);
   const int32_t self = exclude_self ? (int32_t)qi : -1;
   const TriRec *__restrict__ tr = (const TriRec *)elems;
   const PointRec *__restrict__ pr = (const PointRec *)elems;
-  // lane-distributed stack slot
   int32_t s_ref = 0;
   float s_lx = 0, s_ly = 0, s_lz = 0, s_hx = 0, s_hy = 0, s_hz = 0;
   int32_t sp = 0;
   int32_t node = 0;
-  while (true) {
+  while (amask != 0) {
     node = uni(node);
+    if (STATS) { st_nodes++; st_boxes += 2; }
     const BNode nd = nodes[node];
     const float a0 = unif(nd.a.x), a1 = unif(nd.a.y), a2 = unif(nd.a.z), a3 = unif(nd.a.w);
     const float b0 = unif(nd.b.x), b1 = unif(nd.b.y), b2 = unif(nd.b.z), b3 = unif(nd.b.w);
@@ -225,37 +229,60 @@ __global__ void __launch_bounds__(W_BLOCK) k_closest_wave(const BNode *__restric
       const int32_t cnt = c == 0 ? n0 : n1;
       if (r < 0) {
         const int32_t f = ~r;
-        for (int32_t k = f; k < f + cnt; ++k) {
-          float d;
+        for (int32_t k0 = f; k0 < f + cnt; k0 += G) {
+          const int32_t k = k0 + sub;
+          bool valid = k < f + cnt;
+          float d = INFINITY;
           float3 cp;
-          int32_t id;
+          int32_t id = 0x7fffffff;
           if (POINTS) {
-            const float4 e = pr[k].p;
-            id = uni(__float_as_int(e.w));
-            cp = f3(unif(e.x), unif(e.y), unif(e.z));
+            float4 e = pr[valid ? k : f].p;
+            if (G == 1) e = make_float4(unif(e.x), unif(e.y), unif(e.z), unif(e.w));
+            id = __float_as_int(e.w);
+            cp = f3(e.x, e.y, e.z);
             d = sqn3(sub3(cp, p));
           } else {
-            const TriRec *tk = tr + k;
-            const float4 p0 = tk->p0, p1 = tk->p1, p2 = tk->p2, pl = tk->pl, e0 = tk->e0, e1 = tk->e1, e2 = tk->e2;
-            id = uni(tk->ex.x);
-            d = tri_closest(make_float4(unif(p0.x), unif(p0.y), unif(p0.z), unif(p0.w)),
-                            make_float4(unif(p1.x), unif(p1.y), unif(p1.z), unif(p1.w)),
-                            make_float4(unif(p2.x), unif(p2.y), unif(p2.z), unif(p2.w)),
-                            make_float4(unif(pl.x), unif(pl.y), unif(pl.z), unif(pl.w)),
-                            make_float4(unif(e0.x), unif(e0.y), unif(e0.z), unif(e0.w)),
-                            make_float4(unif(e1.x), unif(e1.y), unif(e1.z), unif(e1.w)),
-                            make_float4(unif(e2.x), unif(e2.y), unif(e2.z), unif(e2.w)), p, cp);
+            const TriRec *tk = tr + (valid ? k : f);
+            float4 p0 = tk->p0, p1 = tk->p1, p2 = tk->p2;
+            if (G == 1) {
+              p0 = make_float4(unif(p0.x), unif(p0.y), unif(p0.z), unif(p0.w));
+              p1 = make_float4(unif(p1.x), unif(p1.y), unif(p1.z), unif(p1.w));
+              p2 = make_float4(unif(p2.x), unif(p2.y), unif(p2.z), unif(p2.w));
+            }
+            if (STATS) st_loaded++;
+            if (PRE) {
+              float tlb = box_lb(qlo, qhi, fminf(p0.x, fminf(p1.x, p2.x)), fminf(p0.y, fminf(p1.y, p2.y)),
+                                 fminf(p0.z, fminf(p1.z, p2.z)), fmaxf(p0.x, fmaxf(p1.x, p2.x)),
+                                 fmaxf(p0.y, fmaxf(p1.y, p2.y)), fmaxf(p0.z, fmaxf(p1.z, p2.z)));
+              valid = valid && tlb <= best;
+              if (__ballot(valid) == 0) continue;
+            }
+            if (STATS) st_tested++;
+            float4 pl = tk->pl, e0 = tk->e0, e1 = tk->e1, e2 = tk->e2;
+            id = tk->ex.x;
+            if (G == 1) {
+              pl = make_float4(unif(pl.x), unif(pl.y), unif(pl.z), unif(pl.w));
+              e0 = make_float4(unif(e0.x), unif(e0.y), unif(e0.z), unif(e0.w));
+              e1 = make_float4(unif(e1.x), unif(e1.y), unif(e1.z), unif(e1.w));
+              e2 = make_float4(unif(e2.x), unif(e2.y), unif(e2.z), unif(e2.w));
+              id = uni(id);
+            }
+            d = tri_closest(p0, p1, p2, pl, e0, e1, e2, p, cp);
           }
-          if (id != self && better(d, id, best, bidx)) {
-            best = d;
+          if (valid && id != self && better(d, id, own, bidx)) {
+            own = d;
             bidx = id;
             bcp = cp;
           }
         }
+        // group best for pruning
+        float gb = own;
+#pragma unroll
+        for (int off = 1; off < G; off <<= 1) gb = fminf(gb, __shfl_xor(gb, off, WAVE));
+        best = active ? fminf(best, gb) : best;
       } else if (next < 0) {
         next = r;
       } else {
-        // push child c (box and ref) into lane `sp`
         if (lane == sp) {
           s_ref = r;
           if (c == 0) { s_lx = a0; s_ly = a1; s_lz = a2; s_hx = a3; s_hy = b0; s_hz = b1; }
@@ -270,6 +297,7 @@ __global__ void __launch_bounds__(W_BLOCK) k_closest_wave(const BNode *__restric
       --sp;
       const float lx = lane_readf(s_lx, sp), ly = lane_readf(s_ly, sp), lz = lane_readf(s_lz, sp);
       const float hx = lane_readf(s_hx, sp), hy = lane_readf(s_hy, sp), hz = lane_readf(s_hz, sp);
+      if (STATS) st_boxes++;
       if (__ballot(box_lb(qlo, qhi, lx, ly, lz, hx, hy, hz) <= best)) {
         node = lane_read(s_ref, sp);
         found = true;
@@ -278,24 +306,32 @@ __global__ void __launch_bounds__(W_BLOCK) k_closest_wave(const BNode *__restric
     }
     if (!found) break;
   }
-  if (!active) {
-    if (lb_blk < nb_real && t < n) {
-      out_idx[qi] = -1;
-      if (out_cp) { out_cp[3 * qi] = 0; out_cp[3 * qi + 1] = 0; out_cp[3 * qi + 2] = 0; }
-      if (out_dist) out_dist[qi] = -1.0f;
-      if (out_d2) out_d2[qi] = -1.0f;
-    }
-    return;
+  if (STATS && lane == 0 && amask != 0) {
+    atomicAdd(&stats[0], 1ull);
+    atomicAdd(&stats[1], (unsigned long long)nact / G);
+    atomicAdd(&stats[2], (unsigned long long)st_nodes);
+    atomicAdd(&stats[3], (unsigned long long)st_boxes);
+    atomicAdd(&stats[4], (unsigned long long)st_loaded);
+    atomicAdd(&stats[5], (unsigned long long)st_tested);
   }
-  bool hit = bidx != 0x7fffffff;
+  // lexicographic (d, id) minimum over the group's lanes
+#pragma unroll
+  for (int off = 1; off < G; off <<= 1) {
+    float od = __shfl_xor(own, off, WAVE);
+    int32_t oi = __shfl_xor(bidx, off, WAVE);
+    float ox = __shfl_xor(bcp.x, off, WAVE), oy = __shfl_xor(bcp.y, off, WAVE), oz = __shfl_xor(bcp.z, off, WAVE);
+    if (better(od, oi, own, bidx)) { own = od; bidx = oi; bcp = f3(ox, oy, oz); }
+  }
+  if (!inrange || sub != 0) return;
+  bool hit = active && bidx != 0x7fffffff;
   out_idx[qi] = hit ? bidx : -1;
   if (out_cp) {
     out_cp[3 * qi] = hit ? bcp.x : 0.0f;
     out_cp[3 * qi + 1] = hit ? bcp.y : 0.0f;
     out_cp[3 * qi + 2] = hit ? bcp.z : 0.0f;
   }
-  if (out_dist) out_dist[qi] = hit ? sqrtf(best) : -1.0f;
-  if (out_d2) out_d2[qi] = hit ? best : -1.0f;
+  if (out_dist) out_dist[qi] = hit ? sqrtf(own) : -1.0f;
+  if (out_d2) out_d2[qi] = hit ? own : -1.0f;
 }
 
 //--------------------------------------------------------------------------------------------------
@@ -599,6 +635,19 @@ bool use_wave_traversal(const Bvh &b) {
   return want && b.max_depth < W_STACK;
 }
 
+// Lanes per query and per-element prefilter of the wave kernel (THG_WAVE_G, THG_PREFILTER override).
+struct WaveConfig {
+  int g;
+  bool pre;
+};
+static WaveConfig wave_config(const Bvh &b) {
+  WaveConfig c{b.wave_g, b.wave_pre};
+  if (const char *e = getenv("THG_WAVE_G")) c.g = atoi(e);
+  if (const char *e = getenv("THG_PREFILTER")) c.pre = atoi(e) != 0;
+  if (c.g != 1 && c.g != 2 && c.g != 4) c.g = 1;
+  return c;
+}
+
 // Largest float strictly below the squared bound, or +inf when unbounded: with the lowest-index tie
 // rule, initialising the best to (this, INT_MAX) accepts exactly d^2 < dist_bound^2, as Thea's
 // double comparison does (BvhN.hpp:446-447, :459-466).
@@ -618,15 +667,21 @@ void launch_closest(const Bvh &b, const float *q, int64_t n, double dist_bound, 
   uint32_t grid = round_up8(nb);
   float best = bound_to_best(dist_bound);
   if (use_wave_traversal(b)) {
-    uint32_t wnb = ceil_div(n, W_BLOCK);
+    const WaveConfig wc = wave_config(b);
+    uint32_t wnb = ceil_div(n, (uint64_t)W_WAVES * (WAVE / wc.g));
     uint32_t wgrid = round_up8(wnb);
     KernelTimer kt(b.ctx, s, "closest");
-    if (b.points)
-      hipLaunchKernelGGL((k_closest_wave<true>), dim3(wgrid), dim3(W_BLOCK), 0, s, b.nodes.as<BNode>(), b.elems.p, q,
-                         ws.order, n, wnb, best, b.scale, exclude_self, idx, cp, dist, d2);
-    else
-      hipLaunchKernelGGL((k_closest_wave<false>), dim3(wgrid), dim3(W_BLOCK), 0, s, b.nodes.as<BNode>(), b.elems.p, q,
-                         ws.order, n, wnb, best, b.scale, exclude_self, idx, cp, dist, d2);
+#define THG_CW(P, G, PRE)                                                                                              \
+  hipLaunchKernelGGL((k_closest_wave<P, G, PRE>), dim3(wgrid), dim3(W_BLOCK), 0, s, b.nodes.as<BNode>(), b.elems.p, q, \
+                     ws.order, n, wnb, best, b.scale, exclude_self, idx, cp, dist, d2)
+    if (b.points) {
+      if (wc.g == 1) THG_CW(true, 1, false); else if (wc.g == 2) THG_CW(true, 2, false); else THG_CW(true, 4, false);
+    } else if (wc.pre) {
+      if (wc.g == 1) THG_CW(false, 1, true); else if (wc.g == 2) THG_CW(false, 2, true); else THG_CW(false, 4, true);
+    } else {
+      if (wc.g == 1) THG_CW(false, 1, false); else if (wc.g == 2) THG_CW(false, 2, false); else THG_CW(false, 4, false);
+    }
+#undef THG_CW
     launch_check("k_closest_wave");
     kt.stop();
     return;
@@ -660,6 +715,23 @@ static void launch_knn_k(const Bvh &b, int st, uint32_t grid, uint32_t nb, const
     if (st <= 32) THG_KNN(32, false); else if (st == 64) THG_KNN(64, false); else THG_KNN(128, false);
   }
 #undef THG_KNN
+}
+
+void closest_wave_stats(const Bvh &b, const float *q, int64_t n, unsigned long long *stats, QueryScratch &ws,
+                        hipStream_t s) {
+  if (n <= 0 || b.n == 0) return;
+  order_queries(b, q, n, ws, s);
+  AsyncBuf idx(n * 4, s);
+  uint32_t wnb = ceil_div(n, W_BLOCK);
+  uint32_t wgrid = round_up8(wnb);
+  float best = INFINITY;
+  if (b.points)
+    hipLaunchKernelGGL((k_closest_wave<true, 1, false, true>), dim3(wgrid), dim3(W_BLOCK), 0, s, b.nodes.as<BNode>(),
+                       b.elems.p, q, ws.order, n, wnb, best, b.scale, 0, idx.as<int32_t>(), nullptr, nullptr, nullptr, stats);
+  else
+    hipLaunchKernelGGL((k_closest_wave<false, 1, true, true>), dim3(wgrid), dim3(W_BLOCK), 0, s, b.nodes.as<BNode>(),
+                       b.elems.p, q, ws.order, n, wnb, best, b.scale, 0, idx.as<int32_t>(), nullptr, nullptr, nullptr, stats);
+  launch_check("k_closest_wave(stats)");
 }
 
 void launch_knn(const Bvh &b, const float *q, int64_t n, int k, double dist_bound, int exclude_self, int32_t *idx,
