@@ -200,12 +200,13 @@ def main():
     }
 
     if wl in ("c3", "c1"):
-        st = (C.c_int64 * 6)()
+        st = (C.c_int64 * 8)()
         capi.check(L.thg_closest_stats(bvh.h, P(qd), nq, stream, st))
         waves = max(st[0], 1)
         result["gpu_traversal"] = {
             "per_wave": {"nodes": st[2] / waves, "box_tests": st[3] / waves, "elements_loaded": st[4] / waves,
-                         "elements_evaluated": st[5] / waves},
+                         "elements_evaluated": st[5] / waves, "plane_admitted": st[6] / waves,
+                         "near_best": st[7] / waves},
             "queries_per_wave": st[1] / waves,
             "note": "wave-cooperative traversal: every lane evaluates each element its wave evaluates"}
     if not args.no_cpu_baseline and wl in ("c3", "c1", "c3ray", "c2knn"):

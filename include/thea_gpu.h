@@ -147,9 +147,11 @@ int thg_ctx_kernel_time(thg_ctx ctx, const char *kernel, double *total_ms, int64
 
 /* Traversal statistics of the closest-point kernel (diagnostics): runs the wave-cooperative
  * traversal over n queries (device pointer) and returns out[0] waves, [1] queries,
- * [2] nodes loaded, [3] child-box tests, [4] elements loaded, [5] elements evaluated exactly
- * (sums over waves; every lane of a wave evaluates each element the wave evaluates). */
-int thg_closest_stats(thg_bvh bvh, const float *q, int64_t n, void *stream, int64_t out[6]);
+ * [2] nodes loaded, [3] child-box tests, [4] elements loaded, [5] elements evaluated exactly,
+ * [6] exact evaluations where some lane's plane distance admitted the element, [7] exact evaluations
+ * whose result came within 2^-8 of some lane's best (sums over waves; every lane of a wave evaluates
+ * each element the wave evaluates). */
+int thg_closest_stats(thg_bvh bvh, const float *q, int64_t n, void *stream, int64_t out[8]);
 
 /* Many meshes, one call: build a BVH for each of `nm` meshes and answer `nq[m]` closest-point
  * queries for mesh m, with one persistent launch for all builds' per-element work and one for all

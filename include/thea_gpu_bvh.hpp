@@ -1,0 +1,294 @@
+// thea_gpu_bvh.hpp -- header-only C++ host API over libthea_gpu's C ABI (thea_gpu.h), keeping the
+// names and argument conventions of Thea's BVH query and sampling path so templated callers
+// (Algorithms/Icp3.hpp:282-296, ShapeDiameter.hpp:214-217, Tools/MeshSample, SampleGraph, ShapeAlign)
+// compile against it with a typedef change:
+//
+//   Thea                                                  here
+//   MeshBvh<Mesh>::closestElement<MetricL2>(q, bound,     MeshBvh::closestElement<MetricL2>(q, bound,
+//       compat, &dist, &closest_point)                        compat, &dist, &closest_point)
+//     (Algorithms/BvhN.hpp:1050-1063)
+//   BvhN::closestPair<MetricL2>(q, bound, compat, true)   MeshBvh::closestPair<MetricL2>(...)
+//   BvhN::kClosestPairs<MetricL2>(q, set, bound, compat)  MeshBvh::kClosestPairs<MetricL2>(q, set, bound)
+//   BvhN::rayStructureIntersection<Tester>(ray, max_t)    MeshBvh::rayStructureIntersection<Tester>(ray, max_t)
+//   BvhN::rayIntersectionTime / rayIntersects             same names
+//   MeshSampler::sampleEvenlyByArea(n, pos, nrm, tris)    MeshSampler::sampleEvenlyByArea(n, pos, nrm, tris)
+//
+// Single-query calls run as a batch of one (correct, latency bound); the batch overloads
+// (closestElements, kClosestElements, rayIntersections, ...) are the fast path. Vector types only
+// need operator[] (Thea's Eigen Vector3 works). Failures throw thea_gpu::Error (Thea throws
+// Thea::Error, Error.hpp:27); "no result" is index -1 / time -1 as in Thea.
+#ifndef THEA_GPU_BVH_HPP
+#define THEA_GPU_BVH_HPP
+
+#include <cmath>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "thea_gpu.h"
+
+namespace thea_gpu {
+
+typedef std::int64_t intx;
+
+class Error : public std::runtime_error {
+ public:
+  Error(int code, const std::string &msg) : std::runtime_error(msg), code_(code) {}
+  int code() const { return code_; }
+
+ private:
+  int code_;
+};
+
+inline void check(int status) {
+  if (status != THG_OK) throw Error(status, thg_last_error());
+}
+
+// Tag types mirroring Thea's template arguments (Algorithms/MetricL2.hpp, Filter.hpp:68-73,
+// RayIntersectionTester.hpp). Only the L2 metric and universal compatibility exist on this path.
+struct MetricL2 {};
+struct UniversalCompatibility {};
+struct RayIntersectionTester {};
+
+// Result of a k-nearest query (Thea's NeighborPair, ProximityQueryStructureN.hpp:48-144).
+struct NeighborPair {
+  intx query_index = -1, target_index = -1;
+  double mon_approx_dist = -1;  // squared distance (MetricL2's monotone approximation)
+  bool isValid() const { return query_index >= 0 && target_index >= 0; }
+  intx getTargetIndex() const { return target_index; }
+  intx getQueryIndex() const { return query_index; }
+  double getMonotoneApproxDistance() const { return mon_approx_dist; }
+  template <typename MetricT = MetricL2> double getDistance() const { return mon_approx_dist < 0 ? -1 : std::sqrt(mon_approx_dist); }
+};
+
+// Result of a ray query (Thea's RayStructureIntersectionN, RayQueryStructureN.hpp:29-53).
+struct RayStructureIntersection3 {
+  float time = -1;
+  float normal[3] = {0, 0, 0};
+  intx element_index = -1;
+  bool isValid() const { return time >= 0; }
+  float getTime() const { return time; }
+  intx getElementIndex() const { return element_index; }
+};
+
+// Thea's Ray3 stand-in: origin and direction with operator[].
+struct Ray3 {
+  float o[3], d[3];
+  Ray3(float ox, float oy, float oz, float dx, float dy, float dz) : o{ox, oy, oz}, d{dx, dy, dz} {}
+  template <typename V> Ray3(V const &origin, V const &dir)
+      : o{(float)origin[0], (float)origin[1], (float)origin[2]}, d{(float)dir[0], (float)dir[1], (float)dir[2]} {}
+};
+
+class Context {
+ public:
+  explicit Context(int device = 0) { check(thg_ctx_create(device, &h_)); }
+  ~Context() { if (h_) thg_ctx_destroy(h_); }
+  Context(const Context &) = delete;
+  Context &operator=(const Context &) = delete;
+  thg_ctx handle() const { return h_; }
+
+ private:
+  thg_ctx h_ = nullptr;
+};
+
+// Triangle mesh (vertices float3, triangles int32 triplets), resident on the GPU.
+class Mesh {
+ public:
+  Mesh(Context &ctx, const float *verts, intx nv, const std::int32_t *tris, intx nt) {
+    check(thg_mesh_create(ctx.handle(), verts, nv, tris, nt, THG_MEM_HOST, &h_));
+  }
+  ~Mesh() { if (h_) thg_mesh_destroy(h_); }
+  Mesh(const Mesh &) = delete;
+  Mesh &operator=(const Mesh &) = delete;
+  thg_mesh handle() const { return h_; }
+  intx numTriangles() const { return thg_mesh_num_triangles(h_); }
+
+ private:
+  thg_mesh h_ = nullptr;
+};
+
+// Shared query methods of mesh and point BVHs (BvhN.hpp:1050-1146).
+class BvhBase {
+ public:
+  BvhBase(const BvhBase &) = delete;
+  BvhBase &operator=(const BvhBase &) = delete;
+  virtual ~BvhBase() { if (h_) thg_bvh_destroy(h_); }
+  thg_bvh handle() const { return h_; }
+
+  // Batch: idx[n], optional dist[n], closest points cp[3n].
+  void closestElements(intx n, const float *q, std::int32_t *idx, float *dist = nullptr, float *cp = nullptr,
+                       double dist_bound = -1) const {
+    check(thg_closest_point(h_, q, n, dist_bound, idx, cp, dist, nullptr, THG_MEM_HOST, nullptr));
+  }
+
+  // BvhN::closestElement (BvhN.hpp:1050-1063): index of the nearest element or -1.
+  template <typename MetricT = MetricL2, typename QueryT, typename CompatT = UniversalCompatibility,
+            typename VectorT = QueryT>
+  intx closestElement(QueryT const &query, double dist_bound = -1, CompatT = CompatT(), double *dist = nullptr,
+                      VectorT *closest_point = nullptr) const {
+    float q[3] = {(float)query[0], (float)query[1], (float)query[2]}, cp[3], d;
+    std::int32_t idx;
+    closestElements(1, q, &idx, &d, cp, dist_bound);
+    if (idx >= 0) {
+      if (dist) *dist = (double)d;
+      if (closest_point) { (*closest_point)[0] = cp[0]; (*closest_point)[1] = cp[1]; (*closest_point)[2] = cp[2]; }
+    }
+    return idx;
+  }
+
+  // BvhN::closestElementDistance-style helper: distance or -1.
+  template <typename MetricT = MetricL2, typename QueryT>
+  double closestElementDistance(QueryT const &query, double dist_bound = -1) const {
+    double d = -1;
+    return closestElement<MetricT>(query, dist_bound, UniversalCompatibility(), &d) >= 0 ? d : -1;
+  }
+
+  // BvhN::closestPair (BvhN.hpp:1082-1114) with a point query.
+  template <typename MetricT = MetricL2, typename QueryT, typename CompatT = UniversalCompatibility>
+  NeighborPair closestPair(QueryT const &query, double dist_bound = -1, CompatT = CompatT(),
+                           bool get_closest_points = false) const {
+    (void)get_closest_points;
+    float q[3] = {(float)query[0], (float)query[1], (float)query[2]}, d2;
+    std::int32_t idx;
+    check(thg_closest_point(h_, q, 1, dist_bound, &idx, nullptr, nullptr, &d2, THG_MEM_HOST, nullptr));
+    NeighborPair p;
+    if (idx >= 0) { p.query_index = 0; p.target_index = idx; p.mon_approx_dist = d2; }
+    return p;
+  }
+
+  // Batch k-NN: idx[n*k], dist[n*k] (rows sorted by (distance, index); -1 padded).
+  void kClosestElements(intx n, const float *q, int k, std::int32_t *idx, float *dist = nullptr, double dist_bound = -1,
+                        bool exclude_self = false) const {
+    check(thg_k_closest(h_, q, n, k, dist_bound, exclude_self ? 1 : 0, idx, dist, nullptr, THG_MEM_HOST, nullptr));
+  }
+
+  // BvhN::kClosestPairs (BvhN.hpp:1139-1146) into any container with clear()/push_back() of
+  // NeighborPair and a capacity given by `k` (Thea passes a BoundedSortedArray of capacity k).
+  template <typename MetricT = MetricL2, typename QueryT, typename PairSetT, typename CompatT = UniversalCompatibility>
+  intx kClosestPairs(QueryT const &query, PairSetT &k_closest_pairs, int k, double dist_bound = -1,
+                     CompatT = CompatT()) const {
+    float q[3] = {(float)query[0], (float)query[1], (float)query[2]};
+    std::vector<std::int32_t> idx(k);
+    std::vector<float> d2(k);
+    check(thg_k_closest(h_, q, 1, k, dist_bound, 0, idx.data(), nullptr, d2.data(), THG_MEM_HOST, nullptr));
+    intx found = 0;
+    for (int j = 0; j < k; ++j) {
+      if (idx[j] < 0) break;
+      NeighborPair p;
+      p.query_index = 0;
+      p.target_index = idx[j];
+      p.mon_approx_dist = d2[j];
+      k_closest_pairs.push_back(p);
+      ++found;
+    }
+    return found;
+  }
+
+ protected:
+  BvhBase() = default;
+  thg_bvh h_ = nullptr;
+};
+
+// BVH over a triangle mesh (Thea's MeshBvh<Mesh>, Algorithms/MeshBvh.hpp:31-165).
+class MeshBvh : public BvhBase {
+ public:
+  explicit MeshBvh(Mesh &mesh, int max_elems_in_leaf = 4) : mesh_(&mesh) {
+    thg_build_opts o = {};
+    o.leaf_size = max_elems_in_leaf;
+    check(thg_bvh_build_mesh(mesh.handle(), &o, nullptr, &h_));
+  }
+
+  // Batch first hits: idx[n], optional t[n], normal[3n].
+  void rayIntersections(intx n, const float *origins, const float *dirs, std::int32_t *idx, float *t = nullptr,
+                        float *normal = nullptr, float max_time = -1) const {
+    check(thg_ray_first_hit(h_, origins, dirs, n, max_time, idx, t, normal, THG_MEM_HOST, nullptr));
+  }
+
+  // BvhN::rayStructureIntersection (BvhN.hpp:1291-1316).
+  template <typename TesterT = RayIntersectionTester>
+  RayStructureIntersection3 rayStructureIntersection(Ray3 const &ray, float max_time = -1) const {
+    RayStructureIntersection3 r;
+    std::int32_t idx;
+    float t, nrm[3];
+    rayIntersections(1, ray.o, ray.d, &idx, &t, nrm, max_time);
+    if (idx >= 0) { r.time = t; r.element_index = idx; r.normal[0] = nrm[0]; r.normal[1] = nrm[1]; r.normal[2] = nrm[2]; }
+    return r;
+  }
+
+  // BvhN::rayIntersectionTime (BvhN.hpp:1271-1289): hit time or -1.
+  template <typename TesterT = RayIntersectionTester>
+  float rayIntersectionTime(Ray3 const &ray, float max_time = -1) const {
+    return rayStructureIntersection<TesterT>(ray, max_time).time;
+  }
+
+  // BvhN::rayIntersects (BvhN.hpp:1266-1269).
+  template <typename TesterT = RayIntersectionTester> bool rayIntersects(Ray3 const &ray, float max_time = -1) const {
+    std::uint8_t hit = 0;
+    check(thg_ray_any_hit(h_, ray.o, ray.d, 1, max_time, &hit, THG_MEM_HOST, nullptr));
+    return hit != 0;
+  }
+
+  Mesh &mesh() const { return *mesh_; }
+
+ private:
+  Mesh *mesh_;
+};
+
+// BVH over points (Thea's BvhN<Vector3, 3>).
+class PointBvh : public BvhBase {
+ public:
+  PointBvh(Context &ctx, const float *xyz, intx n, int max_elems_in_leaf = 4) {
+    thg_build_opts o = {};
+    o.leaf_size = max_elems_in_leaf;
+    check(thg_bvh_build_points(ctx.handle(), xyz, n, THG_MEM_HOST, &o, nullptr, &h_));
+  }
+};
+
+// Thea's MeshSampler (Algorithms/MeshSampler.hpp:33-336), area-weighted EXACT mode.
+class MeshSampler {
+ public:
+  enum class CountMode { EXACT };
+  explicit MeshSampler(Mesh &mesh, std::uint64_t seed = 0, std::uint64_t stream = 0)
+      : mesh_(&mesh), seed_(seed), stream_(stream) {}
+
+  // sampleEvenlyByArea (MeshSampler.hpp:100-105): positions (and optional face normals and
+  // triangle indices) of n samples; returns the number of samples (0 for an empty mesh). Each
+  // call advances the RNG stream so successive calls draw fresh samples.
+  template <typename VectorT>
+  intx sampleEvenlyByArea(intx n, std::vector<VectorT> &positions, std::vector<VectorT> *face_normals = nullptr,
+                          std::vector<intx> *triangles = nullptr, CountMode = CountMode::EXACT, bool verbose = false) {
+    (void)verbose;
+    std::vector<float> pos(3 * (size_t)(n > 0 ? n : 0)), nrm(face_normals ? pos.size() : 0);
+    std::vector<std::int32_t> tri(triangles ? (size_t)(n > 0 ? n : 0) : 0);
+    std::int64_t got = 0;
+    check(thg_sample_area(mesh_->handle(), n, seed_, stream_++, pos.data(), face_normals ? nrm.data() : nullptr,
+                          triangles ? tri.data() : nullptr, &got, THG_MEM_HOST, nullptr));
+    positions.resize((size_t)got);
+    if (face_normals) face_normals->resize((size_t)got);
+    if (triangles) triangles->resize((size_t)got);
+    for (std::int64_t i = 0; i < got; ++i) {
+      for (int c = 0; c < 3; ++c) {
+        positions[i][c] = pos[3 * i + c];
+        if (face_normals) (*face_normals)[i][c] = nrm[3 * i + c];
+      }
+      if (triangles) (*triangles)[i] = tri[i];
+    }
+    return got;
+  }
+
+  template <typename VectorT>
+  intx sampleUniformRandom(intx n, std::vector<VectorT> &positions, std::vector<VectorT> *face_normals = nullptr,
+                           std::vector<intx> *triangles = nullptr) {
+    return sampleEvenlyByArea(n, positions, face_normals, triangles);
+  }
+
+ private:
+  Mesh *mesh_;
+  std::uint64_t seed_, stream_;
+};
+
+}  // namespace thea_gpu
+
+#endif  // THEA_GPU_BVH_HPP

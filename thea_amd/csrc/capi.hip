@@ -296,23 +296,23 @@ int thg_closest_point(thg_bvh bvh, const float *q, int64_t n, double dist_bound,
   });
 }
 
-int thg_closest_stats(thg_bvh bvh, const float *q, int64_t n, void *stream, int64_t out[6]) {
+int thg_closest_stats(thg_bvh bvh, const float *q, int64_t n, void *stream, int64_t out[8]) {
   return guarded([&] {
     require(bvh && out, "bvh/out is NULL");
     require(n >= 0 && (n == 0 || q), "bad query array");
     Bvh *b = reinterpret_cast<Bvh *>(bvh);
     set_device(b->ctx);
     hipStream_t s = as_stream(stream);
-    AsyncBuf st(6 * sizeof(unsigned long long), s);
-    THG_HIP(hipMemsetAsync(st.p, 0, 6 * sizeof(unsigned long long), s));
+    AsyncBuf st(8 * sizeof(unsigned long long), s);
+    THG_HIP(hipMemsetAsync(st.p, 0, 8 * sizeof(unsigned long long), s));
     {
       QueryScratch ws;
       closest_wave_stats(*b, q, n, st.as<unsigned long long>(), ws, s);
     }
-    unsigned long long h[6];
+    unsigned long long h[8];
     THG_HIP(hipMemcpyAsync(h, st.p, sizeof(h), hipMemcpyDeviceToHost, s));
     THG_HIP(hipStreamSynchronize(s));
-    for (int i = 0; i < 6; ++i) out[i] = (int64_t)h[i];
+    for (int i = 0; i < 8; ++i) out[i] = (int64_t)h[i];
   });
 }
 

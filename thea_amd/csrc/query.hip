@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(W_BLOCK) k_closest_wave(const BNode *__restric
                                                           float *__restrict__ out_d2,
                                                           unsigned long long *__restrict__ stats = nullptr) {
   // STATS: per wave, count nodes loaded, box tests, leaf elements loaded and elements evaluated.
-  uint32_t st_nodes = 0, st_boxes = 0, st_loaded = 0, st_tested = 0;
+  uint32_t st_nodes = 0, st_boxes = 0, st_loaded = 0, st_tested = 0, st_plane = 0, st_near = 0;
   constexpr int QW = WAVE / G;  // queries per wave
   const int lane = lane_id();
   const int sub = lane % G;
@@ -267,7 +267,13 @@ __global__ void __launch_bounds__(W_BLOCK) k_closest_wave(const BNode *__restric
               e2 = make_float4(unif(e2.x), unif(e2.y), unif(e2.z), unif(e2.w));
               id = uni(id);
             }
+            if (STATS) {
+              // how many exact tests an ideal filter would keep: some lane within 2^-8 of its best
+              float sd = dot3(p, f3(p0.w, p1.w, p2.w)) - pl.x;
+              if (__ballot(valid && sd * sd <= best)) st_plane++;
+            }
             d = tri_closest(p0, p1, p2, pl, e0, e1, e2, p, cp);
+            if (STATS && __ballot(valid && d <= best * (1.0f + 1.0f / 256.0f))) st_near++;
           }
           if (valid && id != self && better(d, id, own, bidx)) {
             own = d;
@@ -313,6 +319,8 @@ __global__ void __launch_bounds__(W_BLOCK) k_closest_wave(const BNode *__restric
     atomicAdd(&stats[3], (unsigned long long)st_boxes);
     atomicAdd(&stats[4], (unsigned long long)st_loaded);
     atomicAdd(&stats[5], (unsigned long long)st_tested);
+    atomicAdd(&stats[6], (unsigned long long)st_plane);
+    atomicAdd(&stats[7], (unsigned long long)st_near);
   }
   // lexicographic (d, id) minimum over the group's lanes
 #pragma unroll
