@@ -23,21 +23,38 @@ struct __align__(64) BNode {
 static_assert(sizeof(BNode) == 64, "BNode must be 64 bytes");
 
 // Triangle record. v0/v1/v2 are the input vertices; n, pd are Thea's plane (Hyperplane3.hpp:34-41);
-// the in-plane 2-D area (`ar`) and its reciprocal (`inv`) are the constants of
-// isPointInsideTriangle (Triangle3.hpp:1436-1447); dir_k/len_k are the perimeter edge directions
-// and lengths of closestPointOnTrianglePerimeter (Triangle3.hpp:1401-1409). `meta` packs the
-// primary axis (bits 0-1) and `id` is the caller's triangle index.
+// `inv` is the reciprocal of the in-plane 2-D area of isPointInsideTriangle (Triangle3.hpp:1436-1447);
+// dir_k/len_k are the perimeter edge directions and lengths of closestPointOnTrianglePerimeter
+// (Triangle3.hpp:1401-1409). `meta` packs the primary axis (bits 0-1) and a flag for a zero 2-D area
+// (bit 2); `id` is the caller's triangle index. (cx, cy, cz, r) is a bounding sphere of the
+// triangle (radius rounded up), used only to skip elements that cannot win.
 struct __align__(128) TriRec {
   float4 p0;   // v0.xyz, n.x
   float4 p1;   // v1.xyz, n.y
   float4 p2;   // v2.xyz, n.z
-  float4 pl;   // pd, inv, ar, meta (int bits)
+  float4 pl;   // pd, inv, meta (int bits), id (int bits)
   float4 e0;   // dir0.xyz, len0   (edge v0 -> v1)
   float4 e1;   // dir1.xyz, len1   (edge v1 -> v2)
   float4 e2;   // dir2.xyz, len2   (edge v2 -> v0)
-  int4 ex;     // id, 0, 0, 0
+  float4 sb;   // bounding sphere centre xyz, radius
 };
 static_assert(sizeof(TriRec) == 128, "TriRec must be 128 bytes");
+
+__host__ __device__ __forceinline__ int trirec_axis(const float4 &pl) {
+  union { float f; int i; } m;
+  m.f = pl.z;
+  return m.i & 3;
+}
+__host__ __device__ __forceinline__ bool trirec_flat(const float4 &pl) {
+  union { float f; int i; } m;
+  m.f = pl.z;
+  return (m.i & 4) != 0;
+}
+__host__ __device__ __forceinline__ int trirec_id(const float4 &pl) {
+  union { float f; int i; } m;
+  m.f = pl.w;
+  return m.i;
+}
 
 struct __align__(16) PointRec {
   float4 p;    // xyz, id (int bits)

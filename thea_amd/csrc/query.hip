@@ -19,6 +19,8 @@ namespace thg {
 
 constexpr int Q_BLOCK = 64;
 constexpr float SLACK_REL = 1.0f / 65536.0f;
+// Relative margin of the element reject tests: a bound is compared with sqrt(best) * REJ_REL.
+constexpr float REJ_REL = 1.0f + 1.0f / 4096.0f;
 
 __device__ __forceinline__ float box_lb(float3 qlo, float3 qhi, float lx, float ly, float lz, float hx, float hy,
                                         float hz) {
@@ -103,7 +105,7 @@ __global__ void __launch_bounds__(Q_BLOCK) k_closest(const BNode *__restrict__ n
             d = sqn3(sub3(cp, p));
           } else {
             const TriRec *t = tr + k;
-            id = t->ex.x;
+            id = trirec_id(t->pl);
             d = tri_closest(t->p0, t->p1, t->p2, t->pl, t->e0, t->e1, t->e2, p, cp);
           }
           if (id != self && better(d, id, best, bidx)) {
@@ -198,6 +200,7 @@ __global__ void __launch_bounds__(W_BLOCK) k_closest_wave(const BNode *__restric
   float3 qlo = f3(p.x - sl, p.y - sl, p.z - sl), qhi = f3(p.x + sl, p.y + sl, p.z + sl);
   float best = active ? best_init : -1.0f;  // group best (pruning); inactive lanes admit nothing
   float own = best;                          // this lane's best over the elements it tested
+  float sbest = active ? sqrtf(best_init) : 0.0f;  // sqrt(best), for the G == 1 reject tests
   int32_t bidx = 0x7fffffff;
   float3 bcp = f3(0, 0, 0);
   const int32_t self = exclude_self ? (int32_t)qi : -1;
@@ -229,63 +232,83 @@ __global__ void __launch_bounds__(W_BLOCK) k_closest_wave(const BNode *__restric
       const int32_t cnt = c == 0 ? n0 : n1;
       if (r < 0) {
         const int32_t f = ~r;
-        for (int32_t k0 = f; k0 < f + cnt; k0 += G) {
-          const int32_t k = k0 + sub;
-          bool valid = k < f + cnt;
-          float d = INFINITY;
-          float3 cp;
-          int32_t id = 0x7fffffff;
-          if (POINTS) {
-            float4 e = pr[valid ? k : f].p;
-            if (G == 1) e = make_float4(unif(e.x), unif(e.y), unif(e.z), unif(e.w));
-            id = __float_as_int(e.w);
-            cp = f3(e.x, e.y, e.z);
-            d = sqn3(sub3(cp, p));
-          } else {
-            const TriRec *tk = tr + (valid ? k : f);
-            float4 p0 = tk->p0, p1 = tk->p1, p2 = tk->p2;
-            if (G == 1) {
-              p0 = make_float4(unif(p0.x), unif(p0.y), unif(p0.z), unif(p0.w));
-              p1 = make_float4(unif(p1.x), unif(p1.y), unif(p1.z), unif(p1.w));
-              p2 = make_float4(unif(p2.x), unif(p2.y), unif(p2.z), unif(p2.w));
-            }
+        if (G == 1 && !POINTS) {
+          // Wave-uniform records (scalar loads). Cheap conservative rejects first, exact Thea
+          // arithmetic only if some lane can still be improved (or tied) by this triangle.
+          for (int32_t k = f; k < f + cnt; ++k) {
+            const TriRec *tk = tr + k;
             if (STATS) st_loaded++;
+            const float4 sbv = make_float4(unif(tk->sb.x), unif(tk->sb.y), unif(tk->sb.z), unif(tk->sb.w));
             if (PRE) {
-              float tlb = box_lb(qlo, qhi, fminf(p0.x, fminf(p1.x, p2.x)), fminf(p0.y, fminf(p1.y, p2.y)),
-                                 fminf(p0.z, fminf(p1.z, p2.z)), fmaxf(p0.x, fmaxf(p1.x, p2.x)),
-                                 fmaxf(p0.y, fmaxf(p1.y, p2.y)), fmaxf(p0.z, fmaxf(p1.z, p2.z)));
-              valid = valid && tlb <= best;
-              if (__ballot(valid) == 0) continue;
+              const float dx = p.x - sbv.x, dy = p.y - sbv.y, dz = p.z - sbv.z;
+              const float dc2 = dx * dx + dy * dy + dz * dz;
+              const float T = sbest * REJ_REL + sbv.w + 2.0f * sl;
+              if (__ballot(active && !(dc2 > T * T)) == 0) continue;
             }
-            if (STATS) st_tested++;
-            float4 pl = tk->pl, e0 = tk->e0, e1 = tk->e1, e2 = tk->e2;
-            id = tk->ex.x;
-            if (G == 1) {
-              pl = make_float4(unif(pl.x), unif(pl.y), unif(pl.z), unif(pl.w));
-              e0 = make_float4(unif(e0.x), unif(e0.y), unif(e0.z), unif(e0.w));
-              e1 = make_float4(unif(e1.x), unif(e1.y), unif(e1.z), unif(e1.w));
-              e2 = make_float4(unif(e2.x), unif(e2.y), unif(e2.z), unif(e2.w));
-              id = uni(id);
-            }
+            const float4 p0 = make_float4(unif(tk->p0.x), unif(tk->p0.y), unif(tk->p0.z), unif(tk->p0.w));
+            const float4 p1 = make_float4(unif(tk->p1.x), unif(tk->p1.y), unif(tk->p1.z), unif(tk->p1.w));
+            const float4 p2 = make_float4(unif(tk->p2.x), unif(tk->p2.y), unif(tk->p2.z), unif(tk->p2.w));
+            const float4 pl = make_float4(unif(tk->pl.x), unif(tk->pl.y), unif(tk->pl.z), unif(tk->pl.w));
+            // HyperplaneN.hpp:145-160 signedDistance, shared with the exact test below
+            const float sd = dot3(p, f3(p0.w, p1.w, p2.w)) - pl.x;
+            if (PRE && __ballot(active && !(fabsf(sd) - 2.0f * sl > sbest * REJ_REL)) == 0) continue;
             if (STATS) {
-              // how many exact tests an ideal filter would keep: some lane within 2^-8 of its best
-              float sd = dot3(p, f3(p0.w, p1.w, p2.w)) - pl.x;
-              if (__ballot(valid && sd * sd <= best)) st_plane++;
+              st_tested++;
+              if (__ballot(active && sd * sd <= best)) st_plane++;
             }
-            d = tri_closest(p0, p1, p2, pl, e0, e1, e2, p, cp);
-            if (STATS && __ballot(valid && d <= best * (1.0f + 1.0f / 256.0f))) st_near++;
+            const float4 e0 = make_float4(unif(tk->e0.x), unif(tk->e0.y), unif(tk->e0.z), unif(tk->e0.w));
+            const float4 e1 = make_float4(unif(tk->e1.x), unif(tk->e1.y), unif(tk->e1.z), unif(tk->e1.w));
+            const float4 e2 = make_float4(unif(tk->e2.x), unif(tk->e2.y), unif(tk->e2.z), unif(tk->e2.w));
+            const int32_t id = trirec_id(pl);
+            const int ax = trirec_axis(pl);
+            float3 cp;
+            float d;
+            if (ax == 0) d = tri_closest_ax<0>(p0, p1, p2, pl, e0, e1, e2, p, sd, cp);
+            else if (ax == 1) d = tri_closest_ax<1>(p0, p1, p2, pl, e0, e1, e2, p, sd, cp);
+            else d = tri_closest_ax<2>(p0, p1, p2, pl, e0, e1, e2, p, sd, cp);
+            if (STATS && __ballot(active && d <= best * (1.0f + 1.0f / 256.0f))) st_near++;
+            if (active && id != self && better(d, id, own, bidx)) {
+              own = d;
+              bidx = id;
+              bcp = cp;
+              best = d;
+              sbest = sqrtf(d);
+            }
           }
-          if (valid && id != self && better(d, id, own, bidx)) {
-            own = d;
-            bidx = id;
-            bcp = cp;
+        } else {
+          for (int32_t k0 = f; k0 < f + cnt; k0 += G) {
+            const int32_t k = k0 + sub;
+            bool valid = active && k < f + cnt;
+            float d = INFINITY;
+            float3 cp;
+            int32_t id = 0x7fffffff;
+            if (POINTS) {
+              float4 e = pr[valid ? k : f].p;
+              if (G == 1) e = make_float4(unif(e.x), unif(e.y), unif(e.z), unif(e.w));
+              id = __float_as_int(e.w);
+              cp = f3(e.x, e.y, e.z);
+              d = sqn3(sub3(cp, p));
+            } else {
+              const TriRec *tk = tr + (valid ? k : f);
+              if (STATS) st_loaded++;
+              id = trirec_id(tk->pl);
+              d = tri_closest(tk->p0, tk->p1, tk->p2, tk->pl, tk->e0, tk->e1, tk->e2, p, cp);
+              if (STATS) st_tested++;
+            }
+            if (valid && id != self && better(d, id, own, bidx)) {
+              own = d;
+              bidx = id;
+              bcp = cp;
+            }
           }
         }
         // group best for pruning
-        float gb = own;
+        if (G > 1 || POINTS) {
+          float gb = own;
 #pragma unroll
-        for (int off = 1; off < G; off <<= 1) gb = fminf(gb, __shfl_xor(gb, off, WAVE));
-        best = active ? fminf(best, gb) : best;
+          for (int off = 1; off < G; off <<= 1) gb = fminf(gb, __shfl_xor(gb, off, WAVE));
+          best = active ? fminf(best, gb) : best;
+        }
       } else if (next < 0) {
         next = r;
       } else {
@@ -425,7 +448,7 @@ __global__ void __launch_bounds__(Q_BLOCK) k_knn(const BNode *__restrict__ nodes
             d = sqn3(sub3(f3(e.x, e.y, e.z), p));
           } else {
             const TriRec *t = tr + kk;
-            id = t->ex.x;
+            id = trirec_id(t->pl);
             float3 cp;
             d = tri_closest(t->p0, t->p1, t->p2, t->pl, t->e0, t->e1, t->e2, p, cp);
           }
@@ -456,6 +479,138 @@ __global__ void __launch_bounds__(Q_BLOCK) k_knn(const BNode *__restrict__ nodes
     out_idx[qi * k + j] = hit ? ki[j] : -1;
     if (out_dist) out_dist[qi * k + j] = hit ? sqrtf(kd[j]) : -1.0f;
     if (out_d2) out_d2[qi * k + j] = hit ? kd[j] : -1.0f;
+  }
+}
+
+//--------------------------------------------------------------------------------------------------
+// k nearest, wave-cooperative: the traversal of k_closest_wave with a per-lane sorted list of K
+// (d^2, id) pairs in registers; a subtree or element is skipped when every lane's k-th best beats
+// its lower bound. Element records are wave-uniform (scalar loads).
+//--------------------------------------------------------------------------------------------------
+template <bool POINTS, int K>
+__global__ void __launch_bounds__(W_BLOCK) k_knn_wave(const BNode *__restrict__ nodes, const void *__restrict__ elems,
+                                                      const float *__restrict__ q, const uint32_t *__restrict__ order,
+                                                      int64_t n, uint32_t nb_real, int k, float best_init, float scale,
+                                                      int exclude_self, int32_t *__restrict__ out_idx,
+                                                      float *__restrict__ out_dist, float *__restrict__ out_d2) {
+  const int lane = lane_id();
+  const uint32_t lb_blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t t = ((int64_t)lb_blk * W_WAVES + (threadIdx.x / WAVE)) * WAVE + lane;
+  const bool inrange = lb_blk < nb_real && t < n;
+  bool active = inrange;
+  int64_t qi = 0;
+  float3 p = f3(0, 0, 0);
+  if (active) {
+    qi = order ? (int64_t)order[t] : t;
+    p = load_q(q, qi);
+    active = !(isnan(p.x) || isnan(p.y) || isnan(p.z));
+  }
+  const uint64_t amask = __ballot(active);
+  const int nact = __popcll(amask);
+  float m = fmaxf(scale, fmaxf(fabsf(p.x), fmaxf(fabsf(p.y), fabsf(p.z))));
+  float sl = m * SLACK_REL;
+  float3 qlo = f3(p.x - sl, p.y - sl, p.z - sl), qhi = f3(p.x + sl, p.y + sl, p.z + sl);
+  float kd[K];
+  int32_t ki[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) { kd[j] = active ? best_init : -1.0f; ki[j] = 0x7fffffff; }
+  // k-th best (slot k - 1): the pruning bound and admission threshold
+  float kb = active ? best_init : -1.0f;
+  int32_t kbi = 0x7fffffff;
+  const int32_t self = exclude_self ? (int32_t)qi : -1;
+  const TriRec *__restrict__ tr = (const TriRec *)elems;
+  const PointRec *__restrict__ pr = (const PointRec *)elems;
+  int32_t s_ref = 0;
+  float s_lx = 0, s_ly = 0, s_lz = 0, s_hx = 0, s_hy = 0, s_hz = 0;
+  int32_t sp = 0;
+  int32_t node = 0;
+  while (amask != 0) {
+    node = uni(node);
+    const BNode nd = nodes[node];
+    const float a0 = unif(nd.a.x), a1 = unif(nd.a.y), a2 = unif(nd.a.z), a3 = unif(nd.a.w);
+    const float b0 = unif(nd.b.x), b1 = unif(nd.b.y), b2 = unif(nd.b.z), b3 = unif(nd.b.w);
+    const float c0 = unif(nd.c.x), c1 = unif(nd.c.y), c2 = unif(nd.c.z), c3 = unif(nd.c.w);
+    const int32_t r0 = uni(nd.d.x), r1 = uni(nd.d.y), n0 = uni(nd.d.z), n1 = uni(nd.d.w);
+    float lb0 = box_lb(qlo, qhi, a0, a1, a2, a3, b0, b1);
+    float lb1 = box_lb(qlo, qhi, b2, b3, c0, c1, c2, c3);
+    uint64_t m0 = __ballot(lb0 <= kb), m1 = __ballot(lb1 <= kb);
+    bool first1 = 2 * __popcll(__ballot(lb1 < lb0) & amask) > nact;
+    int32_t next = -1;
+#pragma unroll
+    for (int cc = 0; cc < 2; ++cc) {
+      const int c = first1 ? 1 - cc : cc;
+      const uint64_t mc = c == 0 ? m0 : m1;
+      if (mc == 0) continue;
+      const int32_t r = c == 0 ? r0 : r1;
+      const int32_t cnt = c == 0 ? n0 : n1;
+      if (r < 0) {
+        const int32_t f = ~r;
+        for (int32_t e = f; e < f + cnt; ++e) {
+          float d;
+          int32_t id;
+          if (POINTS) {
+            const float4 pe = pr[e].p;
+            const float3 pt = f3(unif(pe.x), unif(pe.y), unif(pe.z));
+            id = uni(__float_as_int(pe.w));
+            d = sqn3(sub3(pt, p));
+          } else {
+            const TriRec *tk = tr + e;
+            float3 cp;
+            id = uni(trirec_id(tk->pl));
+            d = tri_closest(make_float4(unif(tk->p0.x), unif(tk->p0.y), unif(tk->p0.z), unif(tk->p0.w)),
+                            make_float4(unif(tk->p1.x), unif(tk->p1.y), unif(tk->p1.z), unif(tk->p1.w)),
+                            make_float4(unif(tk->p2.x), unif(tk->p2.y), unif(tk->p2.z), unif(tk->p2.w)),
+                            make_float4(unif(tk->pl.x), unif(tk->pl.y), unif(tk->pl.z), unif(tk->pl.w)),
+                            make_float4(unif(tk->e0.x), unif(tk->e0.y), unif(tk->e0.z), unif(tk->e0.w)),
+                            make_float4(unif(tk->e1.x), unif(tk->e1.y), unif(tk->e1.z), unif(tk->e1.w)),
+                            make_float4(unif(tk->e2.x), unif(tk->e2.y), unif(tk->e2.z), unif(tk->e2.w)), p, cp);
+          }
+          const bool take = active && id != self && better(d, id, kb, kbi);
+          if (__ballot(take)) {
+            if (take) {
+              knn_insert<K>(kd, ki, d, id);
+              kb = kd[K - 1];
+              kbi = ki[K - 1];
+#pragma unroll
+              for (int j = 0; j < K - 1; ++j)
+                if (j == k - 1) { kb = kd[j]; kbi = ki[j]; }
+            }
+          }
+        }
+      } else if (next < 0) {
+        next = r;
+      } else {
+        if (lane == sp) {
+          s_ref = r;
+          if (c == 0) { s_lx = a0; s_ly = a1; s_lz = a2; s_hx = a3; s_hy = b0; s_hz = b1; }
+          else { s_lx = b2; s_ly = b3; s_lz = c0; s_hx = c1; s_hy = c2; s_hz = c3; }
+        }
+        ++sp;
+      }
+    }
+    if (next >= 0) { node = next; continue; }
+    bool found = false;
+    while (sp > 0) {
+      --sp;
+      const float lx = lane_readf(s_lx, sp), ly = lane_readf(s_ly, sp), lz = lane_readf(s_lz, sp);
+      const float hx = lane_readf(s_hx, sp), hy = lane_readf(s_hy, sp), hz = lane_readf(s_hz, sp);
+      if (__ballot(box_lb(qlo, qhi, lx, ly, lz, hx, hy, hz) <= kb)) {
+        node = lane_read(s_ref, sp);
+        found = true;
+        break;
+      }
+    }
+    if (!found) break;
+  }
+  if (!inrange) return;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    if (j < k) {
+      bool hit = active && ki[j] != 0x7fffffff;
+      out_idx[qi * k + j] = hit ? ki[j] : -1;
+      if (out_dist) out_dist[qi * k + j] = hit ? sqrtf(kd[j]) : -1.0f;
+      if (out_d2) out_d2[qi * k + j] = hit ? kd[j] : -1.0f;
+    }
   }
 }
 
@@ -525,7 +680,7 @@ __global__ void __launch_bounds__(Q_BLOCK) k_ray(const BNode *__restrict__ nodes
           float4 a = t->p0, b = t->p1, cc = t->p2;
           float th = tri_ray(f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), f3(cc.x, cc.y, cc.z), o, d, max_time);
           if (th >= 0) {
-            int32_t id = t->ex.x;
+            int32_t id = trirec_id(t->pl);
             if (better(th, id, best, bidx)) {
               best = th;
               bidx = id;
@@ -749,6 +904,23 @@ void launch_knn(const Bvh &b, const float *q, int64_t n, int k, double dist_boun
   uint32_t nb = ceil_div(n, Q_BLOCK);
   uint32_t grid = round_up8(nb);
   float best = bound_to_best(dist_bound);
+  if (use_wave_traversal(b)) {
+    uint32_t wnb = ceil_div(n, W_BLOCK);
+    uint32_t wgrid = round_up8(wnb);
+    KernelTimer kt(b.ctx, s, "knn");
+#define THG_KW(P, K)                                                                                                 \
+  hipLaunchKernelGGL((k_knn_wave<P, K>), dim3(wgrid), dim3(W_BLOCK), 0, s, b.nodes.as<BNode>(), b.elems.p, q, ws.order, \
+                     n, wnb, k, best, b.scale, exclude_self, idx, dist, d2)
+    if (b.points) {
+      if (k <= 4) THG_KW(true, 4); else if (k <= 8) THG_KW(true, 8); else if (k <= 16) THG_KW(true, 16); else THG_KW(true, 32);
+    } else {
+      if (k <= 4) THG_KW(false, 4); else if (k <= 8) THG_KW(false, 8); else if (k <= 16) THG_KW(false, 16); else THG_KW(false, 32);
+    }
+#undef THG_KW
+    launch_check("k_knn_wave");
+    kt.stop();
+    return;
+  }
   int st = pick_stack(b.max_depth);
   KernelTimer kt(b.ctx, s, "knn");
   if (k <= 4) launch_knn_k<4>(b, st, grid, nb, q, n, k, best, exclude_self, idx, dist, d2, ws.order, s);

@@ -87,13 +87,27 @@ THG_HD TriRec make_trirec(float3 v0, float3 v1, float3 v2, int32_t id) {
   r.p0 = make_float4(v0.x, v0.y, v0.z, n.x);
   r.p1 = make_float4(v1.x, v1.y, v1.z, n.y);
   r.p2 = make_float4(v2.x, v2.y, v2.z, n.z);
-  union { int i; float f; } m;
-  m.i = axis;
-  r.pl = make_float4(pd, inv, ar, m.f);
+  union { int i; float f; } m, idb;
+  m.i = axis | (ar == 0 ? 4 : 0);
+  idb.i = id;
+  r.pl = make_float4(pd, inv, m.f, idb.f);
   r.e0 = make_float4(d0.x, d0.y, d0.z, l0);
   r.e1 = make_float4(d1.x, d1.y, d1.z, l1);
   r.e2 = make_float4(d2.x, d2.y, d2.z, l2);
-  r.ex = make_int4(id, 0, 0, 0);
+  // Bounding sphere: box centre, radius = largest vertex distance (double), rounded up.
+  double cx = 0.5 * ((double)fminf(v0.x, fminf(v1.x, v2.x)) + (double)fmaxf(v0.x, fmaxf(v1.x, v2.x)));
+  double cy = 0.5 * ((double)fminf(v0.y, fminf(v1.y, v2.y)) + (double)fmaxf(v0.y, fmaxf(v1.y, v2.y)));
+  double cz = 0.5 * ((double)fminf(v0.z, fminf(v1.z, v2.z)) + (double)fmaxf(v0.z, fmaxf(v1.z, v2.z)));
+  float fcx = (float)cx, fcy = (float)cy, fcz = (float)cz;
+  double rr = 0;
+  const float3 vv[3] = {v0, v1, v2};
+  for (int k = 0; k < 3; ++k) {
+    double dx = (double)vv[k].x - fcx, dy = (double)vv[k].y - fcy, dz = (double)vv[k].z - fcz;
+    double dd = dx * dx + dy * dy + dz * dz;
+    rr = dd > rr ? dd : rr;
+  }
+  float rad = (float)(sqrt(rr) * (1.0 + 1e-6));
+  r.sb = make_float4(fcx, fcy, fcz, rad);
   return r;
 }
 
@@ -115,13 +129,11 @@ THG_HD float tri_closest(const float4 p0, const float4 p1, const float4 p2, cons
   // HyperplaneN.hpp:145-160: signedDistance = p.n - d; closestPoint = p - sd n.
   float sd = dot3(p, n) - pl.x;
   float3 proj = sub3(p, scl3(sd, n));
-  union { float f; int i; } m;
-  m.f = pl.w;
   int i, j;
-  plane_axes(m.i & 3, i, j);
+  plane_axes(trirec_axis(pl), i, j);
   bool inside;
   // isPointInsideTriangle (Triangle3.hpp:1414-1462)
-  if (pl.z == 0) {
+  if (trirec_flat(pl)) {
     inside = (v0.x == proj.x) && (v0.y == proj.y) && (v0.z == proj.z);
   } else {
     float pi = comp(proj, i), pj = comp(proj, j);
@@ -150,6 +162,48 @@ THG_HD float tri_closest(const float4 p0, const float4 p1, const float4 p2, cons
     else cp = (d1 < d2) ? r1 : r2;
   }
   return sqn3(sub3(cp, p));
+}
+
+// The same arithmetic as tri_closest with the primary axis known at compile time (no per-lane
+// component selects), taking the plane-distance term sd = p.n - pd precomputed by the caller, and
+// returning the squared distance the perimeter search already computed for the chosen edge
+// (Thea recomputes (cp - p).squaredNorm() with the same operations, so the bits are the same).
+template <int AX>
+__device__ __forceinline__ float tri_closest_ax(const float4 &p0, const float4 &p1, const float4 &p2, const float4 &pl,
+                                                const float4 &e0, const float4 &e1, const float4 &e2, float3 p,
+                                                float sd, float3 &cp) {
+  constexpr int I = AX == 1 ? 2 : (AX == 2 ? 0 : 1);
+  constexpr int J = AX == 1 ? 0 : (AX == 2 ? 1 : 2);
+  const float3 v0 = f3(p0.x, p0.y, p0.z), v1 = f3(p1.x, p1.y, p1.z), v2 = f3(p2.x, p2.y, p2.z);
+  const float3 proj = f3(p.x - sd * p0.w, p.y - sd * p1.w, p.z - sd * p2.w);
+  bool inside;
+  if (trirec_flat(pl)) {
+    inside = (v0.x == proj.x) && (v0.y == proj.y) && (v0.z == proj.z);
+  } else {
+    const float pi = comp(proj, I), pj = comp(proj, J);
+    const float v0i = comp(v0, I), v0j = comp(v0, J), v1i = comp(v1, I), v1j = comp(v1, J);
+    const float v2i = comp(v2, I), v2j = comp(v2, J);
+    const float b0 = ((v1i - pi) * (v2j - pj) - (v2i - pi) * (v1j - pj)) * pl.y;
+    const float b1 = ((pi - v0i) * (v2j - v0j) - (v2i - v0i) * (pj - v0j)) * pl.y;
+    const float b2 = 1.0f - b0 - b1;
+    inside = !(b0 < 0 || b0 > 1) && !(b1 < 0 || b1 > 1) && (b2 >= 0) && (b2 <= 1);
+  }
+  if (inside) {
+    cp = proj;
+    return sqn3(sub3(proj, p));
+  }
+  const float3 r0 = seg_closest(v0, v1, f3(e0.x, e0.y, e0.z), e0.w, p);
+  const float3 r1 = seg_closest(v1, v2, f3(e1.x, e1.y, e1.z), e1.w, p);
+  const float3 r2 = seg_closest(v2, v0, f3(e2.x, e2.y, e2.z), e2.w, p);
+  const float d0 = sqn3(sub3(r0, p)), d1 = sqn3(sub3(r1, p)), d2 = sqn3(sub3(r2, p));
+  if (d0 < d1) {
+    if (d0 < d2) { cp = r0; return d0; }
+    cp = r2;
+    return d2;
+  }
+  if (d1 < d2) { cp = r1; return d1; }
+  cp = r2;
+  return d2;
 }
 
 THG_HD float tri_closest(const TriRec &r, float3 p, float3 &cp) {
