@@ -29,7 +29,9 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", default="c3", choices=["c3", "c1", "c3ray", "c2knn", "c2sample"])
+    p.add_argument("--workload", default="c3", choices=["c3", "c1", "c3ray", "c2knn", "c2sample", "c5"])
+    p.add_argument("--requests", type=int, default=400, help="c5: requests in the trace")
+    p.add_argument("--load", type=float, default=0.7, help="c5: Poisson rate as a fraction of plain capacity")
     p.add_argument("--queries", type=int, default=None, help="override the query count")
     p.add_argument("--leaf", type=int, default=4)
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -78,6 +80,9 @@ def main():
     ctx = thea_amd.default_context(0)
     counts = load_counts()
     wl = args.workload
+    if wl == "c5":
+        run_c5(args)
+        return
     t_gen = time.time()
     info = {}
     if wl in ("c3", "c3ray"):
@@ -213,6 +218,139 @@ def main():
         result["cpu_baseline"] = cpu_baseline(args, wl, v, t, q if wl in ("c3", "c1") else None,
                                               (ro, rd) if wl == "c3ray" else None, unit)
     print(json.dumps(result), flush=True)
+
+
+def run_c5(args):
+    """Served request stream: 64 resident meshes (log-uniform 10k-4M triangles), requests with a
+    uniform mesh, a log-uniform batch of 256-1M points and a 60/25/15 mix of closest point, k-NN
+    (k = 8) and first-hit rays. Compares the server (continuous batching, multi-batch launches on 4
+    streams) with plain dispatch (one C ABI call per request on one stream, in arrival order):
+    saturation throughput with every request queued at once, then p50/p99 latency under Poisson
+    arrivals at `--load` times the plain path's capacity."""
+    import threading
+
+    import numpy as np
+    import torch
+    import thea_amd
+    from thea_amd import _capi as capi
+    from thea_amd import workloads as W
+    from thea_amd.serve import Server
+
+    dev = torch.device("cuda", 0)
+    t0 = time.time()
+    meshes = W.resident_c5()
+    gen_s = time.time() - t0
+    srv = Server(num_streams=4)
+    t0 = time.perf_counter()
+    ids = [srv.add_mesh(v, t) for v, t in meshes]
+    srv_build_s = time.perf_counter() - t0
+    bvhs = [thea_amd.MeshBvh(thea_amd.Mesh(v, t)) for v, t in meshes]
+    g = W.rng(5, W.STREAM_TRACE)
+    R = args.requests
+    nmax = 1 << 20
+    pools = []
+    for v, _ in meshes:
+        lo, hi = W.inflated_box(v, 1.2)
+        lo_t = torch.tensor(lo, dtype=torch.float32, device=dev)
+        hi_t = torch.tensor(hi, dtype=torch.float32, device=dev)
+        pools.append(lo_t + (hi_t - lo_t) * torch.rand((nmax, 3), device=dev))
+    dirs = torch.nn.functional.normalize(torch.randn((nmax, 3), device=dev), dim=1).contiguous()
+    kinds = g.choice(["closest", "knn", "ray"], size=R, p=[0.6, 0.25, 0.15])
+    mesh_of = g.integers(0, len(meshes), size=R)
+    sizes = W.log_uniform_int(g, 256, 1 << 20, R)
+    reqs = []
+    for i in range(R):
+        n = int(sizes[i])
+        m = int(mesh_of[i])
+        k = 8 if kinds[i] == "knn" else 0
+        out = {"idx": torch.empty((n, k) if k else n, dtype=torch.int32, device=dev),
+               "dist": torch.empty((n, k) if k else n, dtype=torch.float32, device=dev),
+               "cp": None if k else torch.empty((n, 3), dtype=torch.float32, device=dev)}
+        reqs.append(dict(kind=str(kinds[i]), mesh=m, n=n, k=k, q=pools[m][:n], dirs=dirs[:n] if kinds[i] == "ray"
+                         else None, out=out))
+    total_q = int(sizes.sum())
+    L = capi.lib()
+    stream = C.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    P = lambda x: None if x is None else C.c_void_p(x.data_ptr())  # noqa: E731
+
+    def plain_call(r):
+        b = bvhs[r["mesh"]]
+        o = r["out"]
+        if r["kind"] == "closest":
+            capi.check(L.thg_closest_point(b.h, P(r["q"]), r["n"], -1.0, P(o["idx"]), P(o["cp"]), P(o["dist"]), None,
+                                           capi.THG_MEM_DEVICE, stream))
+        elif r["kind"] == "knn":
+            capi.check(L.thg_k_closest(b.h, P(r["q"]), r["n"], r["k"], -1.0, 0, P(o["idx"]), P(o["dist"]), None,
+                                       capi.THG_MEM_DEVICE, stream))
+        else:
+            capi.check(L.thg_ray_first_hit(b.h, P(r["q"]), P(r["dirs"]), r["n"], -1.0, P(o["idx"]), P(o["dist"]),
+                                           P(o["cp"]), capi.THG_MEM_DEVICE, stream))
+        torch.cuda.current_stream(dev).synchronize()
+
+    def plain_run(arrivals):
+        lat = []
+        t_start = time.perf_counter()
+        for r, a in zip(reqs, arrivals):
+            while time.perf_counter() - t_start < a:
+                pass
+            plain_call(r)
+            lat.append(time.perf_counter() - t_start - a)
+        return time.perf_counter() - t_start, np.array(lat) * 1e6
+
+    def server_run(arrivals):
+        srv.stats(reset=True)
+        rids = [None] * R
+        t_start = time.perf_counter()
+
+        def submitter():
+            for i, (r, a) in enumerate(zip(reqs, arrivals)):
+                while time.perf_counter() - t_start < a:
+                    pass
+                rids[i] = srv.submit(r["kind"], ids[r["mesh"]], r["q"], r["dirs"], k=r["k"], outputs=r["out"])[0]
+        th = threading.Thread(target=submitter)
+        th.start()
+        th.join()
+        lat = []
+        for rid in rids:
+            rc, l_us = srv.wait(rid)
+            capi.check(rc)
+            lat.append(l_us)
+        return time.perf_counter() - t_start, np.array(lat)
+
+    for r in reqs[:8]:  # warm up both paths
+        plain_call(r)
+    server_run(np.zeros(8))
+    zero = np.zeros(R)
+    t_plain, _ = plain_run(zero)
+    t_srv, _ = server_run(zero)
+    cap_plain = R / t_plain
+    lam = args.load * cap_plain
+    arrivals = np.cumsum(g.exponential(1.0 / lam, size=R))
+    tp, lat_p = plain_run(arrivals)
+    ts, lat_s = server_run(arrivals)
+    st = srv.stats()
+
+    def summ(t_total, lat):
+        return {"requests_per_s": R / t_total, "queries_per_s": total_q / t_total,
+                "p50_ms": float(np.percentile(lat, 50)) / 1e3, "p99_ms": float(np.percentile(lat, 99)) / 1e3}
+
+    result = {
+        "metric": "served mixed requests/s (64 resident meshes; closest 60%, k-NN k=8 25%, rays 15%)",
+        "value": R / t_srv, "unit": "requests/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": t_srv * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "served request stream", "meshes": len(meshes),
+                   "triangles_total": int(sum(len(t) for _, t in meshes)), "requests": R,
+                   "queries_total": total_q, "batch": "log-uniform 256..1M points", "streams": 4},
+        "saturation": {"server": summ(t_srv, np.zeros(1)) | {"p50_ms": None, "p99_ms": None},
+                       "plain_dispatch": summ(t_plain, np.zeros(1)) | {"p50_ms": None, "p99_ms": None},
+                       "speedup": t_plain / t_srv},
+        "poisson": {"rate_requests_per_s": lam, "load_vs_plain_capacity": args.load,
+                    "server": summ(ts, lat_s), "plain_dispatch": summ(tp, lat_p)},
+        "server_stats": st, "mesh_generation_s": gen_s, "server_build_s": srv_build_s,
+    }
+    print(json.dumps(result), flush=True)
+    srv.close()
 
 
 def cpu_baseline(args, wl, v, t, q, rays, unit):

@@ -163,6 +163,68 @@ int thg_batch_closest(thg_ctx ctx, int32_t nm, const int64_t *nv, const int64_t 
                       const float *verts, const int32_t *tris, const float *queries, int32_t leaf_size,
                       int32_t *idx, float *dist, void *stream, double timings_ms[2]);
 
+/* ---------------------------------------------------------------------------------------------
+ * Request server (Thea has no analogue: its queries run synchronously on the caller's thread).
+ * Keeps meshes and their BVHs resident on one GPU under a memory budget and serves closest-point,
+ * k-NN and ray requests with continuous batching: whenever one of `num_streams` stream workers is
+ * free it takes every queued request that the memory budget admits (FIFO, up to the batch limits)
+ * and runs them as one multi-batch launch per query type (each request a segment against its own
+ * mesh), so requests that arrive while others run are admitted into the next launch and overlap
+ * on the device. Results are bit-identical to the single-call entry points. All request arrays
+ * are device pointers owned by the caller and must stay valid until the request completes.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct thg_server_s *thg_server;
+
+#define THG_REQ_CLOSEST 0
+#define THG_REQ_KNN 1
+#define THG_REQ_RAY 2
+
+#define THG_PENDING 1 /* thg_server_wait: the request has not completed within the timeout */
+
+typedef struct thg_server_opts {
+  int64_t memory_budget_bytes; /* device bytes for resident meshes/BVHs + request working memory;
+                                  0 = 90% of the device memory free at creation */
+  int32_t num_streams;         /* concurrent batches (stream workers); 0 = 4 */
+  int32_t max_batch_requests;  /* requests per launch; 0 = 256 */
+  int64_t max_batch_queries;   /* queries per launch; 0 = 16M */
+  int32_t coalesce;            /* 1 (default) merge admitted requests into one launch per type;
+                                  -1 = one launch per request (still on concurrent streams) */
+  int32_t reserved[7];
+} thg_server_opts;
+
+typedef struct thg_request {
+  int32_t type;         /* THG_REQ_CLOSEST | THG_REQ_KNN | THG_REQ_RAY */
+  int32_t mesh;         /* id from thg_server_add_mesh */
+  int64_t n;            /* number of queries / rays */
+  int32_t k;            /* k-NN: 1..32 */
+  int32_t exclude_self; /* k-NN: as thg_k_closest */
+  double bound;         /* dist_bound (closest, k-NN) or max_time (rays); negative = none */
+  const float *q;       /* n float3: query points or ray origins */
+  const float *dirs;    /* rays: n float3 directions */
+  int32_t *idx;         /* n (closest, rays) or n*k (k-NN) element indices */
+  float *dist;          /* nullable: distances (closest, k-NN) or hit times (rays) */
+  float *cp;            /* nullable: closest points (closest) or hit normals (rays), n float3 */
+} thg_request;
+
+int thg_server_create(thg_ctx ctx, const thg_server_opts *opts, thg_server *out);
+/* Waits for running batches, fails queued requests with THG_E_CANCELLED, frees everything. */
+int thg_server_destroy(thg_server srv);
+/* Upload a mesh and build its BVH; fails with THG_E_BUDGET if it does not fit the budget. */
+int thg_server_add_mesh(thg_server srv, const float *verts, int64_t nv, const int32_t *tris, int64_t nt, int32_t mem,
+                        int32_t leaf_size, int32_t *mesh_id);
+/* Queue a request (returns at once). THG_E_BUDGET if it can never fit the budget. */
+int thg_server_submit(thg_server srv, const thg_request *req, int64_t *req_id);
+/* Wait up to timeout_us (negative = forever). Returns the request's status (THG_OK, an error
+ * code, THG_E_CANCELLED) once it has finished, or THG_PENDING. latency_us (nullable) receives the
+ * time from submit to completion. A finished request is forgotten after a successful wait. */
+int thg_server_wait(thg_server srv, int64_t req_id, int64_t timeout_us, double *latency_us);
+/* Cancel a request that has not started; THG_E_INVALID if it is running or unknown. */
+int thg_server_cancel(thg_server srv, int64_t req_id);
+/* out[0] requests completed, [1] launches, [2] mean requests per launch, [3] p50 latency (us),
+ * [4] p99 latency (us), [5] resident bytes, [6] peak in-flight working bytes, [7] rejected requests,
+ * [8] queries completed, [9] mean latency (us). reset != 0 clears the counters. */
+int thg_server_stats(thg_server srv, double out[10], int32_t reset);
+
 #ifdef __cplusplus
 }
 #endif

@@ -35,6 +35,21 @@ class BuildOpts(C.Structure):
     _fields_ = [("leaf_size", C.c_int32), ("reserved", C.c_int32 * 7)]
 
 
+class ServerOpts(C.Structure):
+    _fields_ = [("memory_budget_bytes", C.c_int64), ("num_streams", C.c_int32), ("max_batch_requests", C.c_int32),
+                ("max_batch_queries", C.c_int64), ("coalesce", C.c_int32), ("reserved", C.c_int32 * 7)]
+
+
+class Request(C.Structure):
+    _fields_ = [("type", C.c_int32), ("mesh", C.c_int32), ("n", C.c_int64), ("k", C.c_int32),
+                ("exclude_self", C.c_int32), ("bound", C.c_double), ("q", C.c_void_p), ("dirs", C.c_void_p),
+                ("idx", C.c_void_p), ("dist", C.c_void_p), ("cp", C.c_void_p)]
+
+
+THG_REQ_CLOSEST, THG_REQ_KNN, THG_REQ_RAY = 0, 1, 2
+THG_PENDING = 1
+
+
 vp, i32, i64, u64, f32, f64 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint64, C.c_float, C.c_double
 
 # name -> (restype, argtypes)
@@ -59,6 +74,13 @@ SIGNATURES = {
     "thg_closest_stats": (C.c_int, [vp, vp, i64, vp, C.POINTER(i64)]),
     "thg_ctx_set_timing": (C.c_int, [vp, i32]),
     "thg_ctx_kernel_time": (C.c_int, [vp, C.c_char_p, C.POINTER(f64), C.POINTER(i64), i32]),
+    "thg_server_create": (C.c_int, [vp, C.POINTER(ServerOpts), C.POINTER(vp)]),
+    "thg_server_destroy": (C.c_int, [vp]),
+    "thg_server_add_mesh": (C.c_int, [vp, vp, i64, vp, i64, i32, i32, C.POINTER(i32)]),
+    "thg_server_submit": (C.c_int, [vp, C.POINTER(Request), C.POINTER(i64)]),
+    "thg_server_wait": (C.c_int, [vp, i64, i64, C.POINTER(f64)]),
+    "thg_server_cancel": (C.c_int, [vp, i64]),
+    "thg_server_stats": (C.c_int, [vp, C.POINTER(f64), i32]),
     "thg_batch_closest": (C.c_int, [vp, i32, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp]),
 }
 

@@ -169,22 +169,21 @@ __device__ __forceinline__ float lane_readf(float x, int32_t l) {
 // and come through scalar loads. PRE adds a per-element box test before the exact arithmetic: the
 // element is skipped by every lane whose box lower bound exceeds its best (never changes results:
 // the box bound never exceeds the element's float distance, see box_lb's slack).
-template <bool POINTS, int G, bool PRE, bool STATS = false>
-__global__ void __launch_bounds__(W_BLOCK) k_closest_wave(const BNode *__restrict__ nodes, const void *__restrict__ elems,
-                                                          const float *__restrict__ q, const uint32_t *__restrict__ order,
-                                                          int64_t n, uint32_t nb_real, float best_init, float scale,
-                                                          int exclude_self, int32_t *__restrict__ out_idx,
-                                                          float *__restrict__ out_cp, float *__restrict__ out_dist,
-                                                          float *__restrict__ out_d2,
-                                                          unsigned long long *__restrict__ stats = nullptr) {
+// One wave's work: queries [wave * 64 / G, ...) of a batch (`wave_ok` false for padding waves).
+template <bool POINTS, int G, bool PRE, bool STATS>
+__device__ __forceinline__ void closest_wave_body(const BNode *__restrict__ nodes, const void *__restrict__ elems,
+                                                  const float *__restrict__ q, const uint32_t *__restrict__ order,
+                                                  int64_t n, int64_t wave, bool wave_ok, float best_init, float scale,
+                                                  int exclude_self, int32_t *__restrict__ out_idx,
+                                                  float *__restrict__ out_cp, float *__restrict__ out_dist,
+                                                  float *__restrict__ out_d2, unsigned long long *__restrict__ stats) {
   // STATS: per wave, count nodes loaded, box tests, leaf elements loaded and elements evaluated.
   uint32_t st_nodes = 0, st_boxes = 0, st_loaded = 0, st_tested = 0, st_plane = 0, st_near = 0;
   constexpr int QW = WAVE / G;  // queries per wave
   const int lane = lane_id();
   const int sub = lane % G;
-  const uint32_t lb_blk = xcd_remap(blockIdx.x, gridDim.x);
-  const int64_t t = ((int64_t)lb_blk * W_WAVES + (threadIdx.x / WAVE)) * QW + lane / G;
-  const bool inrange = lb_blk < nb_real && t < n;
+  const int64_t t = wave * QW + lane / G;
+  const bool inrange = wave_ok && t < n;
   bool active = inrange;
   int64_t qi = 0;
   float3 p = f3(0, 0, 0);
@@ -365,6 +364,45 @@ __global__ void __launch_bounds__(W_BLOCK) k_closest_wave(const BNode *__restric
   if (out_d2) out_d2[qi] = hit ? own : -1.0f;
 }
 
+template <bool POINTS, int G, bool PRE, bool STATS = false>
+__global__ void __launch_bounds__(W_BLOCK) k_closest_wave(const BNode *__restrict__ nodes, const void *__restrict__ elems,
+                                                          const float *__restrict__ q, const uint32_t *__restrict__ order,
+                                                          int64_t n, uint32_t nb_real, float best_init, float scale,
+                                                          int exclude_self, int32_t *__restrict__ out_idx,
+                                                          float *__restrict__ out_cp, float *__restrict__ out_dist,
+                                                          float *__restrict__ out_d2,
+                                                          unsigned long long *__restrict__ stats = nullptr) {
+  const uint32_t lb_blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t wave = (int64_t)lb_blk * W_WAVES + (threadIdx.x / WAVE);
+  closest_wave_body<POINTS, G, PRE, STATS>(nodes, elems, q, order, n, wave, lb_blk < nb_real, best_init, scale,
+                                           exclude_self, out_idx, out_cp, out_dist, out_d2, stats);
+}
+
+// Many batches (each against its own BVH) in one launch: the waves of segment s are
+// [seg[s].wave_begin, seg[s + 1].wave_begin); a wave finds its segment by binary search.
+__device__ __forceinline__ int find_segment(const SegDesc *__restrict__ seg, int nseg, int64_t wave) {
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if ((int64_t)seg[mid].wave_begin <= wave) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <bool POINTS>
+__global__ void __launch_bounds__(W_BLOCK) k_closest_multi(const SegDesc *__restrict__ seg, int nseg,
+                                                           uint64_t total_waves, uint32_t nb_real) {
+  const uint32_t lb_blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t gw = (int64_t)lb_blk * W_WAVES + (threadIdx.x / WAVE);
+  if (lb_blk >= nb_real || gw >= (int64_t)total_waves) return;
+  const int si = uni(find_segment(seg, nseg, gw));
+  const SegDesc &d = seg[si];
+  closest_wave_body<POINTS, 1, true, false>((const BNode *)d.nodes, d.elems, d.q, d.order, d.n,
+                                            gw - (int64_t)d.wave_begin, true, d.best_init, d.scale, d.exclude_self,
+                                            d.idx, d.cp, d.dist, d.d2, nullptr);
+}
+
 //--------------------------------------------------------------------------------------------------
 // k nearest (sorted list of K (d^2, id) pairs per lane in registers)
 //--------------------------------------------------------------------------------------------------
@@ -488,15 +526,14 @@ __global__ void __launch_bounds__(Q_BLOCK) k_knn(const BNode *__restrict__ nodes
 // its lower bound. Element records are wave-uniform (scalar loads).
 //--------------------------------------------------------------------------------------------------
 template <bool POINTS, int K>
-__global__ void __launch_bounds__(W_BLOCK) k_knn_wave(const BNode *__restrict__ nodes, const void *__restrict__ elems,
-                                                      const float *__restrict__ q, const uint32_t *__restrict__ order,
-                                                      int64_t n, uint32_t nb_real, int k, float best_init, float scale,
-                                                      int exclude_self, int32_t *__restrict__ out_idx,
-                                                      float *__restrict__ out_dist, float *__restrict__ out_d2) {
+__device__ __forceinline__ void knn_wave_body(const BNode *__restrict__ nodes, const void *__restrict__ elems,
+                                              const float *__restrict__ q, const uint32_t *__restrict__ order,
+                                              int64_t n, int64_t wave, bool wave_ok, int k, float best_init,
+                                              float scale, int exclude_self, int32_t *__restrict__ out_idx,
+                                              float *__restrict__ out_dist, float *__restrict__ out_d2) {
   const int lane = lane_id();
-  const uint32_t lb_blk = xcd_remap(blockIdx.x, gridDim.x);
-  const int64_t t = ((int64_t)lb_blk * W_WAVES + (threadIdx.x / WAVE)) * WAVE + lane;
-  const bool inrange = lb_blk < nb_real && t < n;
+  const int64_t t = wave * WAVE + lane;
+  const bool inrange = wave_ok && t < n;
   bool active = inrange;
   int64_t qi = 0;
   float3 p = f3(0, 0, 0);
@@ -614,6 +651,30 @@ __global__ void __launch_bounds__(W_BLOCK) k_knn_wave(const BNode *__restrict__ 
   }
 }
 
+template <bool POINTS, int K>
+__global__ void __launch_bounds__(W_BLOCK) k_knn_wave(const BNode *__restrict__ nodes, const void *__restrict__ elems,
+                                                      const float *__restrict__ q, const uint32_t *__restrict__ order,
+                                                      int64_t n, uint32_t nb_real, int k, float best_init, float scale,
+                                                      int exclude_self, int32_t *__restrict__ out_idx,
+                                                      float *__restrict__ out_dist, float *__restrict__ out_d2) {
+  const uint32_t lb_blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t wave = (int64_t)lb_blk * W_WAVES + (threadIdx.x / WAVE);
+  knn_wave_body<POINTS, K>(nodes, elems, q, order, n, wave, lb_blk < nb_real, k, best_init, scale, exclude_self,
+                           out_idx, out_dist, out_d2);
+}
+
+template <bool POINTS, int K>
+__global__ void __launch_bounds__(W_BLOCK) k_knn_multi(const SegDesc *__restrict__ seg, int nseg, uint64_t total_waves,
+                                                       uint32_t nb_real) {
+  const uint32_t lb_blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t gw = (int64_t)lb_blk * W_WAVES + (threadIdx.x / WAVE);
+  if (lb_blk >= nb_real || gw >= (int64_t)total_waves) return;
+  const int si = uni(find_segment(seg, nseg, gw));
+  const SegDesc &d = seg[si];
+  knn_wave_body<POINTS, K>((const BNode *)d.nodes, d.elems, d.q, d.order, d.n, gw - (int64_t)d.wave_begin, true, d.k,
+                           d.best_init, d.scale, d.exclude_self, d.idx, d.dist, d.d2);
+}
+
 //--------------------------------------------------------------------------------------------------
 // Rays
 //--------------------------------------------------------------------------------------------------
@@ -630,17 +691,12 @@ __device__ __forceinline__ float2 slab(float3 o, float3 inv, float w, float lx, 
 
 // ANY = true: stop at the first accepted hit (any-hit query).
 template <int STACK, bool ANY>
-__global__ void __launch_bounds__(Q_BLOCK) k_ray(const BNode *__restrict__ nodes, const TriRec *__restrict__ tr,
-                                                 const float *__restrict__ orig, const float *__restrict__ dirs,
-                                                 const uint32_t *__restrict__ order, int64_t n, uint32_t nb_real,
-                                                 float best_init, float max_time, float scale,
-                                                 int32_t *__restrict__ out_idx, float *__restrict__ out_t,
-                                                 float *__restrict__ out_n, uint8_t *__restrict__ out_any) {
-  __shared__ int32_t s_ref[STACK * Q_BLOCK];
-  __shared__ float s_t[STACK * Q_BLOCK];
-  bool active;
-  int64_t ri = query_slot(nb_real, n, order, active);
-  if (!active) return;
+__device__ __forceinline__ void ray_body(int32_t *__restrict__ s_ref, float *__restrict__ s_t,
+                                         const BNode *__restrict__ nodes, const TriRec *__restrict__ tr,
+                                         const float *__restrict__ orig, const float *__restrict__ dirs, int64_t ri,
+                                         float best_init, float max_time, float scale, int32_t *__restrict__ out_idx,
+                                         float *__restrict__ out_t, float *__restrict__ out_n,
+                                         uint8_t *__restrict__ out_any) {
   const int tid = threadIdx.x;
   float3 o = load_q(orig, ri), d = load_q(dirs, ri);
   float3 inv = f3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
@@ -719,6 +775,39 @@ __global__ void __launch_bounds__(Q_BLOCK) k_ray(const BNode *__restrict__ nodes
     if (hit) { nx = tr[bslot].p0.w; ny = tr[bslot].p1.w; nz = tr[bslot].p2.w; }
     out_n[3 * ri] = nx; out_n[3 * ri + 1] = ny; out_n[3 * ri + 2] = nz;
   }
+}
+
+template <int STACK, bool ANY>
+__global__ void __launch_bounds__(Q_BLOCK) k_ray(const BNode *__restrict__ nodes, const TriRec *__restrict__ tr,
+                                                 const float *__restrict__ orig, const float *__restrict__ dirs,
+                                                 const uint32_t *__restrict__ order, int64_t n, uint32_t nb_real,
+                                                 float best_init, float max_time, float scale,
+                                                 int32_t *__restrict__ out_idx, float *__restrict__ out_t,
+                                                 float *__restrict__ out_n, uint8_t *__restrict__ out_any) {
+  __shared__ int32_t s_ref[STACK * Q_BLOCK];
+  __shared__ float s_t[STACK * Q_BLOCK];
+  bool active;
+  int64_t ri = query_slot(nb_real, n, order, active);
+  if (!active) return;
+  ray_body<STACK, ANY>(s_ref, s_t, nodes, tr, orig, dirs, ri, best_init, max_time, scale, out_idx, out_t, out_n,
+                       out_any);
+}
+
+// Many ray batches in one launch; one 64-lane block per wave of a segment (SEG_WAVE_QUERIES rays).
+template <int STACK>
+__global__ void __launch_bounds__(Q_BLOCK) k_ray_multi(const SegDesc *__restrict__ seg, int nseg, uint64_t total_waves,
+                                                       uint32_t nb_real) {
+  __shared__ int32_t s_ref[STACK * Q_BLOCK];
+  __shared__ float s_t[STACK * Q_BLOCK];
+  const uint32_t lb_blk = xcd_remap(blockIdx.x, gridDim.x);
+  if (lb_blk >= nb_real || (uint64_t)lb_blk >= total_waves) return;
+  const int si = uni(find_segment(seg, nseg, (int64_t)lb_blk));
+  const SegDesc &d = seg[si];
+  const int64_t t = ((int64_t)lb_blk - (int64_t)d.wave_begin) * Q_BLOCK + threadIdx.x;
+  if (t >= d.n) return;
+  const int64_t ri = d.order ? (int64_t)d.order[t] : t;
+  ray_body<STACK, false>(s_ref, s_t, (const BNode *)d.nodes, (const TriRec *)d.elems, d.q, d.q2, ri, d.best_init,
+                         d.max_time, d.scale, d.idx, d.dist, d.cp, nullptr);
 }
 
 //--------------------------------------------------------------------------------------------------
@@ -878,6 +967,39 @@ static void launch_knn_k(const Bvh &b, int st, uint32_t grid, uint32_t nb, const
     if (st <= 32) THG_KNN(32, false); else if (st == 64) THG_KNN(64, false); else THG_KNN(128, false);
   }
 #undef THG_KNN
+}
+
+void launch_closest_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, bool points, hipStream_t s) {
+  if (nseg <= 0 || total_waves == 0) return;
+  uint32_t nb = ceil_div(total_waves, W_WAVES);
+  uint32_t grid = round_up8(nb);
+  if (points)
+    hipLaunchKernelGGL(k_closest_multi<true>, dim3(grid), dim3(W_BLOCK), 0, s, seg_dev, nseg, total_waves, nb);
+  else
+    hipLaunchKernelGGL(k_closest_multi<false>, dim3(grid), dim3(W_BLOCK), 0, s, seg_dev, nseg, total_waves, nb);
+  launch_check("k_closest_multi");
+}
+
+void launch_knn_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, int k, bool points, hipStream_t s) {
+  if (nseg <= 0 || total_waves == 0) return;
+  uint32_t nb = ceil_div(total_waves, W_WAVES);
+  uint32_t grid = round_up8(nb);
+#define THG_KM(P, K) hipLaunchKernelGGL((k_knn_multi<P, K>), dim3(grid), dim3(W_BLOCK), 0, s, seg_dev, nseg, total_waves, nb)
+  if (points) {
+    if (k <= 4) THG_KM(true, 4); else if (k <= 8) THG_KM(true, 8); else if (k <= 16) THG_KM(true, 16); else THG_KM(true, 32);
+  } else {
+    if (k <= 4) THG_KM(false, 4); else if (k <= 8) THG_KM(false, 8); else if (k <= 16) THG_KM(false, 16); else THG_KM(false, 32);
+  }
+#undef THG_KM
+  launch_check("k_knn_multi");
+}
+
+void launch_ray_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, hipStream_t s) {
+  if (nseg <= 0 || total_waves == 0) return;
+  uint32_t nb = (uint32_t)total_waves;
+  uint32_t grid = round_up8(nb);
+  hipLaunchKernelGGL(k_ray_multi<64>, dim3(grid), dim3(Q_BLOCK), 0, s, seg_dev, nseg, total_waves, nb);
+  launch_check("k_ray_multi");
 }
 
 void closest_wave_stats(const Bvh &b, const float *q, int64_t n, unsigned long long *stats, QueryScratch &ws,

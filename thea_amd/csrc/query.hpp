@@ -8,6 +8,29 @@ namespace thg {
 // Batches smaller than this are processed in input order (sorting would cost more than it saves).
 constexpr int64_t ORDER_MIN_QUERIES = 16384;
 
+// One batch of a multi-batch launch (k_closest_multi / k_knn_multi / k_ray_multi): the BVH it
+// queries, its inputs and outputs, and the first wave of the launch that works on it.
+struct SegDesc {
+  const void *nodes;
+  const void *elems;
+  const float *q;        // queries (closest, k-NN) or ray origins
+  const float *q2;       // ray directions
+  const uint32_t *order; // optional spatial order of the batch
+  int32_t *idx;
+  float *cp;             // closest points (closest) / normals (rays)
+  float *dist;           // distances (closest, k-NN) / hit times (rays)
+  float *d2;
+  int64_t n;
+  uint64_t wave_begin;
+  float best_init;       // squared distance bound (closest, k-NN) / bound on t (rays)
+  float scale;
+  float max_time;
+  int32_t exclude_self;
+  int32_t k;
+  int32_t points;
+  int32_t pad[2];
+};
+
 // Per-call temporaries of a query batch (spatial order of the queries).
 struct QueryScratch {
   AsyncBuf keys, vals, tk, tv, hist;
@@ -19,6 +42,12 @@ void order_queries(const Bvh &b, const float *q, int64_t n, QueryScratch &ws, hi
 float bound_to_best(double dist_bound);
 void launch_closest(const Bvh &b, const float *q, int64_t n, double dist_bound, int exclude_self, int32_t *idx,
                     float *cp, float *dist, float *d2, QueryScratch &ws, hipStream_t s);
+// Launch many closest-point batches (segments) as one kernel; `seg_dev` is a device copy of the
+// descriptor table with wave_begin filled in. Requires every segment's BVH to fit the wave stack.
+void launch_closest_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, bool points, hipStream_t s);
+void launch_knn_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, int k, bool points, hipStream_t s);
+void launch_ray_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, hipStream_t s);
+constexpr int SEG_WAVE_QUERIES = 64;  // queries per wave in the multi-batch kernels
 void closest_wave_stats(const Bvh &b, const float *q, int64_t n, unsigned long long *stats, QueryScratch &ws,
                         hipStream_t s);
 void launch_knn(const Bvh &b, const float *q, int64_t n, int k, double dist_bound, int exclude_self, int32_t *idx,
