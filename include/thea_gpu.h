@@ -154,11 +154,28 @@ int thg_ctx_kernel_time(thg_ctx ctx, const char *kernel, double *total_ms, int64
 int thg_closest_stats(thg_bvh bvh, const float *q, int64_t n, void *stream, int64_t out[8]);
 
 /* Many meshes, one call: build a BVH for each of `nm` meshes and answer `nq[m]` closest-point
- * queries for mesh m, with one persistent launch for all builds' per-element work and one for all
- * queries. Thea has no analogue (each BvhN is built and queried by its own caller). Inputs are
- * concatenated: verts (sum nv float3), tris (sum nt int32 triplets, indices local to their mesh),
- * queries (sum nq float3); nv/nt/nq are per-mesh counts (host arrays). Results use local triangle
- * indices. leaf_size 0 = autotuned per size class. All pointers are device pointers. */
+ * queries for mesh m (Thea has no analogue: each BvhN is built and queried by its own caller).
+ * Inputs are concatenated: verts (sum nv float3), tris (sum nt int32 triplets, vertex indices local
+ * to their mesh), queries (sum nq float3, grouped by mesh); nv/nt/nq are per-mesh counts (host
+ * arrays). Results use local triangle indices. All array pointers are device pointers.
+ *
+ * A plan sizes every buffer once; its whole pipeline (all meshes' bounds, one Morton sort over all
+ * meshes, per-mesh radix trees, one refit, one compaction, records, query sort, and one
+ * multi-segment closest-point launch) has no host synchronisation, so the first run captures it into
+ * a HIP graph and later runs replay it. leaf_sizes: per mesh, 1..16 (NULL = 4).
+ * timings_ms (nullable): when given, the run is eager (not graph) and reports [build, query] ms. */
+typedef struct thg_batch_plan_s *thg_batch_plan;
+int thg_batch_plan_create(thg_ctx ctx, int32_t nm, const int64_t *nv, const int64_t *nt, const int64_t *nq,
+                          const int32_t *leaf_sizes, thg_batch_plan *out);
+int thg_batch_plan_run(thg_batch_plan plan, const float *verts, const int32_t *tris, const float *queries,
+                       int32_t *idx, float *dist, void *stream, double timings_ms[2]);
+int thg_batch_plan_destroy(thg_batch_plan plan);
+/* Leaf size per mesh chosen by measurement: for each size class (floor(log2(triangles))) the
+ * closest-point kernel is timed on up to two of its meshes for leaf sizes 2..16 and the fastest is
+ * kept (cached for the process). Same inputs as thg_batch_closest; writes leaf_sizes[nm]. */
+int thg_autotune_leaf(thg_ctx ctx, int32_t nm, const int64_t *nv, const int64_t *nt, const int64_t *nq,
+                      const float *verts, const int32_t *tris, const float *queries, int32_t *leaf_sizes);
+/* One-shot form: autotunes leaf sizes when leaf_size == 0, creates a plan, runs it, frees it. */
 int thg_batch_closest(thg_ctx ctx, int32_t nm, const int64_t *nv, const int64_t *nt, const int64_t *nq,
                       const float *verts, const int32_t *tris, const float *queries, int32_t leaf_size,
                       int32_t *idx, float *dist, void *stream, double timings_ms[2]);

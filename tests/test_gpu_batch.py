@@ -1,0 +1,93 @@
+"""GPU tests of the batched many-mesh plan: every mesh's closest-point results equal the single-mesh
+entry point bit for bit; graph replays with new inputs stay exact; autotuned leaf sizes are valid."""
+import numpy as np
+import pytest
+
+from conftest import random_mesh
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(seed, sizes, qsizes):
+    from thea_amd import workloads as W
+    g = np.random.default_rng(seed)
+    meshes, queries = [], []
+    for i, (nt, nq) in enumerate(zip(sizes, qsizes)):
+        if nt == 0:
+            v, t = np.zeros((0, 3), np.float32), np.zeros((0, 3), np.int32)
+        elif i % 3 == 0 and nt >= 64:
+            v, t = W.uv_sphere(max(2, int(np.sqrt(nt / 4))), 2 * max(2, int(np.sqrt(nt / 4))))
+            v = (v * g.uniform(0.5, 2) + g.uniform(-5, 5, size=3)).astype(np.float32)
+        else:
+            v, t = random_mesh(g, nt, spread=1.0, size=0.05)
+        meshes.append((v.astype(np.float32), t.astype(np.int32)))
+        lo = v.min(0) - 0.3 if len(v) else np.zeros(3)
+        hi = v.max(0) + 0.3 if len(v) else np.ones(3)
+        queries.append((lo + (hi - lo) * g.random((nq, 3))).astype(np.float32))
+    return meshes, queries
+
+
+def _check(meshes, queries, idx, dist):
+    import thea_amd
+    off = 0
+    for (v, t), q in zip(meshes, queries):
+        n = len(q)
+        if len(t) == 0:
+            assert np.all(idx[off:off + n] == -1)
+        elif n:
+            b = thea_amd.MeshBvh(thea_amd.Mesh(v, t))
+            ri, rd = b.closestElementWithDistance(q)
+            assert np.array_equal(idx[off:off + n], ri), f"mesh with {len(t)} triangles"
+            assert np.array_equal(dist[off:off + n].view(np.uint32), rd.view(np.uint32))
+        off += n
+
+
+def test_batch_matches_single_mesh_and_replays():
+    import torch
+    from thea_amd.batch import BatchPlan, concat
+    sizes = [0, 1, 2, 3, 5, 17, 100, 1000, 5000, 20000, 4, 60000, 7, 300]
+    qsizes = [10, 100, 0, 50, 300, 1000, 2000, 5000, 20000, 30000, 64, 40000, 65, 1]
+    print("case batch plan", flush=True)
+    meshes, queries = _batch(1, sizes, qsizes)
+    nv, nt, nq, verts, tris, qs = concat(meshes, queries)
+    V, T, Q = (torch.from_numpy(x).cuda() for x in (verts, tris, qs))
+    leaf = np.array([4, 4, 4, 4, 4, 4, 2, 8, 16, 4, 4, 6, 1, 12], np.int32)
+    plan = BatchPlan(nv, nt, nq, leaf_sizes=leaf)
+    idx, dist, (tb, tq) = plan.run(V, T, Q, timed=True)
+    torch.cuda.synchronize()
+    _check(meshes, queries, idx.cpu().numpy(), dist.cpu().numpy())
+    print(f"eager: build {tb:.3f} ms, query {tq:.3f} ms")
+    # graph capture and two replays with new queries
+    for seed in (2, 3):
+        _, queries2 = _batch(seed, sizes, qsizes)
+        Q2 = torch.from_numpy(np.concatenate(queries2)).cuda()
+        idx2, dist2 = plan.run(V, T, Q2)
+        torch.cuda.synchronize()
+        _check(meshes, queries2, idx2.cpu().numpy(), dist2.cpu().numpy())
+
+
+def test_autotune_and_one_shot():
+    import ctypes as C
+    import torch
+    from thea_amd import _capi as capi
+    from thea_amd import default_context
+    from thea_amd.batch import autotune_leaf, concat
+    sizes = [3000, 12000, 50000, 700]
+    qsizes = [20000, 5000, 30000, 1000]
+    print("case batch autotune", flush=True)
+    meshes, queries = _batch(4, sizes, qsizes)
+    nv, nt, nq, verts, tris, qs = concat(meshes, queries)
+    V, T, Q = (torch.from_numpy(x).cuda() for x in (verts, tris, qs))
+    leaf = autotune_leaf(nv, nt, nq, V, T, Q)
+    assert np.all((leaf >= 1) & (leaf <= 16))
+    n = int(nq.sum())
+    idx = torch.empty(n, dtype=torch.int32, device="cuda")
+    dist = torch.empty(n, dtype=torch.float32, device="cuda")
+    ctx = default_context()
+    capi.check(capi.lib().thg_batch_closest(ctx.h, len(nt), nv.ctypes.data_as(C.c_void_p), nt.ctypes.data_as(C.c_void_p),
+                                            nq.ctypes.data_as(C.c_void_p), C.c_void_p(V.data_ptr()),
+                                            C.c_void_p(T.data_ptr()), C.c_void_p(Q.data_ptr()), 0,
+                                            C.c_void_p(idx.data_ptr()), C.c_void_p(dist.data_ptr()), None, None))
+    torch.cuda.synchronize()
+    _check(meshes, queries, idx.cpu().numpy(), dist.cpu().numpy())
+    print(f"autotuned leaf sizes {leaf.tolist()}")

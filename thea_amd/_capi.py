@@ -81,6 +81,10 @@ SIGNATURES = {
     "thg_server_wait": (C.c_int, [vp, i64, i64, C.POINTER(f64)]),
     "thg_server_cancel": (C.c_int, [vp, i64]),
     "thg_server_stats": (C.c_int, [vp, C.POINTER(f64), i32]),
+    "thg_batch_plan_create": (C.c_int, [vp, i32, vp, vp, vp, vp, C.POINTER(vp)]),
+    "thg_batch_plan_run": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp]),
+    "thg_batch_plan_destroy": (C.c_int, [vp]),
+    "thg_autotune_leaf": (C.c_int, [vp, i32, vp, vp, vp, vp, vp, vp, vp]),
     "thg_batch_closest": (C.c_int, [vp, i32, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp]),
 }
 

@@ -208,8 +208,9 @@ __device__ __forceinline__ void child_box(int32_t c, const uint32_t *sorted, con
   }
 }
 
-// Emit the collapsed tree. `fidx` = exclusive prefix of keep (final node index of kept nodes).
-__global__ void __launch_bounds__(B_BLOCK) k_emit(int64_t n, int leaf_size, const uint32_t *__restrict__ keep,
+// Emit the collapsed tree. `fidx` = exclusive prefix of keep (final node index of kept nodes, plus
+// `base`); a child that is not kept becomes a leaf over its whole element range.
+__global__ void __launch_bounds__(B_BLOCK) k_emit(int64_t n, int32_t base, const uint32_t *__restrict__ keep,
                                                   const uint32_t *__restrict__ fidx, const int32_t *__restrict__ left,
                                                   const int32_t *__restrict__ right, const int32_t *__restrict__ rfirst,
                                                   const int32_t *__restrict__ rlast, const uint32_t *__restrict__ sorted,
@@ -229,10 +230,13 @@ __global__ void __launch_bounds__(B_BLOCK) k_emit(int64_t n, int leaf_size, cons
       ref[c] = ~(ch[c] & 0x7fffffff);
       cnt[c] = 1;
       ++leaves;
+    } else if (!keep[ch[c]]) {
+      ref[c] = ~rfirst[ch[c]];
+      cnt[c] = rlast[ch[c]] - rfirst[ch[c]] + 1;
+      ++leaves;
     } else {
-      int32_t sz = rlast[ch[c]] - rfirst[ch[c]] + 1;
-      if (sz <= leaf_size) { ref[c] = ~rfirst[ch[c]]; cnt[c] = sz; ++leaves; }
-      else { ref[c] = (int32_t)fidx[ch[c]]; cnt[c] = 0; }
+      ref[c] = base + (int32_t)fidx[ch[c]];
+      cnt[c] = 0;
     }
   }
   BNode nd;
@@ -240,7 +244,7 @@ __global__ void __launch_bounds__(B_BLOCK) k_emit(int64_t n, int leaf_size, cons
   nd.b = make_float4(hi[0][1], hi[0][2], lo[1][0], lo[1][1]);
   nd.c = make_float4(lo[1][2], hi[1][0], hi[1][1], hi[1][2]);
   nd.d = make_int4(ref[0], ref[1], cnt[0], cnt[1]);
-  out[fidx[i]] = nd;
+  out[base + fidx[i]] = nd;
   if (leaves) atomicAdd(leaf_count, leaves);
 }
 
@@ -296,8 +300,8 @@ __global__ void k_init_bounds(BoundsAcc *acc) {
 }
 
 // Exclusive scan of n uint32 into out (total in out[n]); uses the radix scan kernel's approach.
-__global__ void __launch_bounds__(1024) k_excl_scan_u32(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
-                                                        int64_t n) {
+// (in and out may be the same array)
+__global__ void __launch_bounds__(1024) k_excl_scan_u32(const uint32_t *in, uint32_t *out, int64_t n) {
   __shared__ uint32_t part[1024 / WAVE];
   __shared__ uint32_t carry;
   int t = threadIdx.x;
@@ -323,6 +327,79 @@ __global__ void __launch_bounds__(1024) k_excl_scan_u32(const uint32_t *__restri
     __syncthreads();
   }
   if (t == 0) out[n] = carry;
+}
+
+// Three-phase exclusive scan of n uint32 (out[n] = total): block sums, scan of the sums, block
+// rescan. `sums` must hold ceil(n / 1024) + 1 words.
+__global__ void __launch_bounds__(1024) k_scan_block_sums(const uint32_t *__restrict__ in, int64_t n,
+                                                          uint32_t *__restrict__ sums) {
+  __shared__ uint32_t part[1024 / WAVE];
+  int64_t i = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+  uint32_t v = i < n ? in[i] : 0;
+  uint32_t s = wave_sum(v);
+  if (lane_id() == 0) part[threadIdx.x / WAVE] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < 1024 / WAVE; ++w) t += part[w];
+    sums[blockIdx.x] = t;
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_scan_block_apply(const uint32_t *__restrict__ in, int64_t n,
+                                                           const uint32_t *__restrict__ offs, uint32_t *__restrict__ out) {
+  __shared__ uint32_t part[1024 / WAVE];
+  int64_t i = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+  uint32_t v = i < n ? in[i] : 0;
+  uint32_t inc = wave_incl_scan(v);
+  int w = threadIdx.x / WAVE;
+  if (lane_id() == WAVE - 1) part[w] = inc;
+  __syncthreads();
+  uint32_t off = offs[blockIdx.x];
+  for (int k = 0; k < w; ++k) off += part[k];
+  if (i < n) out[i] = off + inc - v;
+  if (i == n - 1) out[n] = off + inc;
+}
+
+size_t scan_tmp_words(int64_t n) { return (size_t)ceil_div((uint64_t)std::max<int64_t>(n, 1), 1024) + 1; }
+
+void scan_excl_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *tmp, hipStream_t s) {
+  if (n <= 0) return;
+  uint32_t nb = ceil_div((uint64_t)n, 1024);
+  hipLaunchKernelGGL(k_scan_block_sums, dim3(nb), dim3(1024), 0, s, in, n, tmp);
+  launch_check("k_scan_block_sums");
+  hipLaunchKernelGGL(k_excl_scan_u32, dim3(1), dim3(1024), 0, s, tmp, tmp, (int64_t)nb);
+  launch_check("k_excl_scan_u32");
+  hipLaunchKernelGGL(k_scan_block_apply, dim3(nb), dim3(1024), 0, s, in, n, tmp, out);
+  launch_check("k_scan_block_apply");
+}
+
+// Entry points for the batched many-mesh pipeline (batch.hip), which reuses these kernels over
+// concatenated meshes with global element positions and node indices.
+void launch_refit(int64_t n, const uint32_t *sorted, const float4 *blo, const float4 *bhi, const int32_t *left,
+                  const int32_t *right, const int32_t *nparent, const int32_t *lparent, uint32_t *flags, float *nbox,
+                  hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_refit, dim3(ceil_div(n, B_BLOCK)), dim3(B_BLOCK), 0, s, n, sorted, blo, bhi, left, right, nparent,
+                     lparent, flags, nbox);
+  launch_check("k_refit");
+}
+
+void launch_emit(int64_t ni, int32_t base, const uint32_t *keep, const uint32_t *fidx, const int32_t *left,
+                 const int32_t *right, const int32_t *rfirst, const int32_t *rlast, const uint32_t *sorted,
+                 const float4 *blo, const float4 *bhi, const float *nbox, BNode *out, uint32_t *leaf_count,
+                 hipStream_t s) {
+  if (ni <= 0) return;
+  hipLaunchKernelGGL(k_emit, dim3(ceil_div(ni, B_BLOCK)), dim3(B_BLOCK), 0, s, ni + 1, base, keep, fidx, left, right,
+                     rfirst, rlast, sorted, blo, bhi, nbox, out, leaf_count);
+  launch_check("k_emit");
+}
+
+void launch_depth(int64_t n, const int32_t *lparent, const int32_t *nparent, const uint32_t *keep, int32_t *max_depth,
+                  hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_depth, dim3(ceil_div(n, B_BLOCK)), dim3(B_BLOCK), 0, s, n, lparent, nparent, keep, max_depth);
+  launch_check("k_depth");
 }
 
 // Host driver shared by mesh and point builds.
@@ -403,15 +480,15 @@ void build_bvh(Bvh &b, const float *verts, const int32_t *tris, int64_t n, bool 
   hipLaunchKernelGGL(k_keep, dim3(ceil_div(ni, B_BLOCK)), dim3(B_BLOCK), 0, s, n, rfirst.as<int32_t>(),
                      rlast.as<int32_t>(), leaf_size, keep.as<uint32_t>());
   launch_check("k_keep");
-  hipLaunchKernelGGL(k_excl_scan_u32, dim3(1), dim3(1024), 0, s, keep.as<uint32_t>(), fidx.as<uint32_t>(), ni);
-  launch_check("k_excl_scan_u32");
+  DevBuf stmp(scan_tmp_words(ni) * 4);
+  scan_excl_u32(keep.as<uint32_t>(), fidx.as<uint32_t>(), ni, stmp.as<uint32_t>(), s);
   uint32_t kept = 0;
   THG_HIP(hipMemcpyAsync(&kept, fidx.as<uint32_t>() + ni, 4, hipMemcpyDeviceToHost, s));
   THG_HIP(hipStreamSynchronize(s));
   b.nodes.alloc((size_t)kept * sizeof(BNode));
   uint32_t *leaf_count = counters.as<uint32_t>();
   int32_t *maxd = reinterpret_cast<int32_t *>(counters.as<uint32_t>() + 1);
-  hipLaunchKernelGGL(k_emit, dim3(ceil_div(ni, B_BLOCK)), dim3(B_BLOCK), 0, s, n, leaf_size, keep.as<uint32_t>(),
+  hipLaunchKernelGGL(k_emit, dim3(ceil_div(ni, B_BLOCK)), dim3(B_BLOCK), 0, s, n, 0, keep.as<uint32_t>(),
                      fidx.as<uint32_t>(), left.as<int32_t>(), right.as<int32_t>(), rfirst.as<int32_t>(),
                      rlast.as<int32_t>(), sorted, blo.as<float4>(), bhi.as<float4>(), nbox.as<float>(),
                      b.nodes.as<BNode>(), leaf_count);

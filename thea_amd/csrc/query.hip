@@ -176,7 +176,8 @@ __device__ __forceinline__ void closest_wave_body(const BNode *__restrict__ node
                                                   int64_t n, int64_t wave, bool wave_ok, float best_init, float scale,
                                                   int exclude_self, int32_t *__restrict__ out_idx,
                                                   float *__restrict__ out_cp, float *__restrict__ out_dist,
-                                                  float *__restrict__ out_d2, unsigned long long *__restrict__ stats) {
+                                                  float *__restrict__ out_d2, unsigned long long *__restrict__ stats,
+                                                  int32_t root = 0) {
   // STATS: per wave, count nodes loaded, box tests, leaf elements loaded and elements evaluated.
   uint32_t st_nodes = 0, st_boxes = 0, st_loaded = 0, st_tested = 0, st_plane = 0, st_near = 0;
   constexpr int QW = WAVE / G;  // queries per wave
@@ -208,7 +209,7 @@ __device__ __forceinline__ void closest_wave_body(const BNode *__restrict__ node
   int32_t s_ref = 0;
   float s_lx = 0, s_ly = 0, s_lz = 0, s_hx = 0, s_hy = 0, s_hz = 0;
   int32_t sp = 0;
-  int32_t node = 0;
+  int32_t node = root;
   while (amask != 0) {
     node = uni(node);
     if (STATS) { st_nodes++; st_boxes += 2; }
@@ -400,7 +401,7 @@ __global__ void __launch_bounds__(W_BLOCK) k_closest_multi(const SegDesc *__rest
   const SegDesc &d = seg[si];
   closest_wave_body<POINTS, 1, true, false>((const BNode *)d.nodes, d.elems, d.q, d.order, d.n,
                                             gw - (int64_t)d.wave_begin, true, d.best_init, d.scale, d.exclude_self,
-                                            d.idx, d.cp, d.dist, d.d2, nullptr);
+                                            d.idx, d.cp, d.dist, d.d2, nullptr, d.root);
 }
 
 //--------------------------------------------------------------------------------------------------
@@ -530,7 +531,8 @@ __device__ __forceinline__ void knn_wave_body(const BNode *__restrict__ nodes, c
                                               const float *__restrict__ q, const uint32_t *__restrict__ order,
                                               int64_t n, int64_t wave, bool wave_ok, int k, float best_init,
                                               float scale, int exclude_self, int32_t *__restrict__ out_idx,
-                                              float *__restrict__ out_dist, float *__restrict__ out_d2) {
+                                              float *__restrict__ out_dist, float *__restrict__ out_d2,
+                                              int32_t root = 0) {
   const int lane = lane_id();
   const int64_t t = wave * WAVE + lane;
   const bool inrange = wave_ok && t < n;
@@ -560,7 +562,7 @@ __device__ __forceinline__ void knn_wave_body(const BNode *__restrict__ nodes, c
   int32_t s_ref = 0;
   float s_lx = 0, s_ly = 0, s_lz = 0, s_hx = 0, s_hy = 0, s_hz = 0;
   int32_t sp = 0;
-  int32_t node = 0;
+  int32_t node = root;
   while (amask != 0) {
     node = uni(node);
     const BNode nd = nodes[node];
@@ -672,7 +674,7 @@ __global__ void __launch_bounds__(W_BLOCK) k_knn_multi(const SegDesc *__restrict
   const int si = uni(find_segment(seg, nseg, gw));
   const SegDesc &d = seg[si];
   knn_wave_body<POINTS, K>((const BNode *)d.nodes, d.elems, d.q, d.order, d.n, gw - (int64_t)d.wave_begin, true, d.k,
-                           d.best_init, d.scale, d.exclude_self, d.idx, d.dist, d.d2);
+                           d.best_init, d.scale, d.exclude_self, d.idx, d.dist, d.d2, d.root);
 }
 
 //--------------------------------------------------------------------------------------------------
@@ -696,7 +698,7 @@ __device__ __forceinline__ void ray_body(int32_t *__restrict__ s_ref, float *__r
                                          const float *__restrict__ orig, const float *__restrict__ dirs, int64_t ri,
                                          float best_init, float max_time, float scale, int32_t *__restrict__ out_idx,
                                          float *__restrict__ out_t, float *__restrict__ out_n,
-                                         uint8_t *__restrict__ out_any) {
+                                         uint8_t *__restrict__ out_any, int32_t root = 0) {
   const int tid = threadIdx.x;
   float3 o = load_q(orig, ri), d = load_q(dirs, ri);
   float3 inv = f3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
@@ -706,7 +708,7 @@ __device__ __forceinline__ void ray_body(int32_t *__restrict__ s_ref, float *__r
   int32_t bidx = 0x7fffffff;
   int32_t bslot = -1;
   int sp = 0;
-  int32_t node = 0;
+  int32_t node = root;
   // A box is entered when its (widened) entry time is within this relative margin of the best.
   const float REL = 1.0f + 1.0f / 4096.0f;
   bool done = false;
@@ -807,7 +809,7 @@ __global__ void __launch_bounds__(Q_BLOCK) k_ray_multi(const SegDesc *__restrict
   if (t >= d.n) return;
   const int64_t ri = d.order ? (int64_t)d.order[t] : t;
   ray_body<STACK, false>(s_ref, s_t, (const BNode *)d.nodes, (const TriRec *)d.elems, d.q, d.q2, ri, d.best_init,
-                         d.max_time, d.scale, d.idx, d.dist, d.cp, nullptr);
+                         d.max_time, d.scale, d.idx, d.dist, d.cp, nullptr, d.root);
 }
 
 //--------------------------------------------------------------------------------------------------

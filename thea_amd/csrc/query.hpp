@@ -28,7 +28,8 @@ struct SegDesc {
   int32_t exclude_self;
   int32_t k;
   int32_t points;
-  int32_t pad[2];
+  int32_t root;          // node the traversal starts from
+  int32_t pad;
 };
 
 // Per-call temporaries of a query batch (spatial order of the queries).
@@ -42,6 +43,19 @@ void order_queries(const Bvh &b, const float *q, int64_t n, QueryScratch &ws, hi
 float bound_to_best(double dist_bound);
 void launch_closest(const Bvh &b, const float *q, int64_t n, double dist_bound, int exclude_self, int32_t *idx,
                     float *cp, float *dist, float *d2, QueryScratch &ws, hipStream_t s);
+// Build-pipeline pieces reused by the batched many-mesh plan (batch.hip).
+void launch_refit(int64_t n, const uint32_t *sorted, const float4 *blo, const float4 *bhi, const int32_t *left,
+                  const int32_t *right, const int32_t *nparent, const int32_t *lparent, uint32_t *flags, float *nbox,
+                  hipStream_t s);
+void launch_emit(int64_t ni, int32_t base, const uint32_t *keep, const uint32_t *fidx, const int32_t *left,
+                 const int32_t *right, const int32_t *rfirst, const int32_t *rlast, const uint32_t *sorted,
+                 const float4 *blo, const float4 *bhi, const float *nbox, BNode *out, uint32_t *leaf_count,
+                 hipStream_t s);
+void launch_depth(int64_t n, const int32_t *lparent, const int32_t *nparent, const uint32_t *keep, int32_t *max_depth,
+                  hipStream_t s);
+size_t scan_tmp_words(int64_t n);
+void scan_excl_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *tmp, hipStream_t s);
+
 // Launch many closest-point batches (segments) as one kernel; `seg_dev` is a device copy of the
 // descriptor table with wave_begin filled in. Requires every segment's BVH to fit the wave stack.
 void launch_closest_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, bool points, hipStream_t s);
