@@ -53,6 +53,12 @@ def load_traffic(name):
         return json.load(f).get("hbm_bytes_per_launch")
 
 
+def log(msg):
+    """Progress line on stderr (the JSON result is the only line on stdout)."""
+    sys.stderr.write(f"[bench] {msg}\n")
+    sys.stderr.flush()
+
+
 def cpu_threads(args):
     if args.cpu_threads:
         return args.cpu_threads
@@ -170,9 +176,10 @@ def main():
         metric, unit = "area samples/s (64 ellipsoids, 128k tris)", "samples/s"
         workload = "area-weighted sampling, 1M samples per step on 64 ellipsoids (128k tris)"
 
-    for _ in range(args.warmup):
+    for i in range(args.warmup):
         step()
-    torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        log(f"warmup step {i + 1}/{args.warmup} done")
     ctx.kernel_time(kernel, reset=True)
     ctx.kernel_time("order", reset=True)
     ctx.set_timing(True)
@@ -182,6 +189,7 @@ def main():
         step()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    log(f"{args.steps} timed steps done in {elapsed:.3f} s")
     ctx.set_timing(False)
     k_ms, k_n = ctx.kernel_time(kernel, reset=True)
     o_ms, o_n = ctx.kernel_time("order", reset=True)
@@ -242,11 +250,13 @@ def run_c4(args, counts):
     nv, nt, nq, verts, tris, qs = concat(meshes, queries)
     del meshes, queries
     gen_s = time.time() - t0
+    log(f"generated {len(nt)} meshes in {gen_s:.1f} s")
     V, T, Q = (torch.from_numpy(x).to(dev) for x in (verts, tris, qs))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     leaf = autotune_leaf(nv, nt, nq, V, T, Q)
     tune_s = time.perf_counter() - t0
+    log(f"autotuned leaf sizes in {tune_s:.1f} s")
     t0 = time.perf_counter()
     plan = BatchPlan(nv, nt, nq, leaf_sizes=leaf)
     plan_s = time.perf_counter() - t0
@@ -254,6 +264,7 @@ def run_c4(args, counts):
     for _ in range(3):  # eager runs with per-phase HIP events on the plan's stream
         idx, dist, ph = plan.run(V, T, Q, timed=True)
         phases.append(ph)
+        log(f"eager plan run: build {ph[0]:.2f} ms, query {ph[1]:.2f} ms")
     build_ms = float(np.mean([p[0] for p in phases[1:]]))
     query_ms = float(np.mean([p[1] for p in phases[1:]]))
     for _ in range(args.warmup):  # the first graph run captures, the rest replay
@@ -265,6 +276,7 @@ def run_c4(args, counts):
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed * 1e3 / args.steps
+    log(f"{args.steps} graph replays: {ms_per_step:.2f} ms each")
     NQ, NT = int(nq.sum()), int(nt.sum())
 
     # plain dispatch of the same work, one mesh at a time
@@ -294,6 +306,7 @@ def run_c4(args, counts):
         plain(m)
     torch.cuda.synchronize()
     plain_s = time.perf_counter() - t0
+    log(f"plain dispatch over {len(nt)} meshes: {plain_s:.2f} s")
     mism = int((pidx != idx).sum().item()) + int((pdist.view(torch.int32) != dist.view(torch.int32)).sum().item())
 
     c = counts["c4_closest"]
@@ -422,17 +435,22 @@ def run_c5(args):
             lat.append(l_us)
         return time.perf_counter() - t_start, np.array(lat)
 
+    log(f"{len(meshes)} meshes resident, {R} requests ({total_q} points) prepared")
     for r in reqs[:8]:  # warm up both paths
         plain_call(r)
     server_run(np.zeros(8))
     zero = np.zeros(R)
     t_plain, _ = plain_run(zero)
+    log(f"plain dispatch, all queued: {t_plain:.2f} s")
     t_srv, _ = server_run(zero)
+    log(f"server, all queued: {t_srv:.2f} s")
     cap_plain = R / t_plain
     lam = args.load * cap_plain
     arrivals = np.cumsum(g.exponential(1.0 / lam, size=R))
     tp, lat_p = plain_run(arrivals)
+    log(f"plain dispatch, Poisson arrivals: {tp:.2f} s")
     ts, lat_s = server_run(arrivals)
+    log(f"server, Poisson arrivals: {ts:.2f} s")
     st = srv.stats()
 
     def summ(t_total, lat):

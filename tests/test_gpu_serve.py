@@ -97,24 +97,30 @@ def test_budget_admission_and_rejection():
     from thea_amd import workloads as W
     print("case server budget", flush=True)
     v, t, _ = W.sphere_c1(n_queries=0)
-    srv = Server(memory_budget_bytes=8 << 20, num_streams=2)  # 8 MB
-    mid = srv.add_mesh(v, t)  # ~4.5 MB resident
+    budget = 6 << 20
+    srv = Server(memory_budget_bytes=budget, num_streams=2)
+    mid = srv.add_mesh(v, t)
+    resident = srv.stats()["resident_bytes"]
+    # admission of a mesh uses the bound records + nodes + arrays (~4.3 MB here); once built it is
+    # charged its actual size (~3.5 MB), so a second copy needs more than the 6 MB budget holds
+    assert 3.0e6 < resident < 4.3e6, resident
     with pytest.raises(capi.TheaGpuError) as e:
-        srv.add_mesh(v, t)  # a second copy does not fit
+        srv.add_mesh(v, t)
     assert e.value.code == capi.THG_E_BUDGET
     big = torch.zeros((400_000, 3), dtype=torch.float32, device="cuda")  # needs ~6.4 MB of sort scratch
     with pytest.raises(capi.TheaGpuError) as e:
         srv.submit("closest", mid, big)
     assert e.value.code == capi.THG_E_BUDGET
-    # requests that fit one at a time are all served (admission waits for memory to free up)
-    q = torch.rand((150_000, 3), device="cuda") * 3 - 1.5
+    # requests that fit one at a time (~1.6 MB each of the ~2.5 MB left) are all served: admission
+    # waits for memory to free up
+    q = torch.rand((100_000, 3), device="cuda") * 3 - 1.5
     rids = [srv.submit("closest", mid, q)[0] for _ in range(6)]
     for rid in rids:
         rc, _ = srv.wait(rid)
         assert rc == capi.THG_OK
     st = srv.stats()
     assert st["rejected"] == 1
-    assert st["peak_inflight_bytes"] <= (8 << 20) - st["resident_bytes"]
+    assert 0 < st["peak_inflight_bytes"] <= budget - st["resident_bytes"]
     srv.close()
 
 
