@@ -29,7 +29,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", default="c3", choices=["c3", "c1", "c3ray", "c2knn", "c2sample", "c4", "c5"])
+    p.add_argument("--workload", default="c3", choices=["c3", "c1", "c3ray", "c2knn", "c2sample", "c4", "c5", "copy"])
     p.add_argument("--requests", type=int, default=400, help="c5: requests in the trace")
     p.add_argument("--load", type=float, default=0.7, help="c5: Poisson rate as a fraction of plain capacity")
     p.add_argument("--queries", type=int, default=None, help="override the query count")
@@ -91,6 +91,9 @@ def main():
         return
     if wl == "c4":
         run_c4(args, counts)
+        return
+    if wl == "copy":
+        run_copy(args)
         return
     t_gen = time.time()
     info = {}
@@ -221,14 +224,42 @@ def main():
         waves = max(st[0], 1)
         result["gpu_traversal"] = {
             "per_wave": {"nodes": st[2] / waves, "box_tests": st[3] / waves, "elements_loaded": st[4] / waves,
-                         "elements_evaluated": st[5] / waves, "plane_admitted": st[6] / waves,
+                         "exact_eval_lane_slots": st[5] / waves, "pairs_passing_rejects": st[6] / waves,
                          "near_best": st[7] / waves},
             "queries_per_wave": st[1] / waves,
-            "note": "wave-cooperative traversal: every lane evaluates each element its wave evaluates"}
+            "lane_utilisation_of_exact_evals": st[6] / max(st[5], 1),
+            "note": "one wave walks one tree for its 64 queries; exact evaluation lane slots vs the "
+                    "(lane, element) pairs that needed one"}
     if not args.no_cpu_baseline and wl in ("c3", "c1", "c3ray", "c2knn"):
         result["cpu_baseline"] = cpu_baseline(args, wl, v, t, q if wl in ("c3", "c1") else None,
                                               (ro, rd) if wl == "c3ray" else None, unit)
     print(json.dumps(result), flush=True)
+
+
+def run_copy(args):
+    """Device-to-device copy of 1 GiB (hipMemcpyAsync through torch): the measured HBM ceiling, and
+    the plain copy of known size that scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    n = 1 << 28  # float32 elements = 1 GiB
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(args.warmup):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        y.copy_(x)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    gbps = 2 * n * 4 / (ms * 1e-3) / 1e9
+    print(json.dumps({"metric": "device copy GB/s (read + write)", "value": gbps, "unit": "GB/s", "n_gpus": 1,
+                      "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+                      "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+                      "config": {"workload": "1 GiB device-to-device copy", "bytes_read": n * 4,
+                                 "bytes_written": n * 4},
+                      "frac_of_peak": gbps / HBM_PEAK_GBPS}), flush=True)
 
 
 def run_c4(args, counts):

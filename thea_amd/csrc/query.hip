@@ -239,11 +239,13 @@ __device__ __forceinline__ void closest_wave_body(const BNode *__restrict__ node
             const TriRec *tk = tr + k;
             if (STATS) st_loaded++;
             const float4 sbv = make_float4(unif(tk->sb.x), unif(tk->sb.y), unif(tk->sb.z), unif(tk->sb.w));
+            bool sph = active;
             if (PRE) {
               const float dx = p.x - sbv.x, dy = p.y - sbv.y, dz = p.z - sbv.z;
               const float dc2 = dx * dx + dy * dy + dz * dz;
               const float T = sbest * REJ_REL + sbv.w + 2.0f * sl;
-              if (__ballot(active && !(dc2 > T * T)) == 0) continue;
+              sph = active && !(dc2 > T * T);
+              if (__ballot(sph) == 0) continue;
             }
             const float4 p0 = make_float4(unif(tk->p0.x), unif(tk->p0.y), unif(tk->p0.z), unif(tk->p0.w));
             const float4 p1 = make_float4(unif(tk->p1.x), unif(tk->p1.y), unif(tk->p1.z), unif(tk->p1.w));
@@ -253,8 +255,8 @@ __device__ __forceinline__ void closest_wave_body(const BNode *__restrict__ node
             const float sd = dot3(p, f3(p0.w, p1.w, p2.w)) - pl.x;
             if (PRE && __ballot(active && !(fabsf(sd) - 2.0f * sl > sbest * REJ_REL)) == 0) continue;
             if (STATS) {
-              st_tested++;
-              if (__ballot(active && sd * sd <= best)) st_plane++;
+              st_tested += WAVE;
+              st_plane += __popcll(__ballot(sph && !(fabsf(sd) - 2.0f * sl > sbest * REJ_REL)));
             }
             const float4 e0 = make_float4(unif(tk->e0.x), unif(tk->e0.y), unif(tk->e0.z), unif(tk->e0.w));
             const float4 e1 = make_float4(unif(tk->e1.x), unif(tk->e1.y), unif(tk->e1.z), unif(tk->e1.w));
@@ -363,6 +365,222 @@ __device__ __forceinline__ void closest_wave_body(const BNode *__restrict__ node
   }
   if (out_dist) out_dist[qi] = hit ? sqrtf(own) : -1.0f;
   if (out_d2) out_d2[qi] = hit ? own : -1.0f;
+}
+
+//--------------------------------------------------------------------------------------------------
+// Compacted wave traversal (triangles). The wave walks one tree for its 64 queries as above, but a
+// triangle of an admitted leaf is not evaluated by the whole wave: every lane applies the cheap
+// conservative rejects (bounding sphere, plane distance) against its own best, and each surviving
+// (lane, triangle) pair is appended to a per-wave LDS queue. Whenever 64 pairs are queued the wave
+// evaluates them one per lane with Thea's exact arithmetic and merges each result into its owner
+// lane's best through an LDS 64-bit atomic min on (float bits of d^2) << 32 | triangle id, which
+// orders pairs exactly as the lexicographic (d^2, index) rule does (d^2 >= 0; NaN never wins).
+// Deferring an evaluation only delays the tightening of the pruning bound, never the result.
+//--------------------------------------------------------------------------------------------------
+struct CmpWave {
+  float4 qp[WAVE];                  // owner's query point; w = excluded triangle id as int bits
+  float4 qcp[WAVE];                 // owner's closest point of its current best
+  unsigned long long key[WAVE];     // owner's best (d^2 bits << 32 | id)
+  int32_t qe[2 * WAVE];             // queued triangle record slots
+  uint8_t qo[2 * WAVE];             // queued owner lanes
+};
+
+__device__ __forceinline__ unsigned long long cmp_key(float d, int32_t id) {
+  return ((unsigned long long)(uint32_t)__float_as_int(d + 0.0f) << 32) | (uint32_t)id;
+}
+__device__ __forceinline__ float cmp_key_d(unsigned long long k) { return __int_as_float((int)(uint32_t)(k >> 32)); }
+
+// Evaluates queue entries [0, m) (one per lane), then moves entries [m, qn) to the front.
+__device__ __forceinline__ void cmp_flush(CmpWave &W, const TriRec *__restrict__ tr, int lane, int m, int qn) {
+  if (lane < m) {
+    const int32_t e = W.qe[lane];
+    const int o = W.qo[lane];
+    const float4 po = W.qp[o];
+    const TriRec *tk = tr + e;
+    const float4 p0 = tk->p0, p1 = tk->p1, p2 = tk->p2, pl = tk->pl, e0 = tk->e0, e1 = tk->e1, e2 = tk->e2;
+    float3 cp;
+    const float d = tri_closest(p0, p1, p2, pl, e0, e1, e2, f3(po.x, po.y, po.z), cp);
+    const int32_t id = trirec_id(pl);
+    unsigned long long k = cmp_key(d, id);
+    const bool ok = id != __float_as_int(po.w) && !isnan(d);
+    if (ok) atomicMin(&W.key[o], k);
+    __builtin_amdgcn_wave_barrier();
+    // (d, id) is unique per owner: the pair whose key survived writes the closest point
+    if (ok && W.key[o] == k) W.qcp[o] = make_float4(cp.x, cp.y, cp.z, 0.0f);
+  }
+  __builtin_amdgcn_wave_barrier();
+  const int rest = qn - m;
+  int32_t e = 0;
+  uint8_t o = 0;
+  if (lane < rest) { e = W.qe[m + lane]; o = W.qo[m + lane]; }
+  __builtin_amdgcn_wave_barrier();
+  if (lane < rest) { W.qe[lane] = e; W.qo[lane] = o; }
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <bool STATS>
+__device__ __forceinline__ void closest_cmp_body(CmpWave &W, const BNode *__restrict__ nodes,
+                                                 const TriRec *__restrict__ tr, const float *__restrict__ q,
+                                                 const uint32_t *__restrict__ order, int64_t n, int64_t wave,
+                                                 bool wave_ok, float best_init, float scale, int exclude_self,
+                                                 int32_t *__restrict__ out_idx, float *__restrict__ out_cp,
+                                                 float *__restrict__ out_dist, float *__restrict__ out_d2,
+                                                 unsigned long long *__restrict__ stats, int32_t root = 0) {
+  uint32_t st_nodes = 0, st_boxes = 0, st_loaded = 0, st_flush = 0, st_pairs = 0;
+  const int lane = lane_id();
+  const int64_t t = wave * WAVE + lane;
+  const bool inrange = wave_ok && t < n;
+  bool active = inrange;
+  int64_t qi = 0;
+  float3 p = f3(0, 0, 0);
+  if (active) {
+    qi = order ? (int64_t)order[t] : t;
+    p = load_q(q, qi);
+    active = !(isnan(p.x) || isnan(p.y) || isnan(p.z));
+  }
+  const uint64_t amask = __ballot(active);
+  const int nact = __popcll(amask);
+  const float m = fmaxf(scale, fmaxf(fabsf(p.x), fmaxf(fabsf(p.y), fabsf(p.z))));
+  const float sl = m * SLACK_REL;
+  const float3 qlo = f3(p.x - sl, p.y - sl, p.z - sl), qhi = f3(p.x + sl, p.y + sl, p.z + sl);
+  float best = active ? best_init : -1.0f;  // inactive lanes admit nothing
+  float sbest = active ? sqrtf(best_init) : 0.0f;
+  const int32_t self = exclude_self ? (int32_t)qi : -1;
+  W.qp[lane] = make_float4(p.x, p.y, p.z, __int_as_float(self));
+  W.key[lane] = cmp_key(active ? best_init : 0.0f, 0x7fffffff);
+  W.qcp[lane] = make_float4(0, 0, 0, 0);
+  __builtin_amdgcn_wave_barrier();
+  int qn = 0;  // queued pairs (wave-uniform)
+  int32_t s_ref = 0;
+  float s_lx = 0, s_ly = 0, s_lz = 0, s_hx = 0, s_hy = 0, s_hz = 0;
+  int32_t sp = 0;
+  int32_t node = root;
+  while (amask != 0) {
+    node = uni(node);
+    if (STATS) { st_nodes++; st_boxes += 2; }
+    const BNode nd = nodes[node];
+    const float a0 = unif(nd.a.x), a1 = unif(nd.a.y), a2 = unif(nd.a.z), a3 = unif(nd.a.w);
+    const float b0 = unif(nd.b.x), b1 = unif(nd.b.y), b2 = unif(nd.b.z), b3 = unif(nd.b.w);
+    const float c0 = unif(nd.c.x), c1 = unif(nd.c.y), c2 = unif(nd.c.z), c3 = unif(nd.c.w);
+    const int32_t r0 = uni(nd.d.x), r1 = uni(nd.d.y), n0 = uni(nd.d.z), n1 = uni(nd.d.w);
+    const float lb0 = box_lb(qlo, qhi, a0, a1, a2, a3, b0, b1);
+    const float lb1 = box_lb(qlo, qhi, b2, b3, c0, c1, c2, c3);
+    const uint64_t m0 = __ballot(lb0 <= best), m1 = __ballot(lb1 <= best);
+    const bool first1 = 2 * __popcll(__ballot(lb1 < lb0) & amask) > nact;
+    int32_t next = -1;
+#pragma unroll
+    for (int cc = 0; cc < 2; ++cc) {
+      const int c = first1 ? 1 - cc : cc;
+      const uint64_t mc = c == 0 ? m0 : m1;
+      if (mc == 0) continue;
+      const int32_t r = c == 0 ? r0 : r1;
+      const int32_t cnt = c == 0 ? n0 : n1;
+      if (r < 0) {
+        const bool in_c = (mc >> lane) & 1;
+        const int32_t f = ~r;
+        for (int32_t k = f; k < f + cnt; ++k) {
+          const TriRec *tk = tr + k;
+          if (STATS) st_loaded++;
+          const float4 sbv = make_float4(unif(tk->sb.x), unif(tk->sb.y), unif(tk->sb.z), unif(tk->sb.w));
+          const float dx = p.x - sbv.x, dy = p.y - sbv.y, dz = p.z - sbv.z;
+          const float dc2 = dx * dx + dy * dy + dz * dz;
+          const float T = sbest * REJ_REL + sbv.w + 2.0f * sl;
+          bool need = in_c && !(dc2 > T * T);
+          if (__ballot(need) == 0) continue;
+          const float nx = unif(tk->p0.w), ny = unif(tk->p1.w), nz = unif(tk->p2.w), pd = unif(tk->pl.x);
+          const float sd = dot3(p, f3(nx, ny, nz)) - pd;
+          need = need && !(fabsf(sd) - 2.0f * sl > sbest * REJ_REL);
+          const uint64_t nb = __ballot(need);
+          if (nb == 0) continue;
+          if (need) {
+            const int pos = qn + __popcll(nb & lanemask_lt());
+            W.qe[pos] = k;
+            W.qo[pos] = (uint8_t)lane;
+          }
+          qn += __popcll(nb);
+          if (STATS) st_pairs += __popcll(nb);
+          if (qn >= WAVE) {
+            __builtin_amdgcn_wave_barrier();
+            cmp_flush(W, tr, lane, WAVE, qn);
+            qn -= WAVE;
+            if (STATS) st_flush++;
+            if (active) {
+              best = cmp_key_d(W.key[lane]);
+              sbest = sqrtf(best);
+            }
+          }
+        }
+      } else if (next < 0) {
+        next = r;
+      } else {
+        if (lane == sp) {
+          s_ref = r;
+          if (c == 0) { s_lx = a0; s_ly = a1; s_lz = a2; s_hx = a3; s_hy = b0; s_hz = b1; }
+          else { s_lx = b2; s_ly = b3; s_lz = c0; s_hx = c1; s_hy = c2; s_hz = c3; }
+        }
+        ++sp;
+      }
+    }
+    if (next >= 0) { node = next; continue; }
+    bool found = false;
+    while (sp > 0) {
+      --sp;
+      const float lx = lane_readf(s_lx, sp), ly = lane_readf(s_ly, sp), lz = lane_readf(s_lz, sp);
+      const float hx = lane_readf(s_hx, sp), hy = lane_readf(s_hy, sp), hz = lane_readf(s_hz, sp);
+      if (STATS) st_boxes++;
+      if (__ballot(box_lb(qlo, qhi, lx, ly, lz, hx, hy, hz) <= best)) {
+        node = lane_read(s_ref, sp);
+        found = true;
+        break;
+      }
+    }
+    if (!found) break;
+  }
+  if (qn > 0) {
+    __builtin_amdgcn_wave_barrier();
+    cmp_flush(W, tr, lane, qn, qn);
+    if (STATS) st_flush++;
+  }
+  __builtin_amdgcn_wave_barrier();
+  if (STATS && lane == 0 && amask != 0) {
+    atomicAdd(&stats[0], 1ull);
+    atomicAdd(&stats[1], (unsigned long long)nact);
+    atomicAdd(&stats[2], (unsigned long long)st_nodes);
+    atomicAdd(&stats[3], (unsigned long long)st_boxes);
+    atomicAdd(&stats[4], (unsigned long long)st_loaded);
+    atomicAdd(&stats[5], (unsigned long long)st_flush * WAVE);
+    atomicAdd(&stats[6], (unsigned long long)st_pairs);
+    atomicAdd(&stats[7], 0ull);
+  }
+  if (!inrange) return;
+  const unsigned long long key = W.key[lane];
+  const int32_t bidx = (int32_t)(uint32_t)(key & 0xffffffffull);
+  const float own = cmp_key_d(key);
+  const bool hit = active && bidx != 0x7fffffff;
+  out_idx[qi] = hit ? bidx : -1;
+  if (out_cp) {
+    const float4 c = W.qcp[lane];
+    out_cp[3 * qi] = hit ? c.x : 0.0f;
+    out_cp[3 * qi + 1] = hit ? c.y : 0.0f;
+    out_cp[3 * qi + 2] = hit ? c.z : 0.0f;
+  }
+  if (out_dist) out_dist[qi] = hit ? sqrtf(own) : -1.0f;
+  if (out_d2) out_d2[qi] = hit ? own : -1.0f;
+}
+
+template <bool STATS = false>
+__global__ void __launch_bounds__(W_BLOCK) k_closest_cmp(const BNode *__restrict__ nodes, const TriRec *__restrict__ tr,
+                                                         const float *__restrict__ q, const uint32_t *__restrict__ order,
+                                                         int64_t n, uint32_t nb_real, float best_init, float scale,
+                                                         int exclude_self, int32_t *__restrict__ out_idx,
+                                                         float *__restrict__ out_cp, float *__restrict__ out_dist,
+                                                         float *__restrict__ out_d2,
+                                                         unsigned long long *__restrict__ stats = nullptr) {
+  __shared__ CmpWave s_w[W_WAVES];
+  const uint32_t lb_blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t wave = (int64_t)lb_blk * W_WAVES + (threadIdx.x / WAVE);
+  closest_cmp_body<STATS>(s_w[threadIdx.x / WAVE], nodes, tr, q, order, n, wave, lb_blk < nb_real, best_init, scale,
+                          exclude_self, out_idx, out_cp, out_dist, out_d2, stats);
 }
 
 template <bool POINTS, int G, bool PRE, bool STATS = false>
@@ -893,11 +1111,13 @@ bool use_wave_traversal(const Bvh &b) {
 struct WaveConfig {
   int g;
   bool pre;
+  bool cmp;
 };
 static WaveConfig wave_config(const Bvh &b) {
-  WaveConfig c{b.wave_g, b.wave_pre};
+  WaveConfig c{b.wave_g, b.wave_pre, b.wave_cmp};
   if (const char *e = getenv("THG_WAVE_G")) c.g = atoi(e);
   if (const char *e = getenv("THG_PREFILTER")) c.pre = atoi(e) != 0;
+  if (const char *e = getenv("THG_COMPACT")) c.cmp = atoi(e) != 0;
   if (c.g != 1 && c.g != 2 && c.g != 4) c.g = 1;
   return c;
 }
@@ -925,6 +1145,14 @@ void launch_closest(const Bvh &b, const float *q, int64_t n, double dist_bound, 
     uint32_t wnb = ceil_div(n, (uint64_t)W_WAVES * (WAVE / wc.g));
     uint32_t wgrid = round_up8(wnb);
     KernelTimer kt(b.ctx, s, "closest");
+    if (wc.cmp && !b.points) {
+      hipLaunchKernelGGL(k_closest_cmp<false>, dim3(round_up8(ceil_div(n, (uint64_t)W_BLOCK))), dim3(W_BLOCK), 0, s,
+                         b.nodes.as<BNode>(), b.elems.as<TriRec>(), q, ws.order, n, (uint32_t)ceil_div(n, (uint64_t)W_BLOCK),
+                         best, b.scale, exclude_self, idx, cp, dist, d2, nullptr);
+      launch_check("k_closest_cmp");
+      kt.stop();
+      return;
+    }
 #define THG_CW(P, G, PRE)                                                                                              \
   hipLaunchKernelGGL((k_closest_wave<P, G, PRE>), dim3(wgrid), dim3(W_BLOCK), 0, s, b.nodes.as<BNode>(), b.elems.p, q, \
                      ws.order, n, wnb, best, b.scale, exclude_self, idx, cp, dist, d2)
@@ -1012,7 +1240,10 @@ void closest_wave_stats(const Bvh &b, const float *q, int64_t n, unsigned long l
   uint32_t wnb = ceil_div(n, W_BLOCK);
   uint32_t wgrid = round_up8(wnb);
   float best = INFINITY;
-  if (b.points)
+  if (!b.points && wave_config(b).cmp)
+    hipLaunchKernelGGL(k_closest_cmp<true>, dim3(wgrid), dim3(W_BLOCK), 0, s, b.nodes.as<BNode>(), b.elems.as<TriRec>(),
+                       q, ws.order, n, wnb, best, b.scale, 0, idx.as<int32_t>(), nullptr, nullptr, nullptr, stats);
+  else if (b.points)
     hipLaunchKernelGGL((k_closest_wave<true, 1, false, true>), dim3(wgrid), dim3(W_BLOCK), 0, s, b.nodes.as<BNode>(),
                        b.elems.p, q, ws.order, n, wnb, best, b.scale, 0, idx.as<int32_t>(), nullptr, nullptr, nullptr, stats);
   else
