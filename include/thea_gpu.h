@@ -154,6 +154,12 @@ int thg_ctx_kernel_time(thg_ctx ctx, const char *kernel, double *total_ms, int64
  * compacted kernel). */
 int thg_closest_stats(thg_bvh bvh, const float *q, int64_t n, void *stream, int64_t out[8]);
 
+/* Per-ray traversal counts of the first-hit kernel (diagnostics), rays in input order (device
+ * pointers, asynchronous on `stream`): idx[i] as thg_ray_first_hit, counts[4i..4i+3] = nodes
+ * visited, triangles tested, deepest traversal stack, stack pops. */
+int thg_ray_stats(thg_bvh bvh, const float *origins, const float *dirs, int64_t n, void *stream, int32_t *idx,
+                  int32_t *counts);
+
 /* Many meshes, one call: build a BVH for each of `nm` meshes and answer `nq[m]` closest-point
  * queries for mesh m (Thea has no analogue: each BvhN is built and queried by its own caller).
  * Inputs are concatenated: verts (sum nv float3), tris (sum nt int32 triplets, vertex indices local

@@ -316,6 +316,18 @@ int thg_closest_stats(thg_bvh bvh, const float *q, int64_t n, void *stream, int6
   });
 }
 
+int thg_ray_stats(thg_bvh bvh, const float *origins, const float *dirs, int64_t n, void *stream, int32_t *idx,
+                  int32_t *counts) {
+  return guarded([&] {
+    require(bvh != nullptr, "bvh is NULL");
+    require(n >= 0 && (n == 0 || (origins && dirs && idx && counts)), "bad ray or output array");
+    Bvh *b = reinterpret_cast<Bvh *>(bvh);
+    require(!b->points, "ray queries need a triangle BVH");
+    set_device(b->ctx);
+    ray_stats(*b, origins, dirs, n, idx, counts, as_stream(stream));
+  });
+}
+
 int thg_k_closest(thg_bvh bvh, const float *q, int64_t n, int32_t k, double dist_bound, int32_t exclude_self,
                   int32_t *idx, float *dist, float *d2, int32_t mem, void *stream) {
   return guarded([&] {

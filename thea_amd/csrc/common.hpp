@@ -174,7 +174,7 @@ struct Bvh {
   int32_t leaf_size = 4;
   int32_t wave_g = 1;    // lanes per query of the wave-cooperative kernels
   bool wave_pre = true;   // per-element box prefilter in the wave-cooperative kernels
-  bool wave_cmp = false;  // compacted (lane, triangle) evaluation queue
+  bool wave_cmp = true;   // compacted (lane, triangle) evaluation queue
   float scale = 0;       // max |coordinate| over the elements
   float3 lo{}, hi{};     // scene bounds (element boxes)
   DevBuf nodes;          // BNode[num_nodes]

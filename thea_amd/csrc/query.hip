@@ -617,9 +617,16 @@ __global__ void __launch_bounds__(W_BLOCK) k_closest_multi(const SegDesc *__rest
   if (lb_blk >= nb_real || gw >= (int64_t)total_waves) return;
   const int si = uni(find_segment(seg, nseg, gw));
   const SegDesc &d = seg[si];
-  closest_wave_body<POINTS, 1, true, false>((const BNode *)d.nodes, d.elems, d.q, d.order, d.n,
+  if constexpr (POINTS) {
+    closest_wave_body<true, 1, true, false>((const BNode *)d.nodes, d.elems, d.q, d.order, d.n,
                                             gw - (int64_t)d.wave_begin, true, d.best_init, d.scale, d.exclude_self,
                                             d.idx, d.cp, d.dist, d.d2, nullptr, d.root);
+  } else {
+    __shared__ CmpWave s_w[W_WAVES];
+    closest_cmp_body<false>(s_w[threadIdx.x / WAVE], (const BNode *)d.nodes, (const TriRec *)d.elems, d.q, d.order,
+                            d.n, gw - (int64_t)d.wave_begin, true, d.best_init, d.scale, d.exclude_self, d.idx, d.cp,
+                            d.dist, d.d2, nullptr, d.root);
+  }
 }
 
 //--------------------------------------------------------------------------------------------------
@@ -910,13 +917,15 @@ __device__ __forceinline__ float2 slab(float3 o, float3 inv, float w, float lx, 
 }
 
 // ANY = true: stop at the first accepted hit (any-hit query).
-template <int STACK, bool ANY>
+template <int STACK, bool ANY, bool STATS = false>
 __device__ __forceinline__ void ray_body(int32_t *__restrict__ s_ref, float *__restrict__ s_t,
                                          const BNode *__restrict__ nodes, const TriRec *__restrict__ tr,
                                          const float *__restrict__ orig, const float *__restrict__ dirs, int64_t ri,
                                          float best_init, float max_time, float scale, int32_t *__restrict__ out_idx,
                                          float *__restrict__ out_t, float *__restrict__ out_n,
-                                         uint8_t *__restrict__ out_any, int32_t root = 0) {
+                                         uint8_t *__restrict__ out_any, int32_t root = 0,
+                                         int32_t *__restrict__ counts = nullptr) {
+  int32_t c_nodes = 0, c_tris = 0, c_maxsp = 0, c_pops = 0;
   const int tid = threadIdx.x;
   float3 o = load_q(orig, ri), d = load_q(dirs, ri);
   float3 inv = f3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
@@ -931,16 +940,20 @@ __device__ __forceinline__ void ray_body(int32_t *__restrict__ s_ref, float *__r
   const float REL = 1.0f + 1.0f / 4096.0f;
   bool done = false;
   while (!done) {
+    if (STATS) c_nodes++;
     const BNode nd = nodes[node];
     float2 s0 = slab(o, inv, w, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y);
     float2 s1 = slab(o, inv, w, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w);
+    // A child is visited only if the ray meets its widened box (h) and enters it no later than the
+    // best hit so far; an unbounded best must not admit a box the ray misses.
     bool h0 = s0.x <= s0.y * REL, h1 = s1.x <= s1.y * REL;
-    float t0 = h0 ? s0.x : INFINITY, t1 = h1 ? s1.x : INFINITY;
+    float t0 = s0.x, t1 = s1.x;
     int32_t r0 = nd.d.x, r1 = nd.d.y, c0 = nd.d.z, c1 = nd.d.w;
-    if (t1 < t0) {
+    if (h1 && (!h0 || t1 < t0)) {
       float tt = t0; t0 = t1; t1 = tt;
       int32_t a = r0; r0 = r1; r1 = a;
       int32_t b = c0; c0 = c1; c1 = b;
+      bool hh = h0; h0 = h1; h1 = hh;
     }
     int32_t next = -1;
 #pragma unroll
@@ -948,11 +961,12 @@ __device__ __forceinline__ void ray_body(int32_t *__restrict__ s_ref, float *__r
       float tn = c == 0 ? t0 : t1;
       int32_t r = c == 0 ? r0 : r1;
       int32_t cnt = c == 0 ? c0 : c1;
-      if (!(tn <= best * REL)) continue;
+      if (!((c == 0 ? h0 : h1) && tn <= best * REL)) continue;
       if (r < 0) {
         int32_t f = ~r;
         for (int32_t k = f; k < f + cnt; ++k) {
           const TriRec *t = tr + k;
+          if (STATS) c_tris++;
           float4 a = t->p0, b = t->p1, cc = t->p2;
           float th = tri_ray(f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), f3(cc.x, cc.y, cc.z), o, d, max_time);
           if (th >= 0) {
@@ -972,6 +986,7 @@ __device__ __forceinline__ void ray_body(int32_t *__restrict__ s_ref, float *__r
         s_ref[sp * Q_BLOCK + tid] = r;
         s_t[sp * Q_BLOCK + tid] = tn;
         ++sp;
+        if (STATS) c_maxsp = max(c_maxsp, sp);
       }
     }
     if (done) break;
@@ -979,9 +994,16 @@ __device__ __forceinline__ void ray_body(int32_t *__restrict__ s_ref, float *__r
     bool found = false;
     while (sp > 0) {
       --sp;
+      if (STATS) c_pops++;
       if (s_t[sp * Q_BLOCK + tid] <= best * REL) { node = s_ref[sp * Q_BLOCK + tid]; found = true; break; }
     }
     if (!found) break;
+  }
+  if (STATS) {
+    counts[4 * ri] = c_nodes;
+    counts[4 * ri + 1] = c_tris;
+    counts[4 * ri + 2] = c_maxsp;
+    counts[4 * ri + 3] = c_pops;
   }
   bool hit = bidx != 0x7fffffff;
   if (ANY) {
@@ -1011,6 +1033,20 @@ __global__ void __launch_bounds__(Q_BLOCK) k_ray(const BNode *__restrict__ nodes
   if (!active) return;
   ray_body<STACK, ANY>(s_ref, s_t, nodes, tr, orig, dirs, ri, best_init, max_time, scale, out_idx, out_t, out_n,
                        out_any);
+}
+
+// Per-ray traversal counts (diagnostics): nodes visited, triangles tested, deepest stack, pops.
+template <int STACK>
+__global__ void __launch_bounds__(Q_BLOCK) k_ray_stats(const BNode *__restrict__ nodes, const TriRec *__restrict__ tr,
+                                                       const float *__restrict__ orig, const float *__restrict__ dirs,
+                                                       int64_t n, float scale, int32_t *__restrict__ out_idx,
+                                                       int32_t *__restrict__ counts) {
+  __shared__ int32_t s_ref[STACK * Q_BLOCK];
+  __shared__ float s_t[STACK * Q_BLOCK];
+  const int64_t ri = (int64_t)blockIdx.x * Q_BLOCK + threadIdx.x;
+  if (ri >= n) return;
+  ray_body<STACK, false, true>(s_ref, s_t, nodes, tr, orig, dirs, ri, INFINITY, -1.0f, scale, out_idx, nullptr,
+                               nullptr, nullptr, 0, counts);
 }
 
 // Many ray batches in one launch; one 64-lane block per wave of a segment (SEG_WAVE_QUERIES rays).
@@ -1230,6 +1266,17 @@ void launch_ray_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, hi
   uint32_t grid = round_up8(nb);
   hipLaunchKernelGGL(k_ray_multi<64>, dim3(grid), dim3(Q_BLOCK), 0, s, seg_dev, nseg, total_waves, nb);
   launch_check("k_ray_multi");
+}
+
+void ray_stats(const Bvh &b, const float *o, const float *d, int64_t n, int32_t *idx, int32_t *counts, hipStream_t s) {
+  if (n <= 0 || b.n == 0) return;
+  const uint32_t nb = ceil_div(n, Q_BLOCK);
+  const int st = pick_stack(b.max_depth);
+#define THG_RS(S) hipLaunchKernelGGL(k_ray_stats<S>, dim3(nb), dim3(Q_BLOCK), 0, s, b.nodes.as<BNode>(), \
+                                     b.elems.as<TriRec>(), o, d, n, b.scale, idx, counts)
+  if (st <= 32) THG_RS(32); else if (st == 64) THG_RS(64); else THG_RS(128);
+#undef THG_RS
+  launch_check("k_ray_stats");
 }
 
 void closest_wave_stats(const Bvh &b, const float *q, int64_t n, unsigned long long *stats, QueryScratch &ws,

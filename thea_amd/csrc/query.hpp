@@ -62,6 +62,7 @@ void launch_closest_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves
 void launch_knn_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, int k, bool points, hipStream_t s);
 void launch_ray_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, hipStream_t s);
 constexpr int SEG_WAVE_QUERIES = 64;  // queries per wave in the multi-batch kernels
+void ray_stats(const Bvh &b, const float *o, const float *d, int64_t n, int32_t *idx, int32_t *counts, hipStream_t s);
 void closest_wave_stats(const Bvh &b, const float *q, int64_t n, unsigned long long *stats, QueryScratch &ws,
                         hipStream_t s);
 void launch_knn(const Bvh &b, const float *q, int64_t n, int k, double dist_bound, int exclude_self, int32_t *idx,
