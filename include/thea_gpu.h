@@ -90,6 +90,12 @@ int thg_bvh_destroy(thg_bvh bvh);
 /* Statistics of a built BVH: out[0] nodes, [1] leaves, [2] max depth, [3] elements,
  * [4] device bytes held by the BVH, [5] leaf size. */
 int thg_bvh_info(thg_bvh bvh, int64_t out[6]);
+/* Copy a built BVH to host memory (diagnostics and tests; synchronous). nodes (nullable):
+ * info[0] 64-byte nodes, each {child 0 box lo xyz, hi xyz, child 1 box lo xyz, hi xyz (floats),
+ * ref0, ref1, count0, count1 (int32)}; ref >= 0 is a node index, ref < 0 a leaf holding elements
+ * [~ref, ~ref + count) of the leaf order. elem_ids (nullable): info[3] caller indices in leaf order.
+ * Node 0 is the root. */
+int thg_bvh_export(thg_bvh bvh, void *nodes, int32_t *elem_ids);
 
 /* Closest element and closest point for n query points `q` (float3).
  * Replaces BvhN::closestElement<MetricL2>(query, dist_bound, compat, &dist, &closest_point)

@@ -160,6 +160,15 @@ class _BvhBase:
         self.info = {"nodes": info[0], "leaves": info[1], "max_depth": info[2], "elements": info[3],
                      "device_bytes": info[4], "leaf_size": info[5]}
 
+    def export(self):
+        """(floats, ints, elem_ids): the nodes as float32 (nodes, 16) and int32 (nodes, 16) views
+        (child 0 box in floats 0-5, child 1 box in 6-11; refs and counts in ints 12-15) and the caller
+        indices of the elements in leaf order."""
+        raw = np.zeros((self.info["nodes"], 16), np.int32)
+        ids = np.zeros(self.info["elements"], np.int32)
+        capi.check(capi.lib().thg_bvh_export(self.h, raw.ctypes.data_as(C.c_void_p), ids.ctypes.data_as(C.c_void_p)))
+        return raw.view(np.float32), raw, ids
+
     def close(self):
         if getattr(self, "h", None):
             capi.lib().thg_bvh_destroy(self.h)

@@ -50,7 +50,7 @@ struct BatchPlan {
   DevBuf d_info, d_tchunks, d_qchunks, mbounds;
   DevBuf center, blo, bhi, keys, vals, tk, tv, hist;
   DevBuf left, right, rfirst, rlast, nparent, lparent, flags, nbox, keep, fidx, stmp;
-  DevBuf nodes, recs, seg, out_idx, out_dist, counters;
+  DevBuf nodes, recs, seg, out_idx, out_dist, counters, rent, rcount;
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
   hipStream_t stream = nullptr;  // plan-owned: graphs cannot be captured on the null stream
@@ -341,8 +341,9 @@ static void plan_pipeline(BatchPlan &P, hipStream_t s, hipEvent_t mid = nullptr)
                        P.nparent.as<int32_t>(), P.lparent.as<int32_t>());
     launch_check("k_bkarras");
     launch_refit(P.NT, P.vals.as<uint32_t>(), P.blo.as<float4>(), P.bhi.as<float4>(), P.left.as<int32_t>(),
-                 P.right.as<int32_t>(), P.nparent.as<int32_t>(), P.lparent.as<int32_t>(), P.flags.as<uint32_t>(),
-                 P.nbox.as<float>(), s);
+                 P.right.as<int32_t>(), P.rfirst.as<int32_t>(), P.rlast.as<int32_t>(), P.nparent.as<int32_t>(),
+                 P.lparent.as<int32_t>(), P.flags.as<uint32_t>(), P.nbox.as<float>(), P.rent.as<int32_t>(),
+                 P.rcount.as<uint32_t>(), s);
     if (P.NI > 0) {
       hipLaunchKernelGGL(k_bkeep, dim3(ceil_div(P.NI, BB)), dim3(BB), 0, s, P.rfirst.as<int32_t>(), P.rlast.as<int32_t>(),
                          mi, nm, P.NI, P.keep.as<uint32_t>());
@@ -458,6 +459,8 @@ int thg_batch_plan_create(thg_ctx ctx, int32_t nm, const int64_t *nv, const int6
     P->out_idx.alloc(std::max<int64_t>(qo, 1) * 4);
     P->out_dist.alloc(std::max<int64_t>(qo, 1) * 4);
     P->counters.alloc(16);
+    P->rent.alloc(T1 * 4);
+    P->rcount.alloc(4);
     THG_HIP(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
     *out = reinterpret_cast<thg_batch_plan>(P.release());
   });

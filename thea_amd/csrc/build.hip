@@ -187,6 +187,117 @@ __global__ void __launch_bounds__(B_BLOCK) k_refit(int64_t n, const uint32_t *__
   }
 }
 
+// Two-pass refit. Pass 1 (k_refit_span): a block owns REFIT_SPAN consecutive sorted positions and
+// computes every internal node whose element range lies inside them. A node's split position g
+// (left child = [first, g], right child = [g + 1, last]) is unique, so LDS slot g - span start
+// holds the node's arrival counter and one box per side; the second child to arrive merges, with
+// workgroup-scope ordering only. Node boxes go to global memory as they are computed. A child whose
+// parent straddles the span appends the parent to `entries`. Pass 2 (k_refit_top) finishes the
+// straddling nodes with the device-wide arrival protocol of k_refit; there are O(depth) of them per
+// span boundary, so the costly device-scope fences run for a small fraction of the nodes.
+constexpr int REFIT_SPAN = 1024;
+constexpr int REFIT_THREADS = 256;
+
+__device__ __forceinline__ void elem_box(const uint32_t *__restrict__ sorted, const float4 *__restrict__ blo,
+                                         const float4 *__restrict__ bhi, int64_t pos, float b[6]) {
+  const uint32_t e = sorted[pos];
+  const float4 a = blo[e], c = bhi[e];
+  b[0] = a.x; b[1] = a.y; b[2] = a.z; b[3] = c.x; b[4] = c.y; b[5] = c.z;
+}
+
+__global__ void __launch_bounds__(REFIT_THREADS) k_refit_span(int64_t n, const uint32_t *__restrict__ sorted,
+                                                              const float4 *__restrict__ blo,
+                                                              const float4 *__restrict__ bhi,
+                                                              const int32_t *__restrict__ rfirst,
+                                                              const int32_t *__restrict__ rlast,
+                                                              const int32_t *__restrict__ node_parent,
+                                                              const int32_t *__restrict__ leaf_parent, float *nbox,
+                                                              int32_t *__restrict__ entries,
+                                                              uint32_t *__restrict__ nentries) {
+  __shared__ float s_l[REFIT_SPAN * 6], s_r[REFIT_SPAN * 6];
+  __shared__ uint32_t s_flag[REFIT_SPAN];
+  const int64_t lo = (int64_t)blockIdx.x * REFIT_SPAN;
+  const int64_t hi = lo + REFIT_SPAN < n ? lo + REFIT_SPAN : n;
+  for (int i = threadIdx.x; i < REFIT_SPAN; i += REFIT_THREADS) s_flag[i] = 0;
+  __syncthreads();
+  for (int64_t k = lo + threadIdx.x; k < hi; k += REFIT_THREADS) {
+    float b[6];
+    elem_box(sorted, blo, bhi, k, b);
+    int64_t cf = k, cl = k;  // element range of the completed child
+    int32_t p = leaf_parent[k];
+    while (p >= 0) {
+      const int64_t fp = rfirst[p], lp = rlast[p];
+      if (fp < lo || lp >= hi) {
+        entries[atomicAdd(nentries, 1u)] = p;
+        break;
+      }
+      const bool is_left = cf == fp;
+      const int slot = (int)((is_left ? cl : cf - 1) - lo);
+      float *mine = (is_left ? s_l : s_r) + 6 * slot;
+#pragma unroll
+      for (int a = 0; a < 6; ++a) mine[a] = b[a];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      if (atomicAdd(&s_flag[slot], 1u) == 0) break;  // the sibling's thread continues
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      const float *other = (is_left ? s_r : s_l) + 6 * slot;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        b[a] = fminf(b[a], other[a]);
+        b[3 + a] = fmaxf(b[3 + a], other[3 + a]);
+      }
+      float *o = nbox + 6 * (int64_t)p;
+#pragma unroll
+      for (int a = 0; a < 6; ++a) o[a] = b[a];
+      cf = fp;
+      cl = lp;
+      p = node_parent[p];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(B_BLOCK) k_refit_top(const uint32_t *__restrict__ sorted,
+                                                       const float4 *__restrict__ blo, const float4 *__restrict__ bhi,
+                                                       const int32_t *__restrict__ left,
+                                                       const int32_t *__restrict__ right,
+                                                       const int32_t *__restrict__ node_parent,
+                                                       const int32_t *__restrict__ entries,
+                                                       const uint32_t *__restrict__ nentries, uint32_t *flags,
+                                                       float *nbox) {
+  const uint32_t count = *nentries;
+  for (uint32_t e = blockIdx.x * B_BLOCK + threadIdx.x; e < count; e += gridDim.x * B_BLOCK) {
+    int32_t cur = entries[e];
+    while (cur >= 0) {
+      __threadfence();
+      if (atomicAdd(&flags[cur], 1u) == 0) break;  // first arrival; the other child's thread continues
+      __threadfence();
+      float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      const int32_t ch[2] = {left[cur], right[cur]};
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        float b[6];
+        if (ch[c] < 0) {
+          elem_box(sorted, blo, bhi, ch[c] & 0x7fffffff, b);
+        } else {
+          const float *q = nbox + 6 * (int64_t)ch[c];
+#pragma unroll
+          for (int a = 0; a < 6; ++a) b[a] = ld_coherent(q + a);
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          lo[a] = fminf(lo[a], b[a]);
+          hi[a] = fmaxf(hi[a], b[3 + a]);
+        }
+      }
+      float *o = nbox + 6 * (int64_t)cur;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) { st_coherent(o + a, lo[a]); st_coherent(o + 3 + a, hi[a]); }
+      cur = node_parent[cur];
+    }
+  }
+}
+
+__global__ void k_zero_u32(uint32_t *p) { *p = 0; }
+
 // keep[i] = 1 if internal node i stays an internal node of the collapsed tree (range > leaf_size).
 __global__ void __launch_bounds__(B_BLOCK) k_keep(int64_t n, const int32_t *__restrict__ rfirst,
                                                   const int32_t *__restrict__ rlast, int leaf_size,
@@ -376,13 +487,36 @@ void scan_excl_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *tmp, 
 
 // Entry points for the batched many-mesh pipeline (batch.hip), which reuses these kernels over
 // concatenated meshes with global element positions and node indices.
+static bool refit_atomic_only() {
+  static const bool v = [] {
+    const char *e = getenv("THG_REFIT");
+    return e && atoi(e) == 1;
+  }();
+  return v;
+}
+
+// `flags` (one per internal node) must be zero; `entries` holds n int32, `nentries` one uint32.
 void launch_refit(int64_t n, const uint32_t *sorted, const float4 *blo, const float4 *bhi, const int32_t *left,
-                  const int32_t *right, const int32_t *nparent, const int32_t *lparent, uint32_t *flags, float *nbox,
+                  const int32_t *right, const int32_t *rfirst, const int32_t *rlast, const int32_t *nparent,
+                  const int32_t *lparent, uint32_t *flags, float *nbox, int32_t *entries, uint32_t *nentries,
                   hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_refit, dim3(ceil_div(n, B_BLOCK)), dim3(B_BLOCK), 0, s, n, sorted, blo, bhi, left, right, nparent,
-                     lparent, flags, nbox);
-  launch_check("k_refit");
+  if (refit_atomic_only()) {
+    hipLaunchKernelGGL(k_refit, dim3(ceil_div(n, B_BLOCK)), dim3(B_BLOCK), 0, s, n, sorted, blo, bhi, left, right,
+                       nparent, lparent, flags, nbox);
+    launch_check("k_refit");
+    return;
+  }
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, nentries);
+  launch_check("k_zero_u32");
+  const uint32_t spans = ceil_div(n, (int64_t)REFIT_SPAN);
+  hipLaunchKernelGGL(k_refit_span, dim3(spans), dim3(REFIT_THREADS), 0, s, n, sorted, blo, bhi, rfirst, rlast, nparent,
+                     lparent, nbox, entries, nentries);
+  launch_check("k_refit_span");
+  const uint32_t grid = std::min<uint32_t>(std::max<uint32_t>(spans / 2, 1), 4096);
+  hipLaunchKernelGGL(k_refit_top, dim3(grid), dim3(B_BLOCK), 0, s, sorted, blo, bhi, left, right, nparent, entries,
+                     nentries, flags, nbox);
+  launch_check("k_refit_top");
 }
 
 void launch_emit(int64_t ni, int32_t base, const uint32_t *keep, const uint32_t *fidx, const int32_t *left,
@@ -467,16 +601,16 @@ void build_bvh(Bvh &b, const float *verts, const int32_t *tris, int64_t n, bool 
   int64_t ni = n - 1;
   DevBuf left(ni * 4), right(ni * 4), rfirst(ni * 4), rlast(ni * 4), nparent(ni * 4), lparent(n * 4);
   DevBuf flags(ni * 4), nbox(ni * 6 * sizeof(float)), keep(ni * 4), fidx((ni + 1) * 4), counters(16);
+  DevBuf entries(n * 4), nentries(4);
   THG_HIP(hipMemsetAsync(flags.p, 0, ni * 4, s));
   THG_HIP(hipMemsetAsync(counters.p, 0, 16, s));
   hipLaunchKernelGGL(k_karras, dim3(ceil_div(ni, B_BLOCK)), dim3(B_BLOCK), 0, s, keys.as<uint32_t>(), n,
                      left.as<int32_t>(), right.as<int32_t>(), rfirst.as<int32_t>(), rlast.as<int32_t>(),
                      nparent.as<int32_t>(), lparent.as<int32_t>());
   launch_check("k_karras");
-  hipLaunchKernelGGL(k_refit, dim3(ceil_div(n, B_BLOCK)), dim3(B_BLOCK), 0, s, n, sorted, blo.as<float4>(),
-                     bhi.as<float4>(), left.as<int32_t>(), right.as<int32_t>(), nparent.as<int32_t>(),
-                     lparent.as<int32_t>(), flags.as<uint32_t>(), nbox.as<float>());
-  launch_check("k_refit");
+  launch_refit(n, sorted, blo.as<float4>(), bhi.as<float4>(), left.as<int32_t>(), right.as<int32_t>(),
+               rfirst.as<int32_t>(), rlast.as<int32_t>(), nparent.as<int32_t>(), lparent.as<int32_t>(),
+               flags.as<uint32_t>(), nbox.as<float>(), entries.as<int32_t>(), nentries.as<uint32_t>(), s);
   hipLaunchKernelGGL(k_keep, dim3(ceil_div(ni, B_BLOCK)), dim3(B_BLOCK), 0, s, n, rfirst.as<int32_t>(),
                      rlast.as<int32_t>(), leaf_size, keep.as<uint32_t>());
   launch_check("k_keep");

@@ -255,6 +255,29 @@ int thg_bvh_info(thg_bvh bvh, int64_t out[6]) {
   });
 }
 
+__global__ void k_record_ids(int64_t n, const void *__restrict__ elems, int points, int32_t *__restrict__ ids) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  ids[i] = points ? __float_as_int(((const PointRec *)elems)[i].p.w) : trirec_id(((const TriRec *)elems)[i].pl);
+}
+
+int thg_bvh_export(thg_bvh bvh, void *nodes, int32_t *elem_ids) {
+  return guarded([&] {
+    require(bvh != nullptr, "bvh is NULL");
+    Bvh *b = reinterpret_cast<Bvh *>(bvh);
+    set_device(b->ctx);
+    if (nodes && b->num_nodes > 0)
+      THG_HIP(hipMemcpy(nodes, b->nodes.p, (size_t)b->num_nodes * sizeof(BNode), hipMemcpyDeviceToHost));
+    if (elem_ids && b->n > 0) {
+      DevBuf ids(b->n * 4);
+      hipLaunchKernelGGL(k_record_ids, dim3(ceil_div(b->n, 256)), dim3(256), 0, 0, b->n, b->elems.p, b->points ? 1 : 0,
+                         ids.as<int32_t>());
+      launch_check("k_record_ids");
+      THG_HIP(hipMemcpy(elem_ids, ids.p, b->n * 4, hipMemcpyDeviceToHost));
+    }
+  });
+}
+
 // Fill per-query "not found" outputs for an empty BVH.
 __global__ void k_fill_missing(int64_t n, int k, int32_t *idx, float *a, float *b, float *cp3) {
   int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;

@@ -45,7 +45,8 @@ void launch_closest(const Bvh &b, const float *q, int64_t n, double dist_bound, 
                     float *cp, float *dist, float *d2, QueryScratch &ws, hipStream_t s);
 // Build-pipeline pieces reused by the batched many-mesh plan (batch.hip).
 void launch_refit(int64_t n, const uint32_t *sorted, const float4 *blo, const float4 *bhi, const int32_t *left,
-                  const int32_t *right, const int32_t *nparent, const int32_t *lparent, uint32_t *flags, float *nbox,
+                  const int32_t *right, const int32_t *rfirst, const int32_t *rlast, const int32_t *nparent,
+                  const int32_t *lparent, uint32_t *flags, float *nbox, int32_t *entries, uint32_t *nentries,
                   hipStream_t s);
 void launch_emit(int64_t ni, int32_t base, const uint32_t *keep, const uint32_t *fidx, const int32_t *left,
                  const int32_t *right, const int32_t *rfirst, const int32_t *rlast, const uint32_t *sorted,
