@@ -448,7 +448,7 @@ def run_c5(args):
 
     def server_run(arrivals):
         srv.stats(reset=True)
-        rids = [None] * R
+        rids = [None] * len(arrivals)
         t_start = time.perf_counter()
 
         def submitter():

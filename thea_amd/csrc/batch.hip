@@ -33,7 +33,9 @@ struct MeshInfo {  // per mesh, device side
   int64_t voff, toff, qoff, ioff;
   int64_t nt, nq;
   int32_t leaf, small_slot;  // small_slot >= 0: index of its single-leaf root node
-  uint64_t wave_begin;
+  uint64_t wave_begin;       // first 64-query wave of the mesh within its traversal group
+  int32_t seg_slot;          // row of the segment table (wave-cooperative meshes first)
+  int32_t lane;              // 1: one query per lane (sparse queries), 0: wave-cooperative
 };
 
 struct BatchPlan {
@@ -44,7 +46,8 @@ struct BatchPlan {
   std::vector<MeshInfo> info;
   int64_t NV = 0, NT = 0, NQ = 0, NI = 0, KMAX = 0;
   int32_t nsmall = 0;
-  uint64_t total_waves = 0;
+  uint64_t total_waves = 0, lane_waves = 0;
+  int32_t nwave_meshes = 0;
   std::vector<int4> tchunks, qchunks;
   DevBuf verts, tris, queries;
   DevBuf d_info, d_tchunks, d_qchunks, mbounds;
@@ -298,7 +301,8 @@ __global__ void k_bsegs(const MeshInfo *__restrict__ mi, int nm, const int *__re
   d.best_init = INFINITY;
   d.scale = sc;
   d.root = root;
-  seg[m] = d;
+  d.nelem = (int32_t)x.nt;
+  seg[x.seg_slot] = d;
 }
 
 // Queries of empty meshes get "not found".
@@ -374,8 +378,20 @@ static void plan_pipeline(BatchPlan &P, hipStream_t s, hipEvent_t mid = nullptr)
     hipLaunchKernelGGL(k_bempty, dim3(ceil_div(P.NQ, BB)), dim3(BB), 0, s, mi, nm, P.NQ, P.out_idx.as<int32_t>(),
                        P.out_dist.as<float>());
     launch_check("k_bempty");
-    launch_closest_multi(P.seg.as<SegDesc>(), nm, P.total_waves, false, s);
+    launch_closest_multi(P.seg.as<SegDesc>(), P.nwave_meshes, P.total_waves, false, s);
+    launch_closest_lane_multi(P.seg.as<SegDesc>() + P.nwave_meshes, nm - P.nwave_meshes, P.lane_waves, false, 64, s);
   }
+}
+
+// Sparse meshes of a plan walk one query per lane when THG_BATCH_LANE=1 (default: every mesh uses
+// the wave-cooperative kernel; the per-lane kernel must reserve a 64-entry stack here because the
+// depth is known only after the captured build).
+static bool batch_lane_split() {
+  static const bool v = [] {
+    const char *e = getenv("THG_BATCH_LANE");
+    return e && atoi(e) == 1;
+  }();
+  return v;
 }
 
 // Leaf sizes chosen by measurement per size class (floor(log2(triangles))), cached for the process.
@@ -419,7 +435,7 @@ int thg_batch_plan_create(thg_ctx ctx, int32_t nm, const int64_t *nv, const int6
     require(P->mesh_bits <= 14, "too many meshes in one batch (at most 16384)");
     P->info.resize(nm);
     int64_t vo = 0, to = 0, qo = 0, io = 0;
-    uint64_t waves = 0;
+    uint64_t waves = 0, lwaves = 0;
     for (int m = 0; m < nm; ++m) {
       require(nv[m] >= 0 && nt[m] >= 0 && nq[m] >= 0, "negative count");
       int leaf = leaf_sizes ? leaf_sizes[m] : 4;
@@ -428,11 +444,17 @@ int thg_batch_plan_create(thg_ctx ctx, int32_t nm, const int64_t *nv, const int6
       x.voff = vo; x.toff = to; x.qoff = qo; x.ioff = io;
       x.nt = nt[m]; x.nq = nq[m]; x.leaf = leaf;
       x.small_slot = (nt[m] >= 1 && nt[m] <= leaf) ? P->nsmall++ : -1;
-      x.wave_begin = waves;
-      if (nt[m] > 0) waves += (uint64_t)((nq[m] + SEG_WAVE_QUERIES - 1) / SEG_WAVE_QUERIES);
+      // The single-call rule (use_wave_traversal): wave-cooperative when there is at least one query
+      // per 8 leaves; a collapsed LBVH has about 1.5 * triangles / leaf_size leaves.
+      const int64_t leaves = (3 * nt[m]) / (2 * leaf) + 1;
+      x.lane = batch_lane_split() && 8 * nq[m] < leaves ? 1 : 0;
+      const uint64_t w = nt[m] > 0 ? (uint64_t)((nq[m] + SEG_WAVE_QUERIES - 1) / SEG_WAVE_QUERIES) : 0;
+      if (x.lane) { x.wave_begin = lwaves; lwaves += w; }
+      else { x.wave_begin = waves; waves += w; P->nwave_meshes++; }
       vo += nv[m]; to += nt[m]; qo += nq[m]; io += nt[m] >= 2 ? nt[m] - 1 : 0;
     }
-    P->NV = vo; P->NT = to; P->NQ = qo; P->NI = io; P->total_waves = waves;
+    for (int m = 0, iw = 0, il = P->nwave_meshes; m < nm; ++m) P->info[m].seg_slot = P->info[m].lane ? il++ : iw++;
+    P->NV = vo; P->NT = to; P->NQ = qo; P->NI = io; P->total_waves = waves; P->lane_waves = lwaves;
     require(to < (int64_t)0x7ffffff0 && qo < (int64_t)0x7ffffff0, "batch too large for 32-bit indices");
     P->nv.assign(nv, nv + nm); P->nt.assign(nt, nt + nm); P->nq.assign(nq, nq + nm);
     make_chunks(P->nt, P->tchunks);

@@ -93,6 +93,15 @@ int thg_ctx_create(int32_t device, thg_ctx *out) {
     THG_HIP(hipGetDeviceCount(&count));
     require(device >= 0 && device < count, "device index out of range");
     THG_HIP(hipSetDevice(device));
+    // Stream-ordered scratch (hipMallocAsync) comes from the device's default pool; keep freed
+    // blocks in the pool instead of returning them to the driver at every synchronisation, so
+    // repeated calls reuse memory rather than allocate it again.
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+      uint64_t keep = UINT64_MAX;
+      (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
+    (void)hipGetLastError();
     auto *c = new Ctx();
     c->device = device;
     c->num_cus = device_cus(device);

@@ -50,18 +50,14 @@ __device__ __forceinline__ int64_t query_slot(uint32_t nb_real, int64_t n, const
 //--------------------------------------------------------------------------------------------------
 // Closest element
 //--------------------------------------------------------------------------------------------------
+// One query per lane, stack in LDS (entry i of lane t at s_ref[i * Q_BLOCK + t]).
 template <int STACK, bool POINTS>
-__global__ void __launch_bounds__(Q_BLOCK) k_closest(const BNode *__restrict__ nodes, const void *__restrict__ elems,
-                                                     const float *__restrict__ q, const uint32_t *__restrict__ order,
-                                                     int64_t n, uint32_t nb_real, float best_init, float scale,
-                                                     int exclude_self, int32_t *__restrict__ out_idx,
-                                                     float *__restrict__ out_cp, float *__restrict__ out_dist,
-                                                     float *__restrict__ out_d2) {
-  __shared__ int32_t s_ref[STACK * Q_BLOCK];
-  __shared__ float s_lb[STACK * Q_BLOCK];
-  bool active;
-  int64_t qi = query_slot(nb_real, n, order, active);
-  if (!active) return;
+__device__ __forceinline__ void closest_lane_body(int32_t *__restrict__ s_ref, float *__restrict__ s_lb,
+                                                  const BNode *__restrict__ nodes, const void *__restrict__ elems,
+                                                  const float *__restrict__ q, int64_t qi, float best_init,
+                                                  float scale, int exclude_self, int32_t *__restrict__ out_idx,
+                                                  float *__restrict__ out_cp, float *__restrict__ out_dist,
+                                                  float *__restrict__ out_d2, int32_t root) {
   const int tid = threadIdx.x;
   float3 p = load_q(q, qi);
   float m = fmaxf(scale, fmaxf(fabsf(p.x), fmaxf(fabsf(p.y), fabsf(p.z))));
@@ -72,7 +68,7 @@ __global__ void __launch_bounds__(Q_BLOCK) k_closest(const BNode *__restrict__ n
   float3 bcp = f3(0, 0, 0);
   const int32_t self = exclude_self ? (int32_t)qi : -1;
   int sp = 0;
-  int32_t node = 0;
+  int32_t node = root;
   const TriRec *__restrict__ tr = (const TriRec *)elems;
   const PointRec *__restrict__ pr = (const PointRec *)elems;
   while (true) {
@@ -116,7 +112,7 @@ __global__ void __launch_bounds__(Q_BLOCK) k_closest(const BNode *__restrict__ n
         }
       } else if (next < 0) {
         next = r;
-      } else {
+      } else if (sp < STACK) {  // never taken when depth < STACK (checked by every caller)
         s_ref[sp * Q_BLOCK + tid] = r;
         s_lb[sp * Q_BLOCK + tid] = lb;
         ++sp;
@@ -141,6 +137,22 @@ __global__ void __launch_bounds__(Q_BLOCK) k_closest(const BNode *__restrict__ n
   if (out_d2) out_d2[qi] = hit ? best : -1.0f;
 }
 
+template <int STACK, bool POINTS>
+__global__ void __launch_bounds__(Q_BLOCK) k_closest(const BNode *__restrict__ nodes, const void *__restrict__ elems,
+                                                     const float *__restrict__ q, const uint32_t *__restrict__ order,
+                                                     int64_t n, uint32_t nb_real, float best_init, float scale,
+                                                     int exclude_self, int32_t *__restrict__ out_idx,
+                                                     float *__restrict__ out_cp, float *__restrict__ out_dist,
+                                                     float *__restrict__ out_d2) {
+  __shared__ int32_t s_ref[STACK * Q_BLOCK];
+  __shared__ float s_lb[STACK * Q_BLOCK];
+  bool active;
+  const int64_t qi = query_slot(nb_real, n, order, active);
+  if (!active) return;
+  closest_lane_body<STACK, POINTS>(s_ref, s_lb, nodes, elems, q, qi, best_init, scale, exclude_self, out_idx, out_cp,
+                                   out_dist, out_d2, 0);
+}
+
 //--------------------------------------------------------------------------------------------------
 // Closest element, wave-cooperative ("packet") traversal
 //
@@ -161,6 +173,32 @@ __device__ __forceinline__ float unif(float x) { return __int_as_float(__builtin
 __device__ __forceinline__ int32_t lane_read(int32_t x, int32_t l) { return __builtin_amdgcn_readlane(x, l); }
 __device__ __forceinline__ float lane_readf(float x, int32_t l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+
+// Per-lane seed of the wave traversals: descend from `root` to the leaf whose box is nearest the
+// query (no backtracking) and return it as [first, first + count). The elements found there give
+// each lane a real upper bound before the shared walk starts, so a wave whose queries are far apart
+// does not admit every node some other lane has not bounded yet. Depth is below 64 for the wave
+// kernels (use_wave_traversal), which bounds the loop.
+__device__ __forceinline__ void seed_leaf(const BNode *__restrict__ nodes, int32_t root, float3 qlo, float3 qhi,
+                                          int32_t &first, int32_t &count) {
+  int32_t nd_i = root;
+  first = 0;
+  count = 0;
+  for (int it = 0; it < 64; ++it) {
+    const BNode nd = nodes[nd_i];
+    const float l0 = box_lb(qlo, qhi, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y);
+    const float l1 = box_lb(qlo, qhi, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w);
+    const bool one = l1 < l0;
+    const int32_t r = one ? nd.d.y : nd.d.x;
+    if (r >= 0) {
+      nd_i = r;
+      continue;
+    }
+    first = ~r;
+    count = one ? nd.d.w : nd.d.z;
+    return;
+  }
 }
 
 // G lanes share one query (64 / G queries per wave): lane g of a group tests elements g, g + G, ...
@@ -206,6 +244,32 @@ __device__ __forceinline__ void closest_wave_body(const BNode *__restrict__ node
   const int32_t self = exclude_self ? (int32_t)qi : -1;
   const TriRec *__restrict__ tr = (const TriRec *)elems;
   const PointRec *__restrict__ pr = (const PointRec *)elems;
+  if (G == 1 && active) {
+    int32_t f, cnt;
+    seed_leaf(nodes, root, qlo, qhi, f, cnt);
+    for (int32_t e = f; e < f + cnt; ++e) {
+      float d;
+      int32_t id;
+      float3 cp;
+      if (POINTS) {
+        const float4 pe = pr[e].p;
+        cp = f3(pe.x, pe.y, pe.z);
+        id = __float_as_int(pe.w);
+        d = sqn3(sub3(cp, p));
+      } else {
+        const TriRec *tk = tr + e;
+        d = tri_closest(tk->p0, tk->p1, tk->p2, tk->pl, tk->e0, tk->e1, tk->e2, p, cp);
+        id = trirec_id(tk->pl);
+      }
+      if (id != self && better(d, id, own, bidx)) {
+        own = d;
+        bidx = id;
+        bcp = cp;
+      }
+    }
+    best = own;
+    sbest = sqrtf(own);
+  }
   int32_t s_ref = 0;
   float s_lx = 0, s_ly = 0, s_lz = 0, s_hx = 0, s_hy = 0, s_hz = 0;
   int32_t sp = 0;
@@ -313,7 +377,7 @@ __device__ __forceinline__ void closest_wave_body(const BNode *__restrict__ node
         }
       } else if (next < 0) {
         next = r;
-      } else {
+      } else if (sp < W_STACK) {  // never full when depth < 64 (checked by every caller)
         if (lane == sp) {
           s_ref = r;
           if (c == 0) { s_lx = a0; s_ly = a1; s_lz = a2; s_hx = a3; s_hy = b0; s_hz = b1; }
@@ -446,9 +510,29 @@ __device__ __forceinline__ void closest_cmp_body(CmpWave &W, const BNode *__rest
   float best = active ? best_init : -1.0f;  // inactive lanes admit nothing
   float sbest = active ? sqrtf(best_init) : 0.0f;
   const int32_t self = exclude_self ? (int32_t)qi : -1;
+  unsigned long long key0 = cmp_key(active ? best_init : 0.0f, 0x7fffffff);
+  float3 cp0 = f3(0, 0, 0);
+  // best_init < 0 (a zero distance bound) admits nothing; its key must not be compared as bits
+  if (active && best_init >= 0.0f) {
+    int32_t f, cnt;
+    seed_leaf(nodes, root, qlo, qhi, f, cnt);
+    for (int32_t e = f; e < f + cnt; ++e) {
+      const TriRec *tk = tr + e;
+      float3 cp;
+      const float d = tri_closest(tk->p0, tk->p1, tk->p2, tk->pl, tk->e0, tk->e1, tk->e2, p, cp);
+      const int32_t id = trirec_id(tk->pl);
+      const unsigned long long kk = cmp_key(d, id);
+      if (id != self && !isnan(d) && kk < key0) {
+        key0 = kk;
+        cp0 = cp;
+      }
+    }
+    best = cmp_key_d(key0);
+    sbest = sqrtf(best);
+  }
   W.qp[lane] = make_float4(p.x, p.y, p.z, __int_as_float(self));
-  W.key[lane] = cmp_key(active ? best_init : 0.0f, 0x7fffffff);
-  W.qcp[lane] = make_float4(0, 0, 0, 0);
+  W.key[lane] = key0;
+  W.qcp[lane] = make_float4(cp0.x, cp0.y, cp0.z, 0.0f);
   __builtin_amdgcn_wave_barrier();
   int qn = 0;  // queued pairs (wave-uniform)
   int32_t s_ref = 0;
@@ -512,7 +596,7 @@ __device__ __forceinline__ void closest_cmp_body(CmpWave &W, const BNode *__rest
         }
       } else if (next < 0) {
         next = r;
-      } else {
+      } else if (sp < W_STACK) {  // never full when depth < 64 (checked by every caller)
         if (lane == sp) {
           s_ref = r;
           if (c == 0) { s_lx = a0; s_ly = a1; s_lz = a2; s_hx = a3; s_hy = b0; s_hz = b1; }
@@ -645,16 +729,12 @@ __device__ __forceinline__ void knn_insert(float (&kd)[K], int32_t (&ki)[K], flo
 }
 
 template <int STACK, bool POINTS, int K>
-__global__ void __launch_bounds__(Q_BLOCK) k_knn(const BNode *__restrict__ nodes, const void *__restrict__ elems,
-                                                 const float *__restrict__ q, const uint32_t *__restrict__ order,
-                                                 int64_t n, uint32_t nb_real, int k, float best_init, float scale,
-                                                 int exclude_self, int32_t *__restrict__ out_idx,
-                                                 float *__restrict__ out_dist, float *__restrict__ out_d2) {
-  __shared__ int32_t s_ref[STACK * Q_BLOCK];
-  __shared__ float s_lb[STACK * Q_BLOCK];
-  bool active;
-  int64_t qi = query_slot(nb_real, n, order, active);
-  if (!active) return;
+__device__ __forceinline__ void knn_lane_body(int32_t *__restrict__ s_ref, float *__restrict__ s_lb,
+                                              const BNode *__restrict__ nodes, const void *__restrict__ elems,
+                                              const float *__restrict__ q, int64_t qi, int k, float best_init,
+                                              float scale, int exclude_self, int32_t *__restrict__ out_idx,
+                                              float *__restrict__ out_dist, float *__restrict__ out_d2,
+                                              int32_t root) {
   const int tid = threadIdx.x;
   float3 p = load_q(q, qi);
   float m = fmaxf(scale, fmaxf(fabsf(p.x), fmaxf(fabsf(p.y), fabsf(p.z))));
@@ -667,7 +747,7 @@ __global__ void __launch_bounds__(Q_BLOCK) k_knn(const BNode *__restrict__ nodes
   // Only the first k slots are live; the worst live slot is the pruning bound.
   const int32_t self = exclude_self ? (int32_t)qi : -1;
   int sp = 0;
-  int32_t node = 0;
+  int32_t node = root;
   const TriRec *__restrict__ tr = (const TriRec *)elems;
   const PointRec *__restrict__ pr = (const PointRec *)elems;
   // The k-th best pair (slot k - 1): the pruning bound and the admission threshold.
@@ -723,7 +803,7 @@ __global__ void __launch_bounds__(Q_BLOCK) k_knn(const BNode *__restrict__ nodes
         }
       } else if (next < 0) {
         next = r;
-      } else {
+      } else if (sp < STACK) {  // never taken when depth < STACK (checked by every caller)
         s_ref[sp * Q_BLOCK + tid] = r;
         s_lb[sp * Q_BLOCK + tid] = lb;
         ++sp;
@@ -746,6 +826,55 @@ __global__ void __launch_bounds__(Q_BLOCK) k_knn(const BNode *__restrict__ nodes
   }
 }
 
+template <int STACK, bool POINTS, int K>
+__global__ void __launch_bounds__(Q_BLOCK) k_knn(const BNode *__restrict__ nodes, const void *__restrict__ elems,
+                                                 const float *__restrict__ q, const uint32_t *__restrict__ order,
+                                                 int64_t n, uint32_t nb_real, int k, float best_init, float scale,
+                                                 int exclude_self, int32_t *__restrict__ out_idx,
+                                                 float *__restrict__ out_dist, float *__restrict__ out_d2) {
+  __shared__ int32_t s_ref[STACK * Q_BLOCK];
+  __shared__ float s_lb[STACK * Q_BLOCK];
+  bool active;
+  const int64_t qi = query_slot(nb_real, n, order, active);
+  if (!active) return;
+  knn_lane_body<STACK, POINTS, K>(s_ref, s_lb, nodes, elems, q, qi, k, best_init, scale, exclude_self, out_idx,
+                                  out_dist, out_d2, 0);
+}
+
+// Many closest-point or k-NN batches in one launch, one query per lane: one 64-lane block per
+// SEG_WAVE_QUERIES queries of a segment (the segment table of the wave-cooperative multi kernels).
+template <int STACK, bool POINTS>
+__global__ void __launch_bounds__(Q_BLOCK) k_closest_lane_multi(const SegDesc *__restrict__ seg, int nseg,
+                                                                uint64_t total_waves, uint32_t nb_real) {
+  __shared__ int32_t s_ref[STACK * Q_BLOCK];
+  __shared__ float s_lb[STACK * Q_BLOCK];
+  const uint32_t lb_blk = xcd_remap(blockIdx.x, gridDim.x);
+  if (lb_blk >= nb_real || (uint64_t)lb_blk >= total_waves) return;
+  const int si = uni(find_segment(seg, nseg, (int64_t)lb_blk));
+  const SegDesc &d = seg[si];
+  const int64_t t = ((int64_t)lb_blk - (int64_t)d.wave_begin) * Q_BLOCK + threadIdx.x;
+  if (t >= d.n) return;
+  const int64_t qi = d.order ? (int64_t)d.order[t] : t;
+  closest_lane_body<STACK, POINTS>(s_ref, s_lb, (const BNode *)d.nodes, d.elems, d.q, qi, d.best_init, d.scale,
+                                   d.exclude_self, d.idx, d.cp, d.dist, d.d2, d.root);
+}
+
+template <int STACK, bool POINTS, int K>
+__global__ void __launch_bounds__(Q_BLOCK) k_knn_lane_multi(const SegDesc *__restrict__ seg, int nseg,
+                                                            uint64_t total_waves, uint32_t nb_real) {
+  __shared__ int32_t s_ref[STACK * Q_BLOCK];
+  __shared__ float s_lb[STACK * Q_BLOCK];
+  const uint32_t lb_blk = xcd_remap(blockIdx.x, gridDim.x);
+  if (lb_blk >= nb_real || (uint64_t)lb_blk >= total_waves) return;
+  const int si = uni(find_segment(seg, nseg, (int64_t)lb_blk));
+  const SegDesc &d = seg[si];
+  const int64_t t = ((int64_t)lb_blk - (int64_t)d.wave_begin) * Q_BLOCK + threadIdx.x;
+  if (t >= d.n) return;
+  const int64_t qi = d.order ? (int64_t)d.order[t] : t;
+  knn_lane_body<STACK, POINTS, K>(s_ref, s_lb, (const BNode *)d.nodes, d.elems, d.q, qi, d.k, d.best_init, d.scale,
+                                  d.exclude_self, d.idx, d.dist, d.d2, d.root);
+}
+
 //--------------------------------------------------------------------------------------------------
 // k nearest, wave-cooperative: the traversal of k_closest_wave with a per-lane sorted list of K
 // (d^2, id) pairs in registers; a subtree or element is skipped when every lane's k-th best beats
@@ -757,7 +886,7 @@ __device__ __forceinline__ void knn_wave_body(const BNode *__restrict__ nodes, c
                                               int64_t n, int64_t wave, bool wave_ok, int k, float best_init,
                                               float scale, int exclude_self, int32_t *__restrict__ out_idx,
                                               float *__restrict__ out_dist, float *__restrict__ out_d2,
-                                              int32_t root = 0) {
+                                              int32_t nelem, int32_t root = 0) {
   const int lane = lane_id();
   const int64_t t = wave * WAVE + lane;
   const bool inrange = wave_ok && t < n;
@@ -784,6 +913,39 @@ __device__ __forceinline__ void knn_wave_body(const BNode *__restrict__ nodes, c
   const int32_t self = exclude_self ? (int32_t)qi : -1;
   const TriRec *__restrict__ tr = (const TriRec *)elems;
   const PointRec *__restrict__ pr = (const PointRec *)elems;
+  // Seed: the k + 1 elements around the nearest leaf in leaf (Morton) order, which are spatially
+  // close; they bound the k-th distance from the start and are skipped by the walk below.
+  int32_t s0 = 0, s1 = 0;
+  if (active && nelem > 0) {
+    int32_t f, cnt;
+    seed_leaf(nodes, root, qlo, qhi, f, cnt);
+    const int32_t w = nelem < k + 1 ? nelem : k + 1;
+    s0 = f + cnt / 2 - w / 2;
+    s0 = s0 < 0 ? 0 : (s0 > nelem - w ? nelem - w : s0);
+    s1 = s0 + w;
+    for (int32_t e = s0; e < s1; ++e) {
+      float d;
+      int32_t id;
+      if (POINTS) {
+        const float4 pe = pr[e].p;
+        id = __float_as_int(pe.w);
+        d = sqn3(sub3(f3(pe.x, pe.y, pe.z), p));
+      } else {
+        const TriRec *tk = tr + e;
+        float3 cp;
+        id = trirec_id(tk->pl);
+        d = tri_closest(tk->p0, tk->p1, tk->p2, tk->pl, tk->e0, tk->e1, tk->e2, p, cp);
+      }
+      if (id != self && better(d, id, kb, kbi)) {
+        knn_insert<K>(kd, ki, d, id);
+        kb = kd[K - 1];
+        kbi = ki[K - 1];
+#pragma unroll
+        for (int j = 0; j < K - 1; ++j)
+          if (j == k - 1) { kb = kd[j]; kbi = ki[j]; }
+      }
+    }
+  }
   int32_t s_ref = 0;
   float s_lx = 0, s_ly = 0, s_lz = 0, s_hx = 0, s_hy = 0, s_hz = 0;
   int32_t sp = 0;
@@ -829,7 +991,7 @@ __device__ __forceinline__ void knn_wave_body(const BNode *__restrict__ nodes, c
                             make_float4(unif(tk->e1.x), unif(tk->e1.y), unif(tk->e1.z), unif(tk->e1.w)),
                             make_float4(unif(tk->e2.x), unif(tk->e2.y), unif(tk->e2.z), unif(tk->e2.w)), p, cp);
           }
-          const bool take = active && id != self && better(d, id, kb, kbi);
+          const bool take = active && id != self && (e < s0 || e >= s1) && better(d, id, kb, kbi);
           if (__ballot(take)) {
             if (take) {
               knn_insert<K>(kd, ki, d, id);
@@ -843,7 +1005,7 @@ __device__ __forceinline__ void knn_wave_body(const BNode *__restrict__ nodes, c
         }
       } else if (next < 0) {
         next = r;
-      } else {
+      } else if (sp < W_STACK) {  // never full when depth < 64 (checked by every caller)
         if (lane == sp) {
           s_ref = r;
           if (c == 0) { s_lx = a0; s_ly = a1; s_lz = a2; s_hx = a3; s_hy = b0; s_hz = b1; }
@@ -883,11 +1045,12 @@ __global__ void __launch_bounds__(W_BLOCK) k_knn_wave(const BNode *__restrict__ 
                                                       const float *__restrict__ q, const uint32_t *__restrict__ order,
                                                       int64_t n, uint32_t nb_real, int k, float best_init, float scale,
                                                       int exclude_self, int32_t *__restrict__ out_idx,
-                                                      float *__restrict__ out_dist, float *__restrict__ out_d2) {
+                                                      float *__restrict__ out_dist, float *__restrict__ out_d2,
+                                                      int32_t nelem) {
   const uint32_t lb_blk = xcd_remap(blockIdx.x, gridDim.x);
   const int64_t wave = (int64_t)lb_blk * W_WAVES + (threadIdx.x / WAVE);
   knn_wave_body<POINTS, K>(nodes, elems, q, order, n, wave, lb_blk < nb_real, k, best_init, scale, exclude_self,
-                           out_idx, out_dist, out_d2);
+                           out_idx, out_dist, out_d2, nelem);
 }
 
 template <bool POINTS, int K>
@@ -899,7 +1062,7 @@ __global__ void __launch_bounds__(W_BLOCK) k_knn_multi(const SegDesc *__restrict
   const int si = uni(find_segment(seg, nseg, gw));
   const SegDesc &d = seg[si];
   knn_wave_body<POINTS, K>((const BNode *)d.nodes, d.elems, d.q, d.order, d.n, gw - (int64_t)d.wave_begin, true, d.k,
-                           d.best_init, d.scale, d.exclude_self, d.idx, d.dist, d.d2, d.root);
+                           d.best_init, d.scale, d.exclude_self, d.idx, d.dist, d.d2, d.nelem, d.root);
 }
 
 //--------------------------------------------------------------------------------------------------
@@ -982,7 +1145,7 @@ __device__ __forceinline__ void ray_body(int32_t *__restrict__ s_ref, float *__r
         if (ANY && done) break;
       } else if (next < 0) {
         next = r;
-      } else {
+      } else if (sp < STACK) {  // never taken when depth < STACK (checked by every caller)
         s_ref[sp * Q_BLOCK + tid] = r;
         s_t[sp * Q_BLOCK + tid] = tn;
         ++sp;
@@ -1135,12 +1298,19 @@ void order_queries(const Bvh &b, const float *q, int64_t n, QueryScratch &ws, hi
 
 static float prev_float(float x) { return std::nextafter(x, -INFINITY); }
 
-// Traversal strategy for point-proximity queries: wave-cooperative unless THG_TRAVERSAL=lane, or the
-// tree is deeper than the lane-distributed stack.
-bool use_wave_traversal(const Bvh &b) {
+// Traversal strategy for a batch of n point-proximity queries. One wave walking the tree for 64
+// sorted queries pays for the union of their search regions, which stays small only when the batch
+// is dense: at least about one query per 8 leaves (measured on spheres and tori of 0.1M-4M
+// triangles with queries uniform in the inflated box, where the cheaper of the two strategies
+// changes between 1/4 and 1/16 of a query per leaf). Sparser batches, and batches too small to be
+// spatially sorted, walk one query per lane. THG_TRAVERSAL=lane or =wave forces a strategy; trees
+// deeper than the lane-distributed stack always use lanes.
+bool use_wave_traversal(const Bvh &b, int64_t n) {
   static const char *env = getenv("THG_TRAVERSAL");
-  bool want = !(env && std::string(env) == "lane");
-  return want && b.max_depth < W_STACK;
+  if (b.max_depth >= W_STACK) return false;
+  if (env && std::string(env) == "lane") return false;
+  if (env && std::string(env) == "wave") return true;
+  return n >= ORDER_MIN_QUERIES && n >= b.num_leaves / 8;
 }
 
 // Lanes per query and per-element prefilter of the wave kernel (THG_WAVE_G, THG_PREFILTER override).
@@ -1176,7 +1346,7 @@ void launch_closest(const Bvh &b, const float *q, int64_t n, double dist_bound, 
   uint32_t nb = ceil_div(n, Q_BLOCK);
   uint32_t grid = round_up8(nb);
   float best = bound_to_best(dist_bound);
-  if (use_wave_traversal(b)) {
+  if (use_wave_traversal(b, n)) {
     const WaveConfig wc = wave_config(b);
     uint32_t wnb = ceil_div(n, (uint64_t)W_WAVES * (WAVE / wc.g));
     uint32_t wgrid = round_up8(wnb);
@@ -1260,6 +1430,35 @@ void launch_knn_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, in
   launch_check("k_knn_multi");
 }
 
+void launch_closest_lane_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, bool points, int stack,
+                               hipStream_t s) {
+  if (nseg <= 0 || total_waves == 0) return;
+  const uint32_t nb = (uint32_t)total_waves;
+  const uint32_t grid = round_up8(nb);
+#define THG_CLM(S, P) hipLaunchKernelGGL((k_closest_lane_multi<S, P>), dim3(grid), dim3(Q_BLOCK), 0, s, seg_dev, nseg, \
+                                         total_waves, nb)
+  if (points) { if (stack <= 32) THG_CLM(32, true); else THG_CLM(64, true); }
+  else { if (stack <= 32) THG_CLM(32, false); else THG_CLM(64, false); }
+#undef THG_CLM
+  launch_check("k_closest_lane_multi");
+}
+
+void launch_knn_lane_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, int k, bool points, int stack,
+                           hipStream_t s) {
+  if (nseg <= 0 || total_waves == 0) return;
+  const uint32_t nb = (uint32_t)total_waves;
+  const uint32_t grid = round_up8(nb);
+#define THG_KLM(S, P, K) hipLaunchKernelGGL((k_knn_lane_multi<S, P, K>), dim3(grid), dim3(Q_BLOCK), 0, s, seg_dev, nseg, \
+                                            total_waves, nb)
+#define THG_KLM_K(S, P) \
+  if (k <= 4) THG_KLM(S, P, 4); else if (k <= 8) THG_KLM(S, P, 8); else if (k <= 16) THG_KLM(S, P, 16); else THG_KLM(S, P, 32)
+  if (points) { if (stack <= 32) { THG_KLM_K(32, true); } else { THG_KLM_K(64, true); } }
+  else { if (stack <= 32) { THG_KLM_K(32, false); } else { THG_KLM_K(64, false); } }
+#undef THG_KLM_K
+#undef THG_KLM
+  launch_check("k_knn_lane_multi");
+}
+
 void launch_ray_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, hipStream_t s) {
   if (nseg <= 0 || total_waves == 0) return;
   uint32_t nb = (uint32_t)total_waves;
@@ -1306,13 +1505,13 @@ void launch_knn(const Bvh &b, const float *q, int64_t n, int k, double dist_boun
   uint32_t nb = ceil_div(n, Q_BLOCK);
   uint32_t grid = round_up8(nb);
   float best = bound_to_best(dist_bound);
-  if (use_wave_traversal(b)) {
+  if (use_wave_traversal(b, n)) {
     uint32_t wnb = ceil_div(n, W_BLOCK);
     uint32_t wgrid = round_up8(wnb);
     KernelTimer kt(b.ctx, s, "knn");
 #define THG_KW(P, K)                                                                                                 \
   hipLaunchKernelGGL((k_knn_wave<P, K>), dim3(wgrid), dim3(W_BLOCK), 0, s, b.nodes.as<BNode>(), b.elems.p, q, ws.order, \
-                     n, wnb, k, best, b.scale, exclude_self, idx, dist, d2)
+                     n, wnb, k, best, b.scale, exclude_self, idx, dist, d2, (int32_t)b.n)
     if (b.points) {
       if (k <= 4) THG_KW(true, 4); else if (k <= 8) THG_KW(true, 8); else if (k <= 16) THG_KW(true, 16); else THG_KW(true, 32);
     } else {

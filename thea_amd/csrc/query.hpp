@@ -6,7 +6,7 @@
 namespace thg {
 
 // Batches smaller than this are processed in input order (sorting would cost more than it saves).
-constexpr int64_t ORDER_MIN_QUERIES = 16384;
+constexpr int64_t ORDER_MIN_QUERIES = 4096;
 
 // One batch of a multi-batch launch (k_closest_multi / k_knn_multi / k_ray_multi): the BVH it
 // queries, its inputs and outputs, and the first wave of the launch that works on it.
@@ -29,7 +29,7 @@ struct SegDesc {
   int32_t k;
   int32_t points;
   int32_t root;          // node the traversal starts from
-  int32_t pad;
+  int32_t nelem;         // elements of the segment's tree (k-NN seed window)
 };
 
 // Per-call temporaries of a query batch (spatial order of the queries).
@@ -61,6 +61,12 @@ void scan_excl_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *tmp, 
 // descriptor table with wave_begin filled in. Requires every segment's BVH to fit the wave stack.
 void launch_closest_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, bool points, hipStream_t s);
 void launch_knn_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, int k, bool points, hipStream_t s);
+// One query per lane over a segment table (stack >= the deepest tree of the segments, at most 64).
+void launch_closest_lane_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, bool points, int stack,
+                               hipStream_t s);
+void launch_knn_lane_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, int k, bool points, int stack,
+                           hipStream_t s);
+bool use_wave_traversal(const Bvh &b, int64_t n);
 void launch_ray_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, hipStream_t s);
 constexpr int SEG_WAVE_QUERIES = 64;  // queries per wave in the multi-batch kernels
 void ray_stats(const Bvh &b, const float *o, const float *d, int64_t n, int32_t *idx, int32_t *counts, hipStream_t s);

@@ -15,6 +15,7 @@
 #include <condition_variable>
 #include <deque>
 #include <map>
+#include <tuple>
 #include <thread>
 #include <unordered_map>
 
@@ -126,19 +127,24 @@ static void worker_loop(Server *S, int wi) {
     std::string err;
     std::vector<std::unique_ptr<QueryScratch>> scratch;
     try {
-      // group by type (k-NN further by k: one kernel instantiation per launch)
-      std::map<std::pair<int, int>, std::vector<SrvRequest *>> groups;
+      // group by type, traversal (wave-cooperative for dense requests, one query per lane for
+      // sparse ones, as the single-call entry points choose) and, for k-NN, k
+      std::map<std::tuple<int, int, int>, std::vector<SrvRequest *>> groups;
       {
         std::lock_guard<std::mutex> g(S->mu);
         for (int64_t id : batch) {
           SrvRequest &r = S->reqs[id];
           int kk = r.r.type == THG_REQ_KNN ? (r.r.k <= 4 ? 4 : r.r.k <= 8 ? 8 : r.r.k <= 16 ? 16 : 32) : 0;
-          groups[{r.r.type, kk}].push_back(&r);
+          int lane = r.r.type != THG_REQ_RAY && !use_wave_traversal(*S->meshes[r.r.mesh].bvh, r.r.n) ? 1 : 0;
+          groups[{r.r.type, lane, kk}].push_back(&r);
         }
       }
       int region = 0;
       for (auto &kv : groups) {
-        const int type = kv.first.first;
+        const int type = std::get<0>(kv.first);
+        const bool lane = std::get<1>(kv.first) != 0;
+        int stack = 32;
+        for (SrvRequest *r : kv.second) stack = std::max(stack, S->meshes[r->r.mesh].bvh->max_depth + 1);
         // k-NN segments of one launch share k; split by exact k
         std::map<int, std::vector<SrvRequest *>> byk;
         for (SrvRequest *r : kv.second) byk[type == THG_REQ_KNN ? r->r.k : 0].push_back(r);
@@ -162,6 +168,7 @@ static void worker_loop(Server *S, int wi) {
             d.n = q.n;
             d.scale = b.scale;
             d.k = q.k;
+            d.nelem = (int32_t)b.n;
             d.exclude_self = q.exclude_self;
             if (type == THG_REQ_RAY) {
               d.dist = q.dist;
@@ -183,9 +190,15 @@ static void worker_loop(Server *S, int wi) {
             hd[nseg++] = d;
           }
           THG_HIP(hipMemcpyAsync(dd, hd, sizeof(SegDesc) * nseg, hipMemcpyHostToDevice, s));
-          if (type == THG_REQ_CLOSEST) launch_closest_multi(dd, nseg, waves, false, s);
-          else if (type == THG_REQ_KNN) launch_knn_multi(dd, nseg, waves, kk.first, false, s);
-          else launch_ray_multi(dd, nseg, waves, s);
+          if (type == THG_REQ_CLOSEST) {
+            if (lane) launch_closest_lane_multi(dd, nseg, waves, false, stack, s);
+            else launch_closest_multi(dd, nseg, waves, false, s);
+          } else if (type == THG_REQ_KNN) {
+            if (lane) launch_knn_lane_multi(dd, nseg, waves, kk.first, false, stack, s);
+            else launch_knn_multi(dd, nseg, waves, kk.first, false, s);
+          } else {
+            launch_ray_multi(dd, nseg, waves, s);
+          }
         }
       }
       THG_HIP(hipStreamSynchronize(s));
