@@ -22,6 +22,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
+# BVH leaf size per workload, the fastest of 2/4/6/8/12 measured on one MI355X for that query type
+# (closest point prefers 8, k-NN over points 12, incoherent rays 2); the library default is 4.
+LEAF_DEFAULT = {"c3": 8, "c1": 8, "c3ray": 2, "c2knn": 12, "c2sample": 4}
 
 
 def parse():
@@ -33,7 +36,8 @@ def parse():
     p.add_argument("--requests", type=int, default=400, help="c5: requests in the trace")
     p.add_argument("--load", type=float, default=0.7, help="c5: Poisson rate as a fraction of plain capacity")
     p.add_argument("--queries", type=int, default=None, help="override the query count")
-    p.add_argument("--leaf", type=int, default=4)
+    p.add_argument("--leaf", type=int, default=None,
+                   help="BVH leaf size (default: the measured best per workload, LEAF_DEFAULT)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-threads", type=int, default=None)
     return p.parse_args()
@@ -86,6 +90,8 @@ def main():
     ctx = thea_amd.default_context(0)
     counts = load_counts()
     wl = args.workload
+    if args.leaf is None:
+        args.leaf = LEAF_DEFAULT.get(wl, 4)
     if wl == "c5":
         run_c5(args)
         return
@@ -212,7 +218,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
                      "kernel": f"{kernel} traversal", "kernel_avg_ms": k_avg, "kernel_launches": k_n,
-                     "bytes_per_query": bytes_per_query, "thea_traversal_counts": metric_counts},
+                     "bytes_per_query": bytes_per_query, "thea_traversal_counts": metric_counts,
+                     "model": "algorithmic bytes = what Thea's own traversal reads per query (boxes and elements "
+                              "it tests) on the same inputs; they exceed the HBM traffic when re-read from L2/MALL, "
+                              "which `traffic` (measured HBM bytes per launch) shows"},
         "build_ms": build_ms,
         "order_kernels_avg_ms": o_ms / max(o_n, 1) if o_n else 0.0,
         "input_generation_s": gen_s,
