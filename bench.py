@@ -239,7 +239,7 @@ def main():
             "lane_utilisation_of_exact_evals": st[6] / max(st[5], 1),
             "note": "one wave walks one tree for its 64 queries; exact evaluation lane slots vs the "
                     "(lane, element) pairs that needed one"}
-    if not args.no_cpu_baseline and wl in ("c3", "c1", "c3ray", "c2knn"):
+    if not args.no_cpu_baseline and wl in ("c3", "c1", "c3ray", "c2knn", "c2sample"):
         result["cpu_baseline"] = cpu_baseline(args, wl, v, t, q if wl in ("c3", "c1") else None,
                                               (ro, rd) if wl == "c3ray" else None, unit)
     print(json.dumps(result), flush=True)
@@ -527,20 +527,28 @@ def cpu_baseline(args, wl, v, t, q, rays, unit):
         from thea_amd import workloads as W
         pos, _, _ = O.sample_area(v, t, 1_000_000, 2, W.STREAM_SAMPLES)
         kd = O.KdTree(points=pos)
-    else:
+    elif wl != "c2sample":
         kd = O.KdTree(v, t)
     build_s = time.perf_counter() - t0
+    # The whole batch on all threads when that is about 30 s of CPU work or less (C3: ~25 s on 16
+    # threads), and a prefix of it on one thread.
     if wl in ("c3", "c1"):
-        n_multi = min(len(q), 60_000 * nth) if wl == "c3" else len(q)
-        n_single = min(len(q), 60_000) if wl == "c3" else len(q)
+        n_multi = min(len(q), 1_000_000 * nth)
+        n_single = min(len(q), 60_000)
         run = lambda n, th: kd.closest(q[:n], mode=1, nthreads=th, want_cp=True)  # noqa: E731
     elif wl == "c3ray":
         o, d = rays
-        n_multi, n_single = min(len(o), 2_000_000), min(len(o), 300_000)
+        n_multi, n_single = len(o), min(len(o), 1_000_000)
         run = lambda n, th: kd.ray(o[:n], d[:n], mode=1, nthreads=th)  # noqa: E731
-    else:
-        n_multi, n_single = 400_000, 50_000
+    elif wl == "c2knn":
+        n_multi, n_single = len(pos), 100_000
         run = lambda n, th: kd.knn(pos[:n], 16, mode=1, exclude_self=True, nthreads=th)  # noqa: E731
+    else:  # c2sample: Thea's sampler is sequential (one CDF, one RNG stream)
+        t0 = time.perf_counter()
+        O.sample_area(v, t, 1_000_000, 2, 3)
+        sec = time.perf_counter() - t0
+        return {"value": 1_000_000 / sec, "unit": unit, "cores": 1, "kind": "port",
+                "sample": f"1,000,000 samples of the same mesh on one thread ({sec:.2f} s, CDF included)"}
     t0 = time.perf_counter()
     run(n_multi, nth)
     multi_s = time.perf_counter() - t0
