@@ -22,9 +22,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
-# BVH leaf size per workload, the fastest of 2/4/6/8/12 measured on one MI355X for that query type
-# (closest point prefers 8, k-NN over points 12, incoherent rays 2); the library default is 4.
-LEAF_DEFAULT = {"c3": 8, "c1": 8, "c3ray": 2, "c2knn": 12, "c2sample": 4}
+# BVH leaf size per workload, the fastest of 1, 2, 4, 6, 8, 12 measured on one MI355X for that query type
+# (closest point prefers 8, k-NN over points 12, incoherent rays 1); the library default is 4.
+LEAF_DEFAULT = {"c3": 8, "c1": 8, "c3ray": 1, "c2knn": 12, "c2sample": 4}
 
 
 def parse():

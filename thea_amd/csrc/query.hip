@@ -1080,7 +1080,7 @@ __device__ __forceinline__ float2 slab(float3 o, float3 inv, float w, float lx, 
 }
 
 // ANY = true: stop at the first accepted hit (any-hit query).
-template <int STACK, bool ANY, bool STATS = false>
+template <int STACK, bool ANY, bool STATS = false, bool TS = true>
 __device__ __forceinline__ void ray_body(int32_t *__restrict__ s_ref, float *__restrict__ s_t,
                                          const BNode *__restrict__ nodes, const TriRec *__restrict__ tr,
                                          const float *__restrict__ orig, const float *__restrict__ dirs, int64_t ri,
@@ -1147,7 +1147,7 @@ __device__ __forceinline__ void ray_body(int32_t *__restrict__ s_ref, float *__r
         next = r;
       } else if (sp < STACK) {  // never taken when depth < STACK (checked by every caller)
         s_ref[sp * Q_BLOCK + tid] = r;
-        s_t[sp * Q_BLOCK + tid] = tn;
+        if (TS) s_t[sp * Q_BLOCK + tid] = tn;
         ++sp;
         if (STATS) c_maxsp = max(c_maxsp, sp);
       }
@@ -1158,7 +1158,9 @@ __device__ __forceinline__ void ray_body(int32_t *__restrict__ s_ref, float *__r
     while (sp > 0) {
       --sp;
       if (STATS) c_pops++;
-      if (s_t[sp * Q_BLOCK + tid] <= best * REL) { node = s_ref[sp * Q_BLOCK + tid]; found = true; break; }
+      // TS = false keeps only node refs (half the LDS, more waves per CU); a popped node is then
+      // entered and its children are tested against the current best instead.
+      if (!TS || s_t[sp * Q_BLOCK + tid] <= best * REL) { node = s_ref[sp * Q_BLOCK + tid]; found = true; break; }
     }
     if (!found) break;
   }
@@ -1182,7 +1184,7 @@ __device__ __forceinline__ void ray_body(int32_t *__restrict__ s_ref, float *__r
   }
 }
 
-template <int STACK, bool ANY>
+template <int STACK, bool ANY, bool TS = true>
 __global__ void __launch_bounds__(Q_BLOCK) k_ray(const BNode *__restrict__ nodes, const TriRec *__restrict__ tr,
                                                  const float *__restrict__ orig, const float *__restrict__ dirs,
                                                  const uint32_t *__restrict__ order, int64_t n, uint32_t nb_real,
@@ -1190,12 +1192,12 @@ __global__ void __launch_bounds__(Q_BLOCK) k_ray(const BNode *__restrict__ nodes
                                                  int32_t *__restrict__ out_idx, float *__restrict__ out_t,
                                                  float *__restrict__ out_n, uint8_t *__restrict__ out_any) {
   __shared__ int32_t s_ref[STACK * Q_BLOCK];
-  __shared__ float s_t[STACK * Q_BLOCK];
+  __shared__ float s_t[TS ? STACK * Q_BLOCK : 1];
   bool active;
   int64_t ri = query_slot(nb_real, n, order, active);
   if (!active) return;
-  ray_body<STACK, ANY>(s_ref, s_t, nodes, tr, orig, dirs, ri, best_init, max_time, scale, out_idx, out_t, out_n,
-                       out_any);
+  ray_body<STACK, ANY, false, TS>(s_ref, s_t, nodes, tr, orig, dirs, ri, best_init, max_time, scale, out_idx, out_t,
+                                  out_n, out_any);
 }
 
 // Per-ray traversal counts (diagnostics): nodes visited, triangles tested, deepest stack, pops.
@@ -1217,7 +1219,7 @@ template <int STACK>
 __global__ void __launch_bounds__(Q_BLOCK) k_ray_multi(const SegDesc *__restrict__ seg, int nseg, uint64_t total_waves,
                                                        uint32_t nb_real) {
   __shared__ int32_t s_ref[STACK * Q_BLOCK];
-  __shared__ float s_t[STACK * Q_BLOCK];
+  __shared__ float s_t[1];
   const uint32_t lb_blk = xcd_remap(blockIdx.x, gridDim.x);
   if (lb_blk >= nb_real || (uint64_t)lb_blk >= total_waves) return;
   const int si = uni(find_segment(seg, nseg, (int64_t)lb_blk));
@@ -1225,8 +1227,8 @@ __global__ void __launch_bounds__(Q_BLOCK) k_ray_multi(const SegDesc *__restrict
   const int64_t t = ((int64_t)lb_blk - (int64_t)d.wave_begin) * Q_BLOCK + threadIdx.x;
   if (t >= d.n) return;
   const int64_t ri = d.order ? (int64_t)d.order[t] : t;
-  ray_body<STACK, false>(s_ref, s_t, (const BNode *)d.nodes, (const TriRec *)d.elems, d.q, d.q2, ri, d.best_init,
-                         d.max_time, d.scale, d.idx, d.dist, d.cp, nullptr, d.root);
+  ray_body<STACK, false, false, false>(s_ref, s_t, (const BNode *)d.nodes, (const TriRec *)d.elems, d.q, d.q2, ri,
+                                       d.best_init, d.max_time, d.scale, d.idx, d.dist, d.cp, nullptr, d.root);
 }
 
 //--------------------------------------------------------------------------------------------------
@@ -1266,10 +1268,10 @@ static int pick_stack(int depth) {
   fail(THG_E_UNSUPPORTED, "BVH deeper than 128 levels");
 }
 
-// Lowest Morton bit the query sort uses (THG_ORDER_BITS = number of top bits, default all 30).
+// Lowest Morton bit the query sort uses (THG_ORDER_BITS = number of top bits of the 30, default 24).
 static int order_bits_lo() {
   static const char *env = getenv("THG_ORDER_BITS");
-  int bits = env ? atoi(env) : 30;
+  int bits = env ? atoi(env) : 24;  // 3 radix passes; 30 bits (4 passes) is no faster on C3
   bits = bits < 8 ? 8 : (bits > 30 ? 30 : bits);
   return 30 - bits;
 }
@@ -1289,7 +1291,7 @@ void order_queries(const Bvh &b, const float *q, int64_t n, QueryScratch &ws, hi
   hipLaunchKernelGGL(k_query_keys, dim3(ceil_div(n, 256)), dim3(256), 0, s, q, n, b.lo, inv, ws.keys.as<uint32_t>(),
                      ws.vals.as<uint32_t>());
   launch_check("k_query_keys");
-  // The top 16 of the 30 code bits give cells of ~1/32 of the scene per axis: enough coherence.
+  // The top 24 of the 30 code bits give cells of 1/256 of the scene per axis: enough coherence.
   radix_sort_pairs(ws.keys.as<uint32_t>(), ws.vals.as<uint32_t>(), ws.tk.as<uint32_t>(), ws.tv.as<uint32_t>(),
                    ws.hist.as<uint32_t>(), n, order_bits_lo(), 30, s);
   ws.order = ws.vals.as<uint32_t>();
@@ -1542,9 +1544,19 @@ void launch_ray(const Bvh &b, const float *o, const float *d, int64_t n, float m
   float best = max_time >= 0 ? prev_float(max_time) : INFINITY;
   int st = pick_stack(b.max_depth);
   KernelTimer kt(b.ctx, s, any ? "any_hit" : "ray");
+  // Entry times on the stack (THG_RAY_TSTACK=1) cost half the LDS and with it half the resident
+  // waves; on C3 rays the ref-only stack is 1.7x faster.
+  static const bool tstack = [] {
+    const char *e = getenv("THG_RAY_TSTACK");
+    return e && atoi(e) == 1;
+  }();
 #define THG_RAY(S, A)                                                                                                \
-  hipLaunchKernelGGL((k_ray<S, A>), dim3(grid), dim3(Q_BLOCK), 0, s, b.nodes.as<BNode>(), b.elems.as<TriRec>(), o, d,   \
-                     ws.order, n, nb, best, max_time, b.scale, idx, t, nrm, hit)
+  if (tstack)                                                                                                        \
+    hipLaunchKernelGGL((k_ray<S, A, true>), dim3(grid), dim3(Q_BLOCK), 0, s, b.nodes.as<BNode>(), b.elems.as<TriRec>(), \
+                       o, d, ws.order, n, nb, best, max_time, b.scale, idx, t, nrm, hit);                           \
+  else                                                                                                               \
+    hipLaunchKernelGGL((k_ray<S, A, false>), dim3(grid), dim3(Q_BLOCK), 0, s, b.nodes.as<BNode>(), b.elems.as<TriRec>(), \
+                       o, d, ws.order, n, nb, best, max_time, b.scale, idx, t, nrm, hit)
   if (any) {
     if (st <= 32) THG_RAY(32, true); else if (st == 64) THG_RAY(64, true); else THG_RAY(128, true);
   } else {
