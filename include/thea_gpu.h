@@ -194,6 +194,37 @@ int thg_batch_closest(thg_ctx ctx, int32_t nm, const int64_t *nv, const int64_t 
                       int32_t *idx, float *dist, void *stream, double timings_ms[2]);
 
 /* ---------------------------------------------------------------------------------------------
+ * Iterative closest point. Replaces Icp3<double>::align(n, from, to_structure, &error)
+ * (Algorithms/Icp3.hpp:112-117, :193-263) with the BVH as the proximity structure, and runs
+ * num_problems alignments of the same source points from different initial transforms at once
+ * (ShapeAlign's search over starting rotations, Tools/ShapeAlign/ShapeAlign.cpp:343-414).
+ * --------------------------------------------------------------------------------------------- */
+typedef struct thg_icp_opts {
+  double fractional_error_threshold; /* convergence threshold on the fractional error change; negative: 0.0001 */
+  int32_t min_iterations;            /* negative: 3 */
+  int32_t max_iterations;            /* negative: 10 */
+  int32_t has_up;                    /* nonzero: rotations about `up` only (Icp3::setUpVector) */
+  int32_t kabsch;                    /* 0: Thea's fit, rot = U V^T of cov = sum dp dq^T (Icp3.hpp:349-350),
+                                        which is the transpose of the least-squares rotation;
+                                        nonzero: the least-squares rotation V U^T */
+  double up[3];
+  const double *translation_weights; /* nullable: n per-point weights of the means (getTranslationWeight) */
+  const double *rotation_weights;    /* nullable: n per-point weights of the fit (getRotationWeight) */
+  int64_t reserved[4];
+} thg_icp_opts;
+
+/* from: n double3 source points; initial (nullable): num_problems row-major 3x4 transforms applied to
+ * `from` first (NULL: identity, one problem per call makes sense then); from, initial and the weights
+ * live in `mem` memory. Writes, in host memory, transforms[12 k ...] = the row-major 3x4 transform
+ * Icp3 returns for problem k (to be composed after initial[k], as ShapeAlign does), errors[k]
+ * (nullable) = its alignment error, iterations[k] (nullable) = the ICP steps taken. The nearest-point
+ * searches use the BVH's float arithmetic on the positions rounded to float; fits and errors are
+ * double. Fails with THG_E_INVALID if the BVH is empty (Thea throws, Icp3.hpp:249-250). */
+int thg_icp_align(thg_bvh to, const double *from, int64_t n, int32_t num_problems, const double *initial,
+                  const thg_icp_opts *opts, int32_t mem, void *stream, double *transforms, double *errors,
+                  int32_t *iterations);
+
+/* ---------------------------------------------------------------------------------------------
  * Request server (Thea has no analogue: its queries run synchronously on the caller's thread).
  * Keeps meshes and their BVHs resident on one GPU under a memory budget and serves closest-point,
  * k-NN and ray requests with continuous batching: whenever one of `num_streams` stream workers is

@@ -46,6 +46,13 @@ class Request(C.Structure):
                 ("idx", C.c_void_p), ("dist", C.c_void_p), ("cp", C.c_void_p)]
 
 
+class IcpOpts(C.Structure):
+    _fields_ = [("fractional_error_threshold", C.c_double), ("min_iterations", C.c_int32),
+                ("max_iterations", C.c_int32), ("has_up", C.c_int32), ("kabsch", C.c_int32),
+                ("up", C.c_double * 3), ("translation_weights", C.c_void_p), ("rotation_weights", C.c_void_p),
+                ("reserved", C.c_int64 * 4)]
+
+
 THG_REQ_CLOSEST, THG_REQ_KNN, THG_REQ_RAY = 0, 1, 2
 THG_PENDING = 1
 
@@ -76,6 +83,7 @@ SIGNATURES = {
     "thg_ray_stats": (C.c_int, [vp, vp, vp, i64, vp, vp, vp]),
     "thg_ctx_set_timing": (C.c_int, [vp, i32]),
     "thg_ctx_kernel_time": (C.c_int, [vp, C.c_char_p, C.POINTER(f64), C.POINTER(i64), i32]),
+    "thg_icp_align": (C.c_int, [vp, vp, i64, i32, vp, C.POINTER(IcpOpts), i32, vp, vp, vp, vp]),
     "thg_server_create": (C.c_int, [vp, C.POINTER(ServerOpts), C.POINTER(vp)]),
     "thg_server_destroy": (C.c_int, [vp]),
     "thg_server_add_mesh": (C.c_int, [vp, vp, i64, vp, i64, i32, i32, C.POINTER(i32)]),
