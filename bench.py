@@ -24,7 +24,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBPS = 8000.0
 # BVH leaf size per workload, the fastest of 1, 2, 4, 6, 8, 12 measured on one MI355X for that query type
 # (closest point prefers 8, k-NN over points 12, incoherent rays 1); the library default is 4.
-LEAF_DEFAULT = {"c3": 8, "c1": 8, "c3ray": 1, "c2knn": 12, "c2sample": 4}
+LEAF_DEFAULT = {"c3": 8, "c1": 8, "c3ray": 1, "c2knn": 12, "c2sample": 4, "icp": 8}
 
 
 def parse():
@@ -32,13 +32,14 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", default="c3", choices=["c3", "c1", "c3ray", "c2knn", "c2sample", "c4", "c5", "copy"])
+    p.add_argument("--workload", default="c3", choices=["c3", "c1", "c3ray", "c2knn", "c2sample", "c4", "c5", "copy", "icp"])
     p.add_argument("--requests", type=int, default=400, help="c5: requests in the trace")
     p.add_argument("--load", type=float, default=0.7, help="c5: Poisson rate as a fraction of plain capacity")
     p.add_argument("--queries", type=int, default=None, help="override the query count")
     p.add_argument("--leaf", type=int, default=None,
                    help="BVH leaf size (default: the measured best per workload, LEAF_DEFAULT)")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--kabsch", action="store_true", help="icp: least-squares fit instead of Thea's")
     p.add_argument("--cpu-threads", type=int, default=None)
     return p.parse_args()
 
@@ -100,6 +101,9 @@ def main():
         return
     if wl == "copy":
         run_copy(args)
+        return
+    if wl == "icp":
+        run_icp(args)
         return
     t_gen = time.time()
     info = {}
@@ -243,6 +247,97 @@ def main():
         result["cpu_baseline"] = cpu_baseline(args, wl, v, t, q if wl in ("c3", "c1") else None,
                                               (ro, rd) if wl == "c3ray" else None, unit)
     print(json.dumps(result), flush=True)
+
+
+def cube_rotations():
+    """The 24 rotations that map the coordinate axes onto signed axes (ShapeAlign's axis-aligned
+    starting rotations, Tools/ShapeAlign/ShapeAlign.cpp:350-366)."""
+    import itertools
+    import numpy as np
+    out = []
+    for perm in itertools.permutations(range(3)):
+        for signs in itertools.product((1, -1), repeat=3):
+            m = np.zeros((3, 3))
+            for i in range(3):
+                m[i, perm[i]] = signs[i]
+            if np.linalg.det(m) > 0:
+                out.append(m)
+    return np.array(out)
+
+
+def run_icp(args):
+    """ShapeAlign's rotation search: ICP (Icp3<double>::align, Thea's defaults: at most 10 iterations)
+    of 1,000,000 source points onto a point BVH of 1,000,000 target points, from each of the 24
+    axis-aligned rotations about the source centroid, all 24 in one batched call. Both point sets are
+    area samples of the C3 torus (different seeds); the source is moved by 0.1 rad about (1, 2, 3)
+    and (0.05, -0.03, 0.02). One step = one whole search."""
+    import numpy as np
+    import torch
+    import thea_amd
+    from thea_amd import workloads as W
+
+    v, t, _, _, _ = W.torus_c3(n_queries=0, n_rays=0)
+    mesh = thea_amd.Mesh(v, t)
+    sampler = thea_amd.MeshSampler(mesh)
+    n = args.queries or 1_000_000
+    to, _, _ = sampler.sampleEvenlyByArea(n, seed=6, stream_id=W.STREAM_SAMPLES)
+    frm, _, _ = sampler.sampleEvenlyByArea(n, seed=7, stream_id=W.STREAM_SAMPLES)
+    ax = np.array([1, 2, 3.0]) / np.sqrt(14)
+    Kx = np.array([[0, -ax[2], ax[1]], [ax[2], 0, -ax[0]], [-ax[1], ax[0], 0]])
+    R = np.eye(3) + np.sin(0.1) * Kx + (1 - np.cos(0.1)) * Kx @ Kx
+    src = frm.astype(np.float64) @ R.T + np.array([0.05, -0.03, 0.02])
+    c = src.mean(0)
+    inits = np.stack([np.hstack([m, (c - m @ c).reshape(3, 1)]) for m in cube_rotations()])
+    bvh = thea_amd.PointBvh(to, leaf_size=args.leaf)
+    icp = thea_amd.Icp3(kabsch=args.kabsch)
+    for i in range(args.warmup):
+        icp.alignBatch(src, bvh, initial=inits)
+        log(f"warmup search {i + 1}/{args.warmup} done")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        tr, err, its = icp.alignBatch(src, bvh, initial=inits)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / args.steps
+    log(f"{args.steps} searches: {sec * 1e3:.1f} ms each")
+    # nearest-point queries: one initial pass per problem, then one per iteration after the first
+    queries = int(n * (len(inits) + np.maximum(its - 1, 0).sum()))
+    best = int(np.argmin(err))
+    result = {
+        "metric": "ShapeAlign 24-rotation ICP search, 1M source onto 1M target points (seconds per search)",
+        "value": sec, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": sec * 1e3,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64 fit, f32 nearest",
+        "data": "synthetic",
+        "config": {"workload": "icp rotation search on C3 torus samples", "source_points": n, "target_points": n,
+                   "starting_rotations": len(inits), "fit": "least squares" if args.kabsch else "Thea (U V^T)",
+                   "leaf_size": args.leaf},
+        "nearest_queries_per_s": queries / sec,
+        "iterations": its.tolist(), "errors": err.tolist(), "best_rotation": best,
+        "best_transform": (tr[best][:, :3] @ inits[best][:, :3]).tolist(),
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = icp_cpu_baseline(args, to, src, inits[0], sec, len(inits), n)
+    print(json.dumps(result), flush=True)
+
+
+def icp_cpu_baseline(args, to, src, init, gpu_sec, nrot, n):
+    """The oracle ICP (numpy fit, kd-tree nearest points on all threads) on one starting rotation with
+    a 100k-point prefix of the source, scaled to the whole search."""
+    import numpy as np
+    from oracle import icp as I
+    from oracle import oracle as O
+    nth = cpu_threads(args)
+    kd = O.KdTree(points=to)
+    m = min(100_000, n)
+    t0 = time.perf_counter()
+    _, _, its = I.icp_align(lambda x: kd.closest(x, mode=1, nthreads=nth)[1], I.xf_apply(init, src[:m]),
+                            kabsch=args.kabsch)
+    sec = time.perf_counter() - t0
+    est = sec * (n / m) * nrot
+    return {"value": est, "unit": "s", "cores": nth, "kind": "port",
+            "sample": f"one of the {nrot} rotations on the first {m} source points ({sec:.2f} s, {its} iterations), "
+                      f"scaled by {n / m:.0f} x {nrot} to the whole search",
+            "speedup_of_gpu": est / gpu_sec}
 
 
 def run_copy(args):

@@ -21,6 +21,7 @@
 #define THEA_GPU_BVH_HPP
 
 #include <cmath>
+#include <array>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -287,6 +288,59 @@ class MeshSampler {
  private:
   Mesh *mesh_;
   std::uint64_t seed_, stream_;
+};
+
+// Thea's Icp3<double> (Algorithms/Icp3.hpp:34-476) over a BVH as the proximity structure. align()
+// returns the row-major 3x4 transform [L | t]; alignBatch() runs one alignment per initial transform
+// in one call (the search over starting rotations of Tools/ShapeAlign/ShapeAlign.cpp:343-414).
+class Icp3 {
+ public:
+  using Transform = std::array<double, 12>;
+  explicit Icp3(double fractional_error_threshold = -1, intx min_iterations = -1, intx max_iterations = -1,
+                bool verbose = false) {
+    (void)verbose;
+    opts_ = thg_icp_opts{};
+    opts_.fractional_error_threshold = fractional_error_threshold;
+    opts_.min_iterations = (std::int32_t)min_iterations;
+    opts_.max_iterations = (std::int32_t)max_iterations;
+  }
+  void setUpVector(const double up[3]) {
+    opts_.has_up = 1;
+    for (int a = 0; a < 3; ++a) opts_.up[a] = up[a];
+  }
+  void clearUpVector() { opts_.has_up = 0; }
+  bool hasUpVector() const { return opts_.has_up != 0; }
+  // Least-squares rotation instead of Thea's U V^T (its transpose).
+  void setLeastSquaresFit(bool on) { opts_.kabsch = on ? 1 : 0; }
+
+  template <typename VectorT>
+  Transform align(intx n, const VectorT *from, const BvhBase &to, double *error = nullptr) const {
+    std::vector<Transform> tr;
+    std::vector<double> err;
+    alignBatch(n, from, to, std::vector<Transform>(), tr, &err);
+    if (error) *error = err[0];
+    return tr[0];
+  }
+
+  template <typename VectorT>
+  void alignBatch(intx n, const VectorT *from, const BvhBase &to, const std::vector<Transform> &initial,
+                  std::vector<Transform> &transforms, std::vector<double> *errors = nullptr,
+                  std::vector<std::int32_t> *iterations = nullptr) const {
+    std::vector<double> xyz(3 * (size_t)n);
+    for (intx i = 0; i < n; ++i)
+      for (int c = 0; c < 3; ++c) xyz[3 * i + c] = (double)from[i][c];
+    const int k = initial.empty() ? 1 : (int)initial.size();
+    transforms.resize(k);
+    std::vector<double> err(k);
+    std::vector<std::int32_t> its(k);
+    check(thg_icp_align(to.handle(), xyz.data(), n, k, initial.empty() ? nullptr : initial[0].data(), &opts_,
+                        THG_MEM_HOST, nullptr, transforms[0].data(), err.data(), its.data()));
+    if (errors) *errors = err;
+    if (iterations) *iterations = its;
+  }
+
+ private:
+  thg_icp_opts opts_;
 };
 
 }  // namespace thea_gpu

@@ -1,6 +1,7 @@
 // Exercises include/thea_gpu_bvh.hpp the way a Thea caller would (single-query template calls and
 // batch calls) on the unit cube of Data/Models/cube.obj; prints one line per result for
 // tests/test_gpu_shim.py to compare with the Python binding.
+#include <array>
 #include <cstdio>
 #include <vector>
 
@@ -39,6 +40,17 @@ int main() {
     std::vector<intx> tri;
     intx got = sampler.sampleEvenlyByArea(4, pos, &nrm, &tri);
     for (intx i = 0; i < got; ++i) std::printf("sample %lld %.9g %.9g %.9g\n", (long long)tri[i], pos[i][0], pos[i][1], pos[i][2]);
+    // ICP of the cube's surface samples, shifted, back onto the cube (ShapeAlign's Icp3 call)
+    std::vector<V3> surf;
+    sampler.sampleEvenlyByArea(2000, surf);
+    std::vector<std::array<double, 3>> moved(surf.size());
+    for (size_t i = 0; i < surf.size(); ++i)
+      for (int c = 0; c < 3; ++c) moved[i][c] = surf[i][c] + (c == 0 ? 0.05 : 0.0);
+    Icp3 icp;
+    icp.setLeastSquaresFit(true);
+    double err = -1;
+    Icp3::Transform tr = icp.align((intx)moved.size(), moved.data(), bvh, &err);
+    std::printf("icp %.6f %.6f %.6f %.3g\n", tr[3], tr[7], tr[11], err);
   } catch (const Error &e) {
     std::printf("error %d %s\n", e.code(), e.what());
     return 2;
