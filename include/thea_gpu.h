@@ -225,6 +225,28 @@ int thg_icp_align(thg_bvh to, const double *from, int64_t n, int32_t num_problem
                   int32_t *iterations);
 
 /* ---------------------------------------------------------------------------------------------
+ * Sample proximity graph. Replaces PointSet3::updateGraph() for samples without oversampling and
+ * without a surface (Algorithms/PointSet3.hpp:324-386, :586-697), the graph Tools/SampleGraph
+ * builds (Tools/SampleGraph/SampleGraph.cpp:329-334).
+ * --------------------------------------------------------------------------------------------- */
+typedef struct thg_graph_opts {
+  int32_t max_degree;        /* PointSet3::Options::max_degree, 1..32; <= 0: 8 */
+  int32_t separation_trials; /* samples of the initial separation estimate; <= 0: 100 */
+  uint64_t seed;             /* picks of those samples: pick t = (w * n) >> 32 for the first word w of
+                                Philox4x32-10 at counter (t, 0, 0x47524150, 0), key = seed (Thea picks
+                                with its time-seeded Random::common()) */
+  int64_t reserved[4];
+} thg_graph_opts;
+
+/* points: a point BVH over the samples (thg_bvh_build_points); normals (nullable): n float3 in
+ * `mem`. Writes, in `mem` memory, nbr_idx[n * max_degree] (neighbours sorted by separation, then
+ * index; -1 padded), nbr_sep (nullable, same layout; separations are Thea's fastSqrt of the float
+ * squared distance, -1 padded), degree (nullable, n). separation (nullable, host, 2 floats): the
+ * initial RMS separation estimate and the final average edge length (getAverageSeparation). */
+int thg_proximity_graph(thg_bvh points, const float *normals, const thg_graph_opts *opts, int32_t mem, void *stream,
+                        int32_t *nbr_idx, float *nbr_sep, int32_t *degree, float separation[2]);
+
+/* ---------------------------------------------------------------------------------------------
  * Request server (Thea has no analogue: its queries run synchronously on the caller's thread).
  * Keeps meshes and their BVHs resident on one GPU under a memory budget and serves closest-point,
  * k-NN and ray requests with continuous batching: whenever one of `num_streams` stream workers is

@@ -59,6 +59,8 @@ def lib():
         "or_thea_mt_u01": (None, [u32, i64, f32p]),
         "or_sample_area": (i64, [f32p, i32p, i64, i64, u64, u64, f32p, vp, vp, vp]),
         "or_geodesic_sphere": (i64, [i32, f32p, i64, i32p, i64, vp]),
+        "or_fast_sqrt": (None, [f32p, i64, f32p]),
+        "or_proximity_graph": (i32, [f32p, vp, i64, i32, i64p, i64, i32p, f32p, i32p, f32p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -255,3 +257,38 @@ def geodesic_sphere(depth):
     ntr = np.zeros(1, np.int64)
     got = lib().or_geodesic_sphere(depth, v, nv, t, nt, _ptr(ntr))
     return v[:got], t[: int(ntr[0])]
+
+
+def fast_sqrt(x):
+    """Thea's Math::fastSqrt for float32 (Quake III reciprocal square root, one Newton step)."""
+    x = _f32(x).ravel()
+    out = np.zeros_like(x)
+    lib().or_fast_sqrt(x, len(x), out)
+    return out
+
+
+def graph_picks(n, trials, seed):
+    """The separation-estimate picks of the proximity graph: pick t = (w * n) >> 32 for the first word
+    w of Philox4x32-10 at counter (t, 0, 0x47524150, 0), key (seed lo, seed hi)."""
+    t = min(trials, n)
+    key = [seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF]
+    return np.array([(int(philox([i & 0xFFFFFFFF, i >> 32, 0x47524150, 0], key)[0]) * n) >> 32 for i in range(t)],
+                    np.int64)
+
+
+def proximity_graph(points, normals=None, max_degree=8, picks=None):
+    """PointSet3::updateGraph restated (see thea_oracle.cpp): (idx, sep, degree, (initial, final avg))."""
+    pts = _f32(points).reshape(-1, 3)
+    n = len(pts)
+    nrm = None if normals is None else _f32(normals).reshape(-1, 3)
+    picks = np.ascontiguousarray(picks, np.int64)
+    idx = np.zeros((n, max_degree), np.int32)
+    sep = np.zeros((n, max_degree), np.float32)
+    deg = np.zeros(n, np.int32)
+    avg = np.zeros(2, np.float32)
+    rc = lib().or_proximity_graph(pts, None if nrm is None else nrm.ctypes.data_as(C.c_void_p), n, max_degree, picks,
+                                  len(picks), idx, sep, deg, avg)
+    if rc != 0:
+        raise ValueError("or_proximity_graph failed")
+    return idx, sep, deg, (float(avg[0]), float(avg[1]))
+

@@ -809,6 +809,39 @@ struct TheaMT {
 //==================================================================================================
 // C interface (ctypes)
 //==================================================================================================
+// Math::fastRsq / fastSqrt for float (Math.hpp:194-219): Quake III's reciprocal square root, one
+// Newton step; separations of the proximity graph are 1 / fastRsq(d^2).
+inline float fast_rsq(float x) {
+  const float x2 = x * 0.5f;
+  float y = x;
+  int32_t i;
+  std::memcpy(&i, &y, 4);
+  i = 0x5f3759df - (i >> 1);
+  std::memcpy(&y, &i, 4);
+  y = y * (1.5f - (x2 * y * y));
+  return y;
+}
+inline float fast_sqrt(float x) { return 1.0f / fast_rsq(x); }
+
+// PointSet3::updateGraph without oversampling or a surface (PointSet3.hpp:324-386): candidates in
+// the ball of radius r around a sample (|p - c|^2 <= r^2, Ball3::contains, BallN.hpp:83), the
+// NeighborFunctor tests (self, normals n_i . n_j >= -0.5, PointSet3.hpp:612-648) and a bounded sorted
+// set of max_degree (separation, index) pairs (SamplePoint3.hpp:52-56,128); the radius doubles up to
+// 3 times until min_degree neighbours are found (findSampleNeighbors, PointSet3.hpp:660-678).
+// Ball queries use a uniform grid of cell size r0 (exact distance tests), independent of any BVH.
+struct Grid {
+  float cell;
+  V3 lo;
+  int nx, ny, nz;
+  std::vector<int64_t> start, items;
+  int key(V3 p, int &x, int &y, int &z) const {
+    x = std::min(nx - 1, std::max(0, (int)std::floor((p.x - lo.x) / cell)));
+    y = std::min(ny - 1, std::max(0, (int)std::floor((p.y - lo.y) / cell)));
+    z = std::min(nz - 1, std::max(0, (int)std::floor((p.z - lo.z) / cell)));
+    return (z * ny + y) * nx + x;
+  }
+};
+
 extern "C" {
 
 // Per-triangle cached properties (Triangle3.hpp:143-155): out[13*i + ...] =
@@ -1144,4 +1177,97 @@ int64_t or_geodesic_sphere(int depth, float *verts, int64_t vcap, int32_t *tris,
   return (int64_t)v.size();
 }
 
+
+// Thea's fastSqrt (float) of each x.
+void or_fast_sqrt(const float *x, int64_t n, float *out) {
+  for (int64_t i = 0; i < n; ++i) out[i] = fast_sqrt(x[i]);
+}
+
+// Proximity graph of n points (normals nullable). picks: the samples of the separation estimate.
+// Writes idx/sep (n x max_degree, -1 / -1 padded, sorted), deg[n], avg[0] the initial RMS
+// separation estimate and avg[1] the final average edge length (updateAverageSeparation,
+// PointSet3.hpp:683-697).
+int or_proximity_graph(const float *pts, const float *nrm, int64_t n, int max_degree, const int64_t *picks,
+                       int64_t ntrials, int32_t *out_idx, float *out_sep, int32_t *out_deg, float *avg) {
+  if (n <= 1 || max_degree <= 0) return -1;
+  std::vector<V3> P(n), N;
+  for (int64_t i = 0; i < n; ++i) P[i] = ld3(pts + 3 * i);
+  if (nrm) {
+    N.resize(n);
+    for (int64_t i = 0; i < n; ++i) N[i] = ld3(nrm + 3 * i);
+  }
+  // separation estimate: RMS distance of the picks to their nearest other sample (brute force)
+  float acc = 0;
+  for (int64_t t = 0; t < ntrials; ++t) {
+    const int64_t a = picks[t];
+    float best = INFINITY;
+    for (int64_t j = 0; j < n; ++j)
+      if (j != a) best = std::min(best, sqn(sub(P[j], P[a])));
+    acc += best;
+  }
+  const float avg_sep = std::sqrt(acc / (float)ntrials);
+  avg[0] = avg_sep;
+  const float r0 = std::sqrt((float)max_degree) * avg_sep;
+  const int min_degree = std::min(max_degree, std::max(4, (int)(0.25 * max_degree)));
+  // grid of cell r0 over the points
+  Grid g;
+  V3 lo = P[0], hi = P[0];
+  for (auto &p : P) { lo = cmin(lo, p); hi = cmax(hi, p); }
+  const float ext = std::max(hi.x - lo.x, std::max(hi.y - lo.y, hi.z - lo.z));
+  g.cell = std::max(r0 > 0 ? r0 : 1.0f, ext / 1000.0f);
+  g.lo = lo;
+  auto dim = [&](float a, float b) { return std::max(1, std::min(1 << 10, (int)std::floor((b - a) / g.cell) + 1)); };
+  g.nx = dim(lo.x, hi.x); g.ny = dim(lo.y, hi.y); g.nz = dim(lo.z, hi.z);
+  std::vector<int64_t> cnt((size_t)g.nx * g.ny * g.nz + 1, 0);
+  std::vector<int> keys(n);
+  for (int64_t i = 0; i < n; ++i) { int x, y, z; keys[i] = g.key(P[i], x, y, z); cnt[keys[i] + 1]++; }
+  for (size_t c = 1; c < cnt.size(); ++c) cnt[c] += cnt[c - 1];
+  g.start = cnt;
+  g.items.assign(n, 0);
+  {
+    std::vector<int64_t> fill(g.start.begin(), g.start.end() - 1);
+    for (int64_t i = 0; i < n; ++i) g.items[fill[keys[i]]++] = i;
+  }
+  float sum_sep = 0;
+  int64_t num_edges = 0;
+  std::vector<std::pair<float, int64_t>> set;
+  for (int64_t i = 0; i < n; ++i) {
+    float radius = r0;
+    for (int it = 0; it < 3; ++it) {
+      set.clear();
+      const float r2 = radius * radius;
+      int cx, cy, cz;
+      g.key(P[i], cx, cy, cz);
+      const int reach = (int)std::ceil(radius / g.cell) + 1;
+      for (int z = std::max(0, cz - reach); z <= std::min(g.nz - 1, cz + reach); ++z)
+        for (int y = std::max(0, cy - reach); y <= std::min(g.ny - 1, cy + reach); ++y)
+          for (int x = std::max(0, cx - reach); x <= std::min(g.nx - 1, cx + reach); ++x) {
+            const int c = (z * g.ny + y) * g.nx + x;
+            for (int64_t k = g.start[c]; k < g.start[c + 1]; ++k) {
+              const int64_t j = g.items[k];
+              if (j == i) continue;
+              const V3 d = sub(P[j], P[i]);
+              const float d2 = sqn(d);
+              if (!(d2 <= r2)) continue;
+              if (nrm && dot(N[i], N[j]) < -0.5f) continue;
+              set.emplace_back(fast_sqrt(d2), j);
+            }
+          }
+      std::sort(set.begin(), set.end());
+      if ((int)set.size() > max_degree) set.resize(max_degree);
+      if ((int)set.size() >= min_degree) break;
+      radius *= 2.0f;
+    }
+    for (int d = 0; d < max_degree; ++d) {
+      const bool has = d < (int)set.size();
+      out_idx[i * max_degree + d] = has ? (int32_t)set[d].second : -1;
+      out_sep[i * max_degree + d] = has ? set[d].first : -1.0f;
+      if (has) sum_sep += set[d].first;
+    }
+    out_deg[i] = (int32_t)set.size();
+    num_edges += (int64_t)set.size();
+  }
+  avg[1] = num_edges > 0 ? sum_sep / (float)num_edges : 0.0f;
+  return 0;
+}
 }  // extern "C"

@@ -53,6 +53,11 @@ class IcpOpts(C.Structure):
                 ("reserved", C.c_int64 * 4)]
 
 
+class GraphOpts(C.Structure):
+    _fields_ = [("max_degree", C.c_int32), ("separation_trials", C.c_int32), ("seed", C.c_uint64),
+                ("reserved", C.c_int64 * 4)]
+
+
 THG_REQ_CLOSEST, THG_REQ_KNN, THG_REQ_RAY = 0, 1, 2
 THG_PENDING = 1
 
@@ -83,6 +88,7 @@ SIGNATURES = {
     "thg_ray_stats": (C.c_int, [vp, vp, vp, i64, vp, vp, vp]),
     "thg_ctx_set_timing": (C.c_int, [vp, i32]),
     "thg_ctx_kernel_time": (C.c_int, [vp, C.c_char_p, C.POINTER(f64), C.POINTER(i64), i32]),
+    "thg_proximity_graph": (C.c_int, [vp, vp, C.POINTER(GraphOpts), i32, vp, vp, vp, vp, vp]),
     "thg_icp_align": (C.c_int, [vp, vp, i64, i32, vp, C.POINTER(IcpOpts), i32, vp, vp, vp, vp]),
     "thg_server_create": (C.c_int, [vp, C.POINTER(ServerOpts), C.POINTER(vp)]),
     "thg_server_destroy": (C.c_int, [vp]),

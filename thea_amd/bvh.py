@@ -277,6 +277,28 @@ class PointBvh(_BvhBase):
         capi.check(capi.lib().thg_bvh_build_points(self.ctx.h, pp, n, a.mem, C.byref(opts), a.stream, C.byref(h)))
         self._finish_build(h)
 
+    def proximityGraph(self, normals=None, max_degree=8, separation_trials=100, seed=0):
+        """PointSet3::updateGraph (PointSet3.hpp:324-386) over these points as the samples:
+        (neighbour indices (n, max_degree), separations (n, max_degree), degrees (n,),
+        (initial separation estimate, average edge length)). Outputs are numpy arrays, or torch
+        tensors when the normals are a torch device tensor."""
+        n = self.info["elements"]
+        dev = _is_torch(normals) and normals.is_cuda
+        a = _Args(dev, self.ctx.torch_device() if dev else None)
+        if normals is not None:
+            _, np_, nn = a.inp(normals, np.float32, 3)
+            if nn != n:
+                raise ValueError("one normal per point")
+        else:
+            np_ = None
+        idx, ip = a.out((n, max_degree), np.int32)
+        sep, sp = a.out((n, max_degree), np.float32)
+        deg, dp = a.out((n,), np.int32)
+        avg = (C.c_float * 2)()
+        o = capi.GraphOpts(max_degree=max_degree, separation_trials=separation_trials, seed=seed)
+        capi.check(capi.lib().thg_proximity_graph(self.h, np_, C.byref(o), a.mem, a.stream, ip, sp, dp, avg))
+        return idx, sep, deg, (avg[0], avg[1])
+
 
 class MeshSampler:
     """Area-weighted sampling (Thea's MeshSampler::sampleEvenlyByArea, CountMode::EXACT)."""

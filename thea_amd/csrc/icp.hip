@@ -13,8 +13,8 @@
 // Device side: the nearest points of all active problems' positions come from one closest-point
 // launch (the float BVH; positions are rounded to float for the search, the correspondences are
 // the float closest points). Means, covariance and error are double sums over each problem's points,
-// reduced in a fixed order (per-thread sequential runs, an LDS tree per block, then the blocks in
-// order), so a problem's result does not depend on the other problems in the batch. The 3x3 SVD and
+// reduced in a fixed order (per-thread sequential sums of every 256th point, an LDS tree per block,
+// then the blocks in order), so a problem's result does not depend on the other problems in the batch. The 3x3 SVD and
 // the convergence logic run on the host.
 #include <algorithm>
 #include <cfloat>
@@ -108,9 +108,10 @@ __global__ void __launch_bounds__(ICP_THREADS) k_icp_sums(int64_t n, const int32
   double acc[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) acc[c] = 0;
-  const int64_t j0 = (int64_t)b * ICP_CHUNK + (int64_t)threadIdx.x * ICP_PER_THREAD;
+  // thread t sums points b CHUNK + t + 256 i (i = 0..15) in order: a wave's loads are contiguous
+  const int64_t j0 = (int64_t)b * ICP_CHUNK + threadIdx.x;
   for (int t = 0; t < ICP_PER_THREAD; ++t) {
-    const int64_t j = j0 + t;
+    const int64_t j = j0 + (int64_t)t * ICP_THREADS;
     if (j >= n) break;
     const double *sp = src + 3 * ((int64_t)k * n + j);
     double p[3];

@@ -13,26 +13,14 @@
 // make a box prune an element that would have won.
 #include "kernels_common.hpp"
 #include "query.hpp"
+#include "traverse.hpp"
 #include "tri_math.hpp"
 
 namespace thg {
 
 constexpr int Q_BLOCK = 64;
-constexpr float SLACK_REL = 1.0f / 65536.0f;
 // Relative margin of the element reject tests: a bound is compared with sqrt(best) * REJ_REL.
 constexpr float REJ_REL = 1.0f + 1.0f / 4096.0f;
-
-__device__ __forceinline__ float box_lb(float3 qlo, float3 qhi, float lx, float ly, float lz, float hx, float hy,
-                                        float hz) {
-  float dx = fmaxf(fmaxf(lx - qhi.x, qlo.x - hx), 0.0f);
-  float dy = fmaxf(fmaxf(ly - qhi.y, qlo.y - hy), 0.0f);
-  float dz = fmaxf(fmaxf(lz - qhi.z, qlo.z - hz), 0.0f);
-  return dx * dx + (dy * dy + dz * dz);
-}
-
-__device__ __forceinline__ bool better(float d, int32_t i, float bd, int32_t bi) {
-  return d < bd || (d == bd && i < bi);
-}
 
 __device__ __forceinline__ float3 load_q(const float *__restrict__ q, int64_t i) {
   return f3(q[3 * i], q[3 * i + 1], q[3 * i + 2]);
@@ -716,18 +704,6 @@ __global__ void __launch_bounds__(W_BLOCK) k_closest_multi(const SegDesc *__rest
 //--------------------------------------------------------------------------------------------------
 // k nearest (sorted list of K (d^2, id) pairs per lane in registers)
 //--------------------------------------------------------------------------------------------------
-template <int K>
-__device__ __forceinline__ void knn_insert(float (&kd)[K], int32_t (&ki)[K], float d, int32_t id) {
-#pragma unroll
-  for (int j = K - 1; j >= 0; --j) {
-    bool prev = j > 0 ? better(d, id, kd[j > 0 ? j - 1 : 0], ki[j > 0 ? j - 1 : 0]) : false;
-    if (better(d, id, kd[j], ki[j])) {
-      kd[j] = prev ? kd[j > 0 ? j - 1 : 0] : d;
-      ki[j] = prev ? ki[j > 0 ? j - 1 : 0] : id;
-    }
-  }
-}
-
 template <int STACK, bool POINTS, int K>
 __device__ __forceinline__ void knn_lane_body(int32_t *__restrict__ s_ref, float *__restrict__ s_lb,
                                               const BNode *__restrict__ nodes, const void *__restrict__ elems,

@@ -1,0 +1,43 @@
+// Device helpers shared by the traversal kernels: the conservative box lower bound, the canonical
+// (value, index) order, and the sorted K-list insertion.
+#pragma once
+
+#include "kernels_common.hpp"
+#include "tri_math.hpp"
+
+namespace thg {
+
+// Relative half-width of the box a query is widened to (2^-16 of max(|q|, scene scale)), so that box
+// lower bounds stay below the float distances of the elements inside (Thea pads nodes by x1.05 for
+// the same purpose, BvhN.hpp:2458).
+constexpr float SLACK_REL = 1.0f / 65536.0f;
+
+// Squared distance between the query box [qlo, qhi] and a node box: a lower bound of the squared
+// distance from the query to anything in the node.
+__device__ __forceinline__ float box_lb(float3 qlo, float3 qhi, float lx, float ly, float lz, float hx, float hy,
+                                        float hz) {
+  float dx = fmaxf(fmaxf(lx - qhi.x, qlo.x - hx), 0.0f);
+  float dy = fmaxf(fmaxf(ly - qhi.y, qlo.y - hy), 0.0f);
+  float dz = fmaxf(fmaxf(lz - qhi.z, qlo.z - hz), 0.0f);
+  return dx * dx + (dy * dy + dz * dz);
+}
+
+// Lexicographic (value, index) order: exact ties go to the lower index.
+__device__ __forceinline__ bool better(float d, int32_t i, float bd, int32_t bi) {
+  return d < bd || (d == bd && i < bi);
+}
+
+// Insert (d, id) into the ascending list kd/ki of K pairs (the last pair drops out).
+template <int K>
+__device__ __forceinline__ void knn_insert(float (&kd)[K], int32_t (&ki)[K], float d, int32_t id) {
+#pragma unroll
+  for (int j = K - 1; j >= 0; --j) {
+    bool prev = j > 0 ? better(d, id, kd[j > 0 ? j - 1 : 0], ki[j > 0 ? j - 1 : 0]) : false;
+    if (better(d, id, kd[j], ki[j])) {
+      kd[j] = prev ? kd[j > 0 ? j - 1 : 0] : d;
+      ki[j] = prev ? ki[j > 0 ? j - 1 : 0] : id;
+    }
+  }
+}
+
+}  // namespace thg
