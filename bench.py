@@ -24,7 +24,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBPS = 8000.0
 # BVH leaf size per workload, the fastest of 1, 2, 4, 6, 8, 12 measured on one MI355X for that query type
 # (closest point prefers 8, k-NN over points 12, incoherent rays 1); the library default is 4.
-LEAF_DEFAULT = {"c3": 8, "c1": 8, "c3ray": 1, "c2knn": 12, "c2sample": 4, "icp": 8}
+LEAF_DEFAULT = {"c3": 8, "c1": 8, "c3ray": 1, "c2knn": 12, "c2sample": 4, "icp": 8, "graph": 12}
 
 
 def parse():
@@ -32,7 +32,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", default="c3", choices=["c3", "c1", "c3ray", "c2knn", "c2sample", "c4", "c5", "copy", "icp"])
+    p.add_argument("--workload", default="c3", choices=["c3", "c1", "c3ray", "c2knn", "c2sample", "c4", "c5", "copy", "icp", "graph"])
     p.add_argument("--requests", type=int, default=400, help="c5: requests in the trace")
     p.add_argument("--load", type=float, default=0.7, help="c5: Poisson rate as a fraction of plain capacity")
     p.add_argument("--queries", type=int, default=None, help="override the query count")
@@ -104,6 +104,9 @@ def main():
         return
     if wl == "icp":
         run_icp(args)
+        return
+    if wl == "graph":
+        run_graph(args)
         return
     t_gen = time.time()
     info = {}
@@ -246,6 +249,56 @@ def main():
     if not args.no_cpu_baseline and wl in ("c3", "c1", "c3ray", "c2knn", "c2sample"):
         result["cpu_baseline"] = cpu_baseline(args, wl, v, t, q if wl in ("c3", "c1") else None,
                                               (ro, rd) if wl == "c3ray" else None, unit)
+    print(json.dumps(result), flush=True)
+
+
+def run_graph(args):
+    """SampleGraph's proximity graph (PointSet3::updateGraph) over the C2 workload's 1,000,000 area
+    samples with their face normals, max degree 16: one step = the separation estimate (100 picks),
+    the neighbour sets of every sample and the final average separation. The point BVH is built once,
+    outside the timed region."""
+    import numpy as np
+    import torch
+    import thea_amd
+    from thea_amd import workloads as W
+    v, t = W.ellipsoids_c2()
+    n = args.queries or 1_000_000
+    pos, nrm, _ = thea_amd.MeshSampler(thea_amd.Mesh(v, t)).sampleEvenlyByArea(n, seed=2, stream_id=W.STREAM_SAMPLES,
+                                                                               device=True)
+    b = thea_amd.PointBvh(pos, leaf_size=args.leaf)
+    for i in range(args.warmup):
+        b.proximityGraph(normals=nrm, max_degree=16, seed=4, average=False)
+        log(f"warmup graph {i + 1}/{args.warmup} done")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        idx, sep, deg, _ = b.proximityGraph(normals=nrm, max_degree=16, seed=4, average=False)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / args.steps
+    log(f"{args.steps} graphs: {sec * 1e3:.1f} ms each")
+    t0 = time.perf_counter()
+    _, _, _, avg = b.proximityGraph(normals=nrm, max_degree=16, seed=4)
+    avg_sec = time.perf_counter() - t0
+    deg = deg.cpu().numpy()
+    result = {
+        "metric": "SampleGraph proximity graph samples/s (1M area samples of 64 ellipsoids, max degree 16)",
+        "value": n / sec, "unit": "samples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": sec * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "proximity graph of C2 area samples with face normals", "samples": n,
+                   "max_degree": 16, "leaf_size": args.leaf},
+        "mean_degree": float(deg.mean()), "separation": [float(avg[0]), float(avg[1])],
+        "with_final_average_ms": avg_sec * 1e3,
+        "note": "value excludes the final average separation (a sequential float sum of all separations "
+                "on the host, as Thea computes it); with_final_average_ms is one call including it",
+    }
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        t0 = time.perf_counter()
+        O.proximity_graph(pos.cpu().numpy(), nrm.cpu().numpy(), 16, O.graph_picks(n, 100, 4))
+        csec = time.perf_counter() - t0
+        result["cpu_baseline"] = {"value": n / csec, "unit": "samples/s", "cores": 1, "kind": "port",
+                                  "sample": f"the whole graph on one thread ({csec:.1f} s; grid ball queries)"}
     print(json.dumps(result), flush=True)
 
 

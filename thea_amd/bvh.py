@@ -277,11 +277,12 @@ class PointBvh(_BvhBase):
         capi.check(capi.lib().thg_bvh_build_points(self.ctx.h, pp, n, a.mem, C.byref(opts), a.stream, C.byref(h)))
         self._finish_build(h)
 
-    def proximityGraph(self, normals=None, max_degree=8, separation_trials=100, seed=0):
+    def proximityGraph(self, normals=None, max_degree=8, separation_trials=100, seed=0, average=True):
         """PointSet3::updateGraph (PointSet3.hpp:324-386) over these points as the samples:
         (neighbour indices (n, max_degree), separations (n, max_degree), degrees (n,),
         (initial separation estimate, average edge length)). Outputs are numpy arrays, or torch
-        tensors when the normals are a torch device tensor."""
+        tensors when the normals are a torch device tensor. average=False skips the averages (the
+        final one is a sequential float sum on the host, as in Thea) and returns None for them."""
         n = self.info["elements"]
         dev = _is_torch(normals) and normals.is_cuda
         a = _Args(dev, self.ctx.torch_device() if dev else None)
@@ -296,8 +297,9 @@ class PointBvh(_BvhBase):
         deg, dp = a.out((n,), np.int32)
         avg = (C.c_float * 2)()
         o = capi.GraphOpts(max_degree=max_degree, separation_trials=separation_trials, seed=seed)
-        capi.check(capi.lib().thg_proximity_graph(self.h, np_, C.byref(o), a.mem, a.stream, ip, sp, dp, avg))
-        return idx, sep, deg, (avg[0], avg[1])
+        capi.check(capi.lib().thg_proximity_graph(self.h, np_, C.byref(o), a.mem, a.stream, ip, sp, dp,
+                                                  avg if average else None))
+        return idx, sep, deg, ((avg[0], avg[1]) if average else None)
 
 
 class MeshSampler:
