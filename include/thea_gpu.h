@@ -246,6 +246,15 @@ typedef struct thg_graph_opts {
 int thg_proximity_graph(thg_bvh points, const float *normals, const thg_graph_opts *opts, int32_t mem, void *stream,
                         int32_t *nbr_idx, float *nbr_sep, int32_t *degree, float separation[2]);
 
+/* Shape diameter function at n surface points: ShapeDiameter::compute(position, normal,
+ * only_hit_interior_surfaces) (Algorithms/SurfaceFeatures/Local/ShapeDiameter.hpp:185-256), or
+ * compute(position) with the normal of the nearest face when `normals` is NULL (:162-176). scale:
+ * the normalisation length (> 0; Thea defaults it to the bounding-sphere diameter of the mesh).
+ * positions/normals in `mem`; sdf (host, n doubles): the values in [0, 1] or -1 where no ray hit;
+ * ray_times (nullable, host, 30 n floats): each cone ray's hit time, -1 if it does not count. */
+int thg_shape_diameter(thg_bvh mesh_bvh, const float *positions, const float *normals, int64_t n, float scale,
+                       int32_t only_interior, int32_t mem, void *stream, double *sdf, float *ray_times);
+
 /* ---------------------------------------------------------------------------------------------
  * Request server (Thea has no analogue: its queries run synchronously on the caller's thread).
  * Keeps meshes and their BVHs resident on one GPU under a memory budget and serves closest-point,

@@ -60,6 +60,8 @@ def lib():
         "or_sample_area": (i64, [f32p, i32p, i64, i64, u64, u64, f32p, vp, vp, vp]),
         "or_geodesic_sphere": (i64, [i32, f32p, i64, i32p, i64, vp]),
         "or_fast_sqrt": (None, [f32p, i64, f32p]),
+        "or_sdf_rays": (None, [f32p, f32p, i64, f32, f32p, f32p]),
+        "or_sdf_reduce": (None, [f32p, i64, f64, f64p]),
         "or_proximity_graph": (i32, [f32p, vp, i64, i32, i64p, i64, i32p, f32p, i32p, f32p]),
     }
     for name, (res, args) in sig.items():
@@ -291,4 +293,25 @@ def proximity_graph(points, normals=None, max_degree=8, picks=None):
     if rc != 0:
         raise ValueError("or_proximity_graph failed")
     return idx, sep, deg, (float(avg[0]), float(avg[1]))
+
+
+def shape_diameter(kd, tri_normals, positions, normals, scale, only_interior=True):
+    """ShapeDiameter::compute restated: rays from C++ (or_sdf_rays), first hits from the kd-tree
+    (canonical), the interior test on the hit face normal, and the reduction (or_sdf_reduce).
+    Returns (sdf (n,), ray values (n, 30))."""
+    pos = _f32(positions).reshape(-1, 3)
+    nrm = _f32(normals).reshape(-1, 3)
+    n = len(pos)
+    org = np.zeros((30 * n, 3), np.float32)
+    dr = np.zeros((30 * n, 3), np.float32)
+    lib().or_sdf_rays(pos, nrm, n, np.float32(scale), org, dr)
+    idx, t, _ = kd.ray(org, dr, mode=0)
+    val = np.where(idx >= 0, t, np.float32(-1)).astype(np.float32)
+    if only_interior:
+        hn = tri_normals[np.maximum(idx, 0)]
+        dot = hn[:, 0] * dr[:, 0] + (hn[:, 1] * dr[:, 1] + hn[:, 2] * dr[:, 2])
+        val = np.where((idx >= 0) & (dot >= 0), val, np.float32(-1)).astype(np.float32)
+    out = np.zeros(n, np.float64)
+    lib().or_sdf_reduce(np.ascontiguousarray(val), n, float(np.float32(scale)), out)
+    return out, val.reshape(n, 30)
 

@@ -842,6 +842,23 @@ struct Grid {
   }
 };
 
+// ShapeDiameter (Algorithms/SurfaceFeatures/Local/ShapeDiameter.hpp:185-256): the 30 cone
+// directions of :197-228 (data), Thea's orthonormalBasis (MatrixUtil.hpp:218-246), and the reduction.
+static const float kCone[30][3] = {
+    {0.407497f, 0.134086f, 0.903309f},    {0.128722f, -0.822715f, 0.553688f},   {0.359298f, 0.686129f, 0.632560f},
+    {0.813318f, -0.153956f, 0.561081f},   {-0.707169f, 0.296987f, 0.641647f},   {-0.478783f, -0.543441f, 0.689520f},
+    {0.561083f, 0.523634f, 0.641087f},    {-0.380898f, -0.655789f, 0.651811f},  {0.722588f, 0.472155f, 0.504912f},
+    {-0.535205f, 0.421235f, 0.732200f},   {0.577593f, -0.477896f, 0.661817f},   {0.393652f, -0.144484f, 0.907834f},
+    {0.141508f, -0.660777f, 0.737122f},   {-0.274967f, -0.141703f, 0.950954f},  {-0.304935f, 0.474085f, 0.825989f},
+    {0.711150f, 0.301383f, 0.635164f},    {-0.696119f, -0.112341f, 0.709082f},  {-0.248327f, 0.381035f, 0.890588f},
+    {-0.0772394f, 0.0462501f, 0.995939f}, {0.438024f, 0.656463f, 0.614159f},    {-0.483384f, -0.542477f, 0.687065f},
+    {0.498569f, -0.697466f, 0.514752f},   {-0.128902f, -0.0173429f, 0.991506f}, {0.338759f, -0.318772f, 0.885227f},
+    {-0.574364f, -0.463887f, 0.674474f},  {-0.459207f, 0.260125f, 0.849390f},   {-0.628057f, -0.528928f, 0.570771f},
+    {0.232916f, -0.328273f, 0.915416f},   {0.231667f, -0.796882f, 0.557951f},   {-0.270612f, -0.809654f, 0.520797f},
+};
+
+inline V3 vnormalized(V3 v) { const float n = std::sqrt(sqn(v)); return mk(v.x / n, v.y / n, v.z / n); }
+
 extern "C" {
 
 // Per-triangle cached properties (Triangle3.hpp:143-155): out[13*i + ...] =
@@ -1269,5 +1286,51 @@ int or_proximity_graph(const float *pts, const float *nrm, int64_t n, int max_de
   }
   avg[1] = num_edges > 0 ? sum_sep / (float)num_edges : 0.0f;
   return 0;
+}
+
+// Rays of the shape diameter at n points with the given normals: origin and direction of ray r of
+// point i at [3 (30 i + r)].
+void or_sdf_rays(const float *pos, const float *nrm, int64_t n, float scale, float *org, float *dir) {
+  for (int64_t i = 0; i < n; ++i) {
+    const V3 nn = vnormalized(ld3(nrm + 3 * i));
+    const V3 in = mk(-nn.x, -nn.y, -nn.z);
+    const V3 w = vnormalized(in);
+    const V3 u = max_abs_axis(in) == 0 ? vnormalized(mk(in.y, -in.x, 0.0f)) : vnormalized(mk(0.0f, in.z, -in.y));
+    const V3 v = cross(w, u);
+    const float sc = 0.001f * scale;
+    const V3 o = add(ld3(pos + 3 * i), mk(sc * in.x, sc * in.y, sc * in.z));
+    for (int r = 0; r < 30; ++r) {
+      const float c0 = kCone[r][0], c1 = kCone[r][1], c2 = kCone[r][2];
+      const int64_t g = 30 * i + r;
+      org[3 * g] = o.x; org[3 * g + 1] = o.y; org[3 * g + 2] = o.z;
+      dir[3 * g] = u.x * c0 + (v.x * c1 + w.x * c2);
+      dir[3 * g + 1] = u.y * c0 + (v.y * c1 + w.y * c2);
+      dir[3 * g + 2] = u.z * c0 + (v.z * c1 + w.z * c2);
+    }
+  }
+}
+
+// The reduction of ShapeDiameter::compute over the 30 ray values of each point (-1 = not counted).
+void or_sdf_reduce(const float *vals, int64_t n, double scale, double *out) {
+  for (int64_t i = 0; i < n; ++i) {
+    double values[30], weights[30];
+    int nv = 0;
+    for (int r = 0; r < 30; ++r)
+      if (vals[30 * i + r] >= 0) { values[nv] = vals[30 * i + r]; weights[nv] = kCone[r][2]; ++nv; }
+    if (nv <= 0) { out[i] = -1.0; continue; }
+    const int mid = nv / 2;
+    std::nth_element(values, values + mid, values + nv);
+    double sv = 0, sq = 0;
+    for (int k = 0; k < nv; ++k) { sv += values[k]; sq += values[k] * values[k]; }
+    const double avg = sv / nv, var = sq / nv - avg * avg;
+    sv = 0;
+    double sw = 0;
+    for (int k = 0; k < nv; ++k) {
+      const double dv = values[k] - values[mid];
+      if (dv * dv <= var) { sv += weights[k] * values[k]; sw += weights[k]; }
+    }
+    const double x = sw > 0 ? sv / (sw * scale) : values[mid] / scale;
+    out[i] = std::min(1.0, std::max(0.0, x));
+  }
 }
 }  // extern "C"

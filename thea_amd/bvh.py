@@ -235,6 +235,29 @@ class MeshBvh(_BvhBase):
         capi.check(capi.lib().thg_bvh_build_mesh(mesh.h, C.byref(opts), None, C.byref(h)))
         self._finish_build(h)
 
+    def shapeDiameter(self, positions, normals=None, scale=None, only_hit_interior_surfaces=True,
+                      want_ray_times=False):
+        """ShapeDiameter::compute (SurfaceFeatures/Local/ShapeDiameter.hpp:162-256) at each position:
+        SDF in [0, 1] (numpy float64), -1 where no ray counts. normals=None takes the normal of the
+        nearest face. scale (> 0, required): the normalisation length (Thea's default is the bounding
+        sphere diameter of the mesh). want_ray_times adds the 30 per-point ray values (-1: not counted)."""
+        if scale is None or scale <= 0:
+            raise ValueError("scale must be a positive length")
+        dev = _is_torch(positions) and positions.is_cuda
+        a = _Args(dev, self.ctx.torch_device() if dev else None)
+        _, pp, n = a.inp(positions, np.float32, 3)
+        npp = None
+        if normals is not None:
+            _, npp, nn = a.inp(normals, np.float32, 3)
+            if nn != n:
+                raise ValueError("one normal per position")
+        sdf = np.zeros(n, np.float64)
+        times = np.zeros((n, 30), np.float32) if want_ray_times else None
+        capi.check(capi.lib().thg_shape_diameter(self.h, pp, npp, n, float(scale), 1 if only_hit_interior_surfaces else 0,
+                                                 a.mem, a.stream, sdf.ctypes.data_as(C.c_void_p),
+                                                 None if times is None else times.ctypes.data_as(C.c_void_p)))
+        return (sdf, times) if want_ray_times else sdf
+
     # -- rays (BvhN::rayStructureIntersection / rayIntersectionTime / rayIntersects) --------------
     def rayStructureIntersection(self, origins, directions, max_time=-1.0, want_normal=True):
         """(idx, t, normal) of the first hit per ray; misses give idx -1, t -1."""
