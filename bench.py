@@ -24,7 +24,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBPS = 8000.0
 # BVH leaf size per workload, the fastest of 1, 2, 4, 6, 8, 12 measured on one MI355X for that query type
 # (closest point prefers 8, k-NN over points 12, incoherent rays 1); the library default is 4.
-LEAF_DEFAULT = {"c3": 8, "c1": 8, "c3ray": 1, "c2knn": 12, "c2sample": 4, "icp": 8, "graph": 12}
+LEAF_DEFAULT = {"c3": 8, "c1": 8, "c3ray": 1, "c2knn": 12, "c2sample": 4, "icp": 8, "graph": 12, "sdf": 1}
 
 
 def parse():
@@ -32,7 +32,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", default="c3", choices=["c3", "c1", "c3ray", "c2knn", "c2sample", "c4", "c5", "copy", "icp", "graph"])
+    p.add_argument("--workload", default="c3", choices=["c3", "c1", "c3ray", "c2knn", "c2sample", "c4", "c5", "copy", "icp", "graph", "sdf"])
     p.add_argument("--requests", type=int, default=400, help="c5: requests in the trace")
     p.add_argument("--load", type=float, default=0.7, help="c5: Poisson rate as a fraction of plain capacity")
     p.add_argument("--queries", type=int, default=None, help="override the query count")
@@ -107,6 +107,9 @@ def main():
         return
     if wl == "graph":
         run_graph(args)
+        return
+    if wl == "sdf":
+        run_sdf(args)
         return
     t_gen = time.time()
     info = {}
@@ -299,6 +302,58 @@ def run_graph(args):
         csec = time.perf_counter() - t0
         result["cpu_baseline"] = {"value": n / csec, "unit": "samples/s", "cores": 1, "kind": "port",
                                   "sample": f"the whole graph on one thread ({csec:.1f} s; grid ball queries)"}
+    print(json.dumps(result), flush=True)
+
+
+def run_sdf(args):
+    """Shape diameter function (ShapeDiameter::compute) at 1,000,000 area samples of the C3 torus with
+    their face normals: 30 cone rays per point (30M first-hit rays) and Thea's per-point reduction.
+    One step = one batch; the mesh BVH is built once."""
+    import numpy as np
+    import torch
+    import thea_amd
+    from thea_amd import workloads as W
+    v, t, _, _, _ = W.torus_c3(n_queries=0, n_rays=0)
+    mesh = thea_amd.Mesh(v, t)
+    n = args.queries or 1_000_000
+    pos, nrm, _ = thea_amd.MeshSampler(mesh).sampleEvenlyByArea(n, seed=9, stream_id=W.STREAM_SAMPLES, device=True)
+    b = thea_amd.MeshBvh(mesh, leaf_size=args.leaf)
+    scale = float(np.linalg.norm(v.max(0) - v.min(0)))
+    for i in range(args.warmup):
+        b.shapeDiameter(pos, nrm, scale=scale)
+        log(f"warmup sdf {i + 1}/{args.warmup} done")
+    b.ctx.kernel_time("ray", reset=True)
+    b.ctx.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sdf = b.shapeDiameter(pos, nrm, scale=scale)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / args.steps
+    b.ctx.set_timing(False)
+    k_ms, k_n = b.ctx.kernel_time("ray", reset=True)
+    log(f"{args.steps} batches: {sec * 1e3:.1f} ms each")
+    result = {
+        "metric": "shape diameter points/s (C3 torus, 1M surface points, 30 rays each)", "value": n / sec,
+        "unit": "points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": sec * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 rays, f64 reduction",
+        "data": "synthetic",
+        "config": {"workload": "ShapeDiameter at C3 area samples with face normals", "points": n, "rays": 30 * n,
+                   "leaf_size": args.leaf, "scale": scale},
+        "ray_kernel_avg_ms": k_ms / max(k_n, 1), "rays_per_s_kernel": 30 * n / (k_ms / max(k_n, 1) * 1e-3),
+        "median_sdf": float(sdf[sdf >= 0].median()),
+    }
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        m = min(n, 100_000)
+        kd = O.KdTree(v, t)
+        tn = O.tri_props(v, t)[:, :3]
+        t0 = time.perf_counter()
+        O.shape_diameter(kd, tn, pos[:m].cpu().numpy(), nrm[:m].cpu().numpy(), scale)
+        csec = time.perf_counter() - t0
+        result["cpu_baseline"] = {"value": m / csec, "unit": "points/s", "cores": cpu_threads(args), "kind": "port",
+                                  "sample": f"the first {m} points ({csec:.1f} s): rays on all threads, generation "
+                                            f"and reduction on one"}
     print(json.dumps(result), flush=True)
 
 

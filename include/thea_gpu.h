@@ -90,6 +90,10 @@ int thg_bvh_destroy(thg_bvh bvh);
 /* Statistics of a built BVH: out[0] nodes, [1] leaves, [2] max depth, [3] elements,
  * [4] device bytes held by the BVH, [5] leaf size. */
 int thg_bvh_info(thg_bvh bvh, int64_t out[6]);
+/* The union of the element boxes: exact min/max of the coordinates, without padding (an empty BVH
+ * gives zeros). Thea's BvhN::getBounds (BvhN.hpp:991) returns this box scaled by 1.05 about its
+ * centre (BOUNDS_EXPANSION_FACTOR, BvhN.hpp:1437,2458). */
+int thg_bvh_bounds(thg_bvh bvh, float lo[3], float hi[3]);
 /* Copy a built BVH to host memory (diagnostics and tests; synchronous). nodes (nullable):
  * info[0] 64-byte nodes, each {child 0 box lo xyz, hi xyz, child 1 box lo xyz, hi xyz (floats),
  * ref0, ref1, count0, count1 (int32)}; ref >= 0 is a node index, ref < 0 a leaf holding elements
@@ -250,8 +254,9 @@ int thg_proximity_graph(thg_bvh points, const float *normals, const thg_graph_op
  * only_hit_interior_surfaces) (Algorithms/SurfaceFeatures/Local/ShapeDiameter.hpp:185-256), or
  * compute(position) with the normal of the nearest face when `normals` is NULL (:162-176). scale:
  * the normalisation length (> 0; Thea defaults it to the bounding-sphere diameter of the mesh).
- * positions/normals in `mem`; sdf (host, n doubles): the values in [0, 1] or -1 where no ray hit;
- * ray_times (nullable, host, 30 n floats): each cone ray's hit time, -1 if it does not count. */
+ * positions, normals, sdf and ray_times all in `mem`. sdf (n doubles): the values in [0, 1] or -1
+ * where no ray counts; ray_times (nullable, 30 n floats): each cone ray's hit time, -1 if it does
+ * not count. With THG_MEM_DEVICE the call is asynchronous on `stream`. */
 int thg_shape_diameter(thg_bvh mesh_bvh, const float *positions, const float *normals, int64_t n, float scale,
                        int32_t only_interior, int32_t mem, void *stream, double *sdf, float *ray_times);
 

@@ -118,6 +118,20 @@ class BvhBase {
   virtual ~BvhBase() { if (h_) thg_bvh_destroy(h_); }
   thg_bvh handle() const { return h_; }
 
+  // BvhN::getBounds (BvhN.hpp:991): the union of the element boxes scaled by 1.05 about its centre
+  // (BvhN.hpp:1437, AxisAlignedBoxN.hpp:274,316-324), in float arithmetic.
+  void getBounds(float lo[3], float hi[3]) const {
+    check(thg_bvh_bounds(h_, lo, hi));
+    std::int64_t info[6];
+    check(thg_bvh_info(h_, info));
+    if (info[3] == 0) return;
+    for (int a = 0; a < 3; ++a) {
+      const float c = 0.5f * (hi[a] + lo[a]);
+      lo[a] = 1.05f * (lo[a] - c) + c;
+      hi[a] = 1.05f * (hi[a] - c) + c;
+    }
+  }
+
   // Batch: idx[n], optional dist[n], closest points cp[3n].
   void closestElements(intx n, const float *q, std::int32_t *idx, float *dist = nullptr, float *cp = nullptr,
                        double dist_bound = -1) const {
@@ -341,6 +355,53 @@ class Icp3 {
 
  private:
   thg_icp_opts opts_;
+};
+
+// Thea's ShapeDiameter (Algorithms/SurfaceFeatures/Local/ShapeDiameter.hpp:40-265) over a
+// caller-built MeshBvh, as its precomputed-BVH constructor (:110-120): a normalisation scale <= 0
+// defaults to the diagonal of the BVH's bounds. compute() evaluates one point (a batch of one);
+// computeBatch() is the fast path.
+class ShapeDiameter {
+ public:
+  explicit ShapeDiameter(MeshBvh const *bvh, float normalization_scale = -1) : bvh_(bvh), scale_(normalization_scale) {
+    if (!bvh_) throw Error(THG_E_INVALID, "ShapeDiameter: precomputed BVH cannot be null");
+    if (scale_ <= 0) {
+      float lo[3], hi[3];
+      bvh_->getBounds(lo, hi);
+      const float e[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+      scale_ = std::sqrt(e[0] * e[0] + (e[1] * e[1] + e[2] * e[2]));
+    }
+  }
+  float getScale() const { return scale_; }
+
+  // compute(position): the normal of the nearest face (:162-176).
+  template <typename VectorT> double compute(VectorT const &position, bool only_hit_interior_surfaces = true) const {
+    const float p[3] = {(float)position[0], (float)position[1], (float)position[2]};
+    double sdf = -1;
+    computeBatch(1, p, nullptr, &sdf, only_hit_interior_surfaces);
+    return sdf;
+  }
+
+  // compute(position, normal) with an outward normal (:185-256).
+  template <typename VectorT>
+  double compute(VectorT const &position, VectorT const &normal, bool only_hit_interior_surfaces = true) const {
+    const float p[3] = {(float)position[0], (float)position[1], (float)position[2]};
+    const float nm[3] = {(float)normal[0], (float)normal[1], (float)normal[2]};
+    double sdf = -1;
+    computeBatch(1, p, nm, &sdf, only_hit_interior_surfaces);
+    return sdf;
+  }
+
+  // positions[3n], normals[3n] or nullptr (nearest face normals), sdf[n].
+  void computeBatch(intx n, const float *positions, const float *normals, double *sdf,
+                    bool only_hit_interior_surfaces = true) const {
+    check(thg_shape_diameter(bvh_->handle(), positions, normals, n, scale_, only_hit_interior_surfaces ? 1 : 0,
+                             THG_MEM_HOST, nullptr, sdf, nullptr));
+  }
+
+ private:
+  MeshBvh const *bvh_;
+  float scale_;
 };
 
 }  // namespace thea_gpu

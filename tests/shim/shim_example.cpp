@@ -51,6 +51,11 @@ int main() {
     double err = -1;
     Icp3::Transform tr = icp.align((intx)moved.size(), moved.data(), bvh, &err);
     std::printf("icp %.6f %.6f %.6f %.3g\n", tr[3], tr[7], tr[11], err);
+    // Shape diameter at the first samples (ShapeDiameter over a precomputed BVH, default scale)
+    ShapeDiameter sdf(&bvh);
+    std::printf("scale %.9g\n", sdf.getScale());
+    for (intx i = 0; i < got; ++i)
+      std::printf("sdf %lld %.17g %.17g\n", (long long)i, sdf.compute(pos[i], nrm[i]), sdf.compute(pos[i]));
   } catch (const Error &e) {
     std::printf("error %d %s\n", e.code(), e.what());
     return 2;

@@ -1,0 +1,75 @@
+"""The C++ host API (include/thea_gpu_bvh.hpp), driven as a Thea caller would drive it
+(tests/shim/shim_example.cpp, built by the package build), gives the same results as the Python
+binding over the same C ABI."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+VERTS = np.array([0, 0, 0, 1, 1, 0, 1, 0, 0, 0, 1, 0, 0, 1, 1, 1, 1, 1, 0, 0, 1, 1, 0, 1], np.float32).reshape(-1, 3)
+TRIS = np.array([0, 1, 2, 0, 3, 1, 0, 4, 3, 0, 6, 4, 3, 5, 1, 3, 4, 5,
+                 2, 1, 5, 2, 5, 7, 0, 2, 7, 0, 7, 6, 6, 7, 5, 6, 5, 4], np.int32).reshape(-1, 3)
+
+
+def run_shim():
+    exe = os.path.join(ROOT, "tests", "shim", "shim_example")
+    assert os.path.exists(exe), "tests/shim/shim_example is built by __graft_entry__.build()"
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    lines = {}
+    for line in out.stdout.splitlines():
+        key, *rest = line.split()
+        lines.setdefault(key, []).append(rest)
+    return lines
+
+
+def test_cpp_api_matches_python_binding(oracle):
+    import thea_amd
+    print("case C++ shim on the unit cube", flush=True)
+    got = run_shim()
+    mesh = thea_amd.Mesh(VERTS, TRIS)
+    bvh = thea_amd.MeshBvh(mesh)
+    f = np.float32  # the shim builds its queries in float arithmetic
+    q = np.array([[f(-0.5) + f(0.5) * f(i), f(0.3), f(1.7) - f(0.4) * f(i)] for i in range(5)], np.float32)
+    idx, dist, cp, _ = bvh.closest(q)
+    for i, row in enumerate(got["closest"]):
+        assert int(row[0]) == i and int(row[1]) == idx[i]
+        assert np.float32(float(row[2])) == dist[i]
+        assert np.array_equal(np.array([float(x) for x in row[3:]], np.float32), cp[i])
+    kidx, kdist = bvh.kClosestElements(np.array([[0.5, 0.5, 2.0]], np.float32), 3)
+    assert [int(r[0]) for r in got["knn"]] == list(kidx[0])
+    assert np.array_equal(np.array([float(r[1]) for r in got["knn"]], np.float32), kdist[0])
+    ridx, rt, _ = bvh.rayStructureIntersection(np.array([[0.25, 0.5, 3.0]], np.float32),
+                                               np.array([[0.0, 0.0, -1.0]], np.float32))
+    hit = got["ray"][0]
+    assert int(hit[0]) == ridx[0] and np.float32(float(hit[1])) == rt[0] and hit[2] == "0"
+    sampler = thea_amd.MeshSampler(mesh)
+    pos, nrm, tri = sampler.sampleEvenlyByArea(4, seed=7, stream_id=0)
+    assert [int(r[0]) for r in got["sample"]] == list(tri)
+    assert np.array_equal(np.array([[float(x) for x in r[1:]] for r in got["sample"]], np.float32), pos)
+    # ICP of 2000 further samples shifted by +0.05 in x back onto the cube, least-squares fit
+    surf, _, _ = sampler.sampleEvenlyByArea(2000, seed=7, stream_id=1)
+    moved = surf.astype(np.float64) + np.array([0.05, 0.0, 0.0])
+    tr, err = thea_amd.Icp3(kabsch=True).align(moved, bvh)
+    tx, ty, tz, cerr = (float(x) for x in got["icp"][0])
+    assert abs(tx - tr[0, 3]) <= 1e-6 and abs(ty - tr[1, 3]) <= 1e-6 and abs(tz - tr[2, 3]) <= 1e-6
+    assert abs(cerr - err) <= 1e-3 * err
+    assert -0.06 < tx < -0.02  # moved back towards the cube (ICP slides on the faces parallel to x)
+    # ShapeDiameter with the default scale: the diagonal of the 1.05-padded unit cube
+    scale = float(got["scale"][0][0])
+    assert abs(scale - 1.05 * np.sqrt(3.0)) < 1e-6
+    lo, hi = bvh.getBounds()
+    assert np.allclose(lo, -0.025) and np.allclose(hi, 1.025)
+    ref_n = bvh.shapeDiameter(pos, nrm)
+    ref_p = bvh.shapeDiameter(pos)
+    assert [float(r[1]) for r in got["sdf"]] == list(ref_n)
+    assert [float(r[2]) for r in got["sdf"]] == list(ref_p)
+    e = hi - lo
+    oracle_sdf, _ = oracle.shape_diameter(oracle.KdTree(VERTS, TRIS), oracle.tri_props(VERTS, TRIS)[:, :3], pos, nrm,
+                                          float(np.sqrt(e[0] * e[0] + (e[1] * e[1] + e[2] * e[2]))))
+    assert np.array_equal(ref_n, oracle_sdf)
+    assert np.all((ref_n > 0) & (ref_n <= 1.0))

@@ -66,7 +66,8 @@ class _Args:
             return None, None
         if self.device_mode:
             torch = _torch()
-            tdt = {np.float32: torch.float32, np.int32: torch.int32, np.uint8: torch.uint8}[dtype]
+            tdt = {np.float32: torch.float32, np.float64: torch.float64, np.int32: torch.int32,
+                   np.uint8: torch.uint8}[dtype]
             t = torch.empty(shape, dtype=tdt, device=self.torch_device)
             return t, C.c_void_p(t.data_ptr()) if t.numel() else None
         arr = np.empty(shape, dtype=dtype)
@@ -169,6 +170,17 @@ class _BvhBase:
         capi.check(capi.lib().thg_bvh_export(self.h, raw.ctypes.data_as(C.c_void_p), ids.ctypes.data_as(C.c_void_p)))
         return raw.view(np.float32), raw, ids
 
+    def getBounds(self):
+        """(lo, hi) float32: BvhN::getBounds (BvhN.hpp:991), the union of the element boxes scaled by
+        1.05 about its centre in float arithmetic (AxisAlignedBoxN.hpp:274,316-324; BvhN.hpp:1437)."""
+        lo, hi = np.zeros(3, np.float32), np.zeros(3, np.float32)
+        capi.check(capi.lib().thg_bvh_bounds(self.h, lo.ctypes.data_as(C.c_void_p), hi.ctypes.data_as(C.c_void_p)))
+        if self.info["elements"] == 0:
+            return lo, hi
+        c = np.float32(0.5) * (hi + lo)
+        f = np.float32(1.05)
+        return f * (lo - c) + c, f * (hi - c) + c
+
     def close(self):
         if getattr(self, "h", None):
             capi.lib().thg_bvh_destroy(self.h)
@@ -238,11 +250,18 @@ class MeshBvh(_BvhBase):
     def shapeDiameter(self, positions, normals=None, scale=None, only_hit_interior_surfaces=True,
                       want_ray_times=False):
         """ShapeDiameter::compute (SurfaceFeatures/Local/ShapeDiameter.hpp:162-256) at each position:
-        SDF in [0, 1] (numpy float64), -1 where no ray counts. normals=None takes the normal of the
-        nearest face. scale (> 0, required): the normalisation length (Thea's default is the bounding
-        sphere diameter of the mesh). want_ray_times adds the 30 per-point ray values (-1: not counted)."""
+        SDF in [0, 1] (float64), -1 where no ray counts. normals=None takes the normal of the nearest
+        face. scale: the normalisation length; None or <= 0 takes the diagonal of getBounds(), the
+        default of Thea's constructor over a precomputed BVH (ShapeDiameter.hpp:110-120; its mesh
+        constructors default to the bounding-sphere diameter instead). want_ray_times adds the 30
+        per-point ray values (-1: not counted). Results are numpy arrays for host inputs and device
+        tensors for device inputs."""
         if scale is None or scale <= 0:
-            raise ValueError("scale must be a positive length")
+            lo, hi = self.getBounds()
+            e = hi - lo
+            scale = float(np.sqrt(e[0] * e[0] + (e[1] * e[1] + e[2] * e[2])))
+            if not scale > 0:
+                raise ValueError("the BVH has an empty box; pass a positive scale")
         dev = _is_torch(positions) and positions.is_cuda
         a = _Args(dev, self.ctx.torch_device() if dev else None)
         _, pp, n = a.inp(positions, np.float32, 3)
@@ -251,11 +270,10 @@ class MeshBvh(_BvhBase):
             _, npp, nn = a.inp(normals, np.float32, 3)
             if nn != n:
                 raise ValueError("one normal per position")
-        sdf = np.zeros(n, np.float64)
-        times = np.zeros((n, 30), np.float32) if want_ray_times else None
+        sdf, sp = a.out((n,), np.float64)
+        times, tp = a.out((n, 30), np.float32, want_ray_times)
         capi.check(capi.lib().thg_shape_diameter(self.h, pp, npp, n, float(scale), 1 if only_hit_interior_surfaces else 0,
-                                                 a.mem, a.stream, sdf.ctypes.data_as(C.c_void_p),
-                                                 None if times is None else times.ctypes.data_as(C.c_void_p)))
+                                                 a.mem, a.stream, sp, tp))
         return (sdf, times) if want_ray_times else sdf
 
     # -- rays (BvhN::rayStructureIntersection / rayIntersectionTime / rayIntersects) --------------

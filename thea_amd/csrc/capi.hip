@@ -264,6 +264,15 @@ int thg_bvh_info(thg_bvh bvh, int64_t out[6]) {
   });
 }
 
+int thg_bvh_bounds(thg_bvh bvh, float lo[3], float hi[3]) {
+  return guarded([&] {
+    require(bvh && lo && hi, "bvh/lo/hi is NULL");
+    const Bvh *b = reinterpret_cast<const Bvh *>(bvh);
+    lo[0] = b->lo.x, lo[1] = b->lo.y, lo[2] = b->lo.z;
+    hi[0] = b->hi.x, hi[1] = b->hi.y, hi[2] = b->hi.z;
+  });
+}
+
 __global__ void k_record_ids(int64_t n, const void *__restrict__ elems, int points, int32_t *__restrict__ ids) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;

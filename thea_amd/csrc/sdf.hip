@@ -1,22 +1,21 @@
 // Shape diameter function: Thea's ShapeDiameter::compute (Algorithms/SurfaceFeatures/Local/
 // ShapeDiameter.hpp:185-256; Gal, Shamir and Cohen-Or, "Pose-Oblivious Shape Signature", TVCG 2007)
-// for a batch of surface points against a mesh BVH.
+// for a batch of surface points against a mesh BVH, entirely on the device.
 //
 // Per point: in = -normalize(normal); rot = orthonormalBasis(in) (MatrixUtil.hpp:218-246); 30 rays
 // from position + 0.001 scale in along rot * CONE_DIRS[i]; a hit counts if its face normal does not
-// oppose the ray (only_interior). The rays are generated and traced on the GPU (the first-hit
-// kernel); the per-point reduction runs on the host exactly as Thea writes it: the median by
-// std::nth_element on the valid hit times, the variance of all of them, and the weighted mean of
-// the times within one standard deviation of the median, where weight i is the cone direction's z
-// of the i-th valid ray while the times have been permuted by nth_element (ShapeDiameter.hpp:226-252).
-#include <algorithm>
+// oppose the ray (only_interior). The reduction follows Thea exactly: the median by std::nth_element
+// on the valid hit times (the permutation libstdc++ leaves, restated in select.hpp), the variance of
+// all of them, and the weighted mean of the times within one standard deviation of the median, where
+// weight i is the cone direction's z of the i-th valid ray while the times have been permuted by
+// nth_element (ShapeDiameter.hpp:226-252). Sums run in double in Thea's order.
 #include <cmath>
-#include <vector>
 
 #include "common.hpp"
 #include "device_types.hpp"
 #include "kernels_common.hpp"
 #include "query.hpp"
+#include "select.hpp"
 #include "tri_math.hpp"
 
 namespace thg {
@@ -36,13 +35,6 @@ __constant__ float c_cone[SDF_RAYS][3] = {
     {-0.574364f, -0.463887f, 0.674474f},  {-0.459207f, 0.260125f, 0.849390f},   {-0.628057f, -0.528928f, 0.570771f},
     {0.232916f, -0.328273f, 0.915416f},   {0.231667f, -0.796882f, 0.557951f},   {-0.270612f, -0.809654f, 0.520797f},
 };
-// The same table on the host (weights of the reduction).
-static const float h_cone_z[SDF_RAYS] = {0.903309f, 0.553688f, 0.632560f, 0.561081f, 0.641647f, 0.689520f,
-                                         0.641087f, 0.651811f, 0.504912f, 0.732200f, 0.661817f, 0.907834f,
-                                         0.737122f, 0.950954f, 0.825989f, 0.635164f, 0.709082f, 0.890588f,
-                                         0.995939f, 0.614159f, 0.687065f, 0.514752f, 0.991506f, 0.885227f,
-                                         0.674474f, 0.849390f, 0.570771f, 0.915416f, 0.557951f, 0.520797f};
-
 __device__ __forceinline__ float3 normalized3(float3 v) {  // v / |v| (Eigen normalized: divide by norm)
   const float n = sqrtf(sqn3(v));
   return f3(v.x / n, v.y / n, v.z / n);
@@ -109,37 +101,60 @@ __global__ void __launch_bounds__(256) k_sdf_valid(int64_t m, const int32_t *__r
   val[g] = ok ? t[g] : -1.0f;
 }
 
-// Host reduction of one point (ShapeDiameter.hpp:230-256).
-static double sdf_reduce(const float *v30, double scale) {
-  double values[SDF_RAYS], weights[SDF_RAYS];
+constexpr int SDF_RED_THREADS = 256;
+
+// One thread per point: the point's 30 ray values are staged through LDS (coalesced global reads),
+// compacted to the valid ones with their ray numbers, and reduced as ShapeDiameter.hpp:230-256.
+__global__ void __launch_bounds__(SDF_RED_THREADS) k_sdf_reduce(int64_t n, const float *__restrict__ val, double scale,
+                                                                double *__restrict__ sdf) {
+  __shared__ float sv[SDF_RAYS * SDF_RED_THREADS];
+  __shared__ uint8_t sr[SDF_RAYS * SDF_RED_THREADS];
+  const int t = threadIdx.x;
+  const int64_t p0 = (int64_t)blockIdx.x * SDF_RED_THREADS;
+  const int np = (int)min((int64_t)SDF_RED_THREADS, n - p0);
+  const float *src = val + p0 * SDF_RAYS;
+  for (int j = t; j < np * SDF_RAYS; j += SDF_RED_THREADS) sv[j] = src[j];
+  __syncthreads();
+  if (t >= np) return;
+  float *mine = sv + t * SDF_RAYS;
+  uint8_t *rays = sr + t * SDF_RAYS;
   int nv = 0;
-  for (int r = 0; r < SDF_RAYS; ++r)
-    if (v30[r] >= 0) {
-      values[nv] = (double)v30[r];
-      weights[nv] = (double)h_cone_z[r];
+  for (int r = 0; r < SDF_RAYS; ++r) {
+    const float x = mine[r];
+    if (x >= 0) {
+      mine[nv] = x;
+      rays[nv] = (uint8_t)r;
       ++nv;
     }
-  if (nv <= 0) return -1.0;
+  }
+  if (nv <= 0) {
+    sdf[p0 + t] = -1.0;
+    return;
+  }
   const int mid = nv / 2;
-  std::nth_element(values, values + mid, values + nv);
+  nth_element_gcc(StridedF{mine, 1}, mid, nv);
   double sum_values = 0, sum_squares = 0;
   for (int i = 0; i < nv; ++i) {
-    sum_values += values[i];
-    sum_squares += (values[i] * values[i]);
+    const double x = mine[i];
+    sum_values += x;
+    sum_squares += x * x;
   }
   const double avg = sum_values / nv;
   const double var = sum_squares / nv - avg * avg;
+  const double vmid = mine[mid];
   sum_values = 0;
   double sum_weights = 0;
   for (int i = 0; i < nv; ++i) {
-    const double dv = values[i] - values[mid];
+    const double x = mine[i];
+    const double dv = x - vmid;
     if (dv * dv <= var) {
-      sum_values += (weights[i] * values[i]);
-      sum_weights += weights[i];
+      const double w = c_cone[rays[i]][2];
+      sum_values += w * x;
+      sum_weights += w;
     }
   }
-  const double x = sum_weights > 0 ? (double)(sum_values / (sum_weights * scale)) : (double)(values[mid] / scale);
-  return std::min(1.0, std::max(0.0, x));
+  const double y = sum_weights > 0 ? sum_values / (sum_weights * scale) : vmid / scale;
+  sdf[p0 + t] = fmin(1.0, fmax(0.0, y));
 }
 
 }  // namespace thg
@@ -166,8 +181,8 @@ extern "C" int thg_shape_diameter(thg_bvh mesh_bvh, const float *positions, cons
                                   float scale, int32_t only_interior, int32_t mem, void *stream, double *sdf,
                                   float *ray_times) {
   return sdf_guarded([&] {
-    require(mesh_bvh != nullptr && sdf != nullptr, "bvh/sdf is NULL");
-    require(n >= 0 && (n == 0 || positions), "bad point array");
+    require(mesh_bvh != nullptr, "bvh is NULL");
+    require(n >= 0 && (n == 0 || (positions && sdf)), "bad point or result array");
     require(scale > 0, "scale must be positive");
     require(mem == THG_MEM_DEVICE || mem == THG_MEM_HOST, "mem must be THG_MEM_DEVICE or THG_MEM_HOST");
     Bvh *b = reinterpret_cast<Bvh *>(mesh_bvh);
@@ -176,7 +191,7 @@ extern "C" int thg_shape_diameter(thg_bvh mesh_bvh, const float *positions, cons
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (n == 0) return;
     const int64_t m = n * SDF_RAYS;
-    AsyncBuf pos_d, nrm_d, org(m * 12, s), dir(m * 12, s), ridx(m * 4, s), rt(m * 4, s), rn(m * 12, s), val(m * 4, s);
+    AsyncBuf pos_d, nrm_d, org(m * 12, s), dir(m * 12, s), ridx(m * 4, s), rt(m * 4, s), rn(m * 12, s), val;
     const float *d_pos = positions, *d_nrm = normals;
     if (mem == THG_MEM_HOST) {
       pos_d.ensure(n * 12, s);
@@ -206,13 +221,28 @@ extern "C" int thg_shape_diameter(thg_bvh mesh_bvh, const float *positions, cons
     QueryScratch ws;
     launch_ray(*b, org.as<float>(), dir.as<float>(), m, -1.0f, false, ridx.as<int32_t>(), rt.as<float>(), rn.as<float>(),
                nullptr, ws, s);
+    // The per-ray values go straight into ray_times when the caller wants them in device memory.
+    float *d_val = (ray_times && mem == THG_MEM_DEVICE) ? ray_times : nullptr;
+    if (!d_val) {
+      val.ensure(m * 4, s);
+      d_val = val.as<float>();
+    }
     hipLaunchKernelGGL(k_sdf_valid, dim3(ceil_div(m, 256)), dim3(256), 0, s, m, ridx.as<int32_t>(), rt.as<float>(),
-                       rn.as<float>(), dir.as<float>(), only_interior ? 1 : 0, val.as<float>());
+                       rn.as<float>(), dir.as<float>(), only_interior ? 1 : 0, d_val);
     launch_check("k_sdf_valid");
-    std::vector<float> hv((size_t)m);
-    THG_HIP(hipMemcpyAsync(hv.data(), val.p, m * 4, hipMemcpyDeviceToHost, s));
-    THG_HIP(hipStreamSynchronize(s));
-    for (int64_t i = 0; i < n; ++i) sdf[i] = sdf_reduce(&hv[(size_t)i * SDF_RAYS], (double)scale);
-    if (ray_times) std::copy(hv.begin(), hv.end(), ray_times);
+    double *d_sdf = sdf;
+    AsyncBuf sdf_d;
+    if (mem == THG_MEM_HOST) {
+      sdf_d.ensure(n * 8, s);
+      d_sdf = sdf_d.as<double>();
+    }
+    hipLaunchKernelGGL(k_sdf_reduce, dim3(ceil_div(n, SDF_RED_THREADS)), dim3(SDF_RED_THREADS), 0, s, n,
+                       (const float *)d_val, (double)scale, d_sdf);
+    launch_check("k_sdf_reduce");
+    if (mem == THG_MEM_HOST) {
+      THG_HIP(hipMemcpyAsync(sdf, d_sdf, n * 8, hipMemcpyDeviceToHost, s));
+      if (ray_times) THG_HIP(hipMemcpyAsync(ray_times, d_val, m * 4, hipMemcpyDeviceToHost, s));
+      THG_HIP(hipStreamSynchronize(s));
+    }
   });
 }
