@@ -250,6 +250,25 @@ typedef struct thg_graph_opts {
 int thg_proximity_graph(thg_bvh points, const float *normals, const thg_graph_opts *opts, int32_t mem, void *stream,
                         int32_t *nbr_idx, float *nbr_sep, int32_t *degree, float separation[2]);
 
+/* FurthestPointSampling::subsample with geodesic distances (Algorithms/FurthestPointSampling.cpp:
+ * 47-125): the proximity graph of the n points (as thg_proximity_graph without normals; graph_opts
+ * nullable), then k picks starting with point 0, each the point furthest along the graph from the
+ * picks so far (unreachable points first, the highest index among them; at equal maximum distance
+ * the highest index). points in `mem`, selected (k int64) in `mem`; *count (host) = the number of
+ * picks made, fewer than k when every remaining point is at distance 0 (Thea reports an error and
+ * returns that count). Requires k <= n < 2^30. */
+int thg_furthest_point_sampling(thg_ctx ctx, const float *points, int64_t n, int64_t k,
+                                const thg_graph_opts *graph_opts, int32_t mem, void *stream, int64_t *selected,
+                                int64_t *count);
+
+/* MeshSampler::sampleEvenlyBySeparation (Algorithms/MeshSampler.hpp:270-329): ceil(factor * n)
+ * area samples (thg_sample_area with seed/stream_id; factor < 0 selects Thea's 3), then n of them by
+ * thg_furthest_point_sampling. pos (3n), normal (nullable, 3n), tri (nullable, n) in `mem`;
+ * *n_out = n, or 0 when the mesh has no area or fewer than n picks were possible (Thea returns 0). */
+int thg_sample_separation(thg_mesh mesh, int64_t n, float oversampling_factor, uint64_t seed, uint64_t stream_id,
+                          const thg_graph_opts *graph_opts, float *pos, float *normal, int32_t *tri, int64_t *n_out,
+                          int32_t mem, void *stream);
+
 /* Shape diameter function at n surface points: ShapeDiameter::compute(position, normal,
  * only_hit_interior_surfaces) (Algorithms/SurfaceFeatures/Local/ShapeDiameter.hpp:185-256), or
  * compute(position) with the normal of the nearest face when `normals` is NULL (:162-176). scale:

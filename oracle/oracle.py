@@ -63,6 +63,7 @@ def lib():
         "or_sdf_rays": (None, [f32p, f32p, i64, f32, f32p, f32p]),
         "or_sdf_reduce": (None, [f32p, i64, f64, f64p]),
         "or_proximity_graph": (i32, [f32p, vp, i64, i32, i64p, i64, i32p, f32p, i32p, f32p]),
+        "or_fps": (i64, [i32p, f32p, i64, i32, i64, i64p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -293,6 +294,17 @@ def proximity_graph(points, normals=None, max_degree=8, picks=None):
     if rc != 0:
         raise ValueError("or_proximity_graph failed")
     return idx, sep, deg, (float(avg[0]), float(avg[1]))
+
+
+def fps(idx, sep, k):
+    """FurthestPointSampling::subsample over a proximity graph (idx, sep: (n, max_degree)) restated
+    with a per-pick multi-source Dijkstra (see or_fps). Returns the picked indices (int64)."""
+    idx = np.ascontiguousarray(idx, np.int32)
+    sep = _f32(sep)
+    n, md = idx.shape
+    sel = np.zeros(max(k, 1), np.int64)
+    got = lib().or_fps(idx, sep, n, md, k, sel)
+    return sel[:got]
 
 
 def shape_diameter(kd, tri_normals, positions, normals, scale, only_interior=True):

@@ -25,6 +25,7 @@
 #include <functional>
 #include <limits>
 #include <map>
+#include <queue>
 #include <thread>
 #include <utility>
 #include <vector>
@@ -1332,5 +1333,59 @@ void or_sdf_reduce(const float *vals, int64_t n, double scale, double *out) {
     const double x = sw > 0 ? sv / (sw * scale) : values[mid] / scale;
     out[i] = std::min(1.0, std::max(0.0, x));
   }
+}
+
+// FurthestPointSampling::subsample with geodesic distances (FurthestPointSampling.cpp:47-125) over a
+// given directed proximity graph (idx/sep: n x md, -1 padded): pick 0 first; for every later pick,
+// a multi-source Dijkstra from all picks so far (ShortestPaths.hpp:200-360: sources at distance 0,
+// relaxation d(u) + w with w the float separation widened to double, strict <), and the vertex the
+// callback sees last: the last unreachable vertex in index order if any, else the last vertex
+// popped (a binary heap ordered by (distance, index), so among equal maximum distances the highest
+// index -- Thea's Fibonacci heap orders equal keys its own way, which no test can pin). Stops when
+// that vertex is at distance 0 (already picked, FurthestPointSampling.cpp:87-91). Returns the
+// number of picks written to sel.
+int64_t or_fps(const int32_t *idx, const float *sep, int64_t n, int md, int64_t k, int64_t *sel) {
+  if (k <= 0 || n <= 0) return 0;
+  const double INF = std::numeric_limits<double>::infinity();
+  std::vector<double> dist((size_t)n);
+  std::vector<char> black((size_t)n);
+  std::vector<int64_t> picks;
+  picks.push_back(0);
+  sel[0] = 0;
+  typedef std::pair<double, int64_t> Item;
+  for (int64_t i = 1; i < k; ++i) {
+    std::fill(dist.begin(), dist.end(), INF);
+    std::fill(black.begin(), black.end(), 0);
+    std::priority_queue<Item, std::vector<Item>, std::greater<Item>> heap;
+    for (int64_t p : picks) {
+      dist[(size_t)p] = 0;
+      heap.push(Item(0.0, p));
+    }
+    int64_t last = -1;
+    while (!heap.empty()) {
+      const Item top = heap.top();
+      heap.pop();
+      const int64_t u = top.second;
+      if (black[(size_t)u] || top.first != dist[(size_t)u]) continue;  // stale entry
+      for (int e = 0; e < md; ++e) {
+        const int32_t v = idx[u * md + e];
+        if (v < 0) break;
+        if (black[(size_t)v]) continue;
+        const double test = (double)sep[u * md + e] + dist[(size_t)u];
+        if (test < dist[(size_t)v]) {
+          dist[(size_t)v] = test;
+          heap.push(Item(test, v));
+        }
+      }
+      black[(size_t)u] = 1;
+      last = u;
+    }
+    for (int64_t v = 0; v < n; ++v)
+      if (!black[(size_t)v]) last = v;  // include_unreachable (ShortestPaths.hpp:350-359)
+    if (last < 0 || dist[(size_t)last] == 0) return i;
+    picks.push_back(last);
+    sel[i] = last;
+  }
+  return k;
 }
 }  // extern "C"

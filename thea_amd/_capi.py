@@ -90,6 +90,9 @@ SIGNATURES = {
     "thg_ctx_set_timing": (C.c_int, [vp, i32]),
     "thg_ctx_kernel_time": (C.c_int, [vp, C.c_char_p, C.POINTER(f64), C.POINTER(i64), i32]),
     "thg_proximity_graph": (C.c_int, [vp, vp, C.POINTER(GraphOpts), i32, vp, vp, vp, vp, vp]),
+    "thg_furthest_point_sampling": (C.c_int, [vp, vp, i64, i64, C.POINTER(GraphOpts), i32, vp, vp, C.POINTER(i64)]),
+    "thg_sample_separation": (C.c_int, [vp, i64, f32, u64, u64, C.POINTER(GraphOpts), vp, vp, vp, C.POINTER(i64), i32,
+                                        vp]),
     "thg_shape_diameter": (C.c_int, [vp, vp, vp, i64, C.c_float, i32, i32, vp, vp, vp]),
     "thg_icp_align": (C.c_int, [vp, vp, i64, i32, vp, C.POINTER(IcpOpts), i32, vp, vp, vp, vp]),
     "thg_server_create": (C.c_int, [vp, C.POINTER(ServerOpts), C.POINTER(vp)]),

@@ -363,6 +363,46 @@ class MeshSampler:
 
     sampleUniformRandom = sampleEvenlyByArea
 
+    def sampleEvenlyBySeparation(self, n, seed=0, stream_id=0, oversampling_factor=-1.0, max_degree=8,
+                                 separation_trials=100, graph_seed=0, device=False):
+        """MeshSampler::sampleEvenlyBySeparation (MeshSampler.hpp:270-329): (positions[n,3],
+        face_normals[n,3], triangle_indices[n]) of n furthest-point picks among ceil(factor * n)
+        area samples (sampleEvenlyByArea with seed/stream_id; factor < 0: 3). Empty arrays when the
+        mesh has no area or fewer than n picks were possible, as Thea returns 0."""
+        a = _Args(device, self.mesh.ctx.torch_device() if device else None)
+        pos, pp = a.out((n, 3), np.float32)
+        nrm, np_ = a.out((n, 3), np.float32)
+        tri, tp = a.out((n,), np.int32)
+        got = C.c_int64()
+        o = capi.GraphOpts(max_degree=max_degree, separation_trials=separation_trials, seed=graph_seed)
+        capi.check(capi.lib().thg_sample_separation(self.mesh.h, int(n), float(oversampling_factor), C.c_uint64(seed),
+                                                    C.c_uint64(stream_id), C.byref(o), pp, np_, tp, C.byref(got),
+                                                    a.mem, a.stream))
+        m = got.value
+        return pos[:m], nrm[:m], tri[:m]
+
+
+def furthestPointSampling(points, k, max_degree=8, separation_trials=100, graph_seed=0, ctx=None):
+    """FurthestPointSampling::subsample with geodesic distances (FurthestPointSampling.cpp:47-125):
+    indices (int64) of k points picked greedily along the points' proximity graph, starting with
+    point 0. Fewer than k when every remaining point is at distance 0. Device tensors in, device
+    tensor out."""
+    ctx = ctx or default_context()
+    dev = _is_torch(points) and points.is_cuda
+    a = _Args(dev, ctx.torch_device() if dev else None)
+    _, pp, n = a.inp(points, np.float32, 3)
+    if dev:
+        sel = _torch().empty((k,), dtype=_torch().int64, device=ctx.torch_device())
+        sp = C.c_void_p(sel.data_ptr()) if k else None
+    else:
+        sel = np.empty(k, np.int64)
+        sp = sel.ctypes.data_as(C.c_void_p) if k else None
+    got = C.c_int64()
+    o = capi.GraphOpts(max_degree=max_degree, separation_trials=separation_trials, seed=graph_seed)
+    capi.check(capi.lib().thg_furthest_point_sampling(ctx.h, pp, n, int(k), C.byref(o), a.mem, a.stream, sp,
+                                                      C.byref(got)))
+    return sel[:got.value]
+
 
 def version():
     return capi.lib().thg_version().decode()

@@ -37,33 +37,6 @@ template <typename F> static int guarded(F f) {
 
 static hipStream_t as_stream(void *s) { return static_cast<hipStream_t>(s); }
 
-// Host-pointer staging: copies `in` arrays to device and results back.
-struct Stage {
-  hipStream_t s;
-  bool host;
-  std::vector<std::unique_ptr<AsyncBuf>> bufs;
-  struct Out { void *host; void *dev; size_t bytes; };
-  std::vector<Out> outs;
-  Stage(hipStream_t st, bool h) : s(st), host(h) {}
-  template <typename T> const T *in(const T *p, size_t count) {
-    if (!host || !p || count == 0) return p;
-    bufs.emplace_back(new AsyncBuf(count * sizeof(T), s));
-    THG_HIP(hipMemcpyAsync(bufs.back()->p, p, count * sizeof(T), hipMemcpyHostToDevice, s));
-    return bufs.back()->as<T>();
-  }
-  template <typename T> T *out(T *p, size_t count) {
-    if (!host || !p || count == 0) return p;
-    bufs.emplace_back(new AsyncBuf(count * sizeof(T), s));
-    outs.push_back({p, bufs.back()->p, count * sizeof(T)});
-    return bufs.back()->as<T>();
-  }
-  void finish() {
-    if (!host) return;
-    for (auto &o : outs) THG_HIP(hipMemcpyAsync(o.host, o.dev, o.bytes, hipMemcpyDeviceToHost, s));
-    THG_HIP(hipStreamSynchronize(s));
-  }
-};
-
 static void set_device(Ctx *c) { THG_HIP(hipSetDevice(c->device)); }
 
 static void check_mem(int32_t mem) { require(mem == THG_MEM_DEVICE || mem == THG_MEM_HOST, "mem must be THG_MEM_DEVICE or THG_MEM_HOST"); }

@@ -106,6 +106,34 @@ struct AsyncBuf {
   template <typename T> T *as() const { return static_cast<T *>(p); }
 };
 
+// Host-memory staging for one call: inputs copied to device buffers, outputs written to device
+// buffers and copied back in finish().
+struct Stage {
+  hipStream_t s;
+  bool host;
+  std::vector<std::unique_ptr<AsyncBuf>> bufs;
+  struct Out { void *host; void *dev; size_t bytes; };
+  std::vector<Out> outs;
+  Stage(hipStream_t st, bool h) : s(st), host(h) {}
+  template <typename T> const T *in(const T *p, size_t count) {
+    if (!host || !p || count == 0) return p;
+    bufs.emplace_back(new AsyncBuf(count * sizeof(T), s));
+    THG_HIP(hipMemcpyAsync(bufs.back()->p, p, count * sizeof(T), hipMemcpyHostToDevice, s));
+    return bufs.back()->as<T>();
+  }
+  template <typename T> T *out(T *p, size_t count) {
+    if (!host || !p || count == 0) return p;
+    bufs.emplace_back(new AsyncBuf(count * sizeof(T), s));
+    outs.push_back({p, bufs.back()->p, count * sizeof(T)});
+    return bufs.back()->as<T>();
+  }
+  void finish() {
+    if (!host) return;
+    for (auto &o : outs) THG_HIP(hipMemcpyAsync(o.host, o.dev, o.bytes, hipMemcpyDeviceToHost, s));
+    THG_HIP(hipStreamSynchronize(s));
+  }
+};
+
 // Grow-only scratch owned by a context, one per (context, purpose). Calls that use the same
 // context's scratch are serialised by `lock`.
 struct Scratch {

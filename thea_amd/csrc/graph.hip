@@ -170,6 +170,58 @@ template <typename F> static int graph_guarded(F f) {
   }
 }
 
+namespace thg {
+
+// The proximity graph of a point BVH's samples into device arrays (nbr_idx, nbr_sep: n x md; degree:
+// n); returns the initial separation estimate. normals: device, nullable.
+float graph_build(Bvh *b, const float *d_nrm, int md, int64_t trials, uint64_t seed, hipStream_t s, int32_t *d_idx,
+                  float *d_sep, int32_t *d_deg) {
+  const int64_t n = b->n;
+  require(n >= 2, "the proximity graph needs at least two samples (PointSet3.hpp:350-351)");
+  require(md >= 1 && md <= 32, "max_degree must be 1..32");
+  require(b->max_depth < 64, "point BVH deeper than the 64-entry traversal stack");
+  trials = std::min<int64_t>(trials, n);
+  // scatter the samples back to input order
+  AsyncBuf pos(n * 12, s), picks_d(trials * 8, s), near_d(trials * 4, s);
+  hipLaunchKernelGGL(k_scatter_points, dim3(ceil_div(n, 256)), dim3(256), 0, s, b->elems.as<PointRec>(), n,
+                     pos.as<float>());
+  launch_check("k_scatter_points");
+  // separation estimate
+  std::vector<int64_t> picks(trials);
+  for (int64_t t = 0; t < trials; ++t) {
+    uint32_t w[4];
+    philox4x32_10((uint32_t)t, (uint32_t)((uint64_t)t >> 32), GRAPH_PICK_STREAM, 0, (uint32_t)seed,
+                  (uint32_t)(seed >> 32), w);
+    picks[t] = (int64_t)(((uint64_t)w[0] * (uint64_t)n) >> 32);
+  }
+  THG_HIP(hipMemcpyAsync(picks_d.p, picks.data(), trials * 8, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_pick_nearest, dim3((unsigned)trials), dim3(256), 0, s, pos.as<float>(), n,
+                     picks_d.as<int64_t>(), near_d.as<float>());
+  launch_check("k_pick_nearest");
+  std::vector<float> near(trials);
+  THG_HIP(hipMemcpyAsync(near.data(), near_d.p, trials * 4, hipMemcpyDeviceToHost, s));
+  THG_HIP(hipStreamSynchronize(s));
+  float acc = 0;
+  for (int64_t t = 0; t < trials; ++t) acc += near[t];
+  const float avg = std::sqrt(acc / (float)trials);
+  const float r0 = std::sqrt((float)md) * avg;
+  const int min_degree = std::min(md, std::max(4, (int)(0.25 * md)));
+  const int stack = b->max_depth < 32 ? 32 : 64;
+  const uint32_t grid = ceil_div(n, G_BLOCK);
+#define THG_G(S, K)                                                                                                   \
+  hipLaunchKernelGGL((k_graph<S, K>), dim3(grid), dim3(G_BLOCK), 0, s, b->nodes.as<BNode>(), b->elems.as<PointRec>(), \
+                     pos.as<float>(), d_nrm, n, r0, md, min_degree, b->scale, d_idx, d_sep, d_deg)
+#define THG_GK(S) \
+  if (md <= 4) THG_G(S, 4); else if (md <= 8) THG_G(S, 8); else if (md <= 16) THG_G(S, 16); else THG_G(S, 32)
+  if (stack == 32) { THG_GK(32); } else { THG_GK(64); }
+#undef THG_GK
+#undef THG_G
+  launch_check("k_graph");
+  return avg;
+}
+
+}  // namespace thg
+
 extern "C" int thg_proximity_graph(thg_bvh points, const float *normals, const thg_graph_opts *opts, int32_t mem,
                                    void *stream, int32_t *nbr_idx, float *nbr_sep, int32_t *degree,
                                    float separation[2]) {
@@ -184,40 +236,15 @@ extern "C" int thg_proximity_graph(thg_bvh points, const float *normals, const t
     require(md <= 32, "max_degree must be at most 32");
     const int64_t n = b->n;
     require(n >= 2, "the proximity graph needs at least two samples (PointSet3.hpp:350-351)");
-    require(b->max_depth < 64, "point BVH deeper than the 64-entry traversal stack");
-    const int64_t trials = std::min<int64_t>(opts && opts->separation_trials > 0 ? opts->separation_trials : 100, n);
+    const int64_t trials = opts && opts->separation_trials > 0 ? opts->separation_trials : 100;
     const uint64_t seed = opts ? opts->seed : 0;
-    // scatter the samples back to input order
-    AsyncBuf pos(n * 12, s), nrm_d, picks_d(trials * 8, s), near_d(trials * 4, s);
-    hipLaunchKernelGGL(k_scatter_points, dim3(ceil_div(n, 256)), dim3(256), 0, s, b->elems.as<PointRec>(), n,
-                       pos.as<float>());
-    launch_check("k_scatter_points");
+    AsyncBuf nrm_d;
     const float *d_nrm = normals;
     if (normals && mem == THG_MEM_HOST) {
       nrm_d.ensure(n * 12, s);
       THG_HIP(hipMemcpyAsync(nrm_d.p, normals, n * 12, hipMemcpyHostToDevice, s));
       d_nrm = nrm_d.as<float>();
     }
-    // separation estimate
-    std::vector<int64_t> picks(trials);
-    for (int64_t t = 0; t < trials; ++t) {
-      uint32_t w[4];
-      philox4x32_10((uint32_t)t, (uint32_t)((uint64_t)t >> 32), GRAPH_PICK_STREAM, 0, (uint32_t)seed,
-                    (uint32_t)(seed >> 32), w);
-      picks[t] = (int64_t)(((uint64_t)w[0] * (uint64_t)n) >> 32);
-    }
-    THG_HIP(hipMemcpyAsync(picks_d.p, picks.data(), trials * 8, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_pick_nearest, dim3((unsigned)trials), dim3(256), 0, s, pos.as<float>(), n,
-                       picks_d.as<int64_t>(), near_d.as<float>());
-    launch_check("k_pick_nearest");
-    std::vector<float> near(trials);
-    THG_HIP(hipMemcpyAsync(near.data(), near_d.p, trials * 4, hipMemcpyDeviceToHost, s));
-    THG_HIP(hipStreamSynchronize(s));
-    float acc = 0;
-    for (int64_t t = 0; t < trials; ++t) acc += near[t];
-    const float avg = std::sqrt(acc / (float)trials);
-    const float r0 = std::sqrt((float)md) * avg;
-    const int min_degree = std::min(md, std::max(4, (int)(0.25 * md)));
     // neighbours (host outputs go through device buffers)
     const bool host = mem == THG_MEM_HOST;
     const bool want_avg = separation != nullptr;
@@ -237,17 +264,7 @@ extern "C" int thg_proximity_graph(thg_bvh points, const float *normals, const t
       deg_tmp.ensure((size_t)n * 4, s);
       d_deg = deg_tmp.as<int32_t>();
     }
-    const int stack = b->max_depth < 32 ? 32 : 64;
-    const uint32_t grid = ceil_div(n, G_BLOCK);
-#define THG_G(S, K)                                                                                                   \
-  hipLaunchKernelGGL((k_graph<S, K>), dim3(grid), dim3(G_BLOCK), 0, s, b->nodes.as<BNode>(), b->elems.as<PointRec>(), \
-                     pos.as<float>(), d_nrm, n, r0, md, min_degree, b->scale, d_idx, d_sep, d_deg)
-#define THG_GK(S) \
-  if (md <= 4) THG_G(S, 4); else if (md <= 8) THG_G(S, 8); else if (md <= 16) THG_G(S, 16); else THG_G(S, 32)
-    if (stack == 32) { THG_GK(32); } else { THG_GK(64); }
-#undef THG_GK
-#undef THG_G
-    launch_check("k_graph");
+    const float avg = graph_build(b, d_nrm, md, trials, seed, s, d_idx, d_sep, d_deg);
     if (want_avg) {
       std::vector<float> hs((size_t)n * md);
       std::vector<int32_t> hd(n);
