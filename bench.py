@@ -24,7 +24,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBPS = 8000.0
 # BVH leaf size per workload, the fastest of 1, 2, 4, 6, 8, 12 measured on one MI355X for that query type
 # (closest point prefers 8, k-NN over points 12, incoherent rays 1); the library default is 4.
-LEAF_DEFAULT = {"c3": 8, "c1": 8, "c3ray": 1, "c2knn": 12, "c2sample": 4, "icp": 8, "graph": 12, "sdf": 1}
+LEAF_DEFAULT = {"c3": 8, "c1": 8, "c3ray": 1, "c2knn": 12, "c2sample": 4, "icp": 8, "graph": 12, "sdf": 1,
+                "vis": 1}
 
 
 def parse():
@@ -32,7 +33,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", default="c3", choices=["c3", "c1", "c3ray", "c2knn", "c2sample", "c4", "c5", "copy", "icp", "graph", "sdf"])
+    p.add_argument("--workload", default="c3", choices=["c3", "c1", "c3ray", "c2knn", "c2sample", "c4", "c5", "copy", "icp", "graph", "sdf", "sep", "vis"])
     p.add_argument("--requests", type=int, default=400, help="c5: requests in the trace")
     p.add_argument("--load", type=float, default=0.7, help="c5: Poisson rate as a fraction of plain capacity")
     p.add_argument("--queries", type=int, default=None, help="override the query count")
@@ -110,6 +111,12 @@ def main():
         return
     if wl == "sdf":
         run_sdf(args)
+        return
+    if wl == "sep":
+        run_sep(args)
+        return
+    if wl == "vis":
+        run_vis(args)
         return
     t_gen = time.time()
     info = {}
@@ -354,6 +361,112 @@ def run_sdf(args):
         result["cpu_baseline"] = {"value": m / csec, "unit": "points/s", "cores": cpu_threads(args), "kind": "port",
                                   "sample": f"the first {m} points ({csec:.1f} s): rays on all threads, generation "
                                             f"and reduction on one"}
+    print(json.dumps(result), flush=True)
+
+
+def run_vis(args):
+    """External visibility (Visibility::compute) at 1,000,000 area samples of the C3 torus, 100 rays
+    each (100M any-hit rays, processed in 2^24-ray chunks). One step = one batch; the BVH is built
+    once."""
+    import numpy as np
+    import torch
+    import thea_amd
+    from thea_amd import workloads as W
+    v, t, _, _, _ = W.torus_c3(n_queries=0, n_rays=0)
+    mesh = thea_amd.Mesh(v, t)
+    n = args.queries or 1_000_000
+    pos, _, _ = thea_amd.MeshSampler(mesh).sampleEvenlyByArea(n, seed=10, stream_id=W.STREAM_SAMPLES, device=True)
+    b = thea_amd.MeshBvh(mesh, leaf_size=args.leaf)
+    for i in range(args.warmup):
+        b.visibility(pos, num_rays=100, seed=1)
+        log(f"warmup vis {i + 1}/{args.warmup} done")
+    b.ctx.kernel_time("any_hit", reset=True)
+    b.ctx.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        vis = b.visibility(pos, num_rays=100, seed=1)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / args.steps
+    b.ctx.set_timing(False)
+    k_ms, k_n = b.ctx.kernel_time("any_hit", reset=True)
+    log(f"{args.steps} batches: {sec * 1e3:.1f} ms each")
+    result = {
+        "metric": "visibility points/s (C3 torus, 1M surface points, 100 rays each)", "value": n / sec,
+        "unit": "points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": sec * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 rays, f64 fractions",
+        "data": "synthetic",
+        "config": {"workload": "Visibility at C3 area samples", "points": n, "rays": 100 * n, "leaf_size": args.leaf},
+        "any_hit_ms_per_batch": k_ms / max(args.steps, 1), "any_hit_launches_per_batch": k_n / max(args.steps, 1),
+        "rays_per_s_kernel": 100 * n * args.steps / (k_ms * 1e-3) if k_ms > 0 else None,
+        "mean_visibility": float(vis.mean()),
+    }
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        m = min(n, 200_000)
+        kd = O.KdTree(v, t)
+        lo, hi = b.getBounds()
+        e = hi - lo
+        scale = float(np.sqrt(e[0] * e[0] + (e[1] * e[1] + e[2] * e[2])))
+        hpos = pos[:m].cpu().numpy()
+        t0 = time.perf_counter()
+        O.visibility(kd, hpos, 100, scale, 1)
+        csec = time.perf_counter() - t0
+        result["cpu_baseline"] = {"value": m / csec, "unit": "points/s", "cores": cpu_threads(args), "kind": "port",
+                                  "sample": f"the first {m} points ({csec:.1f} s): rays traced on all threads, "
+                                            f"generated on one"}
+    print(json.dumps(result), flush=True)
+
+
+def run_sep(args):
+    """MeshSample's separation mode (MeshSampler::sampleEvenlyBySeparation) on the C2 mesh (64
+    ellipsoids, 128k triangles): 10,000 samples picked by geodesic furthest-point sampling among
+    30,000 area samples. One step = one call: area oversampling, point BVH, proximity graph, the
+    picks and the gather."""
+    import numpy as np
+    import torch
+    import thea_amd
+    from thea_amd import workloads as W
+    v, t = W.ellipsoids_c2()
+    mesh = thea_amd.Mesh(v, t)
+    sampler = thea_amd.MeshSampler(mesh)
+    n = args.queries or 10_000
+    for i in range(args.warmup):
+        sampler.sampleEvenlyBySeparation(n, seed=2, stream_id=W.STREAM_SAMPLES, device=True)
+        log(f"warmup sep {i + 1}/{args.warmup} done")
+    mesh.ctx.kernel_time("fps", reset=True)
+    mesh.ctx.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pos, _, _ = sampler.sampleEvenlyBySeparation(n, seed=2, stream_id=W.STREAM_SAMPLES, device=True)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / args.steps
+    mesh.ctx.set_timing(False)
+    k_ms, k_n = mesh.ctx.kernel_time("fps", reset=True)
+    log(f"{args.steps} calls: {sec * 1e3:.1f} ms each")
+    result = {
+        "metric": "separation samples/s (C2 mesh, 10k geodesic furthest-point picks among 30k area samples)",
+        "value": n / sec, "unit": "samples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": sec * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 positions, f64 geodesic distances", "data": "synthetic",
+        "config": {"workload": "sampleEvenlyBySeparation on the C2 mesh", "samples": n, "oversampling": 3 * n,
+                   "max_degree": 8},
+        "fps_kernel_avg_ms": k_ms / max(k_n, 1), "samples_returned": int(len(pos)),
+    }
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        over = 3 * n
+        opos = O.sample_area(v, t, over, 2, W.STREAM_SAMPLES)[0]
+        idx, sep, _, _ = O.proximity_graph(opos, None, 8, O.graph_picks(over, 100, 0))
+        m = min(n, 1500)
+        t0 = time.perf_counter()
+        O.fps(idx, sep, m)
+        csec = time.perf_counter() - t0
+        result["cpu_baseline"] = {"value": m / csec, "unit": "samples/s", "cores": 1, "kind": "port",
+                                  "sample": f"the first {m} picks over the same {over}-point graph ({csec:.1f} s; "
+                                            f"each pick is one multi-source Dijkstra over the whole graph, as in "
+                                            f"Thea, so the rate holds for all {n})"}
     print(json.dumps(result), flush=True)
 
 

@@ -250,6 +250,16 @@ typedef struct thg_graph_opts {
 int thg_proximity_graph(thg_bvh points, const float *normals, const thg_graph_opts *opts, int32_t mem, void *stream,
                         int32_t *nbr_idx, float *nbr_sep, int32_t *degree, float separation[2]);
 
+/* External visibility at n surface points: Visibility::compute(position, num_rays)
+ * (Algorithms/SurfaceFeatures/Local/Visibility.hpp:99-125): the fraction of num_rays (<= 0: 100)
+ * rays, uniform on the sphere by Knop's method and offset by 0.001 scale along themselves, that hit
+ * nothing. Thea draws the directions from its time-seeded Random::common(); here ray r of point i
+ * uses the uniforms of Philox4x32-10 at counter (i * num_rays + r, 0x56495342, attempt) with key
+ * `seed`. scale <= 0 takes the diagonal of the BVH's bounds grown 1.05x, as Thea's constructors.
+ * positions and visibility (n doubles) in `mem`. */
+int thg_visibility(thg_bvh mesh_bvh, const float *positions, int64_t n, int32_t num_rays, uint64_t seed, float scale,
+                   int32_t mem, void *stream, double *visibility);
+
 /* FurthestPointSampling::subsample with geodesic distances (Algorithms/FurthestPointSampling.cpp:
  * 47-125): the proximity graph of the n points (as thg_proximity_graph without normals; graph_opts
  * nullable), then k picks starting with point 0, each the point furthest along the graph from the

@@ -64,6 +64,7 @@ def lib():
         "or_sdf_reduce": (None, [f32p, i64, f64, f64p]),
         "or_proximity_graph": (i32, [f32p, vp, i64, i32, i64p, i64, i32p, f32p, i32p, f32p]),
         "or_fps": (i64, [i32p, f32p, i64, i32, i64, i64p]),
+        "or_vis_rays": (None, [f32p, i64, i32, f32, u64, f32p, f32p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -294,6 +295,24 @@ def proximity_graph(points, normals=None, max_degree=8, picks=None):
     if rc != 0:
         raise ValueError("or_proximity_graph failed")
     return idx, sep, deg, (float(avg[0]), float(avg[1]))
+
+
+def vis_rays(positions, num_rays, scale, seed):
+    """Visibility::compute's rays (see or_vis_rays): origins and directions (n * num_rays, 3)."""
+    pos = _f32(positions).reshape(-1, 3)
+    m = len(pos) * num_rays
+    org = np.zeros((m, 3), np.float32)
+    dr = np.zeros((m, 3), np.float32)
+    lib().or_vis_rays(pos, len(pos), num_rays, np.float32(scale), seed, org, dr)
+    return org, dr
+
+
+def visibility(kd, positions, num_rays, scale, seed):
+    """Visibility::compute restated: the fraction of each point's rays with no hit (kd-tree)."""
+    org, dr = vis_rays(positions, num_rays, scale, seed)
+    idx, _, _ = kd.ray(org, dr, mode=0)
+    escaped = (idx < 0).reshape(-1, num_rays).sum(1)
+    return escaped.astype(np.float64) / num_rays
 
 
 def fps(idx, sep, k):

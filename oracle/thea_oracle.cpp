@@ -1388,4 +1388,40 @@ int64_t or_fps(const int32_t *idx, const float *sep, int64_t n, int md, int64_t 
   }
   return k;
 }
+
+// Visibility::compute's rays (Visibility.hpp:108-118) with Random::sphere (Random.cpp:226-250,
+// Knop's method) drawing its uniforms from Philox4x32-10: ray g = i * R + r uses counter
+// (g lo, g hi, 0x56495342, attempt), two (x, y) tries per counter; +z after 64 failed counters.
+void or_vis_rays(const float *pos, int64_t n, int32_t R, float scale, uint64_t seed, float *org, float *dir) {
+  const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  for (int64_t i = 0; i < n; ++i)
+    for (int32_t r = 0; r < R; ++r) {
+      const uint64_t g = (uint64_t)i * (uint64_t)R + (uint64_t)r;
+      float x = 0, y = 0, z = 1;
+      bool done = false;
+      for (uint32_t a = 0; a < 64 && !done; ++a) {
+        const uint32_t ctr[4] = {(uint32_t)g, (uint32_t)(g >> 32), 0x56495342u, a};
+        uint32_t w[4];
+        philox4x32_10(ctr, key, w);
+        for (int h = 0; h < 2 && !done; ++h) {
+          float xx = 2 * u01(w[2 * h]) - 1, yy = 2 * u01(w[2 * h + 1]) - 1;
+          const float m2 = xx * xx + yy * yy;
+          if (m2 > 1 || m2 < 1e-10) continue;
+          z = 2 * m2 - 1;
+          const float sc = 2 * std::sqrt(1 - m2);
+          x = xx * sc;
+          y = yy * sc;
+          done = true;
+        }
+      }
+      const float f = 0.001f * scale;
+      const int64_t o = 3 * (int64_t)g;
+      org[o] = pos[3 * i] + f * x;
+      org[o + 1] = pos[3 * i + 1] + f * y;
+      org[o + 2] = pos[3 * i + 2] + f * z;
+      dir[o] = x;
+      dir[o + 1] = y;
+      dir[o + 2] = z;
+    }
+}
 }  // extern "C"

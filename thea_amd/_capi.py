@@ -90,6 +90,7 @@ SIGNATURES = {
     "thg_ctx_set_timing": (C.c_int, [vp, i32]),
     "thg_ctx_kernel_time": (C.c_int, [vp, C.c_char_p, C.POINTER(f64), C.POINTER(i64), i32]),
     "thg_proximity_graph": (C.c_int, [vp, vp, C.POINTER(GraphOpts), i32, vp, vp, vp, vp, vp]),
+    "thg_visibility": (C.c_int, [vp, vp, i64, i32, u64, f32, i32, vp, vp]),
     "thg_furthest_point_sampling": (C.c_int, [vp, vp, i64, i64, C.POINTER(GraphOpts), i32, vp, vp, C.POINTER(i64)]),
     "thg_sample_separation": (C.c_int, [vp, i64, f32, u64, u64, C.POINTER(GraphOpts), vp, vp, vp, C.POINTER(i64), i32,
                                         vp]),

@@ -276,6 +276,19 @@ class MeshBvh(_BvhBase):
                                                  a.mem, a.stream, sp, tp))
         return (sdf, times) if want_ray_times else sdf
 
+    def visibility(self, positions, num_rays=100, seed=0, scale=None):
+        """Visibility::compute (SurfaceFeatures/Local/Visibility.hpp:99-125) at each position: the
+        fraction (float64) of num_rays sphere-uniform rays that escape the mesh. Directions come from
+        Philox with `seed` (Thea uses its time-seeded global generator). scale None: the diagonal of
+        getBounds(), as Thea. numpy in, numpy out; device tensors in, device tensor out."""
+        dev = _is_torch(positions) and positions.is_cuda
+        a = _Args(dev, self.ctx.torch_device() if dev else None)
+        _, pp, n = a.inp(positions, np.float32, 3)
+        vis, vp_ = a.out((n,), np.float64)
+        capi.check(capi.lib().thg_visibility(self.h, pp, n, int(num_rays), C.c_uint64(seed),
+                                             -1.0 if scale is None else float(scale), a.mem, a.stream, vp_))
+        return vis
+
     # -- rays (BvhN::rayStructureIntersection / rayIntersectionTime / rayIntersects) --------------
     def rayStructureIntersection(self, origins, directions, max_time=-1.0, want_normal=True):
         """(idx, t, normal) of the first hit per ray; misses give idx -1, t -1."""
