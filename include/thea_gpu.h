@@ -250,6 +250,36 @@ typedef struct thg_graph_opts {
 int thg_proximity_graph(thg_bvh points, const float *normals, const thg_graph_opts *opts, int32_t mem, void *stream,
                         int32_t *nbr_idx, float *nbr_sep, int32_t *degree, float separation[2]);
 
+/* ---- Mesh and point-sample files (host only; no device involved) --------------------------------
+ * A mesh file read into one flat triangle list: Thea's OFF (ASCII or "OFF BINARY") and OBJ readers
+ * (Graphics/MeshCodecOff.hpp:146-430, Graphics/MeshCodecObj.hpp:210-530, with MeshSample's options:
+ * no normals or texture coordinates, non-strict), then Thea's triangulation of each face
+ * (Algorithms/MeshTriangles.hpp:173-228: triangles as they are, quads split by
+ * Polygon3::triangulateQuad, larger polygons by Polygon3::triangulate's ear clipping). Triangles
+ * come in face order; triangle_faces holds the index Thea stores for each triangle's face (its
+ * position in the file). Arrays are allocated by the library: release them with
+ * thg_mesh_file_free. */
+typedef struct thg_mesh_file {
+  int64_t num_vertices;
+  int64_t num_faces;       /* faces kept (>= 3 vertices) */
+  int64_t num_triangles;
+  float *vertices;         /* 3 * num_vertices, file order */
+  int32_t *triangles;      /* 3 * num_triangles, indices into vertices */
+  int64_t *triangle_faces; /* num_triangles */
+} thg_mesh_file;
+/* Reads `path` by its extension (.off or .obj, any case). Replaces MeshGroup::load
+ * (Graphics/MeshGroup.hpp:378-415) followed by MeshTriangles::add. */
+int thg_mesh_file_read(const char *path, thg_mesh_file *out);
+void thg_mesh_file_free(thg_mesh_file *mesh);
+/* MeshSample's point file (Tools/MeshSample/MeshSample.cpp:560-601): one line per sample,
+ * "x y z nx ny nz [face_id]" in the C++ stream's default float format, values below 1e-35 in
+ * magnitude written as 0. nrm and ids nullable. */
+int thg_pts_write(const char *path, int64_t n, const float *pos, const float *nrm, const int64_t *ids);
+/* The first three numbers of every line that is not empty or a '#' comment (MeshSample.cpp:
+ * 314-348); *pos is allocated by the library (3 * *n floats): release it with thg_free. */
+int thg_pts_read(const char *path, int64_t *n, float **pos);
+void thg_free(void *p);
+
 /* External visibility at n surface points: Visibility::compute(position, num_rays)
  * (Algorithms/SurfaceFeatures/Local/Visibility.hpp:99-125): the fraction of num_rays (<= 0: 100)
  * rays, uniform on the sphere by Knop's method and offset by 0.001 scale along themselves, that hit

@@ -58,6 +58,12 @@ class GraphOpts(C.Structure):
                 ("reserved", C.c_int64 * 4)]
 
 
+class MeshFile(C.Structure):
+    _fields_ = [("num_vertices", C.c_int64), ("num_faces", C.c_int64), ("num_triangles", C.c_int64),
+                ("vertices", C.POINTER(C.c_float)), ("triangles", C.POINTER(C.c_int32)),
+                ("triangle_faces", C.POINTER(C.c_int64))]
+
+
 THG_REQ_CLOSEST, THG_REQ_KNN, THG_REQ_RAY = 0, 1, 2
 THG_PENDING = 1
 
@@ -90,6 +96,11 @@ SIGNATURES = {
     "thg_ctx_set_timing": (C.c_int, [vp, i32]),
     "thg_ctx_kernel_time": (C.c_int, [vp, C.c_char_p, C.POINTER(f64), C.POINTER(i64), i32]),
     "thg_proximity_graph": (C.c_int, [vp, vp, C.POINTER(GraphOpts), i32, vp, vp, vp, vp, vp]),
+    "thg_mesh_file_read": (C.c_int, [C.c_char_p, C.POINTER(MeshFile)]),
+    "thg_mesh_file_free": (None, [C.POINTER(MeshFile)]),
+    "thg_pts_write": (C.c_int, [C.c_char_p, i64, vp, vp, vp]),
+    "thg_pts_read": (C.c_int, [C.c_char_p, C.POINTER(i64), C.POINTER(C.POINTER(C.c_float))]),
+    "thg_free": (None, [vp]),
     "thg_visibility": (C.c_int, [vp, vp, i64, i32, u64, f32, i32, vp, vp]),
     "thg_furthest_point_sampling": (C.c_int, [vp, vp, i64, i64, C.POINTER(GraphOpts), i32, vp, vp, C.POINTER(i64)]),
     "thg_sample_separation": (C.c_int, [vp, i64, f32, u64, u64, C.POINTER(GraphOpts), vp, vp, vp, C.POINTER(i64), i32,
