@@ -146,6 +146,15 @@ int thg_ray_any_hit(thg_bvh bvh, const float *origins, const float *dirs, int64_
  *  pos: n float3. normal (nullable): n float3 face normals. tri (nullable): n int32. */
 int thg_sample_area(thg_mesh mesh, int64_t n, uint64_t seed, uint64_t stream_id, float *pos, float *normal,
                     int32_t *tri, int64_t *n_out, int32_t mem, void *stream);
+/* sampleEvenlyByArea in CountMode::APPROXIMATE (Algorithms/MeshSampler.hpp:206-240): triangle i
+ * gets floor(frac) samples, frac = (n * area_i) / total_area, plus one more when a uniform draw is
+ * at most frac's fractional part; samples come in triangle order. Thea draws from its time-seeded
+ * generator; here triangle i's draw is word 0 of Philox4x32-10 at counter (i, stream_id) with key
+ * (seed lo ^ 0x41505052, seed hi), and sample j's barycentric uniforms are words 1 and 2 at counter
+ * (j, stream_id) with key seed. The count is random: with pos NULL only *n_out is set; otherwise
+ * the call fails with THG_E_INVALID if the count exceeds `capacity` (rows of pos/normal/tri). */
+int thg_sample_area_approx(thg_mesh mesh, int64_t n, uint64_t seed, uint64_t stream_id, int64_t capacity, float *pos,
+                           float *normal, int32_t *tri, int64_t *n_out, int32_t mem, void *stream);
 
 /* Kernel timing (diagnostics; Thea's analogue is Stopwatch, Stopwatch.hpp:41-76). While enabled on
  * a context, each query or sampling call records HIP events around its main kernel on the call's

@@ -63,6 +63,7 @@ def lib():
         "or_sdf_rays": (None, [f32p, f32p, i64, f32, f32p, f32p]),
         "or_sdf_reduce": (None, [f32p, i64, f64, f64p]),
         "or_proximity_graph": (i32, [f32p, vp, i64, i32, i64p, i64, i32p, f32p, i32p, f32p]),
+        "or_sample_area_approx": (i64, [f32p, i32p, i64, i64, u64, u64, i64, f32p, vp, vp]),
         "or_fps": (i64, [i32p, f32p, i64, i32, i64, i64p]),
         "or_vis_rays": (None, [f32p, i64, i32, f32, u64, f32p, f32p]),
     }
@@ -251,6 +252,18 @@ def sample_area(verts, tris, n, seed, stream=0, want_cdf=False):
     m = lib().or_sample_area(verts, tris, len(tris), n, seed, stream, pos, _ptr(nrm), _ptr(tri), _ptr(cdf))
     res = (pos[:m], nrm[:m], tri[:m])
     return res + (cdf,) if want_cdf else res
+
+
+def sample_area_approx(verts, tris, n, seed, stream=0):
+    """APPROXIMATE-mode area sampling restated (see or_sample_area_approx): (pos, nrm, tri)."""
+    verts, tris = _f32(verts), _i32(tris)
+    cap = int(n + 10 * np.sqrt(max(n, 1)) + len(tris) + 16)
+    pos = np.zeros((cap, 3), np.float32)
+    nrm = np.zeros((cap, 3), np.float32)
+    tri = np.zeros(cap, np.int32)
+    m = lib().or_sample_area_approx(verts, tris, len(tris), n, seed, stream, cap, pos, _ptr(nrm), _ptr(tri))
+    assert m <= cap
+    return pos[:m], nrm[:m], tri[:m]
 
 
 def geodesic_sphere(depth):

@@ -1424,4 +1424,48 @@ void or_vis_rays(const float *pos, int64_t n, int32_t R, float scale, uint64_t s
       dir[o + 2] = z;
     }
 }
+
+// MeshSampler::sampleEvenlyByArea in CountMode::APPROXIMATE (MeshSampler.hpp:206-240), Thea's
+// loop as written, drawing from Philox: triangle i's "last sample" draw is word 0 at counter
+// (i, stream) with key (seed lo ^ 0x41505052, seed hi); the j-th sample overall takes its
+// barycentric uniforms from words 1 and 2 at counter (j, stream) with key seed. Returns the count
+// (writes at most cap samples).
+int64_t or_sample_area_approx(const float *verts, const int32_t *tris, int64_t nt, int64_t n, uint64_t seed,
+                              uint64_t stream, int64_t cap, float *pos, float *nrm, int32_t *tri_out) {
+  if (n <= 0 || nt <= 0) return 0;
+  std::vector<Tri> t;
+  gather_tris(verts, tris, nt, t);
+  double total = 0;
+  for (int64_t i = 0; i < nt; ++i) total += t[i].area;
+  if (total <= 0) return 0;
+  const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  const uint32_t dkey[2] = {(uint32_t)seed ^ 0x41505052u, (uint32_t)(seed >> 32)};
+  int64_t j = 0;
+  for (int64_t i = 0; i < nt; ++i) {
+    double frac = ((double)n * t[i].area) / total;
+    while (frac > 0) {
+      if (frac < 1) {  // at most once per triangle: frac is negative after this sample
+        const uint32_t ctr[4] = {(uint32_t)i, (uint32_t)((uint64_t)i >> 32), (uint32_t)stream, (uint32_t)(stream >> 32)};
+        uint32_t w[4];
+        philox4x32_10(ctr, dkey, w);
+        if (u01(w[0]) > frac) break;
+      }
+      if (j < cap) {
+        const uint32_t ctr[4] = {(uint32_t)j, (uint32_t)((uint64_t)j >> 32), (uint32_t)stream, (uint32_t)(stream >> 32)};
+        uint32_t w[4];
+        philox4x32_10(ctr, key, w);
+        const Tri &tr = t[i];
+        float s = u01(w[1]), u = u01(w[2]);
+        if (s + u > 1) { u = 1 - u; s = 1 - s; }
+        V3 p = add(add(mul(s, tr.e01), mul(u, tr.e02)), tr.v[0]);
+        pos[3 * j] = p.x; pos[3 * j + 1] = p.y; pos[3 * j + 2] = p.z;
+        if (nrm) { nrm[3 * j] = tr.n.x; nrm[3 * j + 1] = tr.n.y; nrm[3 * j + 2] = tr.n.z; }
+        if (tri_out) tri_out[j] = (int32_t)i;
+      }
+      ++j;
+      frac -= 1.0;
+    }
+  }
+  return j;
+}
 }  // extern "C"

@@ -91,6 +91,7 @@ SIGNATURES = {
     "thg_ray_first_hit": (C.c_int, [vp, vp, vp, i64, f32, vp, vp, vp, i32, vp]),
     "thg_ray_any_hit": (C.c_int, [vp, vp, vp, i64, f32, vp, i32, vp]),
     "thg_sample_area": (C.c_int, [vp, i64, u64, u64, vp, vp, vp, C.POINTER(i64), i32, vp]),
+    "thg_sample_area_approx": (C.c_int, [vp, i64, u64, u64, i64, vp, vp, vp, C.POINTER(i64), i32, vp]),
     "thg_closest_stats": (C.c_int, [vp, vp, i64, vp, C.POINTER(i64)]),
     "thg_ray_stats": (C.c_int, [vp, vp, vp, i64, vp, vp, vp]),
     "thg_ctx_set_timing": (C.c_int, [vp, i32]),

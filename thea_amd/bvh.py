@@ -362,13 +362,27 @@ class MeshSampler:
     def __init__(self, mesh):
         self.mesh = mesh
 
-    def sampleEvenlyByArea(self, n, seed=0, stream_id=0, device=False):
-        """(positions[n,3], face_normals[n,3], triangle_indices[n]); fewer rows for an empty mesh."""
+    def sampleEvenlyByArea(self, n, seed=0, stream_id=0, device=False, count_mode="exact"):
+        """(positions[n,3], face_normals[n,3], triangle_indices[n]); fewer rows for an empty mesh.
+        count_mode "approximate" (Thea's CountMode::APPROXIMATE) returns about n samples, in
+        triangle order."""
         a = _Args(device, self.mesh.ctx.torch_device() if device else None)
+        got = C.c_int64()
+        if count_mode == "approximate":
+            capi.check(capi.lib().thg_sample_area_approx(self.mesh.h, int(n), C.c_uint64(seed), C.c_uint64(stream_id),
+                                                         0, None, None, None, C.byref(got), a.mem, None))
+            m = got.value
+            pos, pp = a.out((m, 3), np.float32)
+            nrm, np_ = a.out((m, 3), np.float32)
+            tri, tp = a.out((m,), np.int32)
+            capi.check(capi.lib().thg_sample_area_approx(self.mesh.h, int(n), C.c_uint64(seed), C.c_uint64(stream_id),
+                                                         m, pp, np_, tp, C.byref(got), a.mem, a.stream))
+            return pos, nrm, tri
+        if count_mode != "exact":
+            raise ValueError("count_mode is 'exact' or 'approximate'")
         pos, pp = a.out((n, 3), np.float32)
         nrm, np_ = a.out((n, 3), np.float32)
         tri, tp = a.out((n,), np.int32)
-        got = C.c_int64()
         capi.check(capi.lib().thg_sample_area(self.mesh.h, int(n), C.c_uint64(seed), C.c_uint64(stream_id), pp, np_,
                                               tp, C.byref(got), a.mem, a.stream))
         m = got.value

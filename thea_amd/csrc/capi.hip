@@ -449,4 +449,32 @@ int thg_sample_area(thg_mesh mesh, int64_t n, uint64_t seed, uint64_t stream_id,
   });
 }
 
+int thg_sample_area_approx(thg_mesh mesh, int64_t n, uint64_t seed, uint64_t stream_id, int64_t capacity, float *pos,
+                           float *normal, int32_t *tri, int64_t *n_out, int32_t mem, void *stream) {
+  return guarded([&] {
+    require(mesh != nullptr && n_out != nullptr, "mesh/n_out is NULL");
+    require(n >= 0, "negative sample count");
+    require(pos || (!normal && !tri), "normal/tri need pos");
+    check_mem(mem);
+    Mesh *m = reinterpret_cast<Mesh *>(mesh);
+    set_device(m->ctx);
+    hipStream_t s = as_stream(stream);
+    mesh_cdf(*m, s);
+    *n_out = 0;
+    const int64_t total = launch_sample_approx(*m, n, seed, stream_id, 0, nullptr, nullptr, nullptr, s);
+    if (!pos || total == 0) {
+      *n_out = total;
+      return;
+    }
+    require(total <= capacity, "APPROXIMATE sampling produced " + std::to_string(total) +
+                                   " samples, more than the output capacity " + std::to_string(capacity));
+    Stage st(s, mem == THG_MEM_HOST);
+    float *dp = st.out(pos, total * 3), *dn = st.out(normal, total * 3);
+    int32_t *dt = st.out(tri, total);
+    launch_sample_approx(*m, n, seed, stream_id, capacity, dp, dn, dt, s);
+    st.finish();
+    *n_out = total;
+  });
+}
+
 }  // extern "C"

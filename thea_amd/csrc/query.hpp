@@ -79,6 +79,8 @@ void launch_ray(const Bvh &b, const float *o, const float *d, int64_t n, float m
 
 // Sampler
 void mesh_cdf(Mesh &m, hipStream_t s);
+int64_t launch_sample_approx(Mesh &m, int64_t n, uint64_t seed, uint64_t stream_id, int64_t capacity, float *pos,
+                             float *nrm, int32_t *tri, hipStream_t s);
 float graph_build(Bvh *b, const float *d_nrm, int md, int64_t trials, uint64_t seed, hipStream_t s, int32_t *d_idx,
                   float *d_sep, int32_t *d_deg);
 void launch_sample(Mesh &m, int64_t n, uint64_t seed, uint64_t stream_id, float *pos, float *nrm, int32_t *tri,

@@ -199,6 +199,96 @@ __global__ void __launch_bounds__(S_BLOCK) k_sample(const float *__restrict__ ve
   if (tri_out) tri_out[i] = (int32_t)ti;
 }
 
+// APPROXIMATE count mode (MeshSampler.hpp:206-240): triangle i gets frac = (n * area) / total
+// samples in expectation: while frac > 0, a last sample (frac < 1) is kept only if a uniform draw is
+// <= frac, and frac drops by 1 per sample. Subtracting 1 from a double below 2^52 is exact, so that
+// is floor(frac) samples plus one more when r = frac - floor(frac) > 0 and the draw is <= r. The
+// draw for triangle i is word 0 of Philox4x32-10 at counter (i, stream) with the seed's low key word
+// xor-ed with APPROX_KEY_TAG; sample j (in triangle order) takes its barycentric uniforms from words
+// 1 and 2 at counter (j, stream), as in EXACT mode.
+constexpr uint32_t APPROX_KEY_TAG = 0x41505052u;
+
+__global__ void __launch_bounds__(S_BLOCK) k_approx_count(const float *__restrict__ area, int64_t nt, double desired,
+                                                          double total, uint32_t k0, uint32_t k1, uint32_t s0,
+                                                          uint32_t s1, uint32_t *__restrict__ count) {
+  const int64_t i = (int64_t)blockIdx.x * S_BLOCK + threadIdx.x;
+  if (i >= nt) return;
+  const double frac = (desired * (double)area[i]) / total;
+  uint32_t c = 0;
+  if (frac > 0) {
+    const double whole = floor(frac);
+    c = (uint32_t)whole;
+    const double r = frac - whole;
+    if (r > 0) {
+      uint32_t w[4];
+      philox4x32_10((uint32_t)i, (uint32_t)((uint64_t)i >> 32), s0, s1, k0 ^ APPROX_KEY_TAG, k1, w);
+      if (!((double)u01(w[0]) > r)) ++c;
+    }
+  }
+  count[i] = c;
+}
+
+__global__ void __launch_bounds__(S_BLOCK) k_approx_sample(const float *__restrict__ verts,
+                                                           const int32_t *__restrict__ tris, int64_t nt,
+                                                           const uint32_t *__restrict__ count,
+                                                           const uint32_t *__restrict__ offset, uint32_t k0,
+                                                           uint32_t k1, uint32_t s0, uint32_t s1,
+                                                           float *__restrict__ pos, float *__restrict__ nrm,
+                                                           int32_t *__restrict__ tri_out) {
+  const int64_t i = (int64_t)blockIdx.x * S_BLOCK + threadIdx.x;
+  if (i >= nt) return;
+  const uint32_t c = count[i];
+  if (c == 0) return;
+  const float3 v0 = vtx(verts, tris[3 * i]), v1 = vtx(verts, tris[3 * i + 1]), v2 = vtx(verts, tris[3 * i + 2]);
+  const float3 e01 = sub3(v1, v0), e02 = sub3(v2, v0);
+  const float3 nn = plane_normal(v0, v1, v2);
+  for (uint32_t q = 0; q < c; ++q) {
+    const int64_t j = (int64_t)offset[i] + q;
+    uint32_t w[4];
+    philox4x32_10((uint32_t)j, (uint32_t)((uint64_t)j >> 32), s0, s1, k0, k1, w);
+    float a = u01(w[1]), b = u01(w[2]);
+    if (a + b > 1) { b = 1 - b; a = 1 - a; }
+    const float3 p = add3(add3(scl3(a, e01), scl3(b, e02)), v0);
+    pos[3 * j] = p.x;
+    pos[3 * j + 1] = p.y;
+    pos[3 * j + 2] = p.z;
+    if (nrm) {
+      nrm[3 * j] = nn.x;
+      nrm[3 * j + 1] = nn.y;
+      nrm[3 * j + 2] = nn.z;
+    }
+    if (tri_out) tri_out[j] = (int32_t)i;
+  }
+}
+
+// Counts (and, with pos set, draws) the APPROXIMATE samples; returns their number.
+int64_t launch_sample_approx(Mesh &m, int64_t n, uint64_t seed, uint64_t stream_id, int64_t capacity, float *pos,
+                             float *nrm, int32_t *tri, hipStream_t s) {
+  if (n <= 0 || m.nt <= 0 || !(m.total_area > 0)) return 0;
+  require(n < ((int64_t)1 << 31), "APPROXIMATE sampling supports fewer than 2^31 samples");
+  AsyncBuf cnt(m.nt * 4, s), off(m.nt * 4, s), tmp(((m.nt + 1023) / 1024 + 1) * 4, s);
+  const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32), s0 = (uint32_t)stream_id,
+                 s1 = (uint32_t)(stream_id >> 32);
+  hipLaunchKernelGGL(k_approx_count, dim3(ceil_div(m.nt, S_BLOCK)), dim3(S_BLOCK), 0, s, m.area.as<float>(), m.nt,
+                     (double)n, m.total_area, k0, k1, s0, s1, cnt.as<uint32_t>());
+  launch_check("k_approx_count");
+  scan_excl_u32(cnt.as<uint32_t>(), off.as<uint32_t>(), m.nt, tmp.as<uint32_t>(), s);
+  uint32_t last[2];
+  THG_HIP(hipMemcpyAsync(&last[0], off.as<uint32_t>() + (m.nt - 1), 4, hipMemcpyDeviceToHost, s));
+  THG_HIP(hipMemcpyAsync(&last[1], cnt.as<uint32_t>() + (m.nt - 1), 4, hipMemcpyDeviceToHost, s));
+  THG_HIP(hipStreamSynchronize(s));
+  const int64_t total = (int64_t)last[0] + (int64_t)last[1];
+  if (!pos) return total;
+  require(total <= capacity, "APPROXIMATE sampling produced " + std::to_string(total) +
+                                 " samples, more than the output capacity " + std::to_string(capacity));
+  KernelTimer kt(m.ctx, s, "sample");
+  hipLaunchKernelGGL(k_approx_sample, dim3(ceil_div(m.nt, S_BLOCK)), dim3(S_BLOCK), 0, s, m.verts.as<float>(),
+                     m.tris.as<int32_t>(), m.nt, cnt.as<uint32_t>(), off.as<uint32_t>(), k0, k1, s0, s1, pos, nrm, tri);
+  launch_check("k_approx_sample");
+  kt.stop();
+  return total;
+}
+
 void launch_sample(Mesh &m, int64_t n, uint64_t seed, uint64_t stream_id, float *pos, float *nrm, int32_t *tri,
                    hipStream_t s) {
   if (n <= 0 || m.nt <= 0 || !(m.total_area > 0)) return;
