@@ -11,12 +11,13 @@
 // Here the graph comes from the proximity-graph kernel and the selection runs in one workgroup
 // that keeps, across picks, each vertex's distance to the picked set. Adding a pick relaxes only
 // the vertices whose distance drops (frontier rounds of Bellman-Ford with atomicMin on the double's
-// bits, which order like the values for non-negative doubles). Since d(u) + w >= d(u) in floating
-// point, the distances are the least fixed point of d(v) = min over in-edges of d(u) + w, the
-// values Dijkstra computes, whatever the relaxation order. A max-tournament tree over (distance,
-// index) then names the furthest vertex; only the ancestors of vertices whose distance changed
-// are recomputed. Among vertices at exactly the same maximum distance the highest index wins:
-// Thea's choice there depends on its Fibonacci heap's order for equal keys.
+// bits, which order like the values for non-negative doubles; one thread per frontier edge).
+// Since d(u) + w >= d(u) in floating point, the distances are the least fixed point of
+// d(v) = min over in-edges of d(u) + w, the values Dijkstra computes, whatever the relaxation
+// order. A 64-ary max-tournament over (distance, index), each node reduced by one wave, then names
+// the furthest vertex; only the ancestors of vertices whose distance changed are recomputed. Among
+// vertices at exactly the same maximum distance the highest index wins: Thea's choice there
+// depends on its Fibonacci heap's order for equal keys.
 #include <cmath>
 
 #include "common.hpp"
@@ -44,37 +45,59 @@ __device__ __forceinline__ void st_i32(int32_t *p, int32_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+constexpr int FPS_FAN = 64;  // children per tournament node: one wave reduces a node
+
 struct FpsWork {
   unsigned long long *dist;  // n: bits of the distance to the picked set
-  int32_t *stamp;            // n: last round that queued the vertex
-  int32_t *cstamp;           // n: last pick whose relaxation listed the vertex as changed
+  int32_t *stamp;            // n: last relaxation round that improved the vertex
   int32_t *qa, *qb;          // n each: frontiers
-  int32_t *changed;          // n
-  int32_t *tree;             // 2P: winners of the max tournament; leaf v at P + v
+  int32_t *changed;          // n: vertices improved by the current pick
+  int32_t *win;              // tournament winners, level l at win + woff[l] (level 0: the vertices)
+  int32_t *wstamp;           // same layout: last pick that marked the node dirty
+  int32_t *da, *db;          // dirty node lists (level 1 size)
   int32_t *status;           // [0] = 1 if a relaxation exceeded its round bound
 };
 
-// The larger of (dist, index) of vertices a and b (-1: none).
-__device__ __forceinline__ int32_t fps_winner(const unsigned long long *dist, int32_t a, int32_t b) {
-  if (a < 0) return b;
-  if (b < 0) return a;
-  const unsigned long long da = ld_u64(dist + a), db = ld_u64(dist + b);
-  return (da > db || (da == db && a > b)) ? a : b;
+struct FpsLevels {
+  int32_t count;                  // levels above the vertices; the root is level `count`
+  int32_t size[8];                // nodes per level (size[0] = n)
+  int32_t off[8];                 // offsets into win / wstamp (level >= 1)
+};
+
+// Max of (distance bits, vertex) over the wave, -1 vertices losing to any real one.
+__device__ __forceinline__ void wave_argmax(unsigned long long &key, int32_t &idx) {
+  for (int o = 32; o >= 1; o >>= 1) {
+    const unsigned long long k2 = __shfl_xor(key, o, WAVE);
+    const int32_t i2 = __shfl_xor(idx, o, WAVE);
+    if (k2 > key || (k2 == key && i2 > idx)) {
+      key = k2;
+      idx = i2;
+    }
+  }
+}
+
+// Winner of node `node` at level l >= 1 from its FPS_FAN children (one wave, all lanes).
+__device__ __forceinline__ int32_t fps_node(const FpsWork &w, const FpsLevels &L, int l, int32_t node) {
+  const int32_t c = node * FPS_FAN + lane_id();
+  int32_t idx = -1;
+  if (c < L.size[l - 1]) idx = l == 1 ? c : ld_i32(w.win + L.off[l - 1] + c);
+  unsigned long long key = idx >= 0 ? ld_u64(w.dist + idx) : 0ull;
+  wave_argmax(key, idx);
+  return idx;
 }
 
 // One workgroup. nbr/sep: n x md neighbour lists (-1 padded) of the directed proximity graph.
 __global__ void __launch_bounds__(FPS_THREADS) k_fps(int32_t n, int32_t md, const int32_t *__restrict__ nbr,
-                                                     const float *__restrict__ sep, int32_t k, int32_t P,
-                                                     int32_t levels, FpsWork w, int64_t *__restrict__ selected,
-                                                     int64_t *__restrict__ count) {
-  __shared__ int32_t s_qn, s_nn, s_cn, s_src;
-  const int t = threadIdx.x;
+                                                     const float *__restrict__ sep, int32_t k, FpsLevels L, FpsWork w,
+                                                     int64_t *__restrict__ selected, int64_t *__restrict__ count) {
+  __shared__ int32_t s_qn, s_nn, s_cn, s_dn, s_src;
+  const int t = threadIdx.x, wave = t / WAVE, nwaves = FPS_THREADS / WAVE;
   for (int32_t v = t; v < n; v += FPS_THREADS) {
     st_u64(w.dist + v, FPS_INF_BITS);
     st_i32(w.stamp + v, -1);
-    st_i32(w.cstamp + v, -1);
   }
-  for (int32_t i = t; i < P; i += FPS_THREADS) st_i32(w.tree + P + i, i < n ? i : -1);
+  for (int l = 1; l <= L.count; ++l)
+    for (int32_t i = t; i < L.size[l]; i += FPS_THREADS) st_i32(w.wstamp + L.off[l] + i, -1);
   if (t == 0) s_src = 0;  // Thea's first pick (FurthestPointSampling.cpp:79-83)
   __syncthreads();
   // Shortest paths are simple (a cycle adds >= 0), so a relaxation settles within n rounds.
@@ -88,11 +111,11 @@ __global__ void __launch_bounds__(FPS_THREADS) k_fps(int32_t n, int32_t md, cons
       st_u64(w.dist + src, 0ull);
       st_i32(w.qa, src);
       st_i32(w.changed, src);
-      st_i32(w.cstamp + src, it);
       s_qn = 1;
       s_cn = 1;
     }
     got = it + 1;
+    const int32_t first_round = round + 1;  // rounds of this pick are >= first_round
     __syncthreads();
     int32_t *cur = w.qa, *nxt = w.qb;
     int32_t rounds = 0;
@@ -108,22 +131,21 @@ __global__ void __launch_bounds__(FPS_THREADS) k_fps(int32_t n, int32_t md, cons
       __syncthreads();  // everyone has read s_qn
       if (t == 0) s_nn = 0;
       __syncthreads();
-      for (int32_t j = t; j < qn; j += FPS_THREADS) {
+      // one thread per (frontier vertex, edge)
+      const int64_t work = (int64_t)qn * md;
+      for (int64_t x = t; x < work; x += FPS_THREADS) {
+        const int32_t j = (int32_t)(x / md), e = (int32_t)(x - (int64_t)j * md);
         const int32_t u = ld_i32(cur + j);
+        const int32_t v = nbr[(int64_t)u * md + e];
+        if (v < 0) continue;
         const double du = __longlong_as_double((long long)ld_u64(w.dist + u));
-        const int32_t *nu = nbr + (int64_t)u * md;
-        const float *su = sep + (int64_t)u * md;
-        for (int e = 0; e < md; ++e) {
-          const int32_t v = nu[e];
-          if (v < 0) break;
-          const double cand = (double)su[e] + du;  // ShortestPaths.hpp:306 (nbr_dist + data->dist)
-          const unsigned long long cb = (unsigned long long)__double_as_longlong(cand);
-          if (cb >= ld_u64(w.dist + v)) continue;
-          const unsigned long long old = atomicMin(w.dist + v, cb);
-          if (old <= cb) continue;
-          if (atomicExch(w.stamp + v, round) != round) st_i32(nxt + atomicAdd(&s_nn, 1), v);
-          if (it > 0 && atomicExch(w.cstamp + v, it) != it) st_i32(w.changed + atomicAdd(&s_cn, 1), v);
-        }
+        const double cand = (double)sep[(int64_t)u * md + e] + du;  // ShortestPaths.hpp:306
+        const unsigned long long cb = (unsigned long long)__double_as_longlong(cand);
+        const unsigned long long old = atomicMin(w.dist + v, cb);
+        if (old <= cb) continue;
+        const int32_t prev = atomicExch(w.stamp + v, round);
+        if (prev != round) st_i32(nxt + atomicAdd(&s_nn, 1), v);
+        if (it > 0 && prev < first_round) st_i32(w.changed + atomicAdd(&s_cn, 1), v);
       }
       __syncthreads();
       if (t == 0) s_qn = s_nn;
@@ -137,23 +159,47 @@ __global__ void __launch_bounds__(FPS_THREADS) k_fps(int32_t n, int32_t md, cons
       break;
     }
     if (it + 1 == k) break;
-    if (it == 0) {  // every vertex's distance is new: build the whole tree
-      for (int32_t lvl = P >> 1; lvl >= 1; lvl >>= 1) {
-        for (int32_t i = lvl + t; i < 2 * lvl; i += FPS_THREADS)
-          st_i32(w.tree + i, fps_winner(w.dist, ld_i32(w.tree + 2 * i), ld_i32(w.tree + 2 * i + 1)));
-        __syncthreads();
-      }
-    } else {  // replay the ancestors of the changed vertices, level by level
-      const int32_t cn = s_cn;
-      for (int32_t l = 1; l <= levels; ++l) {
-        for (int32_t c = t; c < cn; c += FPS_THREADS) {
-          const int32_t node = (P + ld_i32(w.changed + c)) >> l;
-          st_i32(w.tree + node, fps_winner(w.dist, ld_i32(w.tree + 2 * node), ld_i32(w.tree + 2 * node + 1)));
+    if (it == 0) {  // every distance is new: reduce every node, level by level
+      for (int l = 1; l <= L.count; ++l) {
+        for (int32_t node = wave; node < L.size[l]; node += nwaves) {
+          const int32_t win = fps_node(w, L, l, node);
+          if (lane_id() == 0) st_i32(w.win + L.off[l] + node, win);
         }
         __syncthreads();
       }
+    } else {  // only the ancestors of changed vertices: dirty lists, deduplicated per pick
+      if (t == 0) s_dn = 0;
+      __syncthreads();
+      const int32_t cn = s_cn;
+      for (int32_t c = t; c < cn; c += FPS_THREADS) {
+        const int32_t node = ld_i32(w.changed + c) / FPS_FAN;
+        if (atomicExch(w.wstamp + L.off[1] + node, it) != it) st_i32(w.da + atomicAdd(&s_dn, 1), node);
+      }
+      __syncthreads();
+      int32_t *dcur = w.da, *dnext = w.db;
+      for (int l = 1; l <= L.count; ++l) {
+        const int32_t dn = s_dn;
+        __syncthreads();  // everyone has read s_dn
+        if (t == 0) s_dn = 0;
+        __syncthreads();
+        for (int32_t q = wave; q < dn; q += nwaves) {
+          const int32_t node = ld_i32(dcur + q);
+          const int32_t win = fps_node(w, L, l, node);
+          if (lane_id() == 0) {
+            st_i32(w.win + L.off[l] + node, win);
+            if (l < L.count) {
+              const int32_t parent = node / FPS_FAN;
+              if (atomicExch(w.wstamp + L.off[l + 1] + parent, it) != it) st_i32(dnext + atomicAdd(&s_dn, 1), parent);
+            }
+          }
+        }
+        __syncthreads();
+        int32_t *tmp = dcur;
+        dcur = dnext;
+        dnext = tmp;
+      }
     }
-    const int32_t best = ld_i32(w.tree + 1);
+    const int32_t best = ld_i32(w.win + L.off[L.count]);
     // The furthest vertex is at distance 0, so it is already picked (or coincides with a pick):
     // Thea stops with the picks so far (FurthestPointSampling.cpp:87-91).
     if (ld_u64(w.dist + best) == 0ull) break;
@@ -197,27 +243,36 @@ static int64_t fps_select(Ctx *c, const float *d_pts, int64_t n, int64_t k, cons
   build_bvh(b, d_pts, nullptr, n, true, 12, s);
   AsyncBuf idx((size_t)n * md * 4, s), sepb((size_t)n * md * 4, s), deg((size_t)n * 4, s);
   graph_build(&b, nullptr, md, trials, seed, s, idx.as<int32_t>(), sepb.as<float>(), deg.as<int32_t>());
-  int32_t P = 1, levels = 0;
-  while (P < n) {
-    P <<= 1;
-    ++levels;
+  FpsLevels L{};
+  L.size[0] = (int32_t)n;
+  int32_t total = 0;
+  while (L.size[L.count] > 1 || L.count == 0) {
+    require(L.count < 7, "too many points for the furthest-point tournament");
+    ++L.count;
+    L.size[L.count] = (L.size[L.count - 1] + FPS_FAN - 1) / FPS_FAN;
+    L.off[L.count] = total;
+    total += L.size[L.count];
   }
-  AsyncBuf dist((size_t)n * 8, s), stamp((size_t)n * 4, s), cstamp((size_t)n * 4, s), qa((size_t)n * 4, s),
-      qb((size_t)n * 4, s), changed((size_t)n * 4, s), tree((size_t)P * 8, s), small(16, s);
+  const size_t n1 = (size_t)L.size[1];
+  AsyncBuf dist((size_t)n * 8, s), stamp((size_t)n * 4, s), qa((size_t)n * 4, s), qb((size_t)n * 4, s),
+      changed((size_t)n * 4, s), win((size_t)total * 4, s), wstamp((size_t)total * 4, s), da(n1 * 4, s),
+      db(n1 * 4, s), small(16, s);
   THG_HIP(hipMemsetAsync(small.p, 0, 16, s));
   FpsWork w;
   w.dist = dist.as<unsigned long long>();
   w.stamp = stamp.as<int32_t>();
-  w.cstamp = cstamp.as<int32_t>();
   w.qa = qa.as<int32_t>();
   w.qb = qb.as<int32_t>();
   w.changed = changed.as<int32_t>();
-  w.tree = tree.as<int32_t>();
+  w.win = win.as<int32_t>();
+  w.wstamp = wstamp.as<int32_t>();
+  w.da = da.as<int32_t>();
+  w.db = db.as<int32_t>();
   w.status = small.as<int32_t>();
   int64_t *d_count = reinterpret_cast<int64_t *>(small.as<char>() + 8);
   KernelTimer kt(c, s, "fps");
   hipLaunchKernelGGL(k_fps, dim3(1), dim3(FPS_THREADS), 0, s, (int32_t)n, (int32_t)md, idx.as<int32_t>(),
-                     sepb.as<float>(), (int32_t)k, P, levels, w, d_sel, d_count);
+                     sepb.as<float>(), (int32_t)k, L, w, d_sel, d_count);
   launch_check("k_fps");
   kt.stop();
   int64_t h[2];
