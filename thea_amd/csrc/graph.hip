@@ -5,6 +5,8 @@
 //     sample, summed in pick order in float; avg = sqrt(sum / T) (PointSet3.hpp:362-376). Thea picks
 //     with the time-seeded Random::common(); here pick t is the sample (w * n) >> 32 for the first
 //     word w of Philox4x32-10 at counter (t, 0, GRAPH_PICK_STREAM, 0) with the caller's seed as key.
+//     The nearest other sample comes from a 2-NN query on the point BVH (the first neighbour that is
+//     not the pick itself).
 //  2. Neighbours of each sample (findSampleNeighbors, PointSet3.hpp:660-678): the max_degree smallest
 //     (separation, index) pairs among the other samples inside the ball of radius
 //     sqrt(max_degree) * avg (|p - c|^2 <= r^2 in float, BallN.hpp:83) whose normals are not opposed
@@ -53,24 +55,26 @@ __global__ void __launch_bounds__(256) k_scatter_points(const PointRec *__restri
   pos[3 * id + 2] = p.z;
 }
 
-// One block per pick: the smallest squared distance from the picked sample to any other sample.
-__global__ void __launch_bounds__(256) k_pick_nearest(const float *__restrict__ pos, int64_t n, const int64_t *__restrict__ picks,
-                                                      float *__restrict__ out) {
-  __shared__ float s_min[256];
-  const int64_t a = picks[blockIdx.x];
-  const float3 pa = f3(pos[3 * a], pos[3 * a + 1], pos[3 * a + 2]);
-  float best = INFINITY;
-  for (int64_t j = threadIdx.x; j < n; j += 256) {
-    if (j == a) continue;
-    best = fminf(best, sqn3(sub3(f3(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2]), pa)));
-  }
-  s_min[threadIdx.x] = best;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) s_min[threadIdx.x] = fminf(s_min[threadIdx.x], s_min[threadIdx.x + w]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[blockIdx.x] = s_min[0];
+// The separation estimate's queries: the positions of the picked samples.
+__global__ void __launch_bounds__(256) k_pick_points(const float *__restrict__ pos, const int64_t *__restrict__ picks,
+                                                     int64_t trials, float *__restrict__ q) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= trials) return;
+  const int64_t a = picks[t];
+  q[3 * t] = pos[3 * a];
+  q[3 * t + 1] = pos[3 * a + 1];
+  q[3 * t + 2] = pos[3 * a + 2];
+}
+
+// The smallest squared distance from each pick to any other sample: of its two nearest samples
+// (sorted by (d^2, index)), the first that is not the pick itself. A coincident sample of lower
+// index comes before the pick and is the answer (d^2 = 0), as a scan over all other samples gives.
+__global__ void __launch_bounds__(256) k_pick_select(const int64_t *__restrict__ picks, int64_t trials,
+                                                     const int32_t *__restrict__ idx2, const float *__restrict__ d2,
+                                                     float *__restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= trials) return;
+  out[t] = (int64_t)idx2[2 * t] != picks[t] ? d2[2 * t] : d2[2 * t + 1];
 }
 
 template <int STACK, int K>
@@ -195,9 +199,17 @@ float graph_build(Bvh *b, const float *d_nrm, int md, int64_t trials, uint64_t s
     picks[t] = (int64_t)(((uint64_t)w[0] * (uint64_t)n) >> 32);
   }
   THG_HIP(hipMemcpyAsync(picks_d.p, picks.data(), trials * 8, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_pick_nearest, dim3((unsigned)trials), dim3(256), 0, s, pos.as<float>(), n,
-                     picks_d.as<int64_t>(), near_d.as<float>());
-  launch_check("k_pick_nearest");
+  {
+    AsyncBuf qp(trials * 12, s), kidx(trials * 8, s), kd2(trials * 8, s);
+    hipLaunchKernelGGL(k_pick_points, dim3(ceil_div(trials, 256)), dim3(256), 0, s, pos.as<float>(),
+                       picks_d.as<int64_t>(), trials, qp.as<float>());
+    launch_check("k_pick_points");
+    QueryScratch ws;
+    launch_knn(*b, qp.as<float>(), trials, 2, -1.0, 0, kidx.as<int32_t>(), nullptr, kd2.as<float>(), ws, s);
+    hipLaunchKernelGGL(k_pick_select, dim3(ceil_div(trials, 256)), dim3(256), 0, s, picks_d.as<int64_t>(), trials,
+                       kidx.as<int32_t>(), kd2.as<float>(), near_d.as<float>());
+    launch_check("k_pick_select");
+  }
   std::vector<float> near(trials);
   THG_HIP(hipMemcpyAsync(near.data(), near_d.p, trials * 4, hipMemcpyDeviceToHost, s));
   THG_HIP(hipStreamSynchronize(s));
