@@ -1,7 +1,8 @@
 // LBVH construction on the GPU (replaces Thea's kd-tree build, Algorithms/BvhN.hpp:1366-1525).
 //
 //  1. k_elem_prep     element boxes and centres (triangle: Thea's centroid formula (v0+v1+v2)/3,
-//                     Triangle3.hpp:148; point: the point), scene bounds by ordered-int atomics.
+//                     Triangle3.hpp:148; point: the point), per-block bounds reduced by one more
+//                     small kernel (ordered-int min/max).
 //  2. k_morton        30-bit Morton code of each centre in the centre bounds.
 //  3. radix sort      (code, element) pairs, stable, so equal codes keep element order.
 //  4. k_karras        binary radix tree over the sorted codes (Karras, "Maximizing Parallelism in
@@ -31,7 +32,7 @@ __device__ __forceinline__ float3 ld_vert(const float *__restrict__ verts, int64
 __global__ void __launch_bounds__(B_BLOCK) k_elem_prep(const float *__restrict__ verts, const int32_t *__restrict__ tris,
                                                        int64_t n, int points, float4 *__restrict__ center,
                                                        float4 *__restrict__ blo, float4 *__restrict__ bhi,
-                                                       BoundsAcc *__restrict__ acc) {
+                                                       int *__restrict__ part) {
   float mn[6] = {INFINITY, INFINITY, INFINITY, INFINITY, INFINITY, INFINITY};
   float mx[6] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY, -INFINITY, -INFINITY};
   for (int64_t i = (int64_t)blockIdx.x * B_BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * B_BLOCK) {
@@ -55,16 +56,42 @@ __global__ void __launch_bounds__(B_BLOCK) k_elem_prep(const float *__restrict__
     mn[3] = fminf(mn[3], c.x); mn[4] = fminf(mn[4], c.y); mn[5] = fminf(mn[5], c.z);
     mx[3] = fmaxf(mx[3], c.x); mx[4] = fmaxf(mx[4], c.y); mx[5] = fmaxf(mx[5], c.z);
   }
+  // Block partials in BoundsAcc's slot order (no same-address atomics: they serialise in L2).
+  __shared__ float red[B_BLOCK / WAVE][12];
+  const int w = threadIdx.x / WAVE;
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
-    float a = wave_min(mn[k]), b = wave_max(mx[k]);
+    const float a = wave_min(mn[k]), b = wave_max(mx[k]);
     if (lane_id() == 0) {
-      int slot_lo = k < 3 ? k : 6 + (k - 3);
-      int slot_hi = k < 3 ? 3 + k : 9 + (k - 3);
-      atomicMin(&acc->v[slot_lo], float_to_ord(a));
-      atomicMax(&acc->v[slot_hi], float_to_ord(b));
+      red[w][k < 3 ? k : 6 + (k - 3)] = a;
+      red[w][k < 3 ? 3 + k : 9 + (k - 3)] = b;
     }
   }
+  __syncthreads();
+  if (threadIdx.x < 12) {
+    const int k = threadIdx.x;
+    const bool lo_slot = k < 3 || (k >= 6 && k < 9);
+    float v = red[0][k];
+    for (int ww = 1; ww < B_BLOCK / WAVE; ++ww) v = lo_slot ? fminf(v, red[ww][k]) : fmaxf(v, red[ww][k]);
+    part[(int64_t)blockIdx.x * 12 + k] = float_to_ord(v);
+  }
+}
+
+// One wave per slot: the min (lo slots) or max (hi slots) of the block partials.
+__global__ void __launch_bounds__(12 * WAVE) k_bounds_reduce(const int *__restrict__ part, int nb,
+                                                             BoundsAcc *__restrict__ acc) {
+  const int k = threadIdx.x / WAVE, lane = lane_id();
+  const bool lo_slot = k < 3 || (k >= 6 && k < 9);
+  int v = lo_slot ? float_to_ord(INFINITY) : float_to_ord(-INFINITY);
+  for (int b = lane; b < nb; b += WAVE) {
+    const int x = part[(int64_t)b * 12 + k];
+    v = lo_slot ? min(v, x) : max(v, x);
+  }
+  for (int off = 32; off >= 1; off >>= 1) {
+    const int o = __shfl_xor(v, off, WAVE);
+    v = lo_slot ? min(v, o) : max(v, o);
+  }
+  if (lane == 0) acc->v[k] = v;
 }
 
 __device__ __forceinline__ uint32_t spread10(uint32_t v) {
@@ -402,14 +429,6 @@ __global__ void k_single_leaf(int64_t n, float3 lo, float3 hi, BNode *out) {
   out[0] = nd;
 }
 
-__global__ void k_init_bounds(BoundsAcc *acc) {
-  int t = threadIdx.x;
-  if (t < 12) {
-    bool is_lo = (t < 3) || (t >= 6 && t < 9);
-    acc->v[t] = is_lo ? float_to_ord(INFINITY) : float_to_ord(-INFINITY);
-  }
-}
-
 // Exclusive scan of n uint32 into out (total in out[n]); uses the radix scan kernel's approach.
 // (in and out may be the same array)
 __global__ void __launch_bounds__(1024) k_excl_scan_u32(const uint32_t *in, uint32_t *out, int64_t n) {
@@ -547,13 +566,14 @@ void build_bvh(Bvh &b, const float *verts, const int32_t *tris, int64_t n, bool 
     b.max_depth = 0;
     return;
   }
-  DevBuf center(n * sizeof(float4)), blo(n * sizeof(float4)), bhi(n * sizeof(float4)), acc(sizeof(BoundsAcc));
-  hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, s, acc.as<BoundsAcc>());
-  launch_check("k_init_bounds");
   uint32_t grid = std::min<uint32_t>(ceil_div(n, B_BLOCK), 4096);
+  DevBuf center(n * sizeof(float4)), blo(n * sizeof(float4)), bhi(n * sizeof(float4)), acc(sizeof(BoundsAcc)),
+      part((size_t)grid * 12 * sizeof(int));
   hipLaunchKernelGGL(k_elem_prep, dim3(grid), dim3(B_BLOCK), 0, s, verts, tris, n, points ? 1 : 0,
-                     center.as<float4>(), blo.as<float4>(), bhi.as<float4>(), acc.as<BoundsAcc>());
+                     center.as<float4>(), blo.as<float4>(), bhi.as<float4>(), part.as<int>());
   launch_check("k_elem_prep");
+  hipLaunchKernelGGL(k_bounds_reduce, dim3(1), dim3(12 * WAVE), 0, s, part.as<int>(), (int)grid, acc.as<BoundsAcc>());
+  launch_check("k_bounds_reduce");
   BoundsAcc h;
   THG_HIP(hipMemcpyAsync(&h, acc.p, sizeof(h), hipMemcpyDeviceToHost, s));
   THG_HIP(hipStreamSynchronize(s));

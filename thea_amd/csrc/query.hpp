@@ -54,6 +54,8 @@ void launch_emit(int64_t ni, int32_t base, const uint32_t *keep, const uint32_t 
                  hipStream_t s);
 void launch_depth(int64_t n, const int32_t *lparent, const int32_t *nparent, const uint32_t *keep, int32_t *max_depth,
                   hipStream_t s);
+// Exclusive scan of n words: out must hold n + 1 words (the total lands in out[n]) and tmp
+// scan_tmp_words(n) words.
 size_t scan_tmp_words(int64_t n);
 void scan_excl_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *tmp, hipStream_t s);
 

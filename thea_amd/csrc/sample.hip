@@ -23,27 +23,43 @@ __device__ __forceinline__ float3 vtx(const float *__restrict__ v, int64_t i) {
 
 __global__ void __launch_bounds__(S_BLOCK) k_areas(const float *__restrict__ verts, const int32_t *__restrict__ tris,
                                                    int64_t nt, float *__restrict__ area, int *__restrict__ stats) {
-  // stats[0]: min ulp exponent over nonzero areas (as int), stats[1]: max area (ordered int bits)
-  int64_t i = (int64_t)blockIdx.x * S_BLOCK + threadIdx.x;
+  // stats[0]: min ulp exponent over nonzero areas (as int), stats[1]: max area (ordered int bits).
+  // Grid-stride loop, one pair of atomics per block.
   int ue = 1 << 20;
-  float a = 0;
-  if (i < nt) {
+  float mx = 0;
+  for (int64_t i = (int64_t)blockIdx.x * S_BLOCK + threadIdx.x; i < nt; i += (int64_t)gridDim.x * S_BLOCK) {
     float3 v0 = vtx(verts, tris[3 * i]), v1 = vtx(verts, tris[3 * i + 1]), v2 = vtx(verts, tris[3 * i + 2]);
-    a = tri_area(v0, v1, v2);
+    const float a = tri_area(v0, v1, v2);
     area[i] = a;
+    int e = 1 << 20;
     if (a > 0) {
-      if (!isfinite(a)) ue = -(1 << 20);  // forces the sequential path
-      else if (a < 1.17549435e-38f) ue = -149;
-      else ue = ilogbf(a) - 23;
+      if (!isfinite(a)) e = -(1 << 20);  // forces the sequential path
+      else if (a < 1.17549435e-38f) e = -149;
+      else e = ilogbf(a) - 23;
     } else if (!(a == 0)) {
-      ue = -(1 << 20);  // NaN
+      e = -(1 << 20);  // NaN
     }
+    ue = min(ue, e);
+    if (a == a) mx = fmaxf(mx, a);
   }
-  float mx = wave_max(a == a ? a : 0.0f);
-  int mn = (int)wave_min((float)ue);
+  __shared__ int s_ue[S_BLOCK / WAVE];
+  __shared__ float s_mx[S_BLOCK / WAVE];
+  const float wm = wave_max(mx);
+  const int wu = (int)wave_min((float)ue);
   if (lane_id() == 0) {
-    atomicMin(&stats[0], mn);
-    atomicMax(&stats[1], float_to_ord(mx));
+    s_ue[threadIdx.x / WAVE] = wu;
+    s_mx[threadIdx.x / WAVE] = wm;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int bu = s_ue[0];
+    float bm = s_mx[0];
+    for (int w = 1; w < S_BLOCK / WAVE; ++w) {
+      bu = min(bu, s_ue[w]);
+      bm = fmaxf(bm, s_mx[w]);
+    }
+    atomicMin(&stats[0], bu);
+    atomicMax(&stats[1], float_to_ord(bm));
   }
 }
 
@@ -130,7 +146,7 @@ void mesh_cdf(Mesh &m, hipStream_t s) {
   DevBuf stats(8);
   int init[2] = {1 << 20, float_to_ord_host(-INFINITY)};
   THG_HIP(hipMemcpyAsync(stats.p, init, 8, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_areas, dim3(ceil_div(m.nt, S_BLOCK)), dim3(S_BLOCK), 0, s, m.verts.as<float>(),
+  hipLaunchKernelGGL(k_areas, dim3(std::min<uint32_t>(ceil_div(m.nt, S_BLOCK), 1024)), dim3(S_BLOCK), 0, s, m.verts.as<float>(),
                      m.tris.as<int32_t>(), m.nt, m.area.as<float>(), stats.as<int>());
   launch_check("k_areas");
   int st[2];
@@ -266,18 +282,18 @@ int64_t launch_sample_approx(Mesh &m, int64_t n, uint64_t seed, uint64_t stream_
                              float *nrm, int32_t *tri, hipStream_t s) {
   if (n <= 0 || m.nt <= 0 || !(m.total_area > 0)) return 0;
   require(n < ((int64_t)1 << 31), "APPROXIMATE sampling supports fewer than 2^31 samples");
-  AsyncBuf cnt(m.nt * 4, s), off(m.nt * 4, s), tmp(((m.nt + 1023) / 1024 + 1) * 4, s);
+  // the scan writes the total after the last offset: off holds nt + 1 words
+  AsyncBuf cnt(m.nt * 4, s), off((m.nt + 1) * 4, s), tmp(scan_tmp_words(m.nt) * 4, s);
   const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32), s0 = (uint32_t)stream_id,
                  s1 = (uint32_t)(stream_id >> 32);
   hipLaunchKernelGGL(k_approx_count, dim3(ceil_div(m.nt, S_BLOCK)), dim3(S_BLOCK), 0, s, m.area.as<float>(), m.nt,
                      (double)n, m.total_area, k0, k1, s0, s1, cnt.as<uint32_t>());
   launch_check("k_approx_count");
   scan_excl_u32(cnt.as<uint32_t>(), off.as<uint32_t>(), m.nt, tmp.as<uint32_t>(), s);
-  uint32_t last[2];
-  THG_HIP(hipMemcpyAsync(&last[0], off.as<uint32_t>() + (m.nt - 1), 4, hipMemcpyDeviceToHost, s));
-  THG_HIP(hipMemcpyAsync(&last[1], cnt.as<uint32_t>() + (m.nt - 1), 4, hipMemcpyDeviceToHost, s));
+  uint32_t tot = 0;
+  THG_HIP(hipMemcpyAsync(&tot, off.as<uint32_t>() + m.nt, 4, hipMemcpyDeviceToHost, s));
   THG_HIP(hipStreamSynchronize(s));
-  const int64_t total = (int64_t)last[0] + (int64_t)last[1];
+  const int64_t total = (int64_t)tot;
   if (!pos) return total;
   require(total <= capacity, "APPROXIMATE sampling produced " + std::to_string(total) +
                                  " samples, more than the output capacity " + std::to_string(capacity));
