@@ -55,6 +55,37 @@ def test_null_handles_are_invalid_arguments():
     assert lib.thg_version().startswith(b"thea_gpu")
 
 
+def test_null_handles_of_the_feature_entry_points():
+    """The ray features, separation sampling and APPROXIMATE sampling reject NULL handles and bad
+    sizes before touching a device."""
+    from thea_amd import _capi
+    lib = _capi.lib()
+    got = C.c_int64()
+    E = _capi.THG_E_INVALID
+    assert lib.thg_shape_diameter(None, None, None, 0, 1.0, 1, 1, None, None, None) == E
+    assert lib.thg_visibility(None, None, 0, 100, 0, -1.0, 1, None, None) == E
+    assert lib.thg_furthest_point_sampling(None, None, 0, 0, None, 1, None, None, C.byref(got)) == E
+    assert b"NULL" in lib.thg_last_error()
+    assert lib.thg_sample_separation(None, 10, -1.0, 0, 0, None, None, None, None, C.byref(got), 1, None) == E
+    assert lib.thg_sample_area_approx(None, 10, 0, 0, 0, None, None, None, C.byref(got), 1, None) == E
+    assert lib.thg_proximity_graph(None, None, None, 1, None, None, None, None, None) == E
+    assert lib.thg_icp_align(None, None, 0, 1, None, None, 1, None, None, None, None) == E
+    assert lib.thg_bvh_bounds(None, None, None) == E
+
+
+def test_file_entry_points_report_errors(tmp_path):
+    from thea_amd import _capi
+    lib = _capi.lib()
+    m = _capi.MeshFile()
+    assert lib.thg_mesh_file_read(str(tmp_path / "missing.obj").encode(), C.byref(m)) == _capi.THG_E_INVALID
+    assert b"could not open" in lib.thg_last_error()
+    lib.thg_mesh_file_free(C.byref(m))  # zeroed by the failed read: a no-op
+    assert lib.thg_pts_write(None, 0, None, None, None) == _capi.THG_E_INVALID
+    n = C.c_int64()
+    p = C.POINTER(C.c_float)()
+    assert lib.thg_pts_read(str(tmp_path / "missing.pts").encode(), C.byref(n), C.byref(p)) == _capi.THG_E_INVALID
+
+
 def test_no_device_reports_error():
     from thea_amd import _capi
     try:
