@@ -58,14 +58,21 @@ __global__ void __launch_bounds__(256) k_face_normals(const int32_t *__restrict_
   nrm[3 * i + 2] = nn.z;
 }
 
-// One thread per (point, ray): origin and direction of ray r of point i.
+// Rays are laid out cone-major over the points in Morton order: ray g = r * n + k is cone
+// direction r of point order[k] (k itself without an order). A wave then holds one cone direction
+// for 64 neighbouring points, whose rays are nearly parallel, instead of 30 directions of 2 points.
+__device__ __forceinline__ int64_t sdf_point(const uint32_t *__restrict__ order, int64_t k) {
+  return order ? (int64_t)order[k] : k;
+}
+
+// One thread per ray: origin and direction of cone ray r of point i.
 __global__ void __launch_bounds__(256) k_sdf_rays(const float *__restrict__ pos, const float *__restrict__ nrm,
-                                                  int64_t n, float scale, float *__restrict__ org,
-                                                  float *__restrict__ dir) {
+                                                  int64_t n, const uint32_t *__restrict__ order, float scale,
+                                                  float *__restrict__ org, float *__restrict__ dir) {
   const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (g >= n * SDF_RAYS) return;
-  const int64_t i = g / SDF_RAYS;
-  const int r = (int)(g - i * SDF_RAYS);
+  const int r = (int)(g / n);
+  const int64_t i = sdf_point(order, g - (int64_t)r * n);
   const float3 normal = f3(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
   const float3 nn = normalized3(normal);
   const float3 in = f3(-nn.x, -nn.y, -nn.z);
@@ -101,22 +108,31 @@ __global__ void __launch_bounds__(256) k_sdf_valid(int64_t m, const int32_t *__r
   val[g] = ok ? t[g] : -1.0f;
 }
 
+// Per-ray values from the ray layout to the caller's point-major layout (point i, ray r at 30 i + r).
+__global__ void __launch_bounds__(256) k_sdf_times(int64_t n, const uint32_t *__restrict__ order,
+                                                   const float *__restrict__ val, float *__restrict__ times) {
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (g >= n * SDF_RAYS) return;
+  const int r = (int)(g / n);
+  times[sdf_point(order, g - (int64_t)r * n) * SDF_RAYS + r] = val[g];
+}
+
 constexpr int SDF_RED_THREADS = 256;
 
-// One thread per point: the point's 30 ray values are staged through LDS (coalesced global reads),
-// compacted to the valid ones with their ray numbers, and reduced as ShapeDiameter.hpp:230-256.
-__global__ void __launch_bounds__(SDF_RED_THREADS) k_sdf_reduce(int64_t n, const float *__restrict__ val, double scale,
+// One thread per point (in ray-layout order k): the point's 30 ray values are read with coalesced
+// loads (ray r of the points k.. k+63 are consecutive), compacted in LDS to the valid ones with their
+// ray numbers, and reduced as ShapeDiameter.hpp:230-256.
+__global__ void __launch_bounds__(SDF_RED_THREADS) k_sdf_reduce(int64_t n, const float *__restrict__ val,
+                                                                const uint32_t *__restrict__ order, double scale,
                                                                 double *__restrict__ sdf) {
   __shared__ float sv[SDF_RAYS * SDF_RED_THREADS];
   __shared__ uint8_t sr[SDF_RAYS * SDF_RED_THREADS];
   const int t = threadIdx.x;
-  const int64_t p0 = (int64_t)blockIdx.x * SDF_RED_THREADS;
-  const int np = (int)min((int64_t)SDF_RED_THREADS, n - p0);
-  const float *src = val + p0 * SDF_RAYS;
-  for (int j = t; j < np * SDF_RAYS; j += SDF_RED_THREADS) sv[j] = src[j];
-  __syncthreads();
-  if (t >= np) return;
+  const int64_t k = (int64_t)blockIdx.x * SDF_RED_THREADS + t;
+  if (k >= n) return;
   float *mine = sv + t * SDF_RAYS;
+  for (int r = 0; r < SDF_RAYS; ++r) mine[r] = val[(int64_t)r * n + k];
+  const int64_t out = sdf_point(order, k);
   uint8_t *rays = sr + t * SDF_RAYS;
   int nv = 0;
   for (int r = 0; r < SDF_RAYS; ++r) {
@@ -128,7 +144,7 @@ __global__ void __launch_bounds__(SDF_RED_THREADS) k_sdf_reduce(int64_t n, const
     }
   }
   if (nv <= 0) {
-    sdf[p0 + t] = -1.0;
+    sdf[out] = -1.0;
     return;
   }
   const int mid = nv / 2;
@@ -154,7 +170,7 @@ __global__ void __launch_bounds__(SDF_RED_THREADS) k_sdf_reduce(int64_t n, const
     }
   }
   const double y = sum_weights > 0 ? sum_values / (sum_weights * scale) : vmid / scale;
-  sdf[p0 + t] = fmin(1.0, fmax(0.0, y));
+  sdf[out] = fmin(1.0, fmax(0.0, y));
 }
 
 }  // namespace thg
@@ -191,7 +207,7 @@ extern "C" int thg_shape_diameter(thg_bvh mesh_bvh, const float *positions, cons
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (n == 0) return;
     const int64_t m = n * SDF_RAYS;
-    AsyncBuf pos_d, nrm_d, org(m * 12, s), dir(m * 12, s), ridx(m * 4, s), rt(m * 4, s), rn(m * 12, s), val;
+    AsyncBuf pos_d, nrm_d, org(m * 12, s), dir(m * 12, s), ridx(m * 4, s), rt(m * 4, s), rn(m * 12, s), val(m * 4, s);
     const float *d_pos = positions, *d_nrm = normals;
     if (mem == THG_MEM_HOST) {
       pos_d.ensure(n * 12, s);
@@ -215,34 +231,30 @@ extern "C" int thg_shape_diameter(thg_bvh mesh_bvh, const float *positions, cons
       launch_check("k_face_normals");
       d_nrm = nrm_d.as<float>();
     }
-    hipLaunchKernelGGL(k_sdf_rays, dim3(ceil_div(m, 256)), dim3(256), 0, s, d_pos, d_nrm, n, scale, org.as<float>(),
-                       dir.as<float>());
+    // Morton order of the points (none for small batches), then the cone-major ray layout.
+    QueryScratch pws;
+    order_queries(*b, d_pos, n, pws, s);
+    const uint32_t *pord = pws.order;
+    hipLaunchKernelGGL(k_sdf_rays, dim3(ceil_div(m, 256)), dim3(256), 0, s, d_pos, d_nrm, n, pord, scale,
+                       org.as<float>(), dir.as<float>());
     launch_check("k_sdf_rays");
     QueryScratch ws;
     launch_ray(*b, org.as<float>(), dir.as<float>(), m, -1.0f, false, ridx.as<int32_t>(), rt.as<float>(), rn.as<float>(),
-               nullptr, ws, s);
-    // The per-ray values go straight into ray_times when the caller wants them in device memory.
-    float *d_val = (ray_times && mem == THG_MEM_DEVICE) ? ray_times : nullptr;
-    if (!d_val) {
-      val.ensure(m * 4, s);
-      d_val = val.as<float>();
-    }
+               nullptr, ws, s, /*sort_rays=*/false);
     hipLaunchKernelGGL(k_sdf_valid, dim3(ceil_div(m, 256)), dim3(256), 0, s, m, ridx.as<int32_t>(), rt.as<float>(),
-                       rn.as<float>(), dir.as<float>(), only_interior ? 1 : 0, d_val);
+                       rn.as<float>(), dir.as<float>(), only_interior ? 1 : 0, val.as<float>());
     launch_check("k_sdf_valid");
-    double *d_sdf = sdf;
-    AsyncBuf sdf_d;
-    if (mem == THG_MEM_HOST) {
-      sdf_d.ensure(n * 8, s);
-      d_sdf = sdf_d.as<double>();
-    }
+    Stage st(s, mem == THG_MEM_HOST);
+    double *d_sdf = st.out(sdf, (size_t)n);
+    float *d_times = st.out(ray_times, (size_t)m);
     hipLaunchKernelGGL(k_sdf_reduce, dim3(ceil_div(n, SDF_RED_THREADS)), dim3(SDF_RED_THREADS), 0, s, n,
-                       (const float *)d_val, (double)scale, d_sdf);
+                       (const float *)val.as<float>(), pord, (double)scale, d_sdf);
     launch_check("k_sdf_reduce");
-    if (mem == THG_MEM_HOST) {
-      THG_HIP(hipMemcpyAsync(sdf, d_sdf, n * 8, hipMemcpyDeviceToHost, s));
-      if (ray_times) THG_HIP(hipMemcpyAsync(ray_times, d_val, m * 4, hipMemcpyDeviceToHost, s));
-      THG_HIP(hipStreamSynchronize(s));
+    if (d_times) {
+      hipLaunchKernelGGL(k_sdf_times, dim3(ceil_div(m, 256)), dim3(256), 0, s, n, pord, (const float *)val.as<float>(),
+                         d_times);
+      launch_check("k_sdf_times");
     }
+    st.finish();
   });
 }
