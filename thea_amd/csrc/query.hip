@@ -1511,10 +1511,11 @@ void launch_knn(const Bvh &b, const float *q, int64_t n, int k, double dist_boun
 }
 
 void launch_ray(const Bvh &b, const float *o, const float *d, int64_t n, float max_time, bool any, int32_t *idx,
-                float *t, float *nrm, uint8_t *hit, QueryScratch &ws, hipStream_t s, bool sort_rays) {
+                float *t, float *nrm, uint8_t *hit, QueryScratch &ws, hipStream_t s, bool sort_rays,
+                const uint32_t *given_order) {
   if (n <= 0) return;
   if (sort_rays) order_queries(b, o, n, ws, s);
-  else ws.order = nullptr;
+  else ws.order = given_order;
   uint32_t nb = ceil_div(n, Q_BLOCK);
   uint32_t grid = round_up8(nb);
   // Thea accepts only hits strictly before max_time at the structure level (BvhN.hpp:2247-2258).

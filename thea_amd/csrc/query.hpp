@@ -76,9 +76,12 @@ void closest_wave_stats(const Bvh &b, const float *q, int64_t n, unsigned long l
                         hipStream_t s);
 void launch_knn(const Bvh &b, const float *q, int64_t n, int k, double dist_bound, int exclude_self, int32_t *idx,
                 float *dist, float *d2, QueryScratch &ws, hipStream_t s);
-// sort_rays = false traces the rays in the given order (the caller has already made them coherent).
+// sort_rays = false traces the rays in the order given by given_order (ray given_order[j] in slot j;
+// nullptr: input order), for callers that make the rays coherent themselves. Results are always
+// written at the rays' input indices.
 void launch_ray(const Bvh &b, const float *o, const float *d, int64_t n, float max_time, bool any, int32_t *idx,
-                float *t, float *nrm, uint8_t *hit, QueryScratch &ws, hipStream_t s, bool sort_rays = true);
+                float *t, float *nrm, uint8_t *hit, QueryScratch &ws, hipStream_t s, bool sort_rays = true,
+                const uint32_t *given_order = nullptr);
 
 // Sampler
 void mesh_cdf(Mesh &m, hipStream_t s);

@@ -60,6 +60,30 @@ __global__ void __launch_bounds__(256) k_vis_rays(const float *__restrict__ pos,
   dir[3 * g + 2] = u.z;
 }
 
+__device__ __forceinline__ uint32_t vis_spread7(uint32_t v) {  // 7 bits -> every third bit
+  uint32_t r = 0;
+  for (int i = 0; i < 7; ++i) r |= ((v >> i) & 1u) << (3 * i);
+  return r;
+}
+
+// Tracing order of the rays: key = direction octant (3 bits) above the 21-bit Morton code of the
+// origin (7 bits per axis in the scene box), so a wave traces rays of one octant from nearby
+// points rather than 64 random directions from one point (74.4 -> 67.6 ms per 100M C3 rays; an
+// 8 x 8 octahedral direction grid over a 6-bit-per-axis origin code measured 70.9 ms).
+__global__ void __launch_bounds__(256) k_vis_keys(const float *__restrict__ org, const float *__restrict__ dir,
+                                                  int64_t m, float3 lo, float3 inv, uint32_t *__restrict__ keys,
+                                                  uint32_t *__restrict__ vals) {
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (g >= m) return;
+  const float x = (org[3 * g] - lo.x) * inv.x, y = (org[3 * g + 1] - lo.y) * inv.y, z = (org[3 * g + 2] - lo.z) * inv.z;
+  const uint32_t ix = (uint32_t)fminf(fmaxf(x * 128.0f, 0.0f), 127.0f);
+  const uint32_t iy = (uint32_t)fminf(fmaxf(y * 128.0f, 0.0f), 127.0f);
+  const uint32_t iz = (uint32_t)fminf(fmaxf(z * 128.0f, 0.0f), 127.0f);
+  const uint32_t oct = (dir[3 * g] < 0 ? 4u : 0u) | (dir[3 * g + 1] < 0 ? 2u : 0u) | (dir[3 * g + 2] < 0 ? 1u : 0u);
+  keys[g] = (oct << 21) | (vis_spread7(ix) << 2) | (vis_spread7(iy) << 1) | vis_spread7(iz);
+  vals[g] = (uint32_t)g;
+}
+
 // One thread per point: the fraction of its rays that escaped.
 __global__ void __launch_bounds__(256) k_vis_count(const uint8_t *__restrict__ hit, int64_t np, int32_t R,
                                                    double *__restrict__ vis) {
@@ -116,15 +140,23 @@ extern "C" int thg_visibility(thg_bvh mesh_bvh, const float *positions, int64_t 
     double *d_vis = st.out(visibility, (size_t)n);
     const int64_t per = std::max<int64_t>(1, VIS_CHUNK_RAYS / R);
     const int64_t cap = std::min<int64_t>(n, per) * R;
-    AsyncBuf org(cap * 12, s), dir(cap * 12, s), hit(cap, s);
+    AsyncBuf org(cap * 12, s), dir(cap * 12, s), hit(cap, s), keys(cap * 4, s), vals(cap * 4, s), tk(cap * 4, s),
+        tv(cap * 4, s), hist(radix_hist_words(cap) * 4, s);
+    const float3 ext = f3(b->hi.x - b->lo.x, b->hi.y - b->lo.y, b->hi.z - b->lo.z);
+    const float3 inv = f3(ext.x > 0 ? 1.0f / ext.x : 0.0f, ext.y > 0 ? 1.0f / ext.y : 0.0f, ext.z > 0 ? 1.0f / ext.z : 0.0f);
     QueryScratch ws;
     for (int64_t p0 = 0; p0 < n; p0 += per) {
       const int64_t np = std::min(per, n - p0), m = np * R;
       hipLaunchKernelGGL(k_vis_rays, dim3(ceil_div(m, 256)), dim3(256), 0, s, d_pos, p0, np, R, scale, seed,
                          org.as<float>(), dir.as<float>());
       launch_check("k_vis_rays");
+      hipLaunchKernelGGL(k_vis_keys, dim3(ceil_div(m, 256)), dim3(256), 0, s, org.as<float>(), dir.as<float>(), m, b->lo,
+                         inv, keys.as<uint32_t>(), vals.as<uint32_t>());
+      launch_check("k_vis_keys");
+      radix_sort_pairs(keys.as<uint32_t>(), vals.as<uint32_t>(), tk.as<uint32_t>(), tv.as<uint32_t>(),
+                       hist.as<uint32_t>(), m, 0, 24, s);
       launch_ray(*b, org.as<float>(), dir.as<float>(), m, -1.0f, true, nullptr, nullptr, nullptr, hit.as<uint8_t>(), ws,
-                 s);
+                 s, /*sort_rays=*/false, vals.as<uint32_t>());
       hipLaunchKernelGGL(k_vis_count, dim3(ceil_div(np, 256)), dim3(256), 0, s, hit.as<uint8_t>(), np, R,
                          d_vis + p0);
       launch_check("k_vis_count");
