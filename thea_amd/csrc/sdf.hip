@@ -65,14 +65,27 @@ __device__ __forceinline__ int64_t sdf_point(const uint32_t *__restrict__ order,
   return order ? (int64_t)order[k] : k;
 }
 
-// One thread per ray: origin and direction of cone ray r of point i.
+// Positions and normals in the ray layout's point order (k-th row = point order[k]).
+__global__ void __launch_bounds__(256) k_sdf_gather(int64_t n, const uint32_t *__restrict__ order,
+                                                    const float *__restrict__ pos, const float *__restrict__ nrm,
+                                                    float *__restrict__ spos, float *__restrict__ snrm) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  const int64_t i = sdf_point(order, k);
+  for (int c = 0; c < 3; ++c) {
+    spos[3 * k + c] = pos[3 * i + c];
+    snrm[3 * k + c] = nrm[3 * i + c];
+  }
+}
+
+// One thread per ray: origin and direction of cone ray r of the k-th point (pos/nrm in that order).
 __global__ void __launch_bounds__(256) k_sdf_rays(const float *__restrict__ pos, const float *__restrict__ nrm,
-                                                  int64_t n, const uint32_t *__restrict__ order, float scale,
-                                                  float *__restrict__ org, float *__restrict__ dir) {
+                                                  int64_t n, float scale, float *__restrict__ org,
+                                                  float *__restrict__ dir) {
   const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (g >= n * SDF_RAYS) return;
   const int r = (int)(g / n);
-  const int64_t i = sdf_point(order, g - (int64_t)r * n);
+  const int64_t i = g - (int64_t)r * n;
   const float3 normal = f3(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
   const float3 nn = normalized3(normal);
   const float3 in = f3(-nn.x, -nn.y, -nn.z);
@@ -235,8 +248,12 @@ extern "C" int thg_shape_diameter(thg_bvh mesh_bvh, const float *positions, cons
     QueryScratch pws;
     order_queries(*b, d_pos, n, pws, s);
     const uint32_t *pord = pws.order;
-    hipLaunchKernelGGL(k_sdf_rays, dim3(ceil_div(m, 256)), dim3(256), 0, s, d_pos, d_nrm, n, pord, scale,
-                       org.as<float>(), dir.as<float>());
+    AsyncBuf spos(n * 12, s), snrm(n * 12, s);
+    hipLaunchKernelGGL(k_sdf_gather, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, pord, d_pos, d_nrm, spos.as<float>(),
+                       snrm.as<float>());
+    launch_check("k_sdf_gather");
+    hipLaunchKernelGGL(k_sdf_rays, dim3(ceil_div(m, 256)), dim3(256), 0, s, (const float *)spos.as<float>(),
+                       (const float *)snrm.as<float>(), n, scale, org.as<float>(), dir.as<float>());
     launch_check("k_sdf_rays");
     QueryScratch ws;
     launch_ray(*b, org.as<float>(), dir.as<float>(), m, -1.0f, false, ridx.as<int32_t>(), rt.as<float>(), rn.as<float>(),
