@@ -874,7 +874,25 @@ def cpu_baseline(args, wl, v, t, q, rays, unit):
     return {"value": n_multi / multi_s, "unit": unit, "cores": nth, "kind": "port",
             "sample": f"first {n_multi} queries on {nth} threads ({multi_s:.1f} s) and first {n_single} on 1 thread "
                       f"of the same batch; kd-tree build {build_s:.2f} s excluded",
-            "single_thread_value": n_single / single_s, "build_s": build_s}
+            "single_thread_value": n_single / single_s, "build_s": build_s, "host": host_info()}
+
+
+def host_info():
+    """CPU model and the cores this process may use (the GPU box's share, not the machine's)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"cpu_model": model, "usable_cores": usable, "nproc": os.cpu_count()}
 
 
 if __name__ == "__main__":
