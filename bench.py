@@ -347,6 +347,7 @@ def run_sdf(args):
         "data": "synthetic",
         "config": {"workload": "ShapeDiameter at C3 area samples with face normals", "points": n, "rays": 30 * n,
                    "leaf_size": args.leaf, "scale": scale},
+        "roofline": ray_roofline("c3_sdf_ray", 30 * n, k_ms / max(k_n, 1), "first-hit traversal"),
         "ray_kernel_avg_ms": k_ms / max(k_n, 1), "rays_per_s_kernel": 30 * n / (k_ms / max(k_n, 1) * 1e-3),
         "median_sdf": float(sdf[sdf >= 0].median()),
     }
@@ -362,6 +363,21 @@ def run_sdf(args):
                                   "sample": f"the first {m} points ({csec:.1f} s): rays on all threads, generation "
                                             f"and reduction on one"}
     print(json.dumps(result), flush=True)
+
+
+def ray_roofline(counts_key, rays, kernel_ms, kernel):
+    """Roofline object of a ray feature's traversal: Thea's per-ray box and triangle tests on the same
+    rays (tests/golden/workload_counts.json) priced at 32 + 24 V_box + 36 V_tri bytes per ray, over
+    the traversal kernels' time for the batch (HIP events on their stream)."""
+    c = load_counts().get(counts_key)
+    if not c or kernel_ms <= 0:
+        return None
+    bpr = 32 + 24 * c["V_box"] + 36 * c["V_tri"]
+    achieved = bpr * rays / (kernel_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS, "traffic": None, "kernel": kernel, "kernel_ms_per_step": kernel_ms,
+            "bytes_per_ray": bpr, "thea_traversal_counts": c,
+            "model": "algorithmic bytes of Thea's own first-hit traversal of the same rays (sample of the batch)"}
 
 
 def run_vis(args):
@@ -397,6 +413,7 @@ def run_vis(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 rays, f64 fractions",
         "data": "synthetic",
         "config": {"workload": "Visibility at C3 area samples", "points": n, "rays": 100 * n, "leaf_size": args.leaf},
+        "roofline": ray_roofline("c3_vis_ray", 100 * n, k_ms / max(args.steps, 1), "any-hit traversal"),
         "any_hit_ms_per_batch": k_ms / max(args.steps, 1), "any_hit_launches_per_batch": k_n / max(args.steps, 1),
         "rays_per_s_kernel": 100 * n * args.steps / (k_ms * 1e-3) if k_ms > 0 else None,
         "mean_visibility": float(vis.mean()),
