@@ -15,6 +15,7 @@
 #include <cstring>
 #include <fstream>
 #include <limits>
+#include <memory>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -419,12 +420,16 @@ extern "C" int thg_mesh_file_read(const char *path, thg_mesh_file *out) {
     std::vector<int32_t> tris;
     std::vector<int64_t> tri_face;
     triangulate(m, tris, tri_face);
+    // Allocate all three arrays before publishing any, so a failed allocation leaks nothing.
+    std::unique_ptr<float, decltype(&std::free)> pv(copy_out(m.verts), &std::free);
+    std::unique_ptr<int32_t, decltype(&std::free)> pt(copy_out(tris), &std::free);
+    std::unique_ptr<int64_t, decltype(&std::free)> pf(copy_out(tri_face), &std::free);
     out->num_vertices = (int64_t)(m.verts.size() / 3);
     out->num_faces = (int64_t)m.faces.size();
     out->num_triangles = (int64_t)tri_face.size();
-    out->vertices = copy_out(m.verts);
-    out->triangles = copy_out(tris);
-    out->triangle_faces = copy_out(tri_face);
+    out->vertices = pv.release();
+    out->triangles = pt.release();
+    out->triangle_faces = pf.release();
   });
 }
 
