@@ -1056,7 +1056,10 @@ __device__ __forceinline__ float2 slab(float3 o, float3 inv, float w, float lx, 
 }
 
 // ANY = true: stop at the first accepted hit (any-hit query).
-template <int STACK, bool ANY, bool STATS = false, bool TS = true>
+// REG > 0 (with TS = false): the top REG stack entries live in registers as a shift register
+// (reg[0] on top) and only deeper entries in LDS (STACK - REG per lane), so a block needs less LDS
+// and more waves fit on a CU.
+template <int STACK, bool ANY, bool STATS = false, bool TS = true, int REG = 0>
 __device__ __forceinline__ void ray_body(int32_t *__restrict__ s_ref, float *__restrict__ s_t,
                                          const BNode *__restrict__ nodes, const TriRec *__restrict__ tr,
                                          const float *__restrict__ orig, const float *__restrict__ dirs, int64_t ri,
@@ -1074,6 +1077,7 @@ __device__ __forceinline__ void ray_body(int32_t *__restrict__ s_ref, float *__r
   int32_t bidx = 0x7fffffff;
   int32_t bslot = -1;
   int sp = 0;
+  int32_t reg[REG > 0 ? REG : 1];
   int32_t node = root;
   // A box is entered when its (widened) entry time is within this relative margin of the best.
   const float REL = 1.0f + 1.0f / 4096.0f;
@@ -1122,8 +1126,15 @@ __device__ __forceinline__ void ray_body(int32_t *__restrict__ s_ref, float *__r
       } else if (next < 0) {
         next = r;
       } else if (sp < STACK) {  // never taken when depth < STACK (checked by every caller)
-        s_ref[sp * Q_BLOCK + tid] = r;
-        if (TS) s_t[sp * Q_BLOCK + tid] = tn;
+        if (REG > 0 && !TS) {
+          if (sp >= REG) s_ref[(sp - REG) * Q_BLOCK + tid] = reg[REG - 1];
+#pragma unroll
+          for (int j = REG - 1; j > 0; --j) reg[j] = reg[j - 1];
+          reg[0] = r;
+        } else {
+          s_ref[sp * Q_BLOCK + tid] = r;
+          if (TS) s_t[sp * Q_BLOCK + tid] = tn;
+        }
         ++sp;
         if (STATS) c_maxsp = max(c_maxsp, sp);
       }
@@ -1136,6 +1147,14 @@ __device__ __forceinline__ void ray_body(int32_t *__restrict__ s_ref, float *__r
       if (STATS) c_pops++;
       // TS = false keeps only node refs (half the LDS, more waves per CU); a popped node is then
       // entered and its children are tested against the current best instead.
+      if (REG > 0 && !TS) {
+        node = reg[0];
+#pragma unroll
+        for (int j = 0; j < REG - 1; ++j) reg[j] = reg[j + 1];
+        if (sp >= REG) reg[REG - 1] = s_ref[(sp - REG) * Q_BLOCK + tid];
+        found = true;
+        break;
+      }
       if (!TS || s_t[sp * Q_BLOCK + tid] <= best * REL) { node = s_ref[sp * Q_BLOCK + tid]; found = true; break; }
     }
     if (!found) break;
@@ -1160,20 +1179,20 @@ __device__ __forceinline__ void ray_body(int32_t *__restrict__ s_ref, float *__r
   }
 }
 
-template <int STACK, bool ANY, bool TS = true>
+template <int STACK, bool ANY, bool TS = true, int REG = 0>
 __global__ void __launch_bounds__(Q_BLOCK) k_ray(const BNode *__restrict__ nodes, const TriRec *__restrict__ tr,
                                                  const float *__restrict__ orig, const float *__restrict__ dirs,
                                                  const uint32_t *__restrict__ order, int64_t n, uint32_t nb_real,
                                                  float best_init, float max_time, float scale,
                                                  int32_t *__restrict__ out_idx, float *__restrict__ out_t,
                                                  float *__restrict__ out_n, uint8_t *__restrict__ out_any) {
-  __shared__ int32_t s_ref[STACK * Q_BLOCK];
+  __shared__ int32_t s_ref[(STACK - (TS ? 0 : REG)) * Q_BLOCK];
   __shared__ float s_t[TS ? STACK * Q_BLOCK : 1];
   bool active;
   int64_t ri = query_slot(nb_real, n, order, active);
   if (!active) return;
-  ray_body<STACK, ANY, false, TS>(s_ref, s_t, nodes, tr, orig, dirs, ri, best_init, max_time, scale, out_idx, out_t,
-                                  out_n, out_any);
+  ray_body<STACK, ANY, false, TS, TS ? 0 : REG>(s_ref, s_t, nodes, tr, orig, dirs, ri, best_init, max_time, scale,
+                                                out_idx, out_t, out_n, out_any);
 }
 
 // Per-ray traversal counts (diagnostics): nodes visited, triangles tested, deepest stack, pops.
@@ -1533,8 +1552,8 @@ void launch_ray(const Bvh &b, const float *o, const float *d, int64_t n, float m
     hipLaunchKernelGGL((k_ray<S, A, true>), dim3(grid), dim3(Q_BLOCK), 0, s, b.nodes.as<BNode>(), b.elems.as<TriRec>(), \
                        o, d, ws.order, n, nb, best, max_time, b.scale, idx, t, nrm, hit);                           \
   else                                                                                                               \
-    hipLaunchKernelGGL((k_ray<S, A, false>), dim3(grid), dim3(Q_BLOCK), 0, s, b.nodes.as<BNode>(), b.elems.as<TriRec>(), \
-                       o, d, ws.order, n, nb, best, max_time, b.scale, idx, t, nrm, hit)
+    hipLaunchKernelGGL((k_ray<S, A, false, 16>), dim3(grid), dim3(Q_BLOCK), 0, s, b.nodes.as<BNode>(),               \
+                       b.elems.as<TriRec>(), o, d, ws.order, n, nb, best, max_time, b.scale, idx, t, nrm, hit)
   if (any) {
     if (st <= 32) THG_RAY(32, true); else if (st == 64) THG_RAY(64, true); else THG_RAY(128, true);
   } else {
