@@ -39,6 +39,66 @@ __device__ __forceinline__ int64_t query_slot(uint32_t nb_real, int64_t n, const
 // Closest element
 //--------------------------------------------------------------------------------------------------
 // One query per lane, stack in LDS (entry i of lane t at s_ref[i * Q_BLOCK + t]).
+// Per-lane traversal stack of node references (and, with LB, their lower bounds): the top REG
+// entries live in registers as a shift register ([0] on top), deeper ones in LDS at
+// (i - REG) * Q_BLOCK + tid. Registers and dropping the bounds both shrink a block's LDS, so more
+// waves fit on a CU; without bounds a popped node is entered and its children are tested instead.
+constexpr int LANE_REG = 16;
+
+template <int STACK, int REG> constexpr int lane_lds_entries() { return STACK > REG ? STACK - REG : 1; }
+
+template <int REG, bool LB> struct LaneStack {
+  int32_t r[REG > 0 ? REG : 1];
+  float l[REG > 0 && LB ? REG : 1];
+  // Push onto a stack currently holding sp entries.
+  __device__ __forceinline__ void push(int sp, int32_t ref, float lb, int32_t *__restrict__ s_ref,
+                                       float *__restrict__ s_lb, int tid) {
+    if (REG == 0) {
+      s_ref[sp * Q_BLOCK + tid] = ref;
+      if (LB) s_lb[sp * Q_BLOCK + tid] = lb;
+      return;
+    }
+    if (sp >= REG) {
+      s_ref[(sp - REG) * Q_BLOCK + tid] = r[REG - 1];
+      if (LB) s_lb[(sp - REG) * Q_BLOCK + tid] = l[REG - 1];
+    }
+#pragma unroll
+    for (int j = REG - 1; j > 0; --j) {
+      r[j] = r[j - 1];
+      if (LB) l[j] = l[j - 1];
+    }
+    r[0] = ref;
+    if (LB) l[0] = lb;
+  }
+  // Pop the top; sp is the count after the pop. Without LB the bound returned is -inf (enter).
+  __device__ __forceinline__ void pop(int sp, int32_t &ref, float &lb, const int32_t *__restrict__ s_ref,
+                                      const float *__restrict__ s_lb, int tid) {
+    if (REG == 0) {
+      ref = s_ref[sp * Q_BLOCK + tid];
+      lb = LB ? s_lb[sp * Q_BLOCK + tid] : -INFINITY;
+      return;
+    }
+    ref = r[0];
+    lb = LB ? l[0] : -INFINITY;
+#pragma unroll
+    for (int j = 0; j < REG - 1; ++j) {
+      r[j] = r[j + 1];
+      if (LB) l[j] = l[j + 1];
+    }
+    if (sp >= REG) {
+      r[REG - 1] = s_ref[(sp - REG) * Q_BLOCK + tid];
+      if (LB) l[REG - 1] = s_lb[(sp - REG) * Q_BLOCK + tid];
+    }
+  }
+};
+
+// Stack configurations: closest-point lanes keep the bounds (a popped node is pruned without a
+// load); k-NN lanes, whose k-list already fills registers, keep references only in LDS.
+constexpr int CL_REG = 16;
+constexpr bool CL_LB = true;
+constexpr int KN_REG = 0;
+constexpr bool KN_LB = false;
+
 template <int STACK, bool POINTS>
 __device__ __forceinline__ void closest_lane_body(int32_t *__restrict__ s_ref, float *__restrict__ s_lb,
                                                   const BNode *__restrict__ nodes, const void *__restrict__ elems,
@@ -56,6 +116,7 @@ __device__ __forceinline__ void closest_lane_body(int32_t *__restrict__ s_ref, f
   float3 bcp = f3(0, 0, 0);
   const int32_t self = exclude_self ? (int32_t)qi : -1;
   int sp = 0;
+  LaneStack<CL_REG, CL_LB> stk;
   int32_t node = root;
   const TriRec *__restrict__ tr = (const TriRec *)elems;
   const PointRec *__restrict__ pr = (const PointRec *)elems;
@@ -101,8 +162,7 @@ __device__ __forceinline__ void closest_lane_body(int32_t *__restrict__ s_ref, f
       } else if (next < 0) {
         next = r;
       } else if (sp < STACK) {  // never taken when depth < STACK (checked by every caller)
-        s_ref[sp * Q_BLOCK + tid] = r;
-        s_lb[sp * Q_BLOCK + tid] = lb;
+        stk.push(sp, r, lb, s_ref, s_lb, tid);
         ++sp;
       }
     }
@@ -110,7 +170,10 @@ __device__ __forceinline__ void closest_lane_body(int32_t *__restrict__ s_ref, f
     bool found = false;
     while (sp > 0) {
       --sp;
-      if (s_lb[sp * Q_BLOCK + tid] <= best) { node = s_ref[sp * Q_BLOCK + tid]; found = true; break; }
+      int32_t pr_ref;
+      float pr_lb;
+      stk.pop(sp, pr_ref, pr_lb, s_ref, s_lb, tid);
+      if (pr_lb <= best) { node = pr_ref; found = true; break; }
     }
     if (!found) break;
   }
@@ -132,8 +195,8 @@ __global__ void __launch_bounds__(Q_BLOCK) k_closest(const BNode *__restrict__ n
                                                      int exclude_self, int32_t *__restrict__ out_idx,
                                                      float *__restrict__ out_cp, float *__restrict__ out_dist,
                                                      float *__restrict__ out_d2) {
-  __shared__ int32_t s_ref[STACK * Q_BLOCK];
-  __shared__ float s_lb[STACK * Q_BLOCK];
+  __shared__ int32_t s_ref[lane_lds_entries<STACK, CL_REG>() * Q_BLOCK];
+  __shared__ float s_lb[CL_LB ? lane_lds_entries<STACK, CL_REG>() * Q_BLOCK : 1];
   bool active;
   const int64_t qi = query_slot(nb_real, n, order, active);
   if (!active) return;
@@ -723,6 +786,7 @@ __device__ __forceinline__ void knn_lane_body(int32_t *__restrict__ s_ref, float
   // Only the first k slots are live; the worst live slot is the pruning bound.
   const int32_t self = exclude_self ? (int32_t)qi : -1;
   int sp = 0;
+  LaneStack<KN_REG, KN_LB> stk;
   int32_t node = root;
   const TriRec *__restrict__ tr = (const TriRec *)elems;
   const PointRec *__restrict__ pr = (const PointRec *)elems;
@@ -780,8 +844,7 @@ __device__ __forceinline__ void knn_lane_body(int32_t *__restrict__ s_ref, float
       } else if (next < 0) {
         next = r;
       } else if (sp < STACK) {  // never taken when depth < STACK (checked by every caller)
-        s_ref[sp * Q_BLOCK + tid] = r;
-        s_lb[sp * Q_BLOCK + tid] = lb;
+        stk.push(sp, r, lb, s_ref, s_lb, tid);
         ++sp;
       }
     }
@@ -789,7 +852,10 @@ __device__ __forceinline__ void knn_lane_body(int32_t *__restrict__ s_ref, float
     bool found = false;
     while (sp > 0) {
       --sp;
-      if (s_lb[sp * Q_BLOCK + tid] <= bound()) { node = s_ref[sp * Q_BLOCK + tid]; found = true; break; }
+      int32_t pr_ref;
+      float pr_lb;
+      stk.pop(sp, pr_ref, pr_lb, s_ref, s_lb, tid);
+      if (pr_lb <= bound()) { node = pr_ref; found = true; break; }
     }
     if (!found) break;
   }
@@ -808,8 +874,8 @@ __global__ void __launch_bounds__(Q_BLOCK) k_knn(const BNode *__restrict__ nodes
                                                  int64_t n, uint32_t nb_real, int k, float best_init, float scale,
                                                  int exclude_self, int32_t *__restrict__ out_idx,
                                                  float *__restrict__ out_dist, float *__restrict__ out_d2) {
-  __shared__ int32_t s_ref[STACK * Q_BLOCK];
-  __shared__ float s_lb[STACK * Q_BLOCK];
+  __shared__ int32_t s_ref[lane_lds_entries<STACK, KN_REG>() * Q_BLOCK];
+  __shared__ float s_lb[KN_LB ? lane_lds_entries<STACK, KN_REG>() * Q_BLOCK : 1];
   bool active;
   const int64_t qi = query_slot(nb_real, n, order, active);
   if (!active) return;
@@ -822,8 +888,8 @@ __global__ void __launch_bounds__(Q_BLOCK) k_knn(const BNode *__restrict__ nodes
 template <int STACK, bool POINTS>
 __global__ void __launch_bounds__(Q_BLOCK) k_closest_lane_multi(const SegDesc *__restrict__ seg, int nseg,
                                                                 uint64_t total_waves, uint32_t nb_real) {
-  __shared__ int32_t s_ref[STACK * Q_BLOCK];
-  __shared__ float s_lb[STACK * Q_BLOCK];
+  __shared__ int32_t s_ref[lane_lds_entries<STACK, CL_REG>() * Q_BLOCK];
+  __shared__ float s_lb[CL_LB ? lane_lds_entries<STACK, CL_REG>() * Q_BLOCK : 1];
   const uint32_t lb_blk = xcd_remap(blockIdx.x, gridDim.x);
   if (lb_blk >= nb_real || (uint64_t)lb_blk >= total_waves) return;
   const int si = uni(find_segment(seg, nseg, (int64_t)lb_blk));
@@ -838,8 +904,8 @@ __global__ void __launch_bounds__(Q_BLOCK) k_closest_lane_multi(const SegDesc *_
 template <int STACK, bool POINTS, int K>
 __global__ void __launch_bounds__(Q_BLOCK) k_knn_lane_multi(const SegDesc *__restrict__ seg, int nseg,
                                                             uint64_t total_waves, uint32_t nb_real) {
-  __shared__ int32_t s_ref[STACK * Q_BLOCK];
-  __shared__ float s_lb[STACK * Q_BLOCK];
+  __shared__ int32_t s_ref[lane_lds_entries<STACK, KN_REG>() * Q_BLOCK];
+  __shared__ float s_lb[KN_LB ? lane_lds_entries<STACK, KN_REG>() * Q_BLOCK : 1];
   const uint32_t lb_blk = xcd_remap(blockIdx.x, gridDim.x);
   if (lb_blk >= nb_real || (uint64_t)lb_blk >= total_waves) return;
   const int si = uni(find_segment(seg, nseg, (int64_t)lb_blk));
@@ -1213,7 +1279,7 @@ __global__ void __launch_bounds__(Q_BLOCK) k_ray_stats(const BNode *__restrict__
 template <int STACK>
 __global__ void __launch_bounds__(Q_BLOCK) k_ray_multi(const SegDesc *__restrict__ seg, int nseg, uint64_t total_waves,
                                                        uint32_t nb_real) {
-  __shared__ int32_t s_ref[STACK * Q_BLOCK];
+  __shared__ int32_t s_ref[lane_lds_entries<STACK, LANE_REG>() * Q_BLOCK];
   __shared__ float s_t[1];
   const uint32_t lb_blk = xcd_remap(blockIdx.x, gridDim.x);
   if (lb_blk >= nb_real || (uint64_t)lb_blk >= total_waves) return;
@@ -1222,8 +1288,9 @@ __global__ void __launch_bounds__(Q_BLOCK) k_ray_multi(const SegDesc *__restrict
   const int64_t t = ((int64_t)lb_blk - (int64_t)d.wave_begin) * Q_BLOCK + threadIdx.x;
   if (t >= d.n) return;
   const int64_t ri = d.order ? (int64_t)d.order[t] : t;
-  ray_body<STACK, false, false, false>(s_ref, s_t, (const BNode *)d.nodes, (const TriRec *)d.elems, d.q, d.q2, ri,
-                                       d.best_init, d.max_time, d.scale, d.idx, d.dist, d.cp, nullptr, d.root);
+  ray_body<STACK, false, false, false, LANE_REG>(s_ref, s_t, (const BNode *)d.nodes, (const TriRec *)d.elems, d.q,
+                                                 d.q2, ri, d.best_init, d.max_time, d.scale, d.idx, d.dist, d.cp,
+                                                 nullptr, d.root);
 }
 
 //--------------------------------------------------------------------------------------------------
@@ -1456,11 +1523,12 @@ void launch_knn_lane_multi(const SegDesc *seg_dev, int nseg, uint64_t total_wave
   launch_check("k_knn_lane_multi");
 }
 
-void launch_ray_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, hipStream_t s) {
+void launch_ray_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, int stack, hipStream_t s) {
   if (nseg <= 0 || total_waves == 0) return;
   uint32_t nb = (uint32_t)total_waves;
   uint32_t grid = round_up8(nb);
-  hipLaunchKernelGGL(k_ray_multi<64>, dim3(grid), dim3(Q_BLOCK), 0, s, seg_dev, nseg, total_waves, nb);
+  if (stack <= 32) hipLaunchKernelGGL(k_ray_multi<32>, dim3(grid), dim3(Q_BLOCK), 0, s, seg_dev, nseg, total_waves, nb);
+  else hipLaunchKernelGGL(k_ray_multi<64>, dim3(grid), dim3(Q_BLOCK), 0, s, seg_dev, nseg, total_waves, nb);
   launch_check("k_ray_multi");
 }
 
@@ -1552,7 +1620,7 @@ void launch_ray(const Bvh &b, const float *o, const float *d, int64_t n, float m
     hipLaunchKernelGGL((k_ray<S, A, true>), dim3(grid), dim3(Q_BLOCK), 0, s, b.nodes.as<BNode>(), b.elems.as<TriRec>(), \
                        o, d, ws.order, n, nb, best, max_time, b.scale, idx, t, nrm, hit);                           \
   else                                                                                                               \
-    hipLaunchKernelGGL((k_ray<S, A, false, 16>), dim3(grid), dim3(Q_BLOCK), 0, s, b.nodes.as<BNode>(),               \
+    hipLaunchKernelGGL((k_ray<S, A, false, LANE_REG>), dim3(grid), dim3(Q_BLOCK), 0, s, b.nodes.as<BNode>(),         \
                        b.elems.as<TriRec>(), o, d, ws.order, n, nb, best, max_time, b.scale, idx, t, nrm, hit)
   if (any) {
     if (st <= 32) THG_RAY(32, true); else if (st == 64) THG_RAY(64, true); else THG_RAY(128, true);

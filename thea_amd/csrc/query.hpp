@@ -69,7 +69,8 @@ void launch_closest_lane_multi(const SegDesc *seg_dev, int nseg, uint64_t total_
 void launch_knn_lane_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, int k, bool points, int stack,
                            hipStream_t s);
 bool use_wave_traversal(const Bvh &b, int64_t n);
-void launch_ray_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, hipStream_t s);
+// stack: the deepest tree among the segments plus one (32 or 64 entries).
+void launch_ray_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, int stack, hipStream_t s);
 constexpr int SEG_WAVE_QUERIES = 64;  // queries per wave in the multi-batch kernels
 void ray_stats(const Bvh &b, const float *o, const float *d, int64_t n, int32_t *idx, int32_t *counts, hipStream_t s);
 void closest_wave_stats(const Bvh &b, const float *q, int64_t n, unsigned long long *stats, QueryScratch &ws,

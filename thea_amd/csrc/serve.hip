@@ -197,7 +197,7 @@ static void worker_loop(Server *S, int wi) {
             if (lane) launch_knn_lane_multi(dd, nseg, waves, kk.first, false, stack, s);
             else launch_knn_multi(dd, nseg, waves, kk.first, false, s);
           } else {
-            launch_ray_multi(dd, nseg, waves, s);
+            launch_ray_multi(dd, nseg, waves, stack, s);
           }
         }
       }
