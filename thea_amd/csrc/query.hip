@@ -918,6 +918,382 @@ __global__ void __launch_bounds__(Q_BLOCK) k_knn_lane_multi(const SegDesc *__res
 }
 
 //--------------------------------------------------------------------------------------------------
+// Sparse batches on a large tree, G lanes per query
+//
+// A lane walk costs its query's whole search: a query far inside a closed surface meets nearly
+// every leaf, and a launch of a few thousand such queries lasts as long as its slowest one. Here G
+// lanes (a power of two, 2..64, aligned within the wave) share a query. Lane gl of the group owns
+// the subtrees reached from the root by taking both children at depths 0..GRP_FREE-1 and child
+// slot bit (depth - GRP_FREE) of gl at the next log2(G) depths, so the lanes of a group partition
+// the tree into 4G subtrees spread over it; an element range met above that depth is evaluated by
+// the one lane whose remaining route bits are zero. Each lane walks its subtrees with its own
+// stack; every step the group's bound (the minimum of the lanes' bounds, a distance some element
+// of the group attains) is refreshed with cross-lane shuffles, so the whole wave runs one uniform
+// loop. The answer is the group's (d^2, index) minimum, the same pair a single lane finds.
+//--------------------------------------------------------------------------------------------------
+constexpr int GRP_FREE = 2;
+
+__device__ __forceinline__ float group_min(float v, int g) {
+  for (int m = 1; m < g; m <<= 1) v = fminf(v, __shfl_xor(v, m));
+  return v;
+}
+
+// Minimum over the group of the (d^2, index) pair packed as d^2's bits above the index (d^2 >= 0,
+// so the bit order is the value order).
+__device__ __forceinline__ uint64_t group_min_key(uint64_t v, int g) {
+  for (int m = 1; m < g; m <<= 1) {
+    const uint32_t lo = __shfl_xor((uint32_t)v, m), hi = __shfl_xor((uint32_t)(v >> 32), m);
+    const uint64_t o = ((uint64_t)hi << 32) | lo;
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint64_t pair_key(float d, int32_t i) {
+  return ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)i;
+}
+
+// The first GRP_FREE + log2(G) levels for lane gl: element ranges on the way go to `leaf`, the
+// subtrees at the route's end (at most 2^GRP_FREE) to `start`. Children whose lower bound exceeds
+// `bound()` are dropped.
+template <typename LB, typename Leaf, typename Start>
+__device__ __forceinline__ void group_route(const BNode *__restrict__ nodes, int32_t root, int lg, int gl, float3 qlo,
+                                            float3 qhi, LB bound, Leaf leaf, Start start) {
+  const int depth = GRP_FREE + lg;
+  for (int pth = 0; pth < (1 << GRP_FREE); ++pth) {
+    const uint32_t route = (uint32_t)pth | ((uint32_t)gl << GRP_FREE);
+    int32_t node = root;
+    for (int j = 0; j < depth; ++j) {
+      // the routes that share this prefix and differ above level j would repeat it: only the one
+      // whose bits above j are zero goes on
+      const BNode nd = nodes[node];
+      const int slot = (route >> j) & 1;
+      const float lb = slot ? box_lb(qlo, qhi, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w)
+                            : box_lb(qlo, qhi, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y);
+      const int32_t r = slot ? nd.d.y : nd.d.x;
+      const int32_t cnt = slot ? nd.d.w : nd.d.z;
+      if (!(lb <= bound())) break;
+      if (r < 0) {
+        if ((route >> (j + 1)) == 0) leaf(~r, cnt);
+        break;
+      }
+      if (j + 1 == depth) {
+        start(r, lb);
+        break;
+      }
+      node = r;
+    }
+  }
+}
+
+template <int STACK, bool POINTS>
+__device__ __forceinline__ void closest_group_body(int32_t *__restrict__ s_ref, float *__restrict__ s_lb,
+                                                   const BNode *__restrict__ nodes, const void *__restrict__ elems,
+                                                   const float *__restrict__ q, int64_t qi, bool live, int lg,
+                                                   float best_init, float scale, int exclude_self,
+                                                   int32_t *__restrict__ out_idx, float *__restrict__ out_cp,
+                                                   float *__restrict__ out_dist, float *__restrict__ out_d2,
+                                                   int32_t root) {
+  const int tid = threadIdx.x;
+  const int g = 1 << lg, gl = tid & (g - 1);
+  float3 p = live ? load_q(q, qi) : f3(0, 0, 0);
+  float m = fmaxf(scale, fmaxf(fabsf(p.x), fmaxf(fabsf(p.y), fabsf(p.z))));
+  float sl = m * SLACK_REL;
+  float3 qlo = f3(p.x - sl, p.y - sl, p.z - sl), qhi = f3(p.x + sl, p.y + sl, p.z + sl);
+  float best = best_init;
+  int32_t bidx = 0x7fffffff;
+  float3 bcp = f3(0, 0, 0);
+  const int32_t self = exclude_self ? (int32_t)qi : -1;
+  const TriRec *__restrict__ tr = (const TriRec *)elems;
+  const PointRec *__restrict__ pr = (const PointRec *)elems;
+  auto eval = [&](int32_t f, int32_t cnt) {
+    for (int32_t k = f; k < f + cnt; ++k) {
+      float d;
+      float3 cp;
+      int32_t id;
+      if (POINTS) {
+        float4 e = pr[k].p;
+        id = __float_as_int(e.w);
+        cp = f3(e.x, e.y, e.z);
+        d = sqn3(sub3(cp, p));
+      } else {
+        const TriRec *t = tr + k;
+        id = trirec_id(t->pl);
+        d = tri_closest(t->p0, t->p1, t->p2, t->pl, t->e0, t->e1, t->e2, p, cp);
+      }
+      if (id != self && better(d, id, best, bidx)) {
+        best = d;
+        bidx = id;
+        bcp = cp;
+      }
+    }
+  };
+  int sp = 0;
+  LaneStack<CL_REG, CL_LB> stk;
+  if (live)
+    group_route(nodes, root, lg, gl, qlo, qhi, [&]() { return best; }, eval, [&](int32_t r, float lb) {
+      stk.push(sp, r, lb, s_ref, s_lb, tid);
+      ++sp;
+    });
+  float gbest = group_min(best, g);
+  int32_t node = -1;
+  while (true) {
+    if (node < 0) {
+      while (sp > 0) {
+        --sp;
+        int32_t pr_ref;
+        float pr_lb;
+        stk.pop(sp, pr_ref, pr_lb, s_ref, s_lb, tid);
+        if (pr_lb <= gbest) { node = pr_ref; break; }
+      }
+    }
+    if (node >= 0) {
+      const BNode nd = nodes[node];
+      float lb0 = box_lb(qlo, qhi, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y);
+      float lb1 = box_lb(qlo, qhi, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w);
+      int32_t r0 = nd.d.x, r1 = nd.d.y, c0 = nd.d.z, c1 = nd.d.w;
+      if (lb1 < lb0) {
+        float tl = lb0; lb0 = lb1; lb1 = tl;
+        int32_t tr_ = r0; r0 = r1; r1 = tr_;
+        int32_t tc = c0; c0 = c1; c1 = tc;
+      }
+      int32_t next = -1;
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        float lb = c == 0 ? lb0 : lb1;
+        int32_t r = c == 0 ? r0 : r1;
+        int32_t cnt = c == 0 ? c0 : c1;
+        if (!(lb <= fminf(gbest, best))) continue;
+        if (r < 0) {
+          eval(~r, cnt);
+        } else if (next < 0) {
+          next = r;
+        } else if (sp < STACK) {  // never taken when depth < STACK (checked by every caller)
+          stk.push(sp, r, lb, s_ref, s_lb, tid);
+          ++sp;
+        }
+      }
+      node = next;
+    }
+    gbest = group_min(best, g);
+    if (!__any(node >= 0 || sp > 0)) break;
+  }
+  const uint64_t key = pair_key(best, bidx);
+  const uint64_t gkey = group_min_key(key, g);
+  if (!live) return;
+  const bool hit = (uint32_t)gkey != 0x7fffffffu;
+  if (hit ? key != gkey : gl != 0) return;
+  out_idx[qi] = hit ? bidx : -1;
+  if (out_cp) {
+    out_cp[3 * qi] = hit ? bcp.x : 0.0f;
+    out_cp[3 * qi + 1] = hit ? bcp.y : 0.0f;
+    out_cp[3 * qi + 2] = hit ? bcp.z : 0.0f;
+  }
+  if (out_dist) out_dist[qi] = hit ? sqrtf(best) : -1.0f;
+  if (out_d2) out_d2[qi] = hit ? best : -1.0f;
+}
+
+template <int STACK, bool POINTS, int K>
+__device__ __forceinline__ void knn_group_body(int32_t *__restrict__ s_ref, float *__restrict__ s_lb,
+                                               const BNode *__restrict__ nodes, const void *__restrict__ elems,
+                                               const float *__restrict__ q, int64_t qi, bool live, int lg, int k,
+                                               float best_init, float scale, int exclude_self,
+                                               int32_t *__restrict__ out_idx, float *__restrict__ out_dist,
+                                               float *__restrict__ out_d2, int32_t root) {
+  const int tid = threadIdx.x;
+  const int g = 1 << lg, gl = tid & (g - 1);
+  float3 p = live ? load_q(q, qi) : f3(0, 0, 0);
+  float m = fmaxf(scale, fmaxf(fabsf(p.x), fmaxf(fabsf(p.y), fabsf(p.z))));
+  float sl = m * SLACK_REL;
+  float3 qlo = f3(p.x - sl, p.y - sl, p.z - sl), qhi = f3(p.x + sl, p.y + sl, p.z + sl);
+  float kd[K];
+  int32_t ki[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) { kd[j] = best_init; ki[j] = 0x7fffffff; }
+  const int32_t self = exclude_self ? (int32_t)qi : -1;
+  const TriRec *__restrict__ tr = (const TriRec *)elems;
+  const PointRec *__restrict__ pr = (const PointRec *)elems;
+  // The lane's k-th best pair (slot k - 1): its admission threshold; the group's pruning bound is
+  // the minimum of these over the lanes (each lane alone holds k pairs at or below its own).
+  auto kth = [&](float &bd, int32_t &bi) {
+    bd = kd[K - 1];
+    bi = ki[K - 1];
+#pragma unroll
+    for (int j = 0; j < K - 1; ++j)
+      if (j == k - 1) { bd = kd[j]; bi = ki[j]; }
+  };
+  auto bound = [&]() {
+    float bd;
+    int32_t bi;
+    kth(bd, bi);
+    return bd;
+  };
+  auto eval = [&](int32_t f, int32_t cnt) {
+    for (int32_t kk = f; kk < f + cnt; ++kk) {
+      float d;
+      int32_t id;
+      if (POINTS) {
+        float4 e = pr[kk].p;
+        id = __float_as_int(e.w);
+        d = sqn3(sub3(f3(e.x, e.y, e.z), p));
+      } else {
+        const TriRec *t = tr + kk;
+        id = trirec_id(t->pl);
+        float3 cp;
+        d = tri_closest(t->p0, t->p1, t->p2, t->pl, t->e0, t->e1, t->e2, p, cp);
+      }
+      float bd;
+      int32_t bi;
+      kth(bd, bi);
+      if (id != self && better(d, id, bd, bi)) knn_insert<K>(kd, ki, d, id);
+    }
+  };
+  int sp = 0;
+  LaneStack<KN_REG, KN_LB> stk;
+  if (live)
+    group_route(nodes, root, lg, gl, qlo, qhi, bound, eval, [&](int32_t r, float lb) {
+      stk.push(sp, r, lb, s_ref, s_lb, tid);
+      ++sp;
+    });
+  float gbound = group_min(bound(), g);
+  int32_t node = -1;
+  while (true) {
+    if (node < 0 && sp > 0) {
+      --sp;
+      int32_t pr_ref;
+      float pr_lb;
+      stk.pop(sp, pr_ref, pr_lb, s_ref, s_lb, tid);
+      if (pr_lb <= gbound) node = pr_ref;
+    }
+    if (node >= 0) {
+      const BNode nd = nodes[node];
+      float lb0 = box_lb(qlo, qhi, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y);
+      float lb1 = box_lb(qlo, qhi, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w);
+      int32_t r0 = nd.d.x, r1 = nd.d.y, c0 = nd.d.z, c1 = nd.d.w;
+      if (lb1 < lb0) {
+        float tl = lb0; lb0 = lb1; lb1 = tl;
+        int32_t tr_ = r0; r0 = r1; r1 = tr_;
+        int32_t tc = c0; c0 = c1; c1 = tc;
+      }
+      int32_t next = -1;
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        float lb = c == 0 ? lb0 : lb1;
+        int32_t r = c == 0 ? r0 : r1;
+        int32_t cnt = c == 0 ? c0 : c1;
+        if (!(lb <= fminf(gbound, bound()))) continue;
+        if (r < 0) {
+          eval(~r, cnt);
+        } else if (next < 0) {
+          next = r;
+        } else if (sp < STACK) {  // never taken when depth < STACK (checked by every caller)
+          stk.push(sp, r, lb, s_ref, s_lb, tid);
+          ++sp;
+        }
+      }
+      node = next;
+    }
+    gbound = group_min(bound(), g);
+    if (!__any(node >= 0 || sp > 0)) break;
+  }
+  // Merge: the group's k smallest pairs are among the lanes' first k; each round takes the least
+  // head and its lane drops it.
+  int left = k;
+  for (int j = 0; j < k; ++j) {
+    const uint64_t head = left > 0 ? pair_key(kd[0], ki[0]) : ~(uint64_t)0;
+    const uint64_t gkey = group_min_key(head, g);
+    const bool hit = (uint32_t)gkey != 0x7fffffffu && gkey != ~(uint64_t)0;
+    if (live && gl == 0) {
+      const float d = __uint_as_float((uint32_t)(gkey >> 32));
+      out_idx[qi * k + j] = hit ? (int32_t)(uint32_t)gkey : -1;
+      if (out_dist) out_dist[qi * k + j] = hit ? sqrtf(d) : -1.0f;
+      if (out_d2) out_d2[qi * k + j] = hit ? d : -1.0f;
+    }
+    if (hit && head == gkey) {
+#pragma unroll
+      for (int t = 0; t < K - 1; ++t) { kd[t] = kd[t + 1]; ki[t] = ki[t + 1]; }
+      --left;
+    }
+  }
+}
+
+// Lanes per query for a sparse batch of n queries: the largest power of two up to 64 that keeps
+// n * G within GRP_LANES lanes (about four waves per SIMD), as log2.
+constexpr int64_t GRP_LANES = (int64_t)1 << 18;
+
+template <int STACK, bool POINTS>
+__global__ void __launch_bounds__(Q_BLOCK) k_closest_group(const BNode *__restrict__ nodes,
+                                                           const void *__restrict__ elems, const float *__restrict__ q,
+                                                           const uint32_t *__restrict__ order, int64_t n, int lg,
+                                                           uint32_t nb_real, float best_init, float scale,
+                                                           int exclude_self, int32_t *__restrict__ out_idx,
+                                                           float *__restrict__ out_cp, float *__restrict__ out_dist,
+                                                           float *__restrict__ out_d2) {
+  __shared__ int32_t s_ref[lane_lds_entries<STACK, CL_REG>() * Q_BLOCK];
+  __shared__ float s_lb[CL_LB ? lane_lds_entries<STACK, CL_REG>() * Q_BLOCK : 1];
+  const uint32_t lb_blk = xcd_remap(blockIdx.x, gridDim.x);
+  if (lb_blk >= nb_real) return;
+  const int64_t s = ((int64_t)lb_blk * Q_BLOCK + threadIdx.x) >> lg;
+  const bool live = s < n;
+  const int64_t qi = live ? (order ? (int64_t)order[s] : s) : 0;
+  closest_group_body<STACK, POINTS>(s_ref, s_lb, nodes, elems, q, qi, live, lg, best_init, scale, exclude_self,
+                                    out_idx, out_cp, out_dist, out_d2, 0);
+}
+
+template <int STACK, bool POINTS, int K>
+__global__ void __launch_bounds__(Q_BLOCK) k_knn_group(const BNode *__restrict__ nodes, const void *__restrict__ elems,
+                                                       const float *__restrict__ q, const uint32_t *__restrict__ order,
+                                                       int64_t n, int lg, uint32_t nb_real, int k, float best_init,
+                                                       float scale, int exclude_self, int32_t *__restrict__ out_idx,
+                                                       float *__restrict__ out_dist, float *__restrict__ out_d2) {
+  __shared__ int32_t s_ref[lane_lds_entries<STACK, KN_REG>() * Q_BLOCK];
+  __shared__ float s_lb[KN_LB ? lane_lds_entries<STACK, KN_REG>() * Q_BLOCK : 1];
+  const uint32_t lb_blk = xcd_remap(blockIdx.x, gridDim.x);
+  if (lb_blk >= nb_real) return;
+  const int64_t s = ((int64_t)lb_blk * Q_BLOCK + threadIdx.x) >> lg;
+  const bool live = s < n;
+  const int64_t qi = live ? (order ? (int64_t)order[s] : s) : 0;
+  knn_group_body<STACK, POINTS, K>(s_ref, s_lb, nodes, elems, q, qi, live, lg, k, best_init, scale, exclude_self,
+                                   out_idx, out_dist, out_d2, 0);
+}
+
+// Server form: one 64-lane block per wave of a segment; a segment of n queries at 2^lg lanes each
+// spans ceil(n 2^lg / 64) waves (SegDesc::lanes_log2).
+template <int STACK, bool POINTS>
+__global__ void __launch_bounds__(Q_BLOCK) k_closest_group_multi(const SegDesc *__restrict__ seg, int nseg,
+                                                                 uint64_t total_waves, uint32_t nb_real) {
+  __shared__ int32_t s_ref[lane_lds_entries<STACK, CL_REG>() * Q_BLOCK];
+  __shared__ float s_lb[CL_LB ? lane_lds_entries<STACK, CL_REG>() * Q_BLOCK : 1];
+  const uint32_t lb_blk = xcd_remap(blockIdx.x, gridDim.x);
+  if (lb_blk >= nb_real || (uint64_t)lb_blk >= total_waves) return;
+  const int si = uni(find_segment(seg, nseg, (int64_t)lb_blk));
+  const SegDesc &d = seg[si];
+  const int lg = d.lanes_log2;
+  const int64_t s = (((int64_t)lb_blk - (int64_t)d.wave_begin) * Q_BLOCK + threadIdx.x) >> lg;
+  const bool live = s < d.n;
+  const int64_t qi = live ? (d.order ? (int64_t)d.order[s] : s) : 0;
+  closest_group_body<STACK, POINTS>(s_ref, s_lb, (const BNode *)d.nodes, d.elems, d.q, qi, live, lg, d.best_init,
+                                    d.scale, d.exclude_self, d.idx, d.cp, d.dist, d.d2, d.root);
+}
+
+template <int STACK, bool POINTS, int K>
+__global__ void __launch_bounds__(Q_BLOCK) k_knn_group_multi(const SegDesc *__restrict__ seg, int nseg,
+                                                             uint64_t total_waves, uint32_t nb_real) {
+  __shared__ int32_t s_ref[lane_lds_entries<STACK, KN_REG>() * Q_BLOCK];
+  __shared__ float s_lb[KN_LB ? lane_lds_entries<STACK, KN_REG>() * Q_BLOCK : 1];
+  const uint32_t lb_blk = xcd_remap(blockIdx.x, gridDim.x);
+  if (lb_blk >= nb_real || (uint64_t)lb_blk >= total_waves) return;
+  const int si = uni(find_segment(seg, nseg, (int64_t)lb_blk));
+  const SegDesc &d = seg[si];
+  const int lg = d.lanes_log2;
+  const int64_t s = (((int64_t)lb_blk - (int64_t)d.wave_begin) * Q_BLOCK + threadIdx.x) >> lg;
+  const bool live = s < d.n;
+  const int64_t qi = live ? (d.order ? (int64_t)d.order[s] : s) : 0;
+  knn_group_body<STACK, POINTS, K>(s_ref, s_lb, (const BNode *)d.nodes, d.elems, d.q, qi, live, lg, d.k,
+                                   d.best_init, d.scale, d.exclude_self, d.idx, d.dist, d.d2, d.root);
+}
+
+//--------------------------------------------------------------------------------------------------
 // k nearest, wave-cooperative: the traversal of k_closest_wave with a per-lane sorted list of K
 // (d^2, id) pairs in registers; a subtree or element is skipped when every lane's k-th best beats
 // its lower bound. Element records are wave-uniform (scalar loads).
@@ -1330,6 +1706,18 @@ static int pick_stack(int depth) {
   fail(THG_E_UNSUPPORTED, "BVH deeper than 128 levels");
 }
 
+// log2 of the lanes per query of a sparse batch of n queries on a tree of the given depth (0: one
+// lane each): the largest power of two up to 64 with n * G <= GRP_LANES. The grouped kernels keep
+// up to 2^GRP_FREE subtree roots on the stack besides the path, so they run on trees up to 59
+// levels. THG_GROUP_LANES overrides the lane budget (0 turns grouping off).
+int group_lanes_log2(int64_t n, int depth) {
+  static const int64_t budget = getenv("THG_GROUP_LANES") ? atoll(getenv("THG_GROUP_LANES")) : GRP_LANES;
+  if (depth + 1 + (1 << GRP_FREE) > 64) return 0;
+  int lg = 0;
+  while (lg < 6 && n * ((int64_t)2 << lg) <= budget) ++lg;
+  return lg;
+}
+
 // Lowest Morton bit the query sort uses (THG_ORDER_BITS = number of top bits of the 30, default 24).
 static int order_bits_lo() {
   static const char *env = getenv("THG_ORDER_BITS");
@@ -1438,6 +1826,20 @@ void launch_closest(const Bvh &b, const float *q, int64_t n, double dist_bound, 
     kt.stop();
     return;
   }
+  if (const int lg = group_lanes_log2(n, b.max_depth)) {
+    const uint32_t gnb = (uint32_t)ceil_div(n << lg, Q_BLOCK);
+    const bool deep = b.max_depth + 1 + (1 << GRP_FREE) > 32;
+    KernelTimer kt(b.ctx, s, "closest");
+#define THG_CG(S, P)                                                                                                  \
+  hipLaunchKernelGGL((k_closest_group<S, P>), dim3(round_up8(gnb)), dim3(Q_BLOCK), 0, s, b.nodes.as<BNode>(), b.elems.p, \
+                     q, ws.order, n, lg, gnb, best, b.scale, exclude_self, idx, cp, dist, d2)
+    if (b.points) { if (deep) THG_CG(64, true); else THG_CG(32, true); }
+    else { if (deep) THG_CG(64, false); else THG_CG(32, false); }
+#undef THG_CG
+    launch_check("k_closest_group");
+    kt.stop();
+    return;
+  }
   int st = pick_stack(b.max_depth);
   KernelTimer kt(b.ctx, s, "closest");
 #define THG_CLOSEST(S, P)                                                                                            \
@@ -1467,6 +1869,18 @@ static void launch_knn_k(const Bvh &b, int st, uint32_t grid, uint32_t nb, const
     if (st <= 32) THG_KNN(32, false); else if (st == 64) THG_KNN(64, false); else THG_KNN(128, false);
   }
 #undef THG_KNN
+}
+
+template <int K>
+static void launch_knn_group_k(const Bvh &b, bool deep, uint32_t gnb, int lg, const float *q, int64_t n, int k,
+                               float best, int exclude_self, int32_t *idx, float *dist, float *d2,
+                               const uint32_t *order, hipStream_t s) {
+#define THG_KG(S, P)                                                                                                \
+  hipLaunchKernelGGL((k_knn_group<S, P, K>), dim3(round_up8(gnb)), dim3(Q_BLOCK), 0, s, b.nodes.as<BNode>(), b.elems.p, \
+                     q, order, n, lg, gnb, k, best, b.scale, exclude_self, idx, dist, d2)
+  if (b.points) { if (deep) THG_KG(64, true); else THG_KG(32, true); }
+  else { if (deep) THG_KG(64, false); else THG_KG(32, false); }
+#undef THG_KG
 }
 
 void launch_closest_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, bool points, hipStream_t s) {
@@ -1521,6 +1935,35 @@ void launch_knn_lane_multi(const SegDesc *seg_dev, int nseg, uint64_t total_wave
 #undef THG_KLM_K
 #undef THG_KLM
   launch_check("k_knn_lane_multi");
+}
+
+void launch_closest_group_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, bool points, int stack,
+                                hipStream_t s) {
+  if (nseg <= 0 || total_waves == 0) return;
+  const uint32_t nb = (uint32_t)total_waves;
+  const uint32_t grid = round_up8(nb);
+#define THG_CGM(S, P) hipLaunchKernelGGL((k_closest_group_multi<S, P>), dim3(grid), dim3(Q_BLOCK), 0, s, seg_dev, nseg, \
+                                         total_waves, nb)
+  if (points) { if (stack <= 32) THG_CGM(32, true); else THG_CGM(64, true); }
+  else { if (stack <= 32) THG_CGM(32, false); else THG_CGM(64, false); }
+#undef THG_CGM
+  launch_check("k_closest_group_multi");
+}
+
+void launch_knn_group_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, int k, bool points, int stack,
+                            hipStream_t s) {
+  if (nseg <= 0 || total_waves == 0) return;
+  const uint32_t nb = (uint32_t)total_waves;
+  const uint32_t grid = round_up8(nb);
+#define THG_KGM(S, P, K) hipLaunchKernelGGL((k_knn_group_multi<S, P, K>), dim3(grid), dim3(Q_BLOCK), 0, s, seg_dev, nseg, \
+                                            total_waves, nb)
+#define THG_KGM_K(S, P) \
+  if (k <= 4) THG_KGM(S, P, 4); else if (k <= 8) THG_KGM(S, P, 8); else if (k <= 16) THG_KGM(S, P, 16); else THG_KGM(S, P, 32)
+  if (points) { if (stack <= 32) { THG_KGM_K(32, true); } else { THG_KGM_K(64, true); } }
+  else { if (stack <= 32) { THG_KGM_K(32, false); } else { THG_KGM_K(64, false); } }
+#undef THG_KGM_K
+#undef THG_KGM
+  launch_check("k_knn_group_multi");
 }
 
 void launch_ray_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, int stack, hipStream_t s) {
@@ -1584,6 +2027,18 @@ void launch_knn(const Bvh &b, const float *q, int64_t n, int k, double dist_boun
     }
 #undef THG_KW
     launch_check("k_knn_wave");
+    kt.stop();
+    return;
+  }
+  if (const int lg = group_lanes_log2(n, b.max_depth)) {
+    const uint32_t gnb = (uint32_t)ceil_div(n << lg, Q_BLOCK);
+    const bool deep = b.max_depth + 1 + (1 << GRP_FREE) > 32;
+    KernelTimer kt(b.ctx, s, "knn");
+    if (k <= 4) launch_knn_group_k<4>(b, deep, gnb, lg, q, n, k, best, exclude_self, idx, dist, d2, ws.order, s);
+    else if (k <= 8) launch_knn_group_k<8>(b, deep, gnb, lg, q, n, k, best, exclude_self, idx, dist, d2, ws.order, s);
+    else if (k <= 16) launch_knn_group_k<16>(b, deep, gnb, lg, q, n, k, best, exclude_self, idx, dist, d2, ws.order, s);
+    else launch_knn_group_k<32>(b, deep, gnb, lg, q, n, k, best, exclude_self, idx, dist, d2, ws.order, s);
+    launch_check("k_knn_group");
     kt.stop();
     return;
   }

@@ -30,6 +30,7 @@ struct SegDesc {
   int32_t points;
   int32_t root;          // node the traversal starts from
   int32_t nelem;         // elements of the segment's tree (k-NN seed window)
+  int32_t lanes_log2;    // lanes per query (log2) of the grouped sparse kernels
 };
 
 // Per-call temporaries of a query batch (spatial order of the queries).
@@ -68,6 +69,14 @@ void launch_closest_lane_multi(const SegDesc *seg_dev, int nseg, uint64_t total_
                                hipStream_t s);
 void launch_knn_lane_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, int k, bool points, int stack,
                            hipStream_t s);
+// 2^lanes_log2 lanes per query over a segment table (SegDesc::lanes_log2 per segment, each
+// segment spanning ceil(n 2^lanes_log2 / 64) waves); stack >= the deepest tree plus 5, at most 64.
+void launch_closest_group_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, bool points, int stack,
+                                hipStream_t s);
+void launch_knn_group_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, int k, bool points, int stack,
+                            hipStream_t s);
+// log2 of the lanes per query the sparse kernels give a batch of n queries on a tree of this depth.
+int group_lanes_log2(int64_t n, int depth);
 bool use_wave_traversal(const Bvh &b, int64_t n);
 // stack: the deepest tree among the segments plus one (32 or 64 entries).
 void launch_ray_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, int stack, hipStream_t s);

@@ -143,8 +143,15 @@ static void worker_loop(Server *S, int wi) {
       for (auto &kv : groups) {
         const int type = std::get<0>(kv.first);
         const bool lane = std::get<1>(kv.first) != 0;
-        int stack = 32;
-        for (SrvRequest *r : kv.second) stack = std::max(stack, S->meshes[r->r.mesh].bvh->max_depth + 1);
+        int depth1 = 0;
+        for (SrvRequest *r : kv.second) depth1 = std::max(depth1, S->meshes[r->r.mesh].bvh->max_depth + 1);
+        // sparse requests share each query among several lanes (group kernels), which keep up to
+        // four subtree roots on the stack besides the path
+        bool grouped = false;
+        if (lane && depth1 + 4 <= 64)
+          for (SrvRequest *r : kv.second)
+            grouped = grouped || group_lanes_log2(r->r.n, S->meshes[r->r.mesh].bvh->max_depth) > 0;
+        const int stack = std::max(32, depth1 + (grouped ? 4 : 0));
         // k-NN segments of one launch share k; split by exact k
         std::map<int, std::vector<SrvRequest *>> byk;
         for (SrvRequest *r : kv.second) byk[type == THG_REQ_KNN ? r->r.k : 0].push_back(r);
@@ -185,16 +192,19 @@ static void worker_loop(Server *S, int wi) {
               order_queries(b, q.q, q.n, *scratch.back(), s);
               d.order = scratch.back()->order;
             }
+            d.lanes_log2 = grouped ? group_lanes_log2(q.n, b.max_depth) : 0;
             d.wave_begin = waves;
-            waves += (uint64_t)((q.n + SEG_WAVE_QUERIES - 1) / SEG_WAVE_QUERIES);
+            waves += (uint64_t)(((q.n << d.lanes_log2) + SEG_WAVE_QUERIES - 1) / SEG_WAVE_QUERIES);
             hd[nseg++] = d;
           }
           THG_HIP(hipMemcpyAsync(dd, hd, sizeof(SegDesc) * nseg, hipMemcpyHostToDevice, s));
           if (type == THG_REQ_CLOSEST) {
-            if (lane) launch_closest_lane_multi(dd, nseg, waves, false, stack, s);
+            if (grouped) launch_closest_group_multi(dd, nseg, waves, false, stack, s);
+            else if (lane) launch_closest_lane_multi(dd, nseg, waves, false, stack, s);
             else launch_closest_multi(dd, nseg, waves, false, s);
           } else if (type == THG_REQ_KNN) {
-            if (lane) launch_knn_lane_multi(dd, nseg, waves, kk.first, false, stack, s);
+            if (grouped) launch_knn_group_multi(dd, nseg, waves, kk.first, false, stack, s);
+            else if (lane) launch_knn_lane_multi(dd, nseg, waves, kk.first, false, stack, s);
             else launch_knn_multi(dd, nseg, waves, kk.first, false, s);
           } else {
             launch_ray_multi(dd, nseg, waves, stack, s);
