@@ -926,8 +926,8 @@ __global__ void __launch_bounds__(Q_BLOCK) k_knn_lane_multi(const SegDesc *__res
 // the subtrees reached from the root by taking both children at depths 0..GRP_FREE-1 and child
 // slot bit (depth - GRP_FREE) of gl at the next log2(G) depths, so the lanes of a group partition
 // the tree into 4G subtrees spread over it; an element range met above that depth is evaluated by
-// the one lane whose remaining route bits are zero. Each lane walks its subtrees with its own
-// stack; every step the group's bound (the minimum of the lanes' bounds, a distance some element
+// the one lane whose remaining route bits are zero. Each lane walks its subtrees one after another
+// with its own stack; every step the group's bound (the minimum of the lanes' bounds, a distance some element
 // of the group attains) is refreshed with cross-lane shuffles, so the whole wave runs one uniform
 // loop. The answer is the group's (d^2, index) minimum, the same pair a single lane finds.
 //--------------------------------------------------------------------------------------------------
@@ -953,37 +953,36 @@ __device__ __forceinline__ uint64_t pair_key(float d, int32_t i) {
   return ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)i;
 }
 
-// The first GRP_FREE + log2(G) levels for lane gl: element ranges on the way go to `leaf`, the
-// subtrees at the route's end (at most 2^GRP_FREE) to `start`. Children whose lower bound exceeds
-// `bound()` are dropped.
-template <typename LB, typename Leaf, typename Start>
-__device__ __forceinline__ void group_route(const BNode *__restrict__ nodes, int32_t root, int lg, int gl, float3 qlo,
-                                            float3 qhi, LB bound, Leaf leaf, Start start) {
+// Route pth (0 .. 2^GRP_FREE - 1) of lane gl through the first GRP_FREE + log2(G) levels: element
+// ranges on the way go to `leaf`; returns the subtree root at the route's end (with its lower bound
+// in lb_out), or -1 when the route ends early or its child's lower bound exceeds `bound()`. A lane
+// walks its routes one after another, so its stack never holds more than one path.
+template <typename LB, typename Leaf>
+__device__ __forceinline__ int32_t group_route(const BNode *__restrict__ nodes, int32_t root, int lg, int gl, int pth,
+                                               float3 qlo, float3 qhi, LB bound, Leaf leaf, float &lb_out) {
   const int depth = GRP_FREE + lg;
-  for (int pth = 0; pth < (1 << GRP_FREE); ++pth) {
-    const uint32_t route = (uint32_t)pth | ((uint32_t)gl << GRP_FREE);
-    int32_t node = root;
-    for (int j = 0; j < depth; ++j) {
-      // the routes that share this prefix and differ above level j would repeat it: only the one
-      // whose bits above j are zero goes on
-      const BNode nd = nodes[node];
-      const int slot = (route >> j) & 1;
-      const float lb = slot ? box_lb(qlo, qhi, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w)
-                            : box_lb(qlo, qhi, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y);
-      const int32_t r = slot ? nd.d.y : nd.d.x;
-      const int32_t cnt = slot ? nd.d.w : nd.d.z;
-      if (!(lb <= bound())) break;
-      if (r < 0) {
-        if ((route >> (j + 1)) == 0) leaf(~r, cnt);
-        break;
-      }
-      if (j + 1 == depth) {
-        start(r, lb);
-        break;
-      }
-      node = r;
+  const uint32_t route = (uint32_t)pth | ((uint32_t)gl << GRP_FREE);
+  int32_t node = root;
+  for (int j = 0; j < depth; ++j) {
+    const BNode nd = nodes[node];
+    const int slot = (route >> j) & 1;
+    const float lb = slot ? box_lb(qlo, qhi, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w)
+                          : box_lb(qlo, qhi, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y);
+    const int32_t r = slot ? nd.d.y : nd.d.x;
+    const int32_t cnt = slot ? nd.d.w : nd.d.z;
+    if (!(lb <= bound())) return -1;
+    if (r < 0) {
+      // every route sharing bits 0..j meets this range; the one whose higher bits are zero takes it
+      if ((route >> (j + 1)) == 0) leaf(~r, cnt);
+      return -1;
     }
+    if (j + 1 == depth) {
+      lb_out = lb;
+      return r;
+    }
+    node = r;
   }
+  return -1;
 }
 
 template <int STACK, bool POINTS>
@@ -1030,12 +1029,8 @@ __device__ __forceinline__ void closest_group_body(int32_t *__restrict__ s_ref, 
   };
   int sp = 0;
   LaneStack<CL_REG, CL_LB> stk;
-  if (live)
-    group_route(nodes, root, lg, gl, qlo, qhi, [&]() { return best; }, eval, [&](int32_t r, float lb) {
-      stk.push(sp, r, lb, s_ref, s_lb, tid);
-      ++sp;
-    });
-  float gbest = group_min(best, g);
+  int pth = live ? 0 : 1 << GRP_FREE;  // next route
+  float gbest = best_init;
   int32_t node = -1;
   while (true) {
     if (node < 0) {
@@ -1045,6 +1040,10 @@ __device__ __forceinline__ void closest_group_body(int32_t *__restrict__ s_ref, 
         float pr_lb;
         stk.pop(sp, pr_ref, pr_lb, s_ref, s_lb, tid);
         if (pr_lb <= gbest) { node = pr_ref; break; }
+      }
+      if (node < 0 && pth < (1 << GRP_FREE)) {
+        float lb = 0;
+        node = group_route(nodes, root, lg, gl, pth++, qlo, qhi, [&]() { return fminf(gbest, best); }, eval, lb);
       }
     }
     if (node >= 0) {
@@ -1076,7 +1075,7 @@ __device__ __forceinline__ void closest_group_body(int32_t *__restrict__ s_ref, 
       node = next;
     }
     gbest = group_min(best, g);
-    if (!__any(node >= 0 || sp > 0)) break;
+    if (!__any(node >= 0 || sp > 0 || pth < (1 << GRP_FREE))) break;
   }
   const uint64_t key = pair_key(best, bidx);
   const uint64_t gkey = group_min_key(key, g);
@@ -1150,12 +1149,8 @@ __device__ __forceinline__ void knn_group_body(int32_t *__restrict__ s_ref, floa
   };
   int sp = 0;
   LaneStack<KN_REG, KN_LB> stk;
-  if (live)
-    group_route(nodes, root, lg, gl, qlo, qhi, bound, eval, [&](int32_t r, float lb) {
-      stk.push(sp, r, lb, s_ref, s_lb, tid);
-      ++sp;
-    });
-  float gbound = group_min(bound(), g);
+  int pth = live ? 0 : 1 << GRP_FREE;  // next route
+  float gbound = best_init;
   int32_t node = -1;
   while (true) {
     if (node < 0 && sp > 0) {
@@ -1164,6 +1159,9 @@ __device__ __forceinline__ void knn_group_body(int32_t *__restrict__ s_ref, floa
       float pr_lb;
       stk.pop(sp, pr_ref, pr_lb, s_ref, s_lb, tid);
       if (pr_lb <= gbound) node = pr_ref;
+    } else if (node < 0 && pth < (1 << GRP_FREE)) {
+      float lb = 0;
+      node = group_route(nodes, root, lg, gl, pth++, qlo, qhi, [&]() { return fminf(gbound, bound()); }, eval, lb);
     }
     if (node >= 0) {
       const BNode nd = nodes[node];
@@ -1194,7 +1192,7 @@ __device__ __forceinline__ void knn_group_body(int32_t *__restrict__ s_ref, floa
       node = next;
     }
     gbound = group_min(bound(), g);
-    if (!__any(node >= 0 || sp > 0)) break;
+    if (!__any(node >= 0 || sp > 0 || pth < (1 << GRP_FREE))) break;
   }
   // Merge: the group's k smallest pairs are among the lanes' first k; each round takes the least
   // head and its lane drops it.
@@ -1222,7 +1220,7 @@ __device__ __forceinline__ void knn_group_body(int32_t *__restrict__ s_ref, floa
 constexpr int64_t GRP_LANES = (int64_t)1 << 18;
 
 template <int STACK, bool POINTS>
-__global__ void __launch_bounds__(Q_BLOCK) k_closest_group(const BNode *__restrict__ nodes,
+__global__ void __launch_bounds__(Q_BLOCK, STACK <= 32 ? 5 : 1) k_closest_group(const BNode *__restrict__ nodes,
                                                            const void *__restrict__ elems, const float *__restrict__ q,
                                                            const uint32_t *__restrict__ order, int64_t n, int lg,
                                                            uint32_t nb_real, float best_init, float scale,
@@ -1260,7 +1258,7 @@ __global__ void __launch_bounds__(Q_BLOCK) k_knn_group(const BNode *__restrict__
 // Server form: one 64-lane block per wave of a segment; a segment of n queries at 2^lg lanes each
 // spans ceil(n 2^lg / 64) waves (SegDesc::lanes_log2).
 template <int STACK, bool POINTS>
-__global__ void __launch_bounds__(Q_BLOCK) k_closest_group_multi(const SegDesc *__restrict__ seg, int nseg,
+__global__ void __launch_bounds__(Q_BLOCK, STACK <= 32 ? 5 : 1) k_closest_group_multi(const SegDesc *__restrict__ seg, int nseg,
                                                                  uint64_t total_waves, uint32_t nb_real) {
   __shared__ int32_t s_ref[lane_lds_entries<STACK, CL_REG>() * Q_BLOCK];
   __shared__ float s_lb[CL_LB ? lane_lds_entries<STACK, CL_REG>() * Q_BLOCK : 1];
@@ -1707,13 +1705,13 @@ static int pick_stack(int depth) {
 }
 
 // log2 of the lanes per query of a sparse batch of n queries on a tree of the given depth (0: one
-// lane each): the largest power of two up to 64 with n * G <= GRP_LANES. The grouped kernels keep
-// up to 2^GRP_FREE subtree roots on the stack besides the path, so they run on trees up to 59
-// levels. THG_GROUP_LANES overrides the lane budget (0 turns grouping off).
+// lane each): the largest power of two up to 64 with n * G <= GRP_LANES, and at least 2 (a batch
+// too large for the budget still halves its slowest query). Trees deeper than 63 levels keep one
+// lane per query. THG_GROUP_LANES overrides the lane budget (0 turns grouping off).
 int group_lanes_log2(int64_t n, int depth) {
   static const int64_t budget = getenv("THG_GROUP_LANES") ? atoll(getenv("THG_GROUP_LANES")) : GRP_LANES;
-  if (depth + 1 + (1 << GRP_FREE) > 64) return 0;
-  int lg = 0;
+  if (depth + 1 > 64 || budget <= 0) return 0;
+  int lg = 1;
   while (lg < 6 && n * ((int64_t)2 << lg) <= budget) ++lg;
   return lg;
 }
@@ -1828,7 +1826,7 @@ void launch_closest(const Bvh &b, const float *q, int64_t n, double dist_bound, 
   }
   if (const int lg = group_lanes_log2(n, b.max_depth)) {
     const uint32_t gnb = (uint32_t)ceil_div(n << lg, Q_BLOCK);
-    const bool deep = b.max_depth + 1 + (1 << GRP_FREE) > 32;
+    const bool deep = b.max_depth + 1 > 32;
     KernelTimer kt(b.ctx, s, "closest");
 #define THG_CG(S, P)                                                                                                  \
   hipLaunchKernelGGL((k_closest_group<S, P>), dim3(round_up8(gnb)), dim3(Q_BLOCK), 0, s, b.nodes.as<BNode>(), b.elems.p, \
@@ -2032,7 +2030,7 @@ void launch_knn(const Bvh &b, const float *q, int64_t n, int k, double dist_boun
   }
   if (const int lg = group_lanes_log2(n, b.max_depth)) {
     const uint32_t gnb = (uint32_t)ceil_div(n << lg, Q_BLOCK);
-    const bool deep = b.max_depth + 1 + (1 << GRP_FREE) > 32;
+    const bool deep = b.max_depth + 1 > 32;
     KernelTimer kt(b.ctx, s, "knn");
     if (k <= 4) launch_knn_group_k<4>(b, deep, gnb, lg, q, n, k, best, exclude_self, idx, dist, d2, ws.order, s);
     else if (k <= 8) launch_knn_group_k<8>(b, deep, gnb, lg, q, n, k, best, exclude_self, idx, dist, d2, ws.order, s);

@@ -70,7 +70,7 @@ void launch_closest_lane_multi(const SegDesc *seg_dev, int nseg, uint64_t total_
 void launch_knn_lane_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, int k, bool points, int stack,
                            hipStream_t s);
 // 2^lanes_log2 lanes per query over a segment table (SegDesc::lanes_log2 per segment, each
-// segment spanning ceil(n 2^lanes_log2 / 64) waves); stack >= the deepest tree plus 5, at most 64.
+// segment spanning ceil(n 2^lanes_log2 / 64) waves); stack >= the deepest tree plus 1, at most 64.
 void launch_closest_group_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, bool points, int stack,
                                 hipStream_t s);
 void launch_knn_group_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, int k, bool points, int stack,

@@ -145,13 +145,12 @@ static void worker_loop(Server *S, int wi) {
         const bool lane = std::get<1>(kv.first) != 0;
         int depth1 = 0;
         for (SrvRequest *r : kv.second) depth1 = std::max(depth1, S->meshes[r->r.mesh].bvh->max_depth + 1);
-        // sparse requests share each query among several lanes (group kernels), which keep up to
-        // four subtree roots on the stack besides the path
+        // sparse requests share each query among several lanes (group kernels)
         bool grouped = false;
-        if (lane && depth1 + 4 <= 64)
+        if (lane)
           for (SrvRequest *r : kv.second)
             grouped = grouped || group_lanes_log2(r->r.n, S->meshes[r->r.mesh].bvh->max_depth) > 0;
-        const int stack = std::max(32, depth1 + (grouped ? 4 : 0));
+        const int stack = std::max(32, depth1);
         // k-NN segments of one launch share k; split by exact k
         std::map<int, std::vector<SrvRequest *>> byk;
         for (SrvRequest *r : kv.second) byk[type == THG_REQ_KNN ? r->r.k : 0].push_back(r);
