@@ -46,6 +46,15 @@ def _check_tree(bvh, elo, ehi):
     # every internal node except the root is referenced exactly once
     refs = np.concatenate([ref[0][ref[0] >= 0], ref[1][ref[1] >= 0]])
     assert np.array_equal(np.sort(refs), np.arange(1, len(f)))
+    # max_depth: nodes on the path from the root to the deepest leaf (the root counts 1), level by level
+    depth, level, frontier = 0, 1, np.array([0])
+    while len(frontier):
+        if np.any(ref[0][frontier] < 0) or np.any(ref[1][frontier] < 0):
+            depth = level
+        kids = np.concatenate([ref[0][frontier], ref[1][frontier]])
+        frontier = kids[kids >= 0]
+        level += 1
+    assert bvh.info["max_depth"] == depth, (bvh.info["max_depth"], depth)
 
 
 def _tri_boxes(v, t):

@@ -35,7 +35,7 @@ struct MeshInfo {  // per mesh, device side
   int32_t leaf, small_slot;  // small_slot >= 0: index of its single-leaf root node
   uint64_t wave_begin;       // first 64-query wave of the mesh within its traversal group
   int32_t seg_slot;          // row of the segment table (wave-cooperative meshes first)
-  int32_t lane;              // 1: one query per lane (sparse queries), 0: wave-cooperative
+  int32_t lane;              // > 0: log2 lanes per query (sparse queries), 0: wave-cooperative
 };
 
 struct BatchPlan {
@@ -302,6 +302,7 @@ __global__ void k_bsegs(const MeshInfo *__restrict__ mi, int nm, const int *__re
   d.scale = sc;
   d.root = root;
   d.nelem = (int32_t)x.nt;
+  d.lanes_log2 = x.lane;
   seg[x.seg_slot] = d;
 }
 
@@ -357,7 +358,7 @@ static void plan_pipeline(BatchPlan &P, hipStream_t s, hipEvent_t mid = nullptr)
                   P.rfirst.as<int32_t>(), P.rlast.as<int32_t>(), P.vals.as<uint32_t>(), P.blo.as<float4>(),
                   P.bhi.as<float4>(), P.nbox.as<float>(), P.nodes.as<BNode>(), P.counters.as<uint32_t>(), s);
       launch_depth(P.NT, P.lparent.as<int32_t>(), P.nparent.as<int32_t>(), P.keep.as<uint32_t>(),
-                   P.counters.as<int32_t>() + 1, s);
+                   P.rfirst.as<int32_t>(), P.counters.as<int32_t>() + 1, s);
     }
     hipLaunchKernelGGL(k_brecords, dim3(ceil_div(P.NT, BB)), dim3(BB), 0, s, P.verts.as<float>(), P.tris.as<int32_t>(),
                        P.vals.as<uint32_t>(), mi, nm, P.NT, P.recs.as<TriRec>());
@@ -379,13 +380,13 @@ static void plan_pipeline(BatchPlan &P, hipStream_t s, hipEvent_t mid = nullptr)
                        P.out_dist.as<float>());
     launch_check("k_bempty");
     launch_closest_multi(P.seg.as<SegDesc>(), P.nwave_meshes, P.total_waves, false, s);
-    launch_closest_lane_multi(P.seg.as<SegDesc>() + P.nwave_meshes, nm - P.nwave_meshes, P.lane_waves, false, 64, s);
+    launch_closest_group_multi(P.seg.as<SegDesc>() + P.nwave_meshes, nm - P.nwave_meshes, P.lane_waves, false, 64, s);
   }
 }
 
-// Sparse meshes of a plan walk one query per lane when THG_BATCH_LANE=1 (default: every mesh uses
-// the wave-cooperative kernel; the per-lane kernel must reserve a 64-entry stack here because the
-// depth is known only after the captured build).
+// Sparse meshes of a plan share each query among 2-64 lanes when THG_BATCH_LANE=1 (default: every
+// mesh uses the wave-cooperative kernel; the per-query kernel must reserve a 64-entry stack here
+// because the depth is known only after the captured build).
 static bool batch_lane_split() {
   static const bool v = [] {
     const char *e = getenv("THG_BATCH_LANE");
@@ -447,8 +448,9 @@ int thg_batch_plan_create(thg_ctx ctx, int32_t nm, const int64_t *nv, const int6
       // The single-call rule (use_wave_traversal): wave-cooperative when there is at least one query
       // per 8 leaves; a collapsed LBVH has about 1.5 * triangles / leaf_size leaves.
       const int64_t leaves = (3 * nt[m]) / (2 * leaf) + 1;
-      x.lane = batch_lane_split() && 8 * nq[m] < leaves ? 1 : 0;
-      const uint64_t w = nt[m] > 0 ? (uint64_t)((nq[m] + SEG_WAVE_QUERIES - 1) / SEG_WAVE_QUERIES) : 0;
+      x.lane = batch_lane_split() && 8 * nq[m] < leaves ? group_lanes_log2(nq[m], 0) : 0;
+      const uint64_t w =
+          nt[m] > 0 ? (uint64_t)(((nq[m] << x.lane) + SEG_WAVE_QUERIES - 1) / SEG_WAVE_QUERIES) : 0;
       if (x.lane) { x.wave_begin = lwaves; lwaves += w; }
       else { x.wave_begin = waves; waves += w; P->nwave_meshes++; }
       vo += nv[m]; to += nt[m]; qo += nq[m]; io += nt[m] >= 2 ? nt[m] - 1 : 0;

@@ -255,7 +255,7 @@ __global__ void __launch_bounds__(REFIT_THREADS) k_refit_span(int64_t n, const u
     while (p >= 0) {
       const int64_t fp = rfirst[p], lp = rlast[p];
       if (fp < lo || lp >= hi) {
-        entries[atomicAdd(nentries, 1u)] = p;
+        entries[wave_append(nentries)] = p;
         break;
       }
       const bool is_left = cf == fp;
@@ -356,46 +356,62 @@ __global__ void __launch_bounds__(B_BLOCK) k_emit(int64_t n, int32_t base, const
                                                   const float *__restrict__ nbox, BNode *__restrict__ out,
                                                   uint32_t *__restrict__ leaf_count) {
   int64_t i = (int64_t)blockIdx.x * B_BLOCK + threadIdx.x;
-  if (i >= n - 1 || !keep[i]) return;
-  int32_t ch[2] = {left[i], right[i]};
-  float lo[2][3], hi[2][3];
-  int32_t ref[2], cnt[2];
   uint32_t leaves = 0;
+  if (i < n - 1 && keep[i]) {
+    int32_t ch[2] = {left[i], right[i]};
+    float lo[2][3], hi[2][3];
+    int32_t ref[2], cnt[2];
 #pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    child_box(ch[c], sorted, blo, bhi, nbox, lo[c], hi[c]);
-    if (ch[c] < 0) {
-      ref[c] = ~(ch[c] & 0x7fffffff);
-      cnt[c] = 1;
-      ++leaves;
-    } else if (!keep[ch[c]]) {
-      ref[c] = ~rfirst[ch[c]];
-      cnt[c] = rlast[ch[c]] - rfirst[ch[c]] + 1;
-      ++leaves;
-    } else {
-      ref[c] = base + (int32_t)fidx[ch[c]];
-      cnt[c] = 0;
+    for (int c = 0; c < 2; ++c) {
+      child_box(ch[c], sorted, blo, bhi, nbox, lo[c], hi[c]);
+      if (ch[c] < 0) {
+        ref[c] = ~(ch[c] & 0x7fffffff);
+        cnt[c] = 1;
+        ++leaves;
+      } else if (!keep[ch[c]]) {
+        ref[c] = ~rfirst[ch[c]];
+        cnt[c] = rlast[ch[c]] - rfirst[ch[c]] + 1;
+        ++leaves;
+      } else {
+        ref[c] = base + (int32_t)fidx[ch[c]];
+        cnt[c] = 0;
+      }
     }
+    BNode nd;
+    nd.a = make_float4(lo[0][0], lo[0][1], lo[0][2], hi[0][0]);
+    nd.b = make_float4(hi[0][1], hi[0][2], lo[1][0], lo[1][1]);
+    nd.c = make_float4(lo[1][2], hi[1][0], hi[1][1], hi[1][2]);
+    nd.d = make_int4(ref[0], ref[1], cnt[0], cnt[1]);
+    out[base + fidx[i]] = nd;
   }
-  BNode nd;
-  nd.a = make_float4(lo[0][0], lo[0][1], lo[0][2], hi[0][0]);
-  nd.b = make_float4(hi[0][1], hi[0][2], lo[1][0], lo[1][1]);
-  nd.c = make_float4(lo[1][2], hi[1][0], hi[1][1], hi[1][2]);
-  nd.d = make_int4(ref[0], ref[1], cnt[0], cnt[1]);
-  out[base + fidx[i]] = nd;
-  if (leaves) atomicAdd(leaf_count, leaves);
+  // one atomic per wave on the shared counter (every lane of the wave takes part in the sum)
+  leaves = wave_sum(leaves);
+  if (lane_id() == 0 && leaves) atomicAdd(leaf_count, leaves);
 }
 
-// Depth (number of kept ancestors) of every element's leaf; max via atomicMax.
+// Maximum leaf depth of the collapsed tree (number of kept ancestors), via atomicMax. The elements
+// of one collapsed leaf share their kept ancestors, so only the leaf's first element walks to the
+// root; the others stop after the non-kept nodes above them (all inside that leaf). Ancestors of a
+// kept node are kept (their ranges are larger).
 __global__ void __launch_bounds__(B_BLOCK) k_depth(int64_t n, const int32_t *__restrict__ leaf_parent,
                                                    const int32_t *__restrict__ node_parent,
-                                                   const uint32_t *__restrict__ keep, int32_t *__restrict__ max_depth) {
+                                                   const uint32_t *__restrict__ keep,
+                                                   const int32_t *__restrict__ rfirst, int32_t *__restrict__ max_depth) {
   int64_t k = (int64_t)blockIdx.x * B_BLOCK + threadIdx.x;
   int d = 0;
-  if (k < n)
-    for (int32_t cur = leaf_parent[k]; cur >= 0; cur = node_parent[cur]) d += keep[cur] ? 1 : 0;
+  if (k < n) {
+    int32_t cur = leaf_parent[k], top = -1;
+    while (cur >= 0 && !keep[cur]) {
+      top = cur;
+      cur = node_parent[cur];
+    }
+    if (top < 0 || rfirst[top] == k)
+      for (; cur >= 0; cur = node_parent[cur]) ++d;
+  }
   int m = (int)wave_max((float)d);  // every lane of the wave takes part in the reduction
-  if (lane_id() == 0) atomicMax(max_depth, (int)m);
+  // one atomic per wave, and none once the maximum is known to be at least this wave's
+  if (lane_id() == 0 && m > __hip_atomic_load(max_depth, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+    atomicMax(max_depth, m);
 }
 
 __global__ void __launch_bounds__(B_BLOCK) k_tri_records(const float *__restrict__ verts, const int32_t *__restrict__ tris,
@@ -548,10 +564,11 @@ void launch_emit(int64_t ni, int32_t base, const uint32_t *keep, const uint32_t 
   launch_check("k_emit");
 }
 
-void launch_depth(int64_t n, const int32_t *lparent, const int32_t *nparent, const uint32_t *keep, int32_t *max_depth,
-                  hipStream_t s) {
+void launch_depth(int64_t n, const int32_t *lparent, const int32_t *nparent, const uint32_t *keep, const int32_t *rfirst,
+                  int32_t *max_depth, hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_depth, dim3(ceil_div(n, B_BLOCK)), dim3(B_BLOCK), 0, s, n, lparent, nparent, keep, max_depth);
+  hipLaunchKernelGGL(k_depth, dim3(ceil_div(n, B_BLOCK)), dim3(B_BLOCK), 0, s, n, lparent, nparent, keep, rfirst,
+                     max_depth);
   launch_check("k_depth");
 }
 
@@ -648,7 +665,7 @@ void build_bvh(Bvh &b, const float *verts, const int32_t *tris, int64_t n, bool 
                      b.nodes.as<BNode>(), leaf_count);
   launch_check("k_emit");
   hipLaunchKernelGGL(k_depth, dim3(ceil_div(n, B_BLOCK)), dim3(B_BLOCK), 0, s, n, lparent.as<int32_t>(),
-                     nparent.as<int32_t>(), keep.as<uint32_t>(), maxd);
+                     nparent.as<int32_t>(), keep.as<uint32_t>(), rfirst.as<int32_t>(), maxd);
   launch_check("k_depth");
   uint32_t cnt[2];
   THG_HIP(hipMemcpyAsync(cnt, counters.p, 8, hipMemcpyDeviceToHost, s));

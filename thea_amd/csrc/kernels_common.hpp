@@ -46,6 +46,17 @@ __device__ __forceinline__ float wave_max(float v) {
   return v;
 }
 
+// A unique slot of `counter` for each calling lane (callable in divergent code): one atomic per
+// wave, by the lowest active lane, instead of one per lane on the same address.
+__device__ __forceinline__ uint32_t wave_append(uint32_t *counter) {
+  const uint64_t act = __ballot(1);
+  const int leader = __ffsll((unsigned long long)act) - 1;
+  uint32_t base = 0;
+  if (lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(act));
+  base = __shfl(base, leader, WAVE);
+  return base + (uint32_t)__popcll(act & lanemask_lt());
+}
+
 // Inclusive scan across the wave.
 template <typename T> __device__ __forceinline__ T wave_incl_scan(T v) {
   int l = lane_id();

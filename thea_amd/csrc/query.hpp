@@ -53,8 +53,8 @@ void launch_emit(int64_t ni, int32_t base, const uint32_t *keep, const uint32_t 
                  const int32_t *right, const int32_t *rfirst, const int32_t *rlast, const uint32_t *sorted,
                  const float4 *blo, const float4 *bhi, const float *nbox, BNode *out, uint32_t *leaf_count,
                  hipStream_t s);
-void launch_depth(int64_t n, const int32_t *lparent, const int32_t *nparent, const uint32_t *keep, int32_t *max_depth,
-                  hipStream_t s);
+void launch_depth(int64_t n, const int32_t *lparent, const int32_t *nparent, const uint32_t *keep, const int32_t *rfirst,
+                  int32_t *max_depth, hipStream_t s);
 // Exclusive scan of n words: out must hold n + 1 words (the total lands in out[n]) and tmp
 // scan_tmp_words(n) words.
 size_t scan_tmp_words(int64_t n);
