@@ -492,6 +492,14 @@ __device__ __forceinline__ void closest_wave_body(const BNode *__restrict__ node
 // orders pairs exactly as the lexicographic (d^2, index) rule does (d^2 >= 0; NaN never wins).
 // Deferring an evaluation only delays the tightening of the pruning bound, never the result.
 //--------------------------------------------------------------------------------------------------
+// Resident waves per SIMD the compacted closest-point kernels are compiled for (single-stream
+// launches). Their walks are chains of dependent loads, so more waves hide more latency: at 8 (a
+// 32-byte spill) the C3 kernel takes 22.9 ms against 26.2 ms at the 6 its 74 registers would
+// otherwise allow. The k-NN wave
+// kernels keep the compiler's choice: their K-pair lists spill at higher targets (K = 16 at 8
+// waves: 14.6 ms against 3.85 ms on C2; at 6: 3.87 ms).
+constexpr int CMP_WAVES = 8;
+
 struct CmpWave {
   float4 qp[WAVE];                  // owner's query point; w = excluded triangle id as int bits
   float4 qcp[WAVE];                 // owner's closest point of its current best
@@ -704,7 +712,7 @@ __device__ __forceinline__ void closest_cmp_body(CmpWave &W, const BNode *__rest
 }
 
 template <bool STATS = false>
-__global__ void __launch_bounds__(W_BLOCK) k_closest_cmp(const BNode *__restrict__ nodes, const TriRec *__restrict__ tr,
+__global__ void __launch_bounds__(W_BLOCK, CMP_WAVES) k_closest_cmp(const BNode *__restrict__ nodes, const TriRec *__restrict__ tr,
                                                          const float *__restrict__ q, const uint32_t *__restrict__ order,
                                                          int64_t n, uint32_t nb_real, float best_init, float scale,
                                                          int exclude_self, int32_t *__restrict__ out_idx,
@@ -744,8 +752,11 @@ __device__ __forceinline__ int find_segment(const SegDesc *__restrict__ seg, int
   return lo;
 }
 
-template <bool POINTS>
-__global__ void __launch_bounds__(W_BLOCK) k_closest_multi(const SegDesc *__restrict__ seg, int nseg,
+// WAVES: resident waves per SIMD to compile for. The request server, whose four streams run these
+// launches concurrently, keeps the spill-free 6 (the 8-wave variant, which spills 32 bytes, cut its
+// saturated throughput 7%); the batched plan's single stream takes the 8-wave variant.
+template <bool POINTS, int WAVES>
+__global__ void __launch_bounds__(W_BLOCK, WAVES) k_closest_multi(const SegDesc *__restrict__ seg, int nseg,
                                                            uint64_t total_waves, uint32_t nb_real) {
   const uint32_t lb_blk = xcd_remap(blockIdx.x, gridDim.x);
   const int64_t gw = (int64_t)lb_blk * W_WAVES + (threadIdx.x / WAVE);
@@ -1881,14 +1892,18 @@ static void launch_knn_group_k(const Bvh &b, bool deep, uint32_t gnb, int lg, co
 #undef THG_KG
 }
 
-void launch_closest_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, bool points, hipStream_t s) {
+void launch_closest_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, bool points, hipStream_t s,
+                          bool concurrent) {
   if (nseg <= 0 || total_waves == 0) return;
   uint32_t nb = ceil_div(total_waves, W_WAVES);
   uint32_t grid = round_up8(nb);
   if (points)
-    hipLaunchKernelGGL(k_closest_multi<true>, dim3(grid), dim3(W_BLOCK), 0, s, seg_dev, nseg, total_waves, nb);
+    hipLaunchKernelGGL((k_closest_multi<true, 1>), dim3(grid), dim3(W_BLOCK), 0, s, seg_dev, nseg, total_waves, nb);
+  else if (concurrent)
+    hipLaunchKernelGGL((k_closest_multi<false, 1>), dim3(grid), dim3(W_BLOCK), 0, s, seg_dev, nseg, total_waves, nb);
   else
-    hipLaunchKernelGGL(k_closest_multi<false>, dim3(grid), dim3(W_BLOCK), 0, s, seg_dev, nseg, total_waves, nb);
+    hipLaunchKernelGGL((k_closest_multi<false, CMP_WAVES>), dim3(grid), dim3(W_BLOCK), 0, s, seg_dev, nseg, total_waves,
+                       nb);
   launch_check("k_closest_multi");
 }
 

@@ -62,7 +62,10 @@ void scan_excl_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *tmp, 
 
 // Launch many closest-point batches (segments) as one kernel; `seg_dev` is a device copy of the
 // descriptor table with wave_begin filled in. Requires every segment's BVH to fit the wave stack.
-void launch_closest_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, bool points, hipStream_t s);
+// concurrent: the launch shares the device with other streams' launches (request server), which
+// selects the variant that needs no scratch memory.
+void launch_closest_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, bool points, hipStream_t s,
+                          bool concurrent = false);
 void launch_knn_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, int k, bool points, hipStream_t s);
 // One query per lane over a segment table (stack >= the deepest tree of the segments, at most 64).
 void launch_closest_lane_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, bool points, int stack,

@@ -200,7 +200,7 @@ static void worker_loop(Server *S, int wi) {
           if (type == THG_REQ_CLOSEST) {
             if (grouped) launch_closest_group_multi(dd, nseg, waves, false, stack, s);
             else if (lane) launch_closest_lane_multi(dd, nseg, waves, false, stack, s);
-            else launch_closest_multi(dd, nseg, waves, false, s);
+            else launch_closest_multi(dd, nseg, waves, false, s, /*concurrent=*/true);
           } else if (type == THG_REQ_KNN) {
             if (grouped) launch_knn_group_multi(dd, nseg, waves, kk.first, false, stack, s);
             else if (lane) launch_knn_lane_multi(dd, nseg, waves, kk.first, false, stack, s);
