@@ -711,8 +711,9 @@ __device__ __forceinline__ void closest_cmp_body(CmpWave &W, const BNode *__rest
   if (out_d2) out_d2[qi] = hit ? own : -1.0f;
 }
 
-template <bool STATS = false>
-__global__ void __launch_bounds__(W_BLOCK, CMP_WAVES) k_closest_cmp(const BNode *__restrict__ nodes, const TriRec *__restrict__ tr,
+// WAVES: resident waves per SIMD to compile for (1: the compiler's choice, 6 here, no spill).
+template <bool STATS = false, int WAVES = 1>
+__global__ void __launch_bounds__(W_BLOCK, WAVES) k_closest_cmp(const BNode *__restrict__ nodes, const TriRec *__restrict__ tr,
                                                          const float *__restrict__ q, const uint32_t *__restrict__ order,
                                                          int64_t n, uint32_t nb_real, float best_init, float scale,
                                                          int exclude_self, int32_t *__restrict__ out_idx,
@@ -1813,9 +1814,18 @@ void launch_closest(const Bvh &b, const float *q, int64_t n, double dist_bound, 
     uint32_t wgrid = round_up8(wnb);
     KernelTimer kt(b.ctx, s, "closest");
     if (wc.cmp && !b.points) {
-      hipLaunchKernelGGL(k_closest_cmp<false>, dim3(round_up8(ceil_div(n, (uint64_t)W_BLOCK))), dim3(W_BLOCK), 0, s,
-                         b.nodes.as<BNode>(), b.elems.as<TriRec>(), q, ws.order, n, (uint32_t)ceil_div(n, (uint64_t)W_BLOCK),
-                         best, b.scale, exclude_self, idx, cp, dist, d2, nullptr);
+      // A batch whose waves all fit at once at 6 per SIMD gains nothing from more resident waves and
+      // would only pay the 8-wave build's spill (C1, 100k queries: 1.56 against 1.64 ms).
+      const bool one_round = ceil_div(n, (uint64_t)WAVE) <= (uint64_t)b.ctx->num_cus * 4 * 6;
+      const uint32_t cgrid = round_up8(ceil_div(n, (uint64_t)W_BLOCK)), cnb = (uint32_t)ceil_div(n, (uint64_t)W_BLOCK);
+      if (one_round)
+        hipLaunchKernelGGL((k_closest_cmp<false, 1>), dim3(cgrid), dim3(W_BLOCK), 0, s, b.nodes.as<BNode>(),
+                           b.elems.as<TriRec>(), q, ws.order, n, cnb, best, b.scale, exclude_self, idx, cp, dist, d2,
+                           nullptr);
+      else
+        hipLaunchKernelGGL((k_closest_cmp<false, CMP_WAVES>), dim3(cgrid), dim3(W_BLOCK), 0, s, b.nodes.as<BNode>(),
+                           b.elems.as<TriRec>(), q, ws.order, n, cnb, best, b.scale, exclude_self, idx, cp, dist, d2,
+                           nullptr);
       launch_check("k_closest_cmp");
       kt.stop();
       return;
