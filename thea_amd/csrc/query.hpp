@@ -41,6 +41,7 @@ struct QueryScratch {
 
 void build_bvh(Bvh &b, const float *verts, const int32_t *tris, int64_t n, bool points, int leaf_size, hipStream_t s);
 void order_queries(const Bvh &b, const float *q, int64_t n, QueryScratch &ws, hipStream_t s);
+void order_rays(const Bvh &b, const float *o, const float *d, int64_t n, QueryScratch &ws, hipStream_t s);
 float bound_to_best(double dist_bound);
 void launch_closest(const Bvh &b, const float *q, int64_t n, double dist_bound, int exclude_self, int32_t *idx,
                     float *cp, float *dist, float *d2, QueryScratch &ws, hipStream_t s);

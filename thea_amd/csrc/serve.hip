@@ -188,7 +188,8 @@ static void worker_loop(Server *S, int wi) {
             }
             if (q.n >= ORDER_MIN_QUERIES) {
               scratch.emplace_back(new QueryScratch());
-              order_queries(b, q.q, q.n, *scratch.back(), s);
+              if (type == THG_REQ_RAY) order_rays(b, q.q, q.dirs, q.n, *scratch.back(), s);
+              else order_queries(b, q.q, q.n, *scratch.back(), s);
               d.order = scratch.back()->order;
             }
             d.lanes_log2 = grouped ? group_lanes_log2(q.n, b.max_depth) : 0;
