@@ -1378,9 +1378,11 @@ __device__ __forceinline__ void knn_wave_body(const BNode *__restrict__ nodes, c
   float s_lx = 0, s_ly = 0, s_lz = 0, s_hx = 0, s_hy = 0, s_hz = 0;
   int32_t sp = 0;
   int32_t node = root;
+  // The node to enter next is known from the ballots before the leaf children are evaluated, so its
+  // record is loaded first and arrives while the wave works through the leaf's points.
+  BNode nd = nodes[uni(node)];
   while (amask != 0) {
     node = uni(node);
-    const BNode nd = nodes[node];
     const float a0 = unif(nd.a.x), a1 = unif(nd.a.y), a2 = unif(nd.a.z), a3 = unif(nd.a.w);
     const float b0 = unif(nd.b.x), b1 = unif(nd.b.y), b2 = unif(nd.b.z), b3 = unif(nd.b.w);
     const float c0 = unif(nd.c.x), c1 = unif(nd.c.y), c2 = unif(nd.c.z), c3 = unif(nd.c.w);
@@ -1389,7 +1391,17 @@ __device__ __forceinline__ void knn_wave_body(const BNode *__restrict__ nodes, c
     float lb1 = box_lb(qlo, qhi, b2, b3, c0, c1, c2, c3);
     uint64_t m0 = __ballot(lb0 <= kb), m1 = __ballot(lb1 <= kb);
     bool first1 = 2 * __popcll(__ballot(lb1 < lb0) & amask) > nact;
+    // the first admitted internal child in visiting order is the one the walk enters next
     int32_t next = -1;
+    {
+      const int cf = first1 ? 1 : 0;
+      const uint64_t mf = cf == 0 ? m0 : m1, ms = cf == 0 ? m1 : m0;
+      const int32_t rf = cf == 0 ? r0 : r1, rs = cf == 0 ? r1 : r0;
+      next = (mf != 0 && rf >= 0) ? rf : ((ms != 0 && rs >= 0) ? rs : -1);
+    }
+    BNode nd_next;
+    if (next >= 0) nd_next = nodes[next];
+    bool entered = false;
 #pragma unroll
     for (int cc = 0; cc < 2; ++cc) {
       const int c = first1 ? 1 - cc : cc;
@@ -1431,8 +1443,8 @@ __device__ __forceinline__ void knn_wave_body(const BNode *__restrict__ nodes, c
             }
           }
         }
-      } else if (next < 0) {
-        next = r;
+      } else if (!entered && r == next) {
+        entered = true;
       } else if (sp < W_STACK) {  // never full when depth < 64 (checked by every caller)
         if (lane == sp) {
           s_ref = r;
@@ -1442,7 +1454,7 @@ __device__ __forceinline__ void knn_wave_body(const BNode *__restrict__ nodes, c
         ++sp;
       }
     }
-    if (next >= 0) { node = next; continue; }
+    if (next >= 0) { node = next; nd = nd_next; continue; }
     bool found = false;
     while (sp > 0) {
       --sp;
@@ -1455,6 +1467,7 @@ __device__ __forceinline__ void knn_wave_body(const BNode *__restrict__ nodes, c
       }
     }
     if (!found) break;
+    nd = nodes[uni(node)];
   }
   if (!inrange) return;
 #pragma unroll
