@@ -20,7 +20,7 @@ namespace thg {
 
 constexpr uint32_t VIS_STREAM = 0x56495342u;  // "VISB"
 constexpr int VIS_MAX_ATTEMPTS = 64;           // 128 tries; each is accepted with probability pi/4
-constexpr int64_t VIS_CHUNK_RAYS = (int64_t)1 << 24;
+constexpr int64_t VIS_CHUNK_RAYS = (int64_t)1 << 26;
 
 // Ray g's direction (Knop's method over Philox uniforms). Exhausting the attempts (probability
 // below 1e-85) gives +z.
