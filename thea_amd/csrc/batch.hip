@@ -31,7 +31,7 @@ constexpr int CHUNK = 4096;
 
 struct MeshInfo {  // per mesh, device side
   int64_t voff, toff, qoff, ioff;
-  int64_t nt, nq;
+  int64_t nt, nq, nv;
   int32_t leaf, small_slot;  // small_slot >= 0: index of its single-leaf root node
   uint64_t wave_begin;       // first 64-query wave of the mesh within its traversal group
   int32_t seg_slot;          // row of the segment table (wave-cooperative meshes first)
@@ -53,7 +53,7 @@ struct BatchPlan {
   DevBuf d_info, d_tchunks, d_qchunks, mbounds;
   DevBuf center, blo, bhi, keys, vals, tk, tv, hist;
   DevBuf left, right, rfirst, rlast, nparent, lparent, flags, nbox, keep, fidx, stmp;
-  DevBuf nodes, recs, seg, out_idx, out_dist, counters, rent, rcount;
+  DevBuf nodes, recs, seg, out_idx, out_dist, counters, rent, rcount, bad;
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
   hipStream_t stream = nullptr;  // plan-owned: graphs cannot be captured on the null stream
@@ -74,6 +74,22 @@ __device__ __forceinline__ int mesh_of(const MeshInfo *__restrict__ mi, int nm, 
     else hi = mid - 1;
   }
   return lo;
+}
+
+// Every triangle's vertex indices against its own mesh's vertex count; any index outside
+// [0, nv) sets *bad (checked on the host before the pipeline reads vertices through them).
+__global__ void __launch_bounds__(BB) k_bcheck(const int32_t *__restrict__ tris, const MeshInfo *__restrict__ mi, int nm,
+                                               int64_t nt, int *__restrict__ bad) {
+  const int64_t e = (int64_t)blockIdx.x * BB + threadIdx.x;
+  if (e >= nt) return;
+  const int64_t nv = mi[mesh_of(mi, nm, e, false, false)].nv;
+  bool ok = true;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int32_t v = tris[3 * e + c];
+    ok = ok && v >= 0 && v < nv;
+  }
+  if (!ok) atomicOr(bad, 1);
 }
 
 // Fill kernels (graph capture records no memsets).
@@ -395,6 +411,29 @@ static bool batch_lane_split() {
   return v;
 }
 
+// Fails with THG_E_INVALID unless every triangle's vertex indices lie in its own mesh's vertices
+// (tris: the concatenated triangles of nm meshes, device memory); synchronises s.
+static void check_batch_indices(const int32_t *tris, int nm, const int64_t *nv, const int64_t *nt, hipStream_t s) {
+  std::vector<MeshInfo> mi(nm);
+  int64_t to = 0;
+  for (int m = 0; m < nm; ++m) {
+    mi[m] = MeshInfo{};
+    mi[m].toff = to;
+    mi[m].nv = nv[m];
+    to += nt[m];
+  }
+  if (to == 0) return;
+  AsyncBuf dmi(sizeof(MeshInfo) * nm, s), bad(sizeof(int), s);
+  THG_HIP(hipMemcpyAsync(dmi.p, mi.data(), sizeof(MeshInfo) * nm, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(BB), 0, s, bad.as<int32_t>(), (int64_t)1, 0);
+  hipLaunchKernelGGL(k_bcheck, dim3(ceil_div(to, BB)), dim3(BB), 0, s, tris, dmi.as<MeshInfo>(), nm, to, bad.as<int>());
+  launch_check("k_bcheck");
+  int hb = 0;
+  THG_HIP(hipMemcpyAsync(&hb, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  THG_HIP(hipStreamSynchronize(s));
+  require(hb == 0, "triangle vertex index out of range of its mesh");
+}
+
 // Leaf sizes chosen by measurement per size class (floor(log2(triangles))), cached for the process.
 static std::mutex g_tune_mu;
 static std::map<int, int> g_tuned_leaf;
@@ -443,7 +482,7 @@ int thg_batch_plan_create(thg_ctx ctx, int32_t nm, const int64_t *nv, const int6
       require(leaf >= 1 && leaf <= 16, "leaf sizes must be in 1..16");
       MeshInfo &x = P->info[m];
       x.voff = vo; x.toff = to; x.qoff = qo; x.ioff = io;
-      x.nt = nt[m]; x.nq = nq[m]; x.leaf = leaf;
+      x.nt = nt[m]; x.nq = nq[m]; x.nv = nv[m]; x.leaf = leaf;
       x.small_slot = (nt[m] >= 1 && nt[m] <= leaf) ? P->nsmall++ : -1;
       // The single-call rule (use_wave_traversal): wave-cooperative when there is at least one query
       // per 8 leaves; a collapsed LBVH has about 1.5 * triangles / leaf_size leaves.
@@ -485,6 +524,7 @@ int thg_batch_plan_create(thg_ctx ctx, int32_t nm, const int64_t *nv, const int6
     P->counters.alloc(16);
     P->rent.alloc(T1 * 4);
     P->rcount.alloc(4);
+    P->bad.alloc(4);
     THG_HIP(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
     *out = reinterpret_cast<thg_batch_plan>(P.release());
   });
@@ -508,6 +548,17 @@ int thg_batch_plan_run(thg_batch_plan plan, const float *verts, const int32_t *t
     if (P->NV) THG_HIP(hipMemcpyAsync(P->verts.p, verts, P->NV * 12, hipMemcpyDeviceToDevice, s));
     if (P->NT) THG_HIP(hipMemcpyAsync(P->tris.p, tris, P->NT * 12, hipMemcpyDeviceToDevice, s));
     if (P->NQ) THG_HIP(hipMemcpyAsync(P->queries.p, queries, P->NQ * 12, hipMemcpyDeviceToDevice, s));
+    if (P->NT) {
+      // the pipeline reads vertices through the caller's indices: reject any outside its mesh
+      hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(BB), 0, s, P->bad.as<int32_t>(), (int64_t)1, 0);
+      hipLaunchKernelGGL(k_bcheck, dim3(ceil_div(P->NT, BB)), dim3(BB), 0, s, P->tris.as<int32_t>(),
+                         P->d_info.as<MeshInfo>(), P->nm, P->NT, P->bad.as<int>());
+      launch_check("k_bcheck");
+      int hb = 0;
+      THG_HIP(hipMemcpyAsync(&hb, P->bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      THG_HIP(hipStreamSynchronize(s));
+      require(hb == 0, "triangle vertex index out of range of its mesh");
+    }
     if (timings_ms) {
       // eager run with an event between build and queries
       hipEvent_t e0, e1, e2;
@@ -569,11 +620,20 @@ int thg_autotune_leaf(thg_ctx ctx, int32_t nm, const int64_t *nv, const int64_t 
       buckets[b].push_back(m);
     }
     const int candidates[] = {2, 4, 6, 8, 12, 16};
-    hipStream_t s;
-    THG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    hipEvent_t e0, e1;
-    THG_HIP(hipEventCreate(&e0));
-    THG_HIP(hipEventCreate(&e1));
+    struct TuneRes {  // stream and events released on every path, including a failed build
+      hipStream_t s = nullptr;
+      hipEvent_t e0 = nullptr, e1 = nullptr;
+      ~TuneRes() {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (s) (void)hipStreamDestroy(s);
+      }
+    } R;
+    THG_HIP(hipStreamCreateWithFlags(&R.s, hipStreamNonBlocking));
+    THG_HIP(hipEventCreate(&R.e0));
+    THG_HIP(hipEventCreate(&R.e1));
+    hipStream_t s = R.s;
+    check_batch_indices(tris, nm, nv, nt, s);
     for (auto &kv : buckets) {
       int best_leaf;
       {
@@ -585,30 +645,37 @@ int thg_autotune_leaf(thg_ctx ctx, int32_t nm, const int64_t *nv, const int64_t 
         double best_ms = 1e30;
         // the largest query batch of the bucket, up to two meshes, up to 262144 queries each
         std::vector<int> ms = kv.second;
+        ms.erase(std::remove_if(ms.begin(), ms.end(), [&](int m) { return nq[m] <= 0; }), ms.end());
         std::sort(ms.begin(), ms.end(), [&](int a, int b) { return nq[a] > nq[b]; });
         if (ms.size() > 2) ms.resize(2);
         for (int leaf : candidates) {
+          // a candidate counts only if it timed every sampled mesh (a tree too deep for the query
+          // kernels' stack, or a mesh without queries, leaves it untimed)
           double tot = 0;
+          bool timed_all = true;
           for (int m : ms) {
             Bvh b;
             b.ctx = c;
             // vertex indices are local to their mesh: offset the vertex pointer
             build_bvh(b, verts + 3 * vo[m], tris + 3 * to[m], nt[m], false, leaf, s);
             int64_t n = std::min<int64_t>(nq[m], 262144);
-            if (n == 0 || b.max_depth >= 64) continue;
+            if (n == 0 || b.max_depth >= 64) {
+              timed_all = false;
+              break;
+            }
             AsyncBuf oi(n * 4, s);
             QueryScratch ws;
             launch_closest(b, queries + 3 * qo[m], n, -1.0, 0, oi.as<int32_t>(), nullptr, nullptr, nullptr, ws, s);
-            THG_HIP(hipEventRecord(e0, s));
+            THG_HIP(hipEventRecord(R.e0, s));
             for (int rep = 0; rep < 3; ++rep)
               launch_closest(b, queries + 3 * qo[m], n, -1.0, 0, oi.as<int32_t>(), nullptr, nullptr, nullptr, ws, s);
-            THG_HIP(hipEventRecord(e1, s));
-            THG_HIP(hipEventSynchronize(e1));
+            THG_HIP(hipEventRecord(R.e1, s));
+            THG_HIP(hipEventSynchronize(R.e1));
             float ms_ = 0;
-            THG_HIP(hipEventElapsedTime(&ms_, e0, e1));
+            THG_HIP(hipEventElapsedTime(&ms_, R.e0, R.e1));
             tot += ms_;
           }
-          if (tot < best_ms) { best_ms = tot; best_leaf = leaf; }
+          if (timed_all && tot < best_ms) { best_ms = tot; best_leaf = leaf; }
         }
         if (best_leaf == 0) best_leaf = 4;
         std::lock_guard<std::mutex> g(g_tune_mu);
@@ -618,9 +685,6 @@ int thg_autotune_leaf(thg_ctx ctx, int32_t nm, const int64_t *nv, const int64_t 
     }
     for (int m = 0; m < nm; ++m)
       if (nt[m] <= 0) leaf_sizes[m] = 4;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipStreamDestroy(s);
   });
 }
 

@@ -49,6 +49,19 @@ __global__ void k_check_tris(const int32_t *__restrict__ tris, int64_t n3, int64
   }
 }
 
+// Fails with THG_E_INVALID unless every index of the nt triangles (device memory) is in [0, nv).
+void check_tri_indices(const int32_t *tris, int64_t nt, int64_t nv, hipStream_t s) {
+  if (nt <= 0) return;
+  AsyncBuf bad(sizeof(int), s);
+  THG_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), s));
+  hipLaunchKernelGGL(k_check_tris, dim3(ceil_div(nt * 3, 256)), dim3(256), 0, s, tris, nt * 3, nv, bad.as<int>());
+  launch_check("k_check_tris");
+  int hb = 0;
+  THG_HIP(hipMemcpyAsync(&hb, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  THG_HIP(hipStreamSynchronize(s));
+  require(hb == 0, "triangle vertex index out of range");
+}
+
 }  // namespace thg
 
 using namespace thg;
@@ -144,14 +157,7 @@ int thg_mesh_create(thg_ctx ctx, const float *verts, int64_t nv, const int32_t *
     if (nv) THG_HIP(hipMemcpy(m->verts.p, verts, nv * 3 * sizeof(float), kind));
     if (nt) {
       THG_HIP(hipMemcpy(m->tris.p, tris, nt * 3 * sizeof(int32_t), kind));
-      DevBuf bad(sizeof(int));
-      THG_HIP(hipMemset(bad.p, 0, sizeof(int)));
-      hipLaunchKernelGGL(k_check_tris, dim3(ceil_div(nt * 3, 256)), dim3(256), 0, 0, m->tris.as<int32_t>(), nt * 3, nv,
-                         bad.as<int>());
-      launch_check("k_check_tris");
-      int hb = 0;
-      THG_HIP(hipMemcpy(&hb, bad.p, sizeof(int), hipMemcpyDeviceToHost));
-      require(hb == 0, "triangle vertex index out of range");
+      check_tri_indices(m->tris.as<int32_t>(), nt, nv, 0);
     }
     *out = reinterpret_cast<thg_mesh>(m.release());
   });

@@ -1766,25 +1766,34 @@ static int order_bits_lo() {
   return 30 - bits;
 }
 
+size_t sort_scratch_words(int64_t cap) { return (size_t)4 * (size_t)std::max<int64_t>(cap, 1) + radix_hist_words(cap); }
+
+QueryScratch::Sort QueryScratch::sort_memory(int64_t n, hipStream_t s) {
+  if (ext) {
+    require(n <= ext_cap, "batch larger than the caller-provided sort memory");
+    const size_t c = (size_t)std::max<int64_t>(ext_cap, 1);
+    return Sort{ext, ext + c, ext + 2 * c, ext + 3 * c, ext + 4 * c};
+  }
+  keys.ensure(n * 4, s);
+  vals.ensure(n * 4, s);
+  tk.ensure(n * 4, s);
+  tv.ensure(n * 4, s);
+  hist.ensure(radix_hist_words(n) * 4, s);
+  return Sort{keys.as<uint32_t>(), vals.as<uint32_t>(), tk.as<uint32_t>(), tv.as<uint32_t>(), hist.as<uint32_t>()};
+}
+
 void order_queries(const Bvh &b, const float *q, int64_t n, QueryScratch &ws, hipStream_t s) {
   ws.order = nullptr;
   if (n < ORDER_MIN_QUERIES) return;
-  KernelTimer kt(b.ctx, s, "order");
-  size_t hw = radix_hist_words(n);
-  ws.keys.ensure(n * 4, s);
-  ws.vals.ensure(n * 4, s);
-  ws.tk.ensure(n * 4, s);
-  ws.tv.ensure(n * 4, s);
-  ws.hist.ensure(hw * 4, s);
+  KernelTimer kt(ws.timer_ctx(b.ctx), s, "order");
+  const QueryScratch::Sort m = ws.sort_memory(n, s);
   float3 ext = f3(b.hi.x - b.lo.x, b.hi.y - b.lo.y, b.hi.z - b.lo.z);
   float3 inv = f3(ext.x > 0 ? 1.0f / ext.x : 0.0f, ext.y > 0 ? 1.0f / ext.y : 0.0f, ext.z > 0 ? 1.0f / ext.z : 0.0f);
-  hipLaunchKernelGGL(k_query_keys, dim3(ceil_div(n, 256)), dim3(256), 0, s, q, n, b.lo, inv, ws.keys.as<uint32_t>(),
-                     ws.vals.as<uint32_t>());
+  hipLaunchKernelGGL(k_query_keys, dim3(ceil_div(n, 256)), dim3(256), 0, s, q, n, b.lo, inv, m.keys, m.vals);
   launch_check("k_query_keys");
   // The top 24 of the 30 code bits give cells of 1/256 of the scene per axis: enough coherence.
-  radix_sort_pairs(ws.keys.as<uint32_t>(), ws.vals.as<uint32_t>(), ws.tk.as<uint32_t>(), ws.tv.as<uint32_t>(),
-                   ws.hist.as<uint32_t>(), n, order_bits_lo(), 30, s);
-  ws.order = ws.vals.as<uint32_t>();
+  radix_sort_pairs(m.keys, m.vals, m.tk, m.tv, m.hist, n, order_bits_lo(), 30, s);
+  ws.order = m.vals;
   kt.stop();
 }
 
@@ -1802,20 +1811,14 @@ void order_rays(const Bvh &b, const float *o, const float *d, int64_t n, QuerySc
   }
   ws.order = nullptr;
   if (n < ORDER_MIN_QUERIES) return;
-  KernelTimer kt(b.ctx, s, "order");
-  ws.keys.ensure(n * 4, s);
-  ws.vals.ensure(n * 4, s);
-  ws.tk.ensure(n * 4, s);
-  ws.tv.ensure(n * 4, s);
-  ws.hist.ensure(radix_hist_words(n) * 4, s);
+  KernelTimer kt(ws.timer_ctx(b.ctx), s, "order");
+  const QueryScratch::Sort m = ws.sort_memory(n, s);
   float3 ext = f3(b.hi.x - b.lo.x, b.hi.y - b.lo.y, b.hi.z - b.lo.z);
   float3 inv = f3(ext.x > 0 ? 1.0f / ext.x : 0.0f, ext.y > 0 ? 1.0f / ext.y : 0.0f, ext.z > 0 ? 1.0f / ext.z : 0.0f);
-  hipLaunchKernelGGL(k_ray_keys, dim3(ceil_div(n, 256)), dim3(256), 0, s, o, d, n, b.lo, inv, ws.keys.as<uint32_t>(),
-                     ws.vals.as<uint32_t>());
+  hipLaunchKernelGGL(k_ray_keys, dim3(ceil_div(n, 256)), dim3(256), 0, s, o, d, n, b.lo, inv, m.keys, m.vals);
   launch_check("k_ray_keys");
-  radix_sort_pairs(ws.keys.as<uint32_t>(), ws.vals.as<uint32_t>(), ws.tk.as<uint32_t>(), ws.tv.as<uint32_t>(),
-                   ws.hist.as<uint32_t>(), n, order_bits_lo(), 30, s);
-  ws.order = ws.vals.as<uint32_t>();
+  radix_sort_pairs(m.keys, m.vals, m.tk, m.tv, m.hist, n, order_bits_lo(), 30, s);
+  ws.order = m.vals;
   kt.stop();
 }
 
@@ -1873,7 +1876,7 @@ void launch_closest(const Bvh &b, const float *q, int64_t n, double dist_bound, 
     const WaveConfig wc = wave_config(b);
     uint32_t wnb = ceil_div(n, (uint64_t)W_WAVES * (WAVE / wc.g));
     uint32_t wgrid = round_up8(wnb);
-    KernelTimer kt(b.ctx, s, "closest");
+    KernelTimer kt(ws.timer_ctx(b.ctx), s, "closest");
     if (wc.cmp && !b.points) {
       // A batch whose waves all fit at once at 6 per SIMD gains nothing from more resident waves and
       // would only pay the 8-wave build's spill (C1, 100k queries: 1.56 against 1.64 ms).
@@ -1909,7 +1912,7 @@ void launch_closest(const Bvh &b, const float *q, int64_t n, double dist_bound, 
   if (const int lg = group_lanes_log2(n, b.max_depth)) {
     const uint32_t gnb = (uint32_t)ceil_div(n << lg, Q_BLOCK);
     const bool deep = b.max_depth + 1 > 32;
-    KernelTimer kt(b.ctx, s, "closest");
+    KernelTimer kt(ws.timer_ctx(b.ctx), s, "closest");
 #define THG_CG(S, P)                                                                                                  \
   hipLaunchKernelGGL((k_closest_group<S, P>), dim3(round_up8(gnb)), dim3(Q_BLOCK), 0, s, b.nodes.as<BNode>(), b.elems.p, \
                      q, ws.order, n, lg, gnb, best, b.scale, exclude_self, idx, cp, dist, d2)
@@ -1921,7 +1924,7 @@ void launch_closest(const Bvh &b, const float *q, int64_t n, double dist_bound, 
     return;
   }
   int st = pick_stack(b.max_depth);
-  KernelTimer kt(b.ctx, s, "closest");
+  KernelTimer kt(ws.timer_ctx(b.ctx), s, "closest");
 #define THG_CLOSEST(S, P)                                                                                            \
   hipLaunchKernelGGL((k_closest<S, P>), dim3(grid), dim3(Q_BLOCK), 0, s, b.nodes.as<BNode>(), b.elems.p, q, ws.order, n, \
                      nb, best, b.scale, exclude_self, idx, cp, dist, d2)

@@ -33,12 +33,28 @@ struct SegDesc {
   int32_t lanes_log2;    // lanes per query (log2) of the grouped sparse kernels
 };
 
-// Per-call temporaries of a query batch (spatial order of the queries).
+// Per-call temporaries of a query batch (spatial order of the queries). By default the sort memory
+// is allocated stream-ordered per call; with `ext` set it comes from caller-provided device memory
+// (ext_cap queries' worth: 4 * ext_cap words of keys, values and their ping-pong copies, then
+// radix_hist_words(ext_cap) histogram words) and nothing is allocated.
 struct QueryScratch {
   AsyncBuf keys, vals, tk, tv, hist;
   const uint32_t *order = nullptr;
+  uint32_t *ext = nullptr;
+  int64_t ext_cap = 0;
+  bool timed = true;  // record kernel timing events when the context has timing enabled
+  struct Sort {
+    uint32_t *keys, *vals, *tk, *tv, *hist;
+  };
+  Sort sort_memory(int64_t n, hipStream_t s);
+  Ctx *timer_ctx(Ctx *c) const { return timed ? c : nullptr; }
 };
+// Device words a QueryScratch::ext region needs for batches of up to `cap` queries.
+size_t sort_scratch_words(int64_t cap);
 
+// Fails with THG_E_INVALID unless every vertex index of the nt triangles (device memory) is in
+// [0, nv); synchronises `s`.
+void check_tri_indices(const int32_t *tris, int64_t nt, int64_t nv, hipStream_t s);
 void build_bvh(Bvh &b, const float *verts, const int32_t *tris, int64_t n, bool points, int leaf_size, hipStream_t s);
 void order_queries(const Bvh &b, const float *q, int64_t n, QueryScratch &ws, hipStream_t s);
 void order_rays(const Bvh &b, const float *o, const float *d, int64_t n, QueryScratch &ws, hipStream_t s);

@@ -38,6 +38,7 @@ enum ReqState { QUEUED, RUNNING, DONE };
 struct SrvRequest {
   int64_t id = 0;
   thg_request r{};
+  const Bvh *bvh = nullptr;  // the mesh's BVH, looked up under the lock at submit (its address is stable)
   ReqState state = QUEUED;
   int status = THG_OK;
   size_t bytes = 0;
@@ -48,7 +49,9 @@ struct Server {
   Ctx *ctx = nullptr;
   thg_server_opts opts{};
   size_t budget = 0, resident = 0, inflight = 0, peak_inflight = 0;
-  std::vector<SrvMesh> meshes;
+  // Owned meshes; the vector is read and grown only under `mu`, and workers use the Bvh pointers
+  // their requests captured at submit, which stay valid until the server is destroyed.
+  std::vector<std::unique_ptr<SrvMesh>> meshes;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<int64_t> queue;
@@ -135,7 +138,7 @@ static void worker_loop(Server *S, int wi) {
         for (int64_t id : batch) {
           SrvRequest &r = S->reqs[id];
           int kk = r.r.type == THG_REQ_KNN ? (r.r.k <= 4 ? 4 : r.r.k <= 8 ? 8 : r.r.k <= 16 ? 16 : 32) : 0;
-          int lane = r.r.type != THG_REQ_RAY && !use_wave_traversal(*S->meshes[r.r.mesh].bvh, r.r.n) ? 1 : 0;
+          int lane = r.r.type != THG_REQ_RAY && !use_wave_traversal(*r.bvh, r.r.n) ? 1 : 0;
           groups[{r.r.type, lane, kk}].push_back(&r);
         }
       }
@@ -144,12 +147,12 @@ static void worker_loop(Server *S, int wi) {
         const int type = std::get<0>(kv.first);
         const bool lane = std::get<1>(kv.first) != 0;
         int depth1 = 0;
-        for (SrvRequest *r : kv.second) depth1 = std::max(depth1, S->meshes[r->r.mesh].bvh->max_depth + 1);
+        for (SrvRequest *r : kv.second) depth1 = std::max(depth1, r->bvh->max_depth + 1);
         // sparse requests share each query among several lanes (group kernels)
         bool grouped = false;
         if (lane)
           for (SrvRequest *r : kv.second)
-            grouped = grouped || group_lanes_log2(r->r.n, S->meshes[r->r.mesh].bvh->max_depth) > 0;
+            grouped = grouped || group_lanes_log2(r->r.n, r->bvh->max_depth) > 0;
         const int stack = std::max(32, depth1);
         // k-NN segments of one launch share k; split by exact k
         std::map<int, std::vector<SrvRequest *>> byk;
@@ -164,7 +167,7 @@ static void worker_loop(Server *S, int wi) {
           int nseg = 0;
           for (SrvRequest *r : list) {
             const thg_request &q = r->r;
-            const Bvh &b = *S->meshes[q.mesh].bvh;
+            const Bvh &b = *r->bvh;
             SegDesc d{};
             d.nodes = b.nodes.p;
             d.elems = b.elems.p;
@@ -325,7 +328,8 @@ int thg_server_add_mesh(thg_server srv, const float *verts, int64_t nv, const in
   if (rc != THG_OK) return rc;
   return srv_guarded([&] {
     Server *S = reinterpret_cast<Server *>(srv);
-    SrvMesh sm;
+    std::unique_ptr<SrvMesh> smp(new SrvMesh());
+    SrvMesh &sm = *smp;
     sm.mesh.reset(reinterpret_cast<Mesh *>(m));
     require(mesh_id != nullptr, "mesh_id is NULL");
     require(nt > 0, "a served mesh needs at least one triangle");
@@ -357,7 +361,7 @@ int thg_server_add_mesh(thg_server srv, const float *verts, int64_t nv, const in
     sm.bytes = mesh_bytes + sm.bvh->device_bytes();
     std::lock_guard<std::mutex> g(S->mu);
     S->resident += sm.bytes;
-    S->meshes.push_back(std::move(sm));
+    S->meshes.push_back(std::move(smp));
     *mesh_id = (int32_t)S->meshes.size() - 1;
   });
 }
@@ -380,7 +384,8 @@ int thg_server_submit(thg_server srv, const thg_request *req, int64_t *req_id) {
     {
       std::lock_guard<std::mutex> g(S->mu);
       require(r.mesh >= 0 && r.mesh < (int32_t)S->meshes.size(), "unknown mesh id");
-      require(r.type != THG_REQ_KNN || !r.exclude_self || r.n <= S->meshes[r.mesh].bvh->n,
+      sr.bvh = S->meshes[r.mesh]->bvh.get();
+      require(r.type != THG_REQ_KNN || !r.exclude_self || r.n <= sr.bvh->n,
               "exclude_self needs the mesh's own elements as queries");
       require(!S->stopping, "server is shutting down");
       if (sr.bytes > S->budget - S->resident) {
