@@ -20,9 +20,12 @@
 #ifndef THEA_GPU_BVH_HPP
 #define THEA_GPU_BVH_HPP
 
-#include <cmath>
+#include <algorithm>
 #include <array>
+#include <cmath>
+#include <cstddef>
 #include <cstdint>
+#include <type_traits>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -53,15 +56,58 @@ struct MetricL2 {};
 struct UniversalCompatibility {};
 struct RayIntersectionTester {};
 
-// Result of a k-nearest query (Thea's NeighborPair, ProximityQueryStructureN.hpp:48-144).
+// Result of a proximity query (Thea's NeighborPair, ProximityQueryStructureN.hpp:48-144). Pairs
+// order by (squared distance, query index, target index), the lowest-index rule of this library.
 struct NeighborPair {
   intx query_index = -1, target_index = -1;
   double mon_approx_dist = -1;  // squared distance (MetricL2's monotone approximation)
+  float query_point[3] = {0, 0, 0}, target_point[3] = {0, 0, 0};  // filled when closest points are asked for
   bool isValid() const { return query_index >= 0 && target_index >= 0; }
   intx getTargetIndex() const { return target_index; }
   intx getQueryIndex() const { return query_index; }
   double getMonotoneApproxDistance() const { return mon_approx_dist; }
   template <typename MetricT = MetricL2> double getDistance() const { return mon_approx_dist < 0 ? -1 : std::sqrt(mon_approx_dist); }
+  const float *getQueryPoint() const { return query_point; }
+  const float *getTargetPoint() const { return target_point; }
+  bool operator<(NeighborPair const &o) const {
+    if (mon_approx_dist != o.mon_approx_dist) return mon_approx_dist < o.mon_approx_dist;
+    if (query_index != o.query_index) return query_index < o.query_index;
+    return target_index < o.target_index;
+  }
+  bool operator==(NeighborPair const &o) const { return query_index == o.query_index && target_index == o.target_index; }
+};
+
+// Largest k the GPU k-nearest kernels keep per query (thg_k_closest).
+constexpr int MAX_K = 32;
+
+// Fixed-capacity sorted array (Thea's BoundedSortedArrayN, BoundedSortedArrayN.hpp): keeps the N
+// smallest values inserted, ascending. Thea's own BoundedSortedArrayN / BoundedSortedArray work with
+// kClosestPairs too; any type with getCapacity(), clear() and insert() does.
+template <std::size_t N, typename T> class BoundedSortedArrayN {
+ public:
+  static constexpr std::size_t getCapacity() { return N; }
+  std::size_t size() const { return n_; }
+  bool empty() const { return n_ == 0; }
+  T const &operator[](std::size_t i) const { return v_[i]; }
+  T const &front() const { return v_[0]; }
+  T const &back() const { return v_[n_ - 1]; }
+  const T *begin() const { return v_; }
+  const T *end() const { return v_ + n_; }
+  void clear() { n_ = 0; }
+  bool isInsertable(T const &t) const { return n_ < N || t < back(); }
+  std::size_t insert(T const &t) {
+    if (!isInsertable(t)) return N;
+    std::size_t pos = (std::size_t)(std::upper_bound(v_, v_ + n_, t) - v_);
+    std::size_t last = n_ < N ? n_ : N - 1;
+    for (std::size_t i = last; i > pos; --i) v_[i] = v_[i - 1];
+    v_[pos] = t;
+    if (n_ < N) ++n_;
+    return pos;
+  }
+
+ private:
+  T v_[N];
+  std::size_t n_ = 0;
 };
 
 // Result of a ray query (Thea's RayStructureIntersectionN, RayQueryStructureN.hpp:29-53).
@@ -110,9 +156,24 @@ class Mesh {
   thg_mesh h_ = nullptr;
 };
 
+class BvhBase;
+class PointBvh;
+
+namespace detail {
+// bounded neighbour sets (Thea's BoundedSortedArrayN / BoundedSortedArray) expose getCapacity()
+template <typename T, typename = void> struct has_capacity : std::false_type {};
+template <typename T>
+struct has_capacity<T, decltype((void)std::declval<T const &>().getCapacity(), void())> : std::true_type {};
+template <typename T> using if_bounded_set = typename std::enable_if<has_capacity<T>::value, int>::type;
+template <typename T> using if_plain_list = typename std::enable_if<!has_capacity<T>::value, int>::type;
+template <typename T> using if_not_bvh =
+    typename std::enable_if<!std::is_base_of<BvhBase, typename std::decay<T>::type>::value, int>::type;
+}  // namespace detail
+
 // Shared query methods of mesh and point BVHs (BvhN.hpp:1050-1146).
 class BvhBase {
  public:
+  typedef thea_gpu::NeighborPair NeighborPair;  // BvhN::NeighborPair
   BvhBase(const BvhBase &) = delete;
   BvhBase &operator=(const BvhBase &) = delete;
   virtual ~BvhBase() { if (h_) thg_bvh_destroy(h_); }
@@ -131,6 +192,13 @@ class BvhBase {
       hi[a] = 1.05f * (hi[a] - c) + c;
     }
   }
+
+  // BvhN::enableNearestNeighborAcceleration / disableNearestNeighborAcceleration (BvhN.hpp:853-888).
+  // Thea's proxy structure only seeds the search bound with an upper bound on the nearest distance,
+  // so it never changes a result; the GPU kernels seed each query from its nearest leaf instead.
+  // Accepted and ignored.
+  void enableNearestNeighborAcceleration(intx num_acceleration_samples = -1) { (void)num_acceleration_samples; }
+  void disableNearestNeighborAcceleration(bool deallocate_memory = true) { (void)deallocate_memory; }
 
   // Batch: idx[n], optional dist[n], closest points cp[3n].
   void closestElements(intx n, const float *q, std::int32_t *idx, float *dist = nullptr, float *cp = nullptr,
@@ -161,7 +229,8 @@ class BvhBase {
   }
 
   // BvhN::closestPair (BvhN.hpp:1082-1114) with a point query.
-  template <typename MetricT = MetricL2, typename QueryT, typename CompatT = UniversalCompatibility>
+  template <typename MetricT = MetricL2, typename QueryT, typename CompatT = UniversalCompatibility,
+            detail::if_not_bvh<QueryT> = 0>
   NeighborPair closestPair(QueryT const &query, double dist_bound = -1, CompatT = CompatT(),
                            bool get_closest_points = false) const {
     (void)get_closest_points;
@@ -179,11 +248,36 @@ class BvhBase {
     check(thg_k_closest(h_, q, n, k, dist_bound, exclude_self ? 1 : 0, idx, dist, nullptr, THG_MEM_HOST, nullptr));
   }
 
-  // BvhN::kClosestPairs (BvhN.hpp:1139-1146) into any container with clear()/push_back() of
-  // NeighborPair and a capacity given by `k` (Thea passes a BoundedSortedArray of capacity k).
-  template <typename MetricT = MetricL2, typename QueryT, typename PairSetT, typename CompatT = UniversalCompatibility>
-  intx kClosestPairs(QueryT const &query, PairSetT &k_closest_pairs, int k, double dist_bound = -1,
+  // BvhN::kClosestPairs (BvhN.hpp:1139-1146) with Thea's signature: the capacity of the bounded set
+  // (Thea's BoundedSortedArrayN<k, NeighborPair> or BoundedSortedArray, or the one above) is k, at
+  // most MAX_K; larger capacities fail with Error(THG_E_UNSUPPORTED). Pairs go in through
+  // insert(), so pairs already in the set compete with the new ones as in Thea. Returns the set's size.
+  template <typename MetricT = MetricL2, typename QueryT, typename BoundedSetT, typename CompatT = UniversalCompatibility,
+            detail::if_bounded_set<BoundedSetT> = 0, detail::if_not_bvh<QueryT> = 0>
+  intx kClosestPairs(QueryT const &query, BoundedSetT &k_closest_pairs, double dist_bound = -1, CompatT = CompatT(),
+                     bool get_closest_points = false) const {
+    (void)get_closest_points;
+    const int k = checked_k((std::size_t)k_closest_pairs.getCapacity());
+    float q[3] = {(float)query[0], (float)query[1], (float)query[2]};
+    std::vector<std::int32_t> idx(k);
+    std::vector<float> d2(k);
+    check(thg_k_closest(h_, q, 1, k, dist_bound, 0, idx.data(), nullptr, d2.data(), THG_MEM_HOST, nullptr));
+    for (int j = 0; j < k && idx[j] >= 0; ++j) {
+      NeighborPair p;
+      p.query_index = 0;
+      p.target_index = idx[j];
+      p.mon_approx_dist = d2[j];
+      k_closest_pairs.insert(p);
+    }
+    return (intx)k_closest_pairs.size();
+  }
+
+  // Explicit-k form into a plain container with push_back() (e.g. std::vector<NeighborPair>).
+  template <typename MetricT = MetricL2, typename QueryT, typename PairListT, typename CompatT = UniversalCompatibility,
+            detail::if_plain_list<PairListT> = 0>
+  intx kClosestPairs(QueryT const &query, PairListT &k_closest_pairs, int k, double dist_bound = -1,
                      CompatT = CompatT()) const {
+    k = checked_k((std::size_t)(k > 0 ? k : 0));
     float q[3] = {(float)query[0], (float)query[1], (float)query[2]};
     std::vector<std::int32_t> idx(k);
     std::vector<float> d2(k);
@@ -201,8 +295,28 @@ class BvhBase {
     return found;
   }
 
+  // BvhN::closestPair and kClosestPairs with another point BVH as the query (BvhN.hpp:1082-1146,
+  // TestBvh.cpp:256-297): the nearest pair, or the k nearest pairs, between the query BVH's points
+  // and this BVH's elements. query_index is the query point's index, target_index this BVH's
+  // element; ties go to the lower (query, target) index pair. Defined after PointBvh.
+  template <typename MetricT = MetricL2, typename CompatT = UniversalCompatibility>
+  NeighborPair closestPair(PointBvh const &query, double dist_bound = -1, CompatT = CompatT(),
+                           bool get_closest_points = false) const;
+  template <typename MetricT = MetricL2, typename BoundedSetT, typename CompatT = UniversalCompatibility,
+            detail::if_bounded_set<BoundedSetT> = 0>
+  intx kClosestPairs(PointBvh const &query, BoundedSetT &k_closest_pairs, double dist_bound = -1, CompatT = CompatT(),
+                     bool get_closest_points = false) const;
+
  protected:
   BvhBase() = default;
+  virtual const float *hostPoints() const { return nullptr; }  // host copy of point elements, if any
+  static int checked_k(std::size_t k) {
+    if (k == 0) throw Error(THG_E_INVALID, "kClosestPairs: the neighbour set has capacity 0");
+    if (k > (std::size_t)MAX_K)
+      throw Error(THG_E_UNSUPPORTED, "kClosestPairs: capacity " + std::to_string(k) +
+                                         " exceeds the GPU k-nearest limit of " + std::to_string(MAX_K));
+    return (int)k;
+  }
   thg_bvh h_ = nullptr;
 };
 
@@ -251,35 +365,122 @@ class MeshBvh : public BvhBase {
   Mesh *mesh_;
 };
 
-// BVH over points (Thea's BvhN<Vector3, 3>).
+// BVH over points (Thea's BvhN<Vector3, 3>). Keeps a host copy of the points, which serve as the
+// queries when this BVH is the query side of closestPair / kClosestPairs.
 class PointBvh : public BvhBase {
  public:
-  PointBvh(Context &ctx, const float *xyz, intx n, int max_elems_in_leaf = 4) {
+  PointBvh(Context &ctx, const float *xyz, intx n, int max_elems_in_leaf = 4) : xyz_(xyz, xyz + 3 * (n > 0 ? n : 0)) {
     thg_build_opts o = {};
     o.leaf_size = max_elems_in_leaf;
     check(thg_bvh_build_points(ctx.handle(), xyz, n, THG_MEM_HOST, &o, nullptr, &h_));
   }
+  intx numElements() const { return (intx)(xyz_.size() / 3); }
+  const float *points() const { return xyz_.data(); }
+
+ protected:
+  const float *hostPoints() const override { return xyz_.data(); }
+
+ private:
+  std::vector<float> xyz_;
 };
 
-// Thea's MeshSampler (Algorithms/MeshSampler.hpp:33-336), area-weighted EXACT mode.
+template <typename MetricT, typename CompatT>
+NeighborPair BvhBase::closestPair(PointBvh const &query, double dist_bound, CompatT, bool get_closest_points) const {
+  const intx n = query.numElements();
+  std::vector<std::int32_t> idx((std::size_t)n);
+  std::vector<float> d2((std::size_t)n), cp(3 * (std::size_t)n);
+  check(thg_closest_point(h_, query.points(), n, dist_bound, idx.data(), cp.data(), nullptr, d2.data(), THG_MEM_HOST,
+                          nullptr));
+  NeighborPair best;
+  for (intx i = 0; i < n; ++i) {
+    if (idx[i] < 0) continue;
+    NeighborPair p;
+    p.query_index = i;
+    p.target_index = idx[i];
+    p.mon_approx_dist = d2[i];
+    if (!best.isValid() || p < best) {
+      if (get_closest_points)
+        for (int c = 0; c < 3; ++c) { p.query_point[c] = query.points()[3 * i + c]; p.target_point[c] = cp[3 * i + c]; }
+      best = p;
+    }
+  }
+  return best;
+}
+
+template <typename MetricT, typename BoundedSetT, typename CompatT, detail::if_bounded_set<BoundedSetT>>
+intx BvhBase::kClosestPairs(PointBvh const &query, BoundedSetT &k_closest_pairs, double dist_bound, CompatT,
+                            bool get_closest_points) const {
+  // the k nearest pairs overall are among the k nearest targets of each query point
+  const int k = checked_k((std::size_t)k_closest_pairs.getCapacity());
+  const intx n = query.numElements();
+  std::vector<std::int32_t> idx((std::size_t)n * k);
+  std::vector<float> d2((std::size_t)n * k);
+  check(thg_k_closest(h_, query.points(), n, k, dist_bound, 0, idx.data(), nullptr, d2.data(), THG_MEM_HOST, nullptr));
+  std::vector<NeighborPair> cand;
+  for (intx i = 0; i < n; ++i)
+    for (int j = 0; j < k && idx[(std::size_t)i * k + j] >= 0; ++j) {
+      NeighborPair p;
+      p.query_index = i;
+      p.target_index = idx[(std::size_t)i * k + j];
+      p.mon_approx_dist = d2[(std::size_t)i * k + j];
+      cand.push_back(p);
+    }
+  const std::size_t keep = std::min<std::size_t>(cand.size(), (std::size_t)k);
+  std::partial_sort(cand.begin(), cand.begin() + keep, cand.end());
+  const float *targets = hostPoints();
+  for (std::size_t j = 0; j < keep; ++j) {
+    NeighborPair p = cand[j];
+    if (get_closest_points) {
+      // point elements: the closest points are the points themselves (target points are known
+      // on the host for a PointBvh target; a MeshBvh target leaves them zero)
+      for (int c = 0; c < 3; ++c) {
+        p.query_point[c] = query.points()[3 * p.query_index + c];
+        if (targets) p.target_point[c] = targets[3 * p.target_index + c];
+      }
+    }
+    k_closest_pairs.insert(p);
+  }
+  return (intx)k_closest_pairs.size();
+}
+
+// Thea's MeshSampler (Algorithms/MeshSampler.hpp:33-336), area-weighted sampling in both count modes.
 class MeshSampler {
  public:
-  enum class CountMode { EXACT };
+  // MeshSampler::CountMode (MeshSampler.hpp:48-54): EXACT returns exactly n samples; APPROXIMATE
+  // gives each triangle floor(n * area / total) samples plus one more with probability equal to the
+  // fractional part (MeshSampler.hpp:205-241), so the count varies around n.
+  enum class CountMode { EXACT, APPROXIMATE };
   explicit MeshSampler(Mesh &mesh, std::uint64_t seed = 0, std::uint64_t stream = 0)
       : mesh_(&mesh), seed_(seed), stream_(stream) {}
 
   // sampleEvenlyByArea (MeshSampler.hpp:100-105): positions (and optional face normals and
-  // triangle indices) of n samples; returns the number of samples (0 for an empty mesh). Each
+  // triangle indices) of the samples; returns their number (0 for an empty or zero-area mesh). Each
   // call advances the RNG stream so successive calls draw fresh samples.
   template <typename VectorT>
-  intx sampleEvenlyByArea(intx n, std::vector<VectorT> &positions, std::vector<VectorT> *face_normals = nullptr,
-                          std::vector<intx> *triangles = nullptr, CountMode = CountMode::EXACT, bool verbose = false) {
+  intx sampleEvenlyByArea(intx n, std::vector<VectorT> &positions,
+                          typename std::common_type<std::vector<VectorT>>::type *face_normals = nullptr,
+                          std::vector<intx> *triangles = nullptr, CountMode count_mode = CountMode::EXACT,
+                          bool verbose = false) {
     (void)verbose;
-    std::vector<float> pos(3 * (size_t)(n > 0 ? n : 0)), nrm(face_normals ? pos.size() : 0);
-    std::vector<std::int32_t> tri(triangles ? (size_t)(n > 0 ? n : 0) : 0);
+    const std::uint64_t stream = stream_++;
+    std::int64_t cap = n > 0 ? n : 0;
+    if (count_mode == CountMode::APPROXIMATE) {
+      // the count is a function of (seed, stream): ask for it, then fill exactly that many rows
+      check(thg_sample_area_approx(mesh_->handle(), n, seed_, stream, 0, nullptr, nullptr, nullptr, &cap, THG_MEM_HOST,
+                                   nullptr));
+    }
+    std::vector<float> pos(3 * (size_t)cap), nrm(face_normals ? pos.size() : 0);
+    std::vector<std::int32_t> tri(triangles ? (size_t)cap : 0);
     std::int64_t got = 0;
-    check(thg_sample_area(mesh_->handle(), n, seed_, stream_++, pos.data(), face_normals ? nrm.data() : nullptr,
-                          triangles ? tri.data() : nullptr, &got, THG_MEM_HOST, nullptr));
+    if (count_mode == CountMode::APPROXIMATE) {
+      if (cap > 0)
+        check(thg_sample_area_approx(mesh_->handle(), n, seed_, stream, cap, pos.data(),
+                                     face_normals ? nrm.data() : nullptr, triangles ? tri.data() : nullptr, &got,
+                                     THG_MEM_HOST, nullptr));
+    } else {
+      check(thg_sample_area(mesh_->handle(), n, seed_, stream, pos.data(), face_normals ? nrm.data() : nullptr,
+                            triangles ? tri.data() : nullptr, &got, THG_MEM_HOST, nullptr));
+    }
     positions.resize((size_t)got);
     if (face_normals) face_normals->resize((size_t)got);
     if (triangles) triangles->resize((size_t)got);

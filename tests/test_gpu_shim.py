@@ -15,10 +15,10 @@ TRIS = np.array([0, 1, 2, 0, 3, 1, 0, 4, 3, 0, 6, 4, 3, 5, 1, 3, 4, 5,
                  2, 1, 5, 2, 5, 7, 0, 2, 7, 0, 7, 6, 6, 7, 5, 6, 5, 4], np.int32).reshape(-1, 3)
 
 
-def run_shim():
+def run_shim(*args):
     exe = os.path.join(ROOT, "tests", "shim", "shim_example")
     assert os.path.exists(exe), "tests/shim/shim_example is built by __graft_entry__.build()"
-    out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    out = subprocess.run([exe, *args], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
     lines = {}
     for line in out.stdout.splitlines():
@@ -51,6 +51,11 @@ def test_cpp_api_matches_python_binding(oracle):
     pos, nrm, tri = sampler.sampleEvenlyByArea(4, seed=7, stream_id=0)
     assert [int(r[0]) for r in got["sample"]] == list(tri)
     assert np.array_equal(np.array([[float(x) for x in r[1:]] for r in got["sample"]], np.float32), pos)
+    # MeshSampler::CountMode::APPROXIMATE through the C++ layer equals the binding's approximate mode
+    apos, _, atri = sampler.sampleEvenlyByArea(40, seed=7, stream_id=5, count_mode="approximate")
+    assert len(got["approx"]) == len(atri) and 30 <= len(atri) <= 50
+    assert [int(r[0]) for r in got["approx"]] == list(atri)
+    assert np.array_equal(np.array([[float(x) for x in r[1:]] for r in got["approx"]], np.float32), apos)
     # ICP of 2000 further samples shifted by +0.05 in x back onto the cube, least-squares fit
     surf, _, _ = sampler.sampleEvenlyByArea(2000, seed=7, stream_id=1)
     moved = surf.astype(np.float64) + np.array([0.05, 0.0, 0.0])
@@ -73,3 +78,50 @@ def test_cpp_api_matches_python_binding(oracle):
                                           float(np.sqrt(e[0] * e[0] + (e[1] * e[1] + e[2] * e[2]))))
     assert np.array_equal(ref_n, oracle_sdf)
     assert np.all((ref_n > 0) & (ref_n <= 1.0))
+
+
+def test_cpp_api_testbvh_call_sites(oracle, tmp_path):
+    """Thea's TestBvh.cpp call sites (point NN with bound 0.25, 3-NN into a BoundedSortedArrayN<3>,
+    BVH-vs-BVH closestPair and kClosestPairs, the triangle NN with closest point after
+    enableNearestNeighborAcceleration, and the three ray queries) through the C++ layer on TestBvh's
+    own regenerated inputs, against the committed oracle answers."""
+    g = np.load(os.path.join(ROOT, "tests", "golden", "testbvh.npz"))
+    for order in ("gcc", "ltr"):
+        print(f"case TestBvh call sites through the C++ layer, {order} argument order", flush=True)
+        pts, pts2, tris = g[f"{order}_points"], g[f"{order}_points2"], g[f"{order}_tris"]
+        blob = np.concatenate([pts.ravel(), pts2.ravel(), tris.ravel(), g[f"{order}_ray_dir"].ravel()]).astype(np.float32)
+        path = tmp_path / f"testbvh_{order}.f32"
+        blob.tofile(path)
+        got = run_shim(str(path))
+        # closestElement with dist_bound 0.25 (TestBvh.cpp:204-215)
+        nn = got["tb_nn"][0]
+        assert int(nn[0]) == int(g[f"{order}_nn_idx"][0])
+        assert np.float32(float(nn[1])) == np.sqrt(g[f"{order}_nn_d2"][0])
+        # kClosestPairs into BoundedSortedArrayN<3, NeighborPair> (TestBvh.cpp:221-236)
+        assert [int(r[1]) for r in got["tb_k3"]] == list(g[f"{order}_k3_idx"][0])
+        assert all(int(r[0]) == 3 for r in got["tb_k3"])
+        assert np.array_equal(np.array([float(r[2]) for r in got["tb_k3"]], np.float32), g[f"{order}_k3_d2"][0])
+        # closestPair(new_bvh) (TestBvh.cpp:256-262): the least (d2, query) over the per-query answers
+        pd2, pidx = g[f"{order}_pair_d2"], g[f"{order}_pair_idx"]
+        q = int(np.argmin(pd2))
+        pr = got["tb_pair"][0]
+        assert (int(pr[0]), int(pr[1])) == (q, int(pidx[q]))
+        assert np.float32(float(pr[2])) == pd2[q]
+        assert np.array_equal(np.array([float(x) for x in pr[3:]], np.float32), pts[pidx[q]])
+        # kClosestPairs(new_bvh, nbrs) (TestBvh.cpp:282-297): the 3 least pairs over all query points
+        kidx, kd2, _ = oracle.KdTree(points=pts).knn(pts2, 3)
+        cand = sorted((float(kd2[i, j]), i, int(kidx[i, j])) for i in range(len(pts2)) for j in range(3))[:3]
+        assert [(int(r[1]), int(r[2])) for r in got["tb_kpairs"]] == [(c[1], c[2]) for c in cand]
+        assert np.array_equal(np.array([float(r[3]) for r in got["tb_kpairs"]], np.float32),
+                              np.array([c[0] for c in cand], np.float32))
+        # k beyond the GPU limit: a clear THG_E_UNSUPPORTED (-5) error, not a silent truncation
+        assert got["tb_bigk"][0][0] == "-5" and "exceeds" in " ".join(got["tb_bigk"][0])
+        # triangles (TestBvh.cpp:338-360): closestElement with closest point
+        t = got["tb_tnn"][0]
+        assert int(t[0]) == int(g[f"{order}_tnn_idx"][0])
+        assert np.float32(float(t[1])) == np.sqrt(g[f"{order}_tnn_d2"][0])
+        assert np.array_equal(np.array([float(x) for x in t[2:]], np.float32), g[f"{order}_tnn_cp"][0])
+        # rays from the origin (TestBvh.cpp:405-439): rayIntersects, rayIntersectionTime, rayStructureIntersection
+        r = got["tb_ray"][0]
+        assert r[0] == "1" and int(r[2]) == int(g[f"{order}_ray_idx"][0])
+        assert np.float32(float(r[1])) == g[f"{order}_ray_t"][0] == np.float32(float(r[3]))
