@@ -50,6 +50,7 @@ def lib():
         "or_brute_closest": (None, [f32p, i32p, i64, f32p, i64, f64, i32, i32p, vp, vp]),
         "or_brute_closest_f64": (None, [f32p, i32p, i64, f32p, i64, i32, i32p, f64p]),
         "or_dist_f64": (None, [f32p, i32p, f32p, i32p, i64, f64p]),
+        "or_d2_to": (None, [f32p, i32p, f32p, i32p, i64, f32p]),
         "or_brute_knn": (None, [f32p, i64, f32p, i64, i32, f64, i32, i32, i32p, f32p]),
         "or_brute_ray": (None, [f32p, i32p, i64, f32p, f32p, i64, f32, i32, i32p, f32p]),
         "or_ray_f64": (None, [f32p, i32p, f32p, f32p, i32p, i64, f64p]),
@@ -190,6 +191,14 @@ def dist_f64(verts, tris, q, which):
     verts, tris, q, which = _f32(verts), _i32(tris), _f32(q).reshape(-1, 3), _i32(which)
     out = np.zeros(len(q), np.float64)
     lib().or_dist_f64(verts, tris, q, which, len(q), out)
+    return out
+
+
+def d2_to(verts, tris, q, which):
+    """Thea's float squared distance from each query to the triangle `which` names (-1 for -1)."""
+    verts, tris, q, which = _f32(verts), _i32(tris), _f32(q).reshape(-1, 3), _i32(which)
+    out = np.zeros(len(q), np.float32)
+    lib().or_d2_to(verts, tris, q, which, len(q), out)
     return out
 
 

@@ -1051,6 +1051,22 @@ void or_dist_f64(const float *verts, const int32_t *tris, const float *q, const 
   }
 }
 
+// Thea's float squared distance from query i to triangle which[i] (Triangle3.hpp:288-304), for checking
+// a reported index and distance without a search (-1 where which[i] < 0).
+void or_d2_to(const float *verts, const int32_t *tris, const float *q, const int32_t *which, int64_t nq, float *d2) {
+  for (int64_t i = 0; i < nq; ++i) {
+    if (which[i] < 0) { d2[i] = -1; continue; }
+    const int64_t j = which[i];
+    Tri t;
+    t.v[0] = ld3(verts + 3 * (int64_t)tris[3 * j]);
+    t.v[1] = ld3(verts + 3 * (int64_t)tris[3 * j + 1]);
+    t.v[2] = ld3(verts + 3 * (int64_t)tris[3 * j + 2]);
+    tri_update(t);
+    V3 c;
+    d2[i] = tri_closest(t, ld3(q + 3 * i), c);
+  }
+}
+
 void or_brute_knn(const float *pts, int64_t n, const float *q, int64_t nq, int k, double dist_bound, int exclude_self,
                   int nthreads, int32_t *idx, float *d2) {
   double bound2 = dist_bound >= 0 ? dist_bound * dist_bound : -1;

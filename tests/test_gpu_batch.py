@@ -91,3 +91,72 @@ def test_autotune_and_one_shot():
     torch.cuda.synchronize()
     _check(meshes, queries, idx.cpu().numpy(), dist.cpu().numpy())
     print(f"autotuned leaf sizes {leaf.tolist()}")
+
+
+def test_c4_full_size_oracle(oracle):
+    """The 512-mesh workload the benchmark runs (42.7M triangles, 20.4M queries), autotuned leaf sizes,
+    through the captured graph: the first 256 queries of every mesh bit-exact against the oracle's
+    kd-tree (canonical ties), and for every query the reported index is a triangle of its mesh whose
+    float distance (Thea's arithmetic) is exactly the reported distance."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import torch
+    from thea_amd import workloads as W
+    from thea_amd.batch import BatchPlan, autotune_leaf, concat
+    meshes, queries = W.many_meshes_c4()
+    nv, nt, nq, verts, tris, qs = concat(meshes, queries)
+    print(f"case C4 full size: {len(nt)} meshes, {int(nt.sum())} tris, {int(nq.sum())} queries", flush=True)
+    V, T, Q = (torch.from_numpy(x).cuda() for x in (verts, tris, qs))
+    leaf = autotune_leaf(nv, nt, nq, V, T, Q)
+    plan = BatchPlan(nv, nt, nq, leaf_sizes=leaf)
+    plan.run(V, T, Q)  # captures the graph
+    idx, dist = plan.run(V, T, Q)  # replays it
+    torch.cuda.synchronize()
+    idx, dist = idx.cpu().numpy(), dist.cpu().numpy()
+    plan.close()
+    qoff = np.concatenate([[0], np.cumsum(nq)])
+    head = 256
+
+    def check(m):
+        v, t = meshes[m]
+        q = queries[m]
+        i, d = idx[qoff[m]:qoff[m + 1]], dist[qoff[m]:qoff[m + 1]]
+        assert i.min() >= 0 and i.max() < len(t), f"mesh {m}: index out of range"
+        d2 = oracle.d2_to(v, t, q, i)
+        assert np.array_equal(d, np.sqrt(d2.astype(np.float64)).astype(np.float32)), f"mesh {m}: distance != index's"
+        ridx, _, rd2, _ = oracle.KdTree(v, t).closest(q[:head], mode=0, nthreads=1, want_cp=False)
+        assert np.array_equal(i[:head], ridx), f"mesh {m}: nearest triangle differs from the oracle"
+        assert np.array_equal(d2[:head].view(np.uint32), rd2.view(np.uint32))
+        return len(t)
+
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        done = list(ex.map(check, range(len(meshes))))
+    assert len(done) == 512
+    print(f"C4: {len(done)} meshes checked; leaf sizes {dict(zip(*np.unique(leaf, return_counts=True)))}")
+
+
+def test_bad_vertex_index_rejected():
+    """A triangle index outside its own mesh's vertices fails the plan run and the autotuner with
+    THG_E_INVALID before any vertex is read through it; a corrected input then runs."""
+    import torch
+    from thea_amd import _capi as capi
+    from thea_amd.batch import BatchPlan, autotune_leaf, concat
+    print("case batch bad index", flush=True)
+    meshes, queries = _batch(6, [500, 800, 300], [100, 200, 50])
+    nv, nt, nq, verts, tris, qs = concat(meshes, queries)
+    for bad in (int(nv[1]), -1):  # one past mesh 1's vertices (a valid index of mesh 2's range), negative
+        t_bad = tris.copy()
+        t_bad[int(nt[0]) + 7, 1] = bad
+        V, T, Q = (torch.from_numpy(x).cuda() for x in (verts, t_bad, qs))
+        plan = BatchPlan(nv, nt, nq)
+        with pytest.raises(capi.TheaGpuError) as e:
+            plan.run(V, T, Q)
+        assert e.value.code == capi.THG_E_INVALID and "out of range" in str(e.value)
+        with pytest.raises(capi.TheaGpuError) as e:
+            autotune_leaf(nv, nt, nq, V, T, Q)
+        assert e.value.code == capi.THG_E_INVALID
+        T_ok = torch.from_numpy(tris).cuda()
+        idx, dist = plan.run(V, T_ok, Q)
+        torch.cuda.synchronize()
+        _check(meshes, queries, idx.cpu().numpy(), dist.cpu().numpy())
+        plan.close()

@@ -178,18 +178,26 @@ def test_host_and_device_pointers_agree():
     assert np.array_equal(b2.closestElement(q), hidx)
 
 
-def test_c3_full_size_properties():
+def test_c3_full_size_oracle(oracle):
+    """The benchmarked configuration itself: all 10M C3 queries at leaf size 8, device pointers, the
+    same call the benchmark times. A batch of more than 256 CUs x 24 waves x 64 queries (393,216) runs
+    the 8-waves-per-SIMD build of the compacted kernel, so this pins that code object bit for bit
+    against the oracle, and then checks size-independent properties."""
     import torch
     import thea_amd
     from thea_amd import workloads as W
     v, t, q, _, _ = W.torus_c3(n_rays=0)
-    print(f"case C3 full size: {len(t)} tris x {len(q)} queries", flush=True)
+    print(f"case C3 full size: {len(t)} tris x {len(q)} queries, leaf 8", flush=True)
+    assert len(q) > 256 * 24 * 64
     m = thea_amd.Mesh(v, t)
-    b = thea_amd.MeshBvh(m)
+    b = thea_amd.MeshBvh(m, leaf_size=8)
     qd = torch.from_numpy(q).cuda()
-    idx, dist, cp = b.closestPoint(qd)
+    idx, dist, cp, d2 = b.closest(qd, want_d2=True)
     torch.cuda.synchronize()
-    idx, dist, cp = idx.cpu().numpy(), dist.cpu().numpy(), cp.cpu().numpy()
+    idx, dist, cp, d2 = idx.cpu().numpy(), dist.cpu().numpy(), cp.cpu().numpy(), d2.cpu().numpy()
+    ridx, rcp, rd2, _ = oracle.KdTree(v, t).closest(q, mode=0)
+    _assert_same(idx, cp, d2, ridx, rcp, rd2, "C3 full")
+    _dist_rule(dist, d2, idx)
     assert idx.min() >= 0 and idx.max() < len(t)
     # closest point lies on the reported triangle's box and at the reported distance
     tri = v[t[idx]]
@@ -201,3 +209,32 @@ def test_c3_full_size_properties():
     # idempotence: the closest point of a closest point is itself, at distance ~0
     i2, d2_, cp2 = b.closestPoint(cp[:100000])
     assert np.max(d2_) < 1e-5
+
+
+def test_teapot_closest_and_rays(oracle):
+    """Thea's own teapot model (Data/Models/teapot.obj, 6,320 triangles): closest point and first hit
+    against the oracle, queries in the 20%-inflated box and rays from random origins."""
+    import os
+    import thea_amd
+    from thea_amd import workloads as W
+    root = os.path.dirname(os.path.abspath(__file__))
+    md = np.load(os.path.join(root, "golden", "models.npz"))
+    v, t = md["teapot_v"].astype(np.float32), md["teapot_t"].astype(np.int32)
+    g = np.random.default_rng(21)
+    lo, hi = W.inflated_box(v, 1.2)
+    q = (lo + (hi - lo) * g.random((200_000, 3))).astype(np.float32)
+    print(f"case teapot: {len(t)} tris x {len(q)} queries", flush=True)
+    kd = oracle.KdTree(v, t)
+    for leaf in (4, 8):
+        idx, dist, cp, d2, _ = _gpu_closest(v, t, q, leaf=leaf)
+        ridx, rcp, rd2, _ = kd.closest(q, mode=0)
+        _assert_same(idx, cp, d2, ridx, rcp, rd2, f"teapot leaf {leaf}")
+        _dist_rule(dist, d2, idx)
+    o = (lo + (hi - lo) * g.random((100_000, 3))).astype(np.float32)
+    d = g.normal(size=(100_000, 3)).astype(np.float32)
+    b = thea_amd.MeshBvh(thea_amd.Mesh(v, t), leaf_size=1)
+    ridx, rt, _ = b.rayStructureIntersection(o, d)
+    oidx, ot, _ = kd.ray(o, d, mode=0)
+    assert np.array_equal(ridx, oidx)
+    assert np.array_equal(rt.view(np.uint32), ot.view(np.uint32))
+    print(f"teapot rays: {np.sum(ridx >= 0)} of {len(o)} hit")

@@ -143,3 +143,131 @@ def test_cancel_queued_request():
         assert srv.wait(rid)[0] == capi.THG_OK
     assert srv.cancel(first) == capi.THG_E_INVALID  # finished requests cannot be cancelled
     srv.close()
+
+
+def test_c5_full_size_oracle(oracle):
+    """The served workload at its full size: 64 resident meshes (41.2M triangles, log-uniform 10k-4M),
+    96 requests drawn like the benchmark's trace (uniform mesh, log-uniform 256..1M points, 60/25/15
+    closest / 8-NN / first-hit rays), all queued at once so the workers coalesce them. Every result
+    equals the same request run alone through the single-call entry points, and the first 64 queries
+    of every request match the oracle's kd-tree (canonical ties) bit for bit."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import torch
+    import thea_amd
+    from thea_amd.serve import Server
+    from thea_amd import _capi as capi
+    from thea_amd import workloads as W
+    meshes = W.resident_c5()
+    print(f"case C5 full size: {len(meshes)} meshes, {sum(len(t) for _, t in meshes)} tris", flush=True)
+    srv = Server(num_streams=4)
+    ids = [srv.add_mesh(v, t) for v, t in meshes]
+    g = W.rng(5, W.STREAM_TRACE)
+    R = 96
+    kinds = g.choice(["closest", "knn", "ray"], size=R, p=[0.6, 0.25, 0.15])
+    mesh_of = g.integers(0, len(meshes), size=R)
+    sizes = W.log_uniform_int(g, 256, 1 << 20, R)
+    reqs = []
+    for i in range(R):
+        m, n = int(mesh_of[i]), int(sizes[i])
+        lo, hi = W.inflated_box(meshes[m][0], 1.2)
+        q = (lo + (hi - lo) * g.random((n, 3))).astype(np.float32)
+        d = g.normal(size=(n, 3)).astype(np.float32) if kinds[i] == "ray" else None
+        reqs.append(dict(kind=str(kinds[i]), mesh=m, q=q, dirs=d, k=8 if kinds[i] == "knn" else 0))
+    dev = [dict(q=torch.from_numpy(r["q"]).cuda(), dirs=None if r["dirs"] is None else torch.from_numpy(r["dirs"]).cuda())
+           for r in reqs]
+    torch.cuda.synchronize()
+    srv.stats(reset=True)
+    handles = [srv.submit(r["kind"], ids[r["mesh"]], d["q"], d["dirs"], k=r["k"]) for r, d in zip(reqs, dev)]
+    for rid, _ in handles:
+        rc, _ = srv.wait(rid)
+        assert rc == capi.THG_OK, capi.lib().thg_last_error()
+    st = srv.stats()
+    print(f"C5: {st['completed']:.0f} requests, {sum(sizes)} points in {st['launches']:.0f} worker batches "
+          f"({st['requests_per_launch']:.2f} requests per batch), p99 {st['p99_us'] / 1e3:.1f} ms", flush=True)
+    assert st["completed"] == R
+    # alone, through the single-call entry points on separately built BVHs (the server's leaf size 4)
+    bvhs = {}
+    for r, d, (rid, out) in zip(reqs, dev, handles):
+        m = r["mesh"]
+        if m not in bvhs:
+            bvhs[m] = thea_amd.MeshBvh(thea_amd.Mesh(*meshes[m]))
+        alone = _alone(bvhs, dict(kind=r["kind"], mesh=m, q=d["q"], dirs=d["dirs"], k=r["k"], bound=-1.0))
+        torch.cuda.synchronize()
+        assert torch.equal(out["idx"], alone[0]), r["kind"]
+        assert _same(out["dist"], alone[1]), r["kind"]
+        if r["kind"] != "knn":
+            assert _same(out["cp"], alone[2]), r["kind"]
+    srv.close()
+    del bvhs
+    # the oracle on the first 64 queries of every request
+    head = 64
+    used = sorted(set(int(m) for m in mesh_of))
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        kds = dict(zip(used, ex.map(lambda m: oracle.KdTree(*meshes[m]), used)))
+    for r, (rid, out) in zip(reqs, handles):
+        kd, q = kds[r["mesh"]], r["q"][:head]
+        idx = out["idx"][:head].cpu().numpy()
+        val = out["dist"][:head].cpu().numpy()
+        if r["kind"] == "closest":
+            ridx, rcp, rd2, _ = kd.closest(q, mode=0, nthreads=1)
+            assert np.array_equal(idx, ridx)
+            assert np.array_equal(val, np.sqrt(rd2.astype(np.float64)).astype(np.float32))
+            assert np.array_equal(out["cp"][:head].cpu().numpy().view(np.uint32), rcp.view(np.uint32))
+        elif r["kind"] == "knn":
+            ridx, rd2, _ = kd.knn(q, 8, mode=0, nthreads=1)
+            assert np.array_equal(idx, ridx)
+            assert np.array_equal(val, np.sqrt(rd2.astype(np.float64)).astype(np.float32))
+        else:
+            ridx, rt, _ = kd.ray(q, r["dirs"][:head], mode=0, nthreads=1)
+            assert np.array_equal(idx, ridx)
+            assert np.array_equal(val.view(np.uint32), rt.view(np.uint32))
+    print(f"C5: {R} requests equal alone and in the mix; first {head} queries of each match the oracle")
+
+
+def test_add_mesh_while_requests_run():
+    """Meshes added from another thread while requests on earlier meshes are queued and running: the
+    workers use the BVH each request captured at submit, so growing the mesh table does not disturb
+    them, and every result equals the request run alone."""
+    import threading
+
+    import torch
+    import thea_amd
+    from thea_amd.serve import Server
+    from thea_amd import _capi as capi
+    from thea_amd import workloads as W
+    print("case server add_mesh during requests", flush=True)
+    base = _meshes()
+    srv = Server(num_streams=4)
+    ids = [srv.add_mesh(v, t) for v, t in base]
+    bvhs = [thea_amd.MeshBvh(thea_amd.Mesh(v, t)) for v, t in base]
+    g = np.random.default_rng(52)
+    reqs = _requests(g, base, 120)
+    extra = [W.torus(48 + 8 * i, 24 + 4 * i, seed=20 + i) for i in range(12)]
+    added = []
+
+    def adder():
+        for v, t in extra:
+            added.append(srv.add_mesh(v, t))
+    th = threading.Thread(target=adder)
+    th.start()
+    handles = [srv.submit(r["kind"], ids[r["mesh"]], r["q"], r["dirs"], k=r["k"], bound=r["bound"]) for r in reqs]
+    th.join()
+    for rid, _ in handles:
+        rc, _ = srv.wait(rid)
+        assert rc == capi.THG_OK, capi.lib().thg_last_error()
+    assert added == list(range(len(base), len(base) + len(extra)))
+    for (rid, out), r in zip(handles, reqs):
+        idx, dist, extra_out = _alone(bvhs, r)
+        torch.cuda.synchronize()
+        assert torch.equal(out["idx"], idx), r["kind"]
+        assert _same(out["dist"], dist), r["kind"]
+    # the new meshes serve requests too
+    v, t = extra[-1]
+    q = torch.from_numpy((v.min(0) + (v.max(0) - v.min(0)) * g.random((5000, 3))).astype(np.float32)).cuda()
+    rid, out = srv.submit("closest", added[-1], q)
+    assert srv.wait(rid)[0] == capi.THG_OK
+    ref = thea_amd.MeshBvh(thea_amd.Mesh(v, t)).closestElement(q)
+    torch.cuda.synchronize()
+    assert torch.equal(out["idx"], ref)
+    srv.close()
