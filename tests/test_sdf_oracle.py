@@ -44,3 +44,63 @@ def test_reduction_closed_forms():
     vals[0, :] = 100.0
     O.lib().or_sdf_reduce(vals.ravel(), 3, 2.0, out)
     assert out[0] == 1.0  # clamped
+
+
+# cone z components (the weights) of rays 5, 9, 17 and 26 of ShapeDiameter's table
+# (Algorithms/SurfaceFeatures/Local/ShapeDiameter.hpp:197-228)
+CONE_Z = {5: 0.689520, 9: 0.732200, 17: 0.890588, 26: 0.570771, 0: 0.903309}
+
+
+def _reduce(vals_by_ray, scale):
+    vals = np.full(30, -1, np.float32)
+    for r, x in vals_by_ray.items():
+        vals[r] = x
+    out = np.zeros(1)
+    O.lib().or_sdf_reduce(vals, 1, float(scale), out)
+    return out[0]
+
+
+def test_reduction_window_and_weights_three_hits():
+    """Three hits (rays 5, 9, 17 at 0.5, 0.125, 0.25). With three values std::nth_element is an
+    insertion sort, so the values become [0.125, 0.25, 0.5] while the weights stay in ray order
+    (ShapeDiameter.hpp:230-252). Median 0.25, variance 7/288: 0.125 is kept ((0.125)^2 <= var),
+    0.5 rejected. Each plausible slip changes the answer: centring on the mean rejects 0.125,
+    comparing |v - median| with the variance rejects it too, comparing the square with the standard
+    deviation keeps 0.5, and pairing each value with its own ray's weight or dropping the weights
+    changes the weighted mean."""
+    w = {r: float(np.float32(z)) for r, z in CONE_Z.items()}
+    got = _reduce({5: 0.5, 9: 0.125, 17: 0.25}, 2.0)
+    want = (w[5] * 0.125 + w[9] * 0.25) / ((w[5] + w[9]) * 2.0)
+    assert got == want
+    assert got != (w[9] * 0.125 + w[17] * 0.25) / ((w[9] + w[17]) * 2.0)  # own-ray weights
+    assert got != (0.125 + 0.25) / 2 / 2.0                                   # unweighted
+
+
+def test_reduction_five_hits_one_outlier():
+    """Five hits, one far outlier (rays 0, 5, 9, 17, 26 at 1, 3, 2, 20, 4): median 3, mean 6,
+    variance 50, so 1, 2, 3, 4 are kept and 20 is dropped. The values are paired with the ray-order
+    weights in the order std::nth_element leaves them, taken from the host library itself."""
+    import os
+    import subprocess
+    import tempfile
+    rays = [0, 5, 9, 17, 26]
+    vals = [1.0, 3.0, 2.0, 20.0, 4.0]
+    src = ("#include <algorithm>\n#include <cstdio>\nint main(){double v[5]={1,3,2,20,4};"
+           "std::nth_element(v,v+2,v+5);for(double x:v)std::printf(\"%.17g \",x);return 0;}\n")
+    with tempfile.TemporaryDirectory() as d:
+        c, exe = os.path.join(d, "nth.cpp"), os.path.join(d, "nth")
+        with open(c, "w") as f:
+            f.write(src)
+        subprocess.run(["g++", "-O2", c, "-o", exe], check=True)
+        perm = [float(x) for x in subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()]
+    assert sorted(perm) == sorted(vals) and perm[2] == 3.0
+    w = [float(np.float32(CONE_Z[r])) for r in rays]
+    sv = sw = 0.0
+    for k in range(5):
+        if (perm[k] - 3.0) ** 2 <= 50.0:
+            sv += w[k] * perm[k]
+            sw += w[k]
+    want = sv / (sw * 8.0)
+    got = _reduce(dict(zip(rays, vals)), 8.0)
+    assert got == want
+    assert abs(got - 2.5 / 8.0) < 0.05  # a weighted mean of 1, 2, 3, 4 over the scale (below the clamp at 1)

@@ -103,13 +103,13 @@ void read_off_ascii(Reader &in, FileMesh &m) {
   const std::string counts_line = in.content_line("the counts");
   std::istringstream counts(counts_line);
   int64_t nv, nf, ne;
-  if (!(counts >> nv >> nf >> ne)) fail(THG_E_INVALID, "OFF: could not read mesh statistics on line '" + counts_line + "'");
+  if (!(counts >> nv >> nf >> ne)) fail(THG_E_INVALID, "OFF: expected vertex, face and edge counts, got '" + counts_line + "'");
   if (nv <= 0) return;  // skip_empty_meshes
   for (int64_t v = 0; v < nv; ++v) {
     const std::string l = in.content_line("a vertex");
     std::istringstream vs(l);
     double x, y, z;
-    if (!(vs >> x >> y >> z)) fail(THG_E_INVALID, "OFF: could not read vertex on line '" + l + "'");
+    if (!(vs >> x >> y >> z)) fail(THG_E_INVALID, "OFF: expected three vertex coordinates, got '" + l + "'");
     m.verts.push_back((float)x);
     m.verts.push_back((float)y);
     m.verts.push_back((float)z);
@@ -118,7 +118,7 @@ void read_off_ascii(Reader &in, FileMesh &m) {
     const std::string l = in.content_line("a face");
     std::istringstream fs(l);
     int64_t k;
-    if (!(fs >> k)) fail(THG_E_INVALID, "OFF: could not read number of vertices in face on line '" + l + "'");
+    if (!(fs >> k)) fail(THG_E_INVALID, "OFF: expected a face's vertex count, got '" + l + "'");
     if (k <= 0) continue;
     std::vector<int64_t> face;
     bool skip = false;
@@ -196,7 +196,7 @@ void read_obj(Reader &in, FileMesh &m) {
     if (c == 'v' && l.size() >= 2 && (l[1] == ' ' || l[1] == '\t')) {
       std::istringstream vs(l.substr(1));
       double x, y, z;
-      if (!(vs >> x >> y >> z)) fail(THG_E_INVALID, "OBJ: could not read vertex position on line '" + l + "'");
+      if (!(vs >> x >> y >> z)) fail(THG_E_INVALID, "OBJ: expected three coordinates after 'v', got '" + l + "'");
       m.verts.push_back((float)x);
       m.verts.push_back((float)y);
       m.verts.push_back((float)z);
