@@ -111,6 +111,21 @@ int thg_bvh_export(thg_bvh bvh, void *nodes, int32_t *elem_ids);
 int thg_closest_point(thg_bvh bvh, const float *q, int64_t n, double dist_bound, int32_t *idx, float *cp, float *dist,
                       float *d2, int32_t mem, void *stream);
 
+/* Largest batch thg_closest_point_noalloc accepts. */
+#define THG_NOALLOC_MAX_QUERIES 4194304
+
+/* thg_closest_point that allocates nothing: no device or host allocation, no HIP events, no
+ * memset. No counterpart in Thea; it is for callers that cannot allocate: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc. Same arguments and results as
+ * thg_closest_point with mem = THG_MEM_DEVICE (all pointers device memory, asynchronous on
+ * `stream`). The spatial sort of the queries uses a fixed per-device array of the library
+ * (68.2 MB, held until the process ends), so n is limited to THG_NOALLOC_MAX_QUERIES: larger n
+ * fails with THG_E_INVALID (split the batch). Every call on one device shares that array: calls
+ * that overlap on one device (other streams, other threads) race and must be serialised by the
+ * caller; calls in order on one stream are safe. A context must have been created on the device. */
+int thg_closest_point_noalloc(thg_bvh bvh, const float *q, int64_t n, double dist_bound, int32_t *idx, float *cp,
+                              float *dist, float *d2, void *stream);
+
 /* k nearest elements per query, sorted by (distance, index) ascending; rows shorter than k are
  * padded with index -1 and distance -1. Replaces BvhN::kClosestPairs<MetricL2>
  * (Algorithms/BvhN.hpp:1139-1146) with a BoundedSortedArray of capacity k.

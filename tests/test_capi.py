@@ -39,6 +39,16 @@ def test_library_exports_every_declared_symbol():
 def test_python_binding_covers_header():
     from thea_amd import _capi
     assert set(declared_functions()) == set(_capi.SIGNATURES)
+    m = re.search(r"#define THG_NOALLOC_MAX_QUERIES (\d+)", open(HEADER).read())
+    assert m and int(m.group(1)) == _capi.THG_NOALLOC_MAX_QUERIES
+
+
+def test_noalloc_entry_point_checks_arguments_first():
+    """The no-allocation entry point rejects a NULL handle and a negative count before any device work."""
+    from thea_amd import _capi
+    lib = _capi.lib()
+    assert lib.thg_closest_point_noalloc(None, None, 0, -1.0, None, None, None, None, None) == _capi.THG_E_INVALID
+    assert b"NULL" in lib.thg_last_error()
 
 
 def test_null_handles_are_invalid_arguments():

@@ -18,6 +18,7 @@ THG_E_UNSUPPORTED = -5
 THG_E_CANCELLED = -6
 THG_MEM_DEVICE = 0
 THG_MEM_HOST = 1
+THG_NOALLOC_MAX_QUERIES = 4194304  # include/thea_gpu.h
 
 _ERR_NAMES = {
     THG_E_INVALID: "THG_E_INVALID", THG_E_OOM: "THG_E_OOM", THG_E_DEVICE: "THG_E_DEVICE",
@@ -87,6 +88,7 @@ SIGNATURES = {
     "thg_bvh_export": (C.c_int, [vp, vp, vp]),
     "thg_bvh_bounds": (C.c_int, [vp, vp, vp]),
     "thg_closest_point": (C.c_int, [vp, vp, i64, f64, vp, vp, vp, vp, i32, vp]),
+    "thg_closest_point_noalloc": (C.c_int, [vp, vp, i64, f64, vp, vp, vp, vp, vp]),
     "thg_k_closest": (C.c_int, [vp, vp, i64, i32, f64, i32, vp, vp, vp, i32, vp]),
     "thg_ray_first_hit": (C.c_int, [vp, vp, vp, i64, f32, vp, vp, vp, i32, vp]),
     "thg_ray_any_hit": (C.c_int, [vp, vp, vp, i64, f32, vp, i32, vp]),

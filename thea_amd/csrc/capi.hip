@@ -62,6 +62,39 @@ void check_tri_indices(const int32_t *tris, int64_t nt, int64_t nv, hipStream_t 
   require(hb == 0, "triangle vertex index out of range");
 }
 
+// Sort memory of thg_closest_point_noalloc: the spatial order of a batch of up to
+// THG_NOALLOC_MAX_QUERIES queries (keys, values and their ping-pong copies, 4 words per query,
+// then the radix histograms: 256 words per 4096-key tile plus 256 totals), as
+// sort_scratch_words(THG_NOALLOC_MAX_QUERIES) counts them. Every word the sort reads is written
+// earlier in the same call, so nothing needs clearing between calls. 68.2 MB per device, held from
+// the library's load on that device until the process ends.
+constexpr size_t NOALLOC_WORDS = 4 * (size_t)THG_NOALLOC_MAX_QUERIES +
+                                 256 * (((size_t)THG_NOALLOC_MAX_QUERIES + 4095) / 4096) + 256;
+static __device__ __align__(256) uint32_t scratch[NOALLOC_WORDS];
+
+constexpr int NOALLOC_DEVICES = 64;
+static std::mutex g_noalloc_mu;
+static uint32_t *g_noalloc_ptr[NOALLOC_DEVICES];  // each device's own address of `scratch`
+
+// Looks up the current device's copy of `scratch` once per device (thg_ctx_create calls it with
+// `device` current). Host code uses only the address found here, never the array's name.
+static void noalloc_init(int device) {
+  require(device >= 0 && device < NOALLOC_DEVICES, "device index beyond the no-allocation table");
+  std::lock_guard<std::mutex> g(g_noalloc_mu);
+  if (g_noalloc_ptr[device]) return;
+  void *p = nullptr;
+  THG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  require(p != nullptr, "no device address for the no-allocation sort memory");
+  g_noalloc_ptr[device] = static_cast<uint32_t *>(p);
+}
+
+static uint32_t *noalloc_ptr(int device) {
+  std::lock_guard<std::mutex> g(g_noalloc_mu);
+  uint32_t *p = device >= 0 && device < NOALLOC_DEVICES ? g_noalloc_ptr[device] : nullptr;
+  require(p != nullptr, "no-allocation sort memory not set up on this device (create a context first)");
+  return p;
+}
+
 }  // namespace thg
 
 using namespace thg;
@@ -88,6 +121,7 @@ int thg_ctx_create(int32_t device, thg_ctx *out) {
       (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
     }
     (void)hipGetLastError();
+    noalloc_init(device);
     auto *c = new Ctx();
     c->device = device;
     c->num_cus = device_cus(device);
@@ -313,6 +347,34 @@ int thg_closest_point(thg_bvh bvh, const float *q, int64_t n, double dist_bound,
       launch_closest(*b, dq, n, dist_bound, 0, didx, dcp, ddist, dd2, ws, s);
     }
     st.finish();
+  });
+}
+
+int thg_closest_point_noalloc(thg_bvh bvh, const float *q, int64_t n, double dist_bound, int32_t *idx, float *cp,
+                              float *dist, float *d2, void *stream) {
+  return guarded([&] {
+    require(bvh != nullptr, "bvh is NULL");
+    require(n >= 0, "negative query count");
+    require(n <= THG_NOALLOC_MAX_QUERIES,
+            "n = " + std::to_string(n) + " exceeds THG_NOALLOC_MAX_QUERIES (" +
+                std::to_string((int64_t)THG_NOALLOC_MAX_QUERIES) + "); split the batch");
+    require(n == 0 || (q && idx), "q/idx is NULL");
+    require(!std::isnan(dist_bound), "dist_bound is NaN");
+    static_assert(NOALLOC_WORDS >= 4 * (size_t)THG_NOALLOC_MAX_QUERIES + 256, "sort memory too small");
+    Bvh *b = reinterpret_cast<Bvh *>(bvh);
+    set_device(b->ctx);
+    require(sort_scratch_words(THG_NOALLOC_MAX_QUERIES) <= NOALLOC_WORDS, "sort memory smaller than the sort needs");
+    hipStream_t s = as_stream(stream);
+    if (n == 0) return;
+    if (b->n == 0) {
+      fill_missing(n, 1, idx, dist, d2, cp, s);
+      return;
+    }
+    QueryScratch ws;
+    ws.ext = noalloc_ptr(b->ctx->device);
+    ws.ext_cap = THG_NOALLOC_MAX_QUERIES;
+    ws.timed = false;  // timing events are host objects created per launch; this path creates none
+    launch_closest(*b, q, n, dist_bound, 0, idx, cp, dist, d2, ws, s);
   });
 }
 
