@@ -111,6 +111,23 @@ int thg_bvh_export(thg_bvh bvh, void *nodes, int32_t *elem_ids);
 int thg_closest_point(thg_bvh bvh, const float *q, int64_t n, double dist_bound, int32_t *idx, float *cp, float *dist,
                       float *d2, int32_t mem, void *stream);
 
+/* Point BVH over double coordinates, for callers of BvhN<Vector3d, 3, double> over points (the
+ * target structure of ShapeAlign's ICP, Tools/ShapeAlign/ShapeAlign.cpp:339; built by BvhN::init,
+ * Algorithms/BvhN.hpp:705-794). `xyz` = n double3. Queries on it go through thg_closest_point_f64
+ * (thg_closest_point and thg_k_closest also accept it and then search the points rounded to float). */
+int thg_bvh_build_points_f64(thg_ctx ctx, const double *xyz, int64_t n, int32_t mem, const thg_build_opts *opts,
+                             void *stream, thg_bvh *out);
+
+/* Nearest point in double precision on a thg_bvh_build_points_f64 BVH. Replaces
+ * BvhN<Vector3d, 3, double>::closestElement<MetricL2>(query, dist_bound, compat, &dist, &closest_point)
+ * (Algorithms/BvhN.hpp:1050-1063; point metric MetricL2.hpp:176-193) as Icp3<double> calls it
+ * (Algorithms/Icp3.hpp:282-296,388-404). q: n double3. The answer is the lowest-index point of least
+ * double squared distance ((dx^2 + dy^2) + dz^2); only points with distance < dist_bound count
+ * (negative = unbounded). idx[i]: point index or -1. cp (nullable): that point's double
+ * coordinates. dist (nullable): sqrt(d^2) in double. d2 (nullable): the double squared distance. */
+int thg_closest_point_f64(thg_bvh bvh, const double *q, int64_t n, double dist_bound, int32_t *idx, double *cp,
+                          double *dist, double *d2, int32_t mem, void *stream);
+
 /* Largest batch thg_closest_point_noalloc accepts. */
 #define THG_NOALLOC_MAX_QUERIES 4194304
 
@@ -188,6 +205,13 @@ int thg_ctx_kernel_time(thg_ctx ctx, const char *kernel, double *total_ms, int64
  * compacted kernel). */
 int thg_closest_stats(thg_bvh bvh, const float *q, int64_t n, void *stream, int64_t out[8]);
 
+/* Per-wave statistics of the 16-nearest-neighbour wave kernel on a point BVH (diagnostics; no
+ * counterpart in Thea): n queries (device pointer), 64 per wave in spatial order; per_wave (device,
+ * 4 int64 per wave) = elapsed wall clock in 10 ns ticks, nodes entered, points tested, insertion
+ * rounds. Asynchronous on `stream`. */
+int thg_knn_wave_stats(thg_bvh bvh, const float *q, int64_t n, int32_t exclude_self, void *stream,
+                       int64_t *per_wave);
+
 /* Per-ray traversal counts of the first-hit kernel (diagnostics), rays in input order (device
  * pointers, asynchronous on `stream`): idx[i] as thg_ray_first_hit, counts[4i..4i+3] = nodes
  * visited, triangles tested, deepest traversal stack, stack pops. */
@@ -212,11 +236,26 @@ int thg_batch_plan_run(thg_batch_plan plan, const float *verts, const int32_t *t
                        int32_t *idx, float *dist, void *stream, double timings_ms[2]);
 int thg_batch_plan_destroy(thg_batch_plan plan);
 /* Leaf size per mesh chosen by measurement: for each size class (floor(log2(triangles))) the
- * closest-point kernel is timed on up to two of its meshes for leaf sizes 2..16 and the fastest is
- * kept (cached for the process). Same inputs as thg_batch_closest; writes leaf_sizes[nm]. */
+ * plan's closest-point kernel is timed on up to two of its meshes for leaf sizes 2..16 and the
+ * fastest is kept (cached for the process). Same inputs as thg_batch_closest; writes leaf_sizes[nm]. */
 int thg_autotune_leaf(thg_ctx ctx, int32_t nm, const int64_t *nv, const int64_t *nt, const int64_t *nq,
                       const float *verts, const int32_t *tris, const float *queries, int32_t *leaf_sizes);
-/* One-shot form: autotunes leaf sizes when leaf_size == 0, creates a plan, runs it, frees it. */
+/* Leaf size and child visiting order per mesh, and the launch shape of the plan, chosen by
+ * measurement. Per size class, as thg_autotune_leaf, then at that leaf size the child order
+ * (child_orders[m]: 0 = the child most of a wave's queries find nearer, 1 = the child with the least
+ * lower bound over the wave's queries, 2 = the build's order; cached per class). Then launch[0] =
+ * waves per workgroup of the query launch (1, 2, 4, 8) and launch[1] = 1 for the build compiled for 8
+ * resident waves per SIMD (0: the spill-free one), the fastest summed over the sampled meshes.
+ * Results never depend on these choices, only the time does. No counterpart in Thea. */
+int thg_autotune_plan(thg_ctx ctx, int32_t nm, const int64_t *nv, const int64_t *nt, const int64_t *nq,
+                      const float *verts, const int32_t *tris, const float *queries, int32_t *leaf_sizes,
+                      int32_t *child_orders, int32_t launch[2]);
+/* Child visiting orders (nullable: 0 for every mesh) and launch shape of a plan's query launch, as
+ * thg_autotune_plan chooses them; the next run captures its graph again. */
+int thg_batch_plan_set_launch(thg_batch_plan plan, const int32_t *child_orders, int32_t block_waves,
+                              int32_t spill_build);
+/* One-shot form: autotunes leaf sizes, child orders and launch shape when leaf_size == 0, creates a
+ * plan, runs it, frees it. */
 int thg_batch_closest(thg_ctx ctx, int32_t nm, const int64_t *nv, const int64_t *nt, const int64_t *nq,
                       const float *verts, const int32_t *tris, const float *queries, int32_t leaf_size,
                       int32_t *idx, float *dist, void *stream, double timings_ms[2]);
@@ -253,9 +292,9 @@ int thg_icp_align(thg_bvh to, const double *from, int64_t n, int32_t num_problem
                   int32_t *iterations);
 
 /* ---------------------------------------------------------------------------------------------
- * Sample proximity graph. Replaces PointSet3::updateGraph() for samples without oversampling and
- * without a surface (Algorithms/PointSet3.hpp:324-386, :586-697), the graph Tools/SampleGraph
- * builds (Tools/SampleGraph/SampleGraph.cpp:329-334).
+ * Sample proximity graph. Replaces PointSet3::updateGraph() without a surface, with or without
+ * oversampling (Algorithms/PointSet3.hpp:324-386, :586-697; PointSet3.cpp:158-178), the graph
+ * Tools/SampleGraph builds (Tools/SampleGraph/SampleGraph.cpp:296-334).
  * --------------------------------------------------------------------------------------------- */
 typedef struct thg_graph_opts {
   int32_t max_degree;        /* PointSet3::Options::max_degree, 1..32; <= 0: 8 */
@@ -263,14 +302,26 @@ typedef struct thg_graph_opts {
   uint64_t seed;             /* picks of those samples: pick t = (w * n) >> 32 for the first word w of
                                 Philox4x32-10 at counter (t, 0, 0x47524150, 0), key = seed (Thea picks
                                 with its time-seeded Random::common()) */
-  int64_t reserved[4];
+  int64_t num_original;      /* thg_proximity_graph: the first num_original points are the samples and
+                                the rest their oversampling (PointSet3::addOversampling); <= 0: all
+                                points are samples. Ignored elsewhere. */
+  thg_bvh surface;           /* thg_proximity_graph (nullable): a mesh BVH for the reachability test of
+                                PointSet3::updateGraph(surface) (PointSet3.hpp:633-641; SampleGraph -r);
+                                needs normals. Ignored elsewhere. */
+  int64_t reserved[2];
 } thg_graph_opts;
 
 /* points: a point BVH over the samples (thg_bvh_build_points); normals (nullable): n float3 in
- * `mem`. Writes, in `mem` memory, nbr_idx[n * max_degree] (neighbours sorted by separation, then
- * index; -1 padded), nbr_sep (nullable, same layout; separations are Thea's fastSqrt of the float
- * squared distance, -1 padded), degree (nullable, n). separation (nullable, host, 2 floats): the
- * initial RMS separation estimate and the final average edge length (getAverageSeparation). */
+ * `mem`. With S = n samples, writes, in `mem` memory, nbr_idx[S * max_degree] (neighbours sorted by
+ * separation, then index; -1 padded), nbr_sep (nullable, same layout; separations are Thea's
+ * fastSqrt of the float squared distance, -1 padded), degree (nullable, S). With oversampling
+ * (opts->num_original = S < n, the SampleGraph tool's case below 50,000 samples) the graph is built
+ * over all n points and each sample's neighbours are the max_degree nearest other samples by
+ * shortest-path distance through it (PointSet3::extractOriginalAdjacencies, PointSet3.cpp:158-178):
+ * S rows, separations = float(path length); exact ties in path length are "parity unpinned" (Thea's
+ * Fibonacci heap orders them). separation (nullable, host, 2 floats): the initial RMS separation
+ * estimate (over all n points) and the final average edge length of the S samples
+ * (getAverageSeparation). */
 int thg_proximity_graph(thg_bvh points, const float *normals, const thg_graph_opts *opts, int32_t mem, void *stream,
                         int32_t *nbr_idx, float *nbr_sep, int32_t *degree, float separation[2]);
 

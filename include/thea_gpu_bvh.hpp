@@ -384,6 +384,44 @@ class PointBvh : public BvhBase {
   std::vector<float> xyz_;
 };
 
+// Thea's BvhN<Vector3d, 3, double> over a point set (the target of ShapeAlign's ICP,
+// Tools/ShapeAlign/ShapeAlign.cpp:339): closestElement / closestElements search in double. The
+// float methods of BvhBase also work on it and search the points rounded to float.
+class PointBvhD : public BvhBase {
+ public:
+  PointBvhD(Context &ctx, const double *xyz, intx n, int max_elems_in_leaf = 4) : n_(n > 0 ? n : 0) {
+    thg_build_opts o = {};
+    o.leaf_size = max_elems_in_leaf;
+    check(thg_bvh_build_points_f64(ctx.handle(), xyz, n, THG_MEM_HOST, &o, nullptr, &h_));
+  }
+  intx numElements() const { return n_; }
+
+  // Batch: idx[n], optional dist[n] and closest points cp[3n], all double.
+  void closestElements(intx n, const double *q, std::int32_t *idx, double *dist = nullptr, double *cp = nullptr,
+                       double dist_bound = -1) const {
+    check(thg_closest_point_f64(h_, q, n, dist_bound, idx, cp, dist, nullptr, THG_MEM_HOST, nullptr));
+  }
+
+  // BvhN<Vector3d>::closestElement (BvhN.hpp:1050-1063): index of the nearest point or -1.
+  template <typename MetricT = MetricL2, typename QueryT, typename CompatT = UniversalCompatibility,
+            typename VectorT = QueryT>
+  intx closestElement(QueryT const &query, double dist_bound = -1, CompatT = CompatT(), double *dist = nullptr,
+                      VectorT *closest_point = nullptr) const {
+    const double q[3] = {(double)query[0], (double)query[1], (double)query[2]};
+    double cp[3], d;
+    std::int32_t idx;
+    closestElements(1, q, &idx, &d, cp, dist_bound);
+    if (idx >= 0) {
+      if (dist) *dist = d;
+      if (closest_point) { (*closest_point)[0] = cp[0]; (*closest_point)[1] = cp[1]; (*closest_point)[2] = cp[2]; }
+    }
+    return idx;
+  }
+
+ private:
+  intx n_;
+};
+
 template <typename MetricT, typename CompatT>
 NeighborPair BvhBase::closestPair(PointBvh const &query, double dist_bound, CompatT, bool get_closest_points) const {
   const intx n = query.numElements();

@@ -3,7 +3,9 @@
 Only tests/ and bench.py's cpu_baseline leg may import this module, and only as the checker.
 
 The nearest points come from the oracle's kd-tree (canonical mode) over the target, queried with the
-source positions rounded to float32, as the GPU path does; every other quantity is double. The 3x3
+source positions rounded to float32, as the GPU path does on a float BVH, or (f64=True) from the
+oracle's double brute force with the positions in double, as on a double point BVH; every other
+quantity is double. The 3x3
 fit uses numpy's SVD (LAPACK) for U V^T, an algorithm independent of the library's Jacobi sweeps.
 Sums are numpy's (pairwise), so results agree with the GPU to rounding, not bit for bit.
 """
@@ -70,14 +72,17 @@ def measure_error(tr, p, q, wt=None, wr=None):
     return err + float((e if wr is None else wr * wr * e).sum())
 
 
-def icp_align(nearest, src, frac_thr=0.0001, min_it=3, max_it=10, wt=None, wr=None, up=None, kabsch=False):
-    """Icp3::align (Icp3.hpp:193-263). nearest(points float32 (n, 3)) -> closest points (n, 3); src:
-    the source points (double). Returns (transform (3, 4), error, iterations)."""
+def icp_align(nearest, src, frac_thr=0.0001, min_it=3, max_it=10, wt=None, wr=None, up=None, kabsch=False,
+              f64=False):
+    """Icp3::align (Icp3.hpp:193-263). nearest(points (n, 3)) -> closest points (n, 3), given the
+    positions rounded to float32, or in double with f64=True (Icp3<double> over a BvhN<Vector3d>,
+    ShapeAlign.cpp:339); src: the source points (double). Returns (transform (3, 4), error,
+    iterations)."""
     if up is not None:
         up = np.asarray(up, np.float64) / np.linalg.norm(up)
 
     def mapping(pts):
-        return nearest(pts.astype(np.float32)).astype(np.float64)
+        return np.asarray(nearest(pts if f64 else pts.astype(np.float32)), np.float64)
 
     from_points = src.copy()
     old_tr = identity()

@@ -49,6 +49,8 @@ def lib():
         "or_kd_ray": (None, [vp, f32p, f32p, i64, f32, i32, f32, i32, i32p, f32p, vp]),
         "or_brute_closest": (None, [f32p, i32p, i64, f32p, i64, f64, i32, i32p, vp, vp]),
         "or_brute_closest_f64": (None, [f32p, i32p, i64, f32p, i64, i32, i32p, f64p]),
+        "or_brute_closest_points_f64": (None, [f64p, i64, f64p, i64, f64, i32, i32p, vp]),
+        "or_extract_original": (None, [i32p, f32p, i32p, i64, i32, i64, i32, i32p, f32p, i32p]),
         "or_dist_f64": (None, [f32p, i32p, f32p, i32p, i64, f64p]),
         "or_d2_to": (None, [f32p, i32p, f32p, i32p, i64, f32p]),
         "or_brute_knn": (None, [f32p, i64, f32p, i64, i32, f64, i32, i32, i32p, f32p]),
@@ -63,7 +65,7 @@ def lib():
         "or_fast_sqrt": (None, [f32p, i64, f32p]),
         "or_sdf_rays": (None, [f32p, f32p, i64, f32, f32p, f32p]),
         "or_sdf_reduce": (None, [f32p, i64, f64, f64p]),
-        "or_proximity_graph": (i32, [f32p, vp, i64, i32, i64p, i64, i32p, f32p, i32p, f32p]),
+        "or_proximity_graph": (i32, [f32p, vp, i64, i32, i64p, i64, i32p, f32p, i32p, f32p, vp]),
         "or_sample_area_approx": (i64, [f32p, i32p, i64, i64, u64, u64, i64, f32p, vp, vp]),
         "or_fps": (i64, [i32p, f32p, i64, i32, i64, i64p]),
         "or_vis_rays": (None, [f32p, i64, i32, f32, u64, f32p, f32p]),
@@ -187,6 +189,17 @@ def brute_closest_f64(verts, tris, q, nthreads=None):
     return idx, dist
 
 
+def brute_closest_points_f64(points, q, dist_bound=-1.0, nthreads=None):
+    """Nearest point in double precision (lowest index on exact ties): (idx int32, d2 float64)."""
+    pts = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+    q = np.ascontiguousarray(q, np.float64).reshape(-1, 3)
+    idx = np.zeros(len(q), np.int32)
+    d2 = np.zeros(len(q), np.float64)
+    lib().or_brute_closest_points_f64(pts, len(pts), q, len(q), float(dist_bound), nthreads or default_threads(), idx,
+                                      _ptr(d2))
+    return idx, d2
+
+
 def dist_f64(verts, tris, q, which):
     verts, tris, q, which = _f32(verts), _i32(tris), _f32(q).reshape(-1, 3), _i32(which)
     out = np.zeros(len(q), np.float64)
@@ -302,8 +315,9 @@ def graph_picks(n, trials, seed):
                     np.int64)
 
 
-def proximity_graph(points, normals=None, max_degree=8, picks=None):
-    """PointSet3::updateGraph restated (see thea_oracle.cpp): (idx, sep, degree, (initial, final avg))."""
+def proximity_graph(points, normals=None, max_degree=8, picks=None, surface=None):
+    """PointSet3::updateGraph restated (see thea_oracle.cpp): (idx, sep, degree, (initial, final avg)).
+    surface: a mesh KdTree for the reachability test (used with normals)."""
     pts = _f32(points).reshape(-1, 3)
     n = len(pts)
     nrm = None if normals is None else _f32(normals).reshape(-1, 3)
@@ -313,10 +327,25 @@ def proximity_graph(points, normals=None, max_degree=8, picks=None):
     deg = np.zeros(n, np.int32)
     avg = np.zeros(2, np.float32)
     rc = lib().or_proximity_graph(pts, None if nrm is None else nrm.ctypes.data_as(C.c_void_p), n, max_degree, picks,
-                                  len(picks), idx, sep, deg, avg)
+                                  len(picks), idx, sep, deg, avg, None if surface is None else surface.h)
     if rc != 0:
         raise ValueError("or_proximity_graph failed")
     return idx, sep, deg, (float(avg[0]), float(avg[1]))
+
+
+def extract_original(nbr_idx, nbr_sep, degree, num_original, nthreads=None):
+    """PointSet3::extractOriginalAdjacencies restated (see or_extract_original): the graph of all n
+    samples (n x md neighbour indices and separations, degrees) -> for the first num_original samples,
+    (idx (S, md), sep (S, md) float32, degree (S,))."""
+    nbr_idx = _i32(nbr_idx)
+    n, md = nbr_idx.shape
+    S = int(num_original)
+    idx = np.zeros((S, md), np.int32)
+    sep = np.zeros((S, md), np.float32)
+    deg = np.zeros(S, np.int32)
+    lib().or_extract_original(nbr_idx, _f32(nbr_sep), _i32(degree), n, md, S, nthreads or default_threads(), idx, sep,
+                              deg)
+    return idx, sep, deg
 
 
 def vis_rays(positions, num_rays, scale, seed):

@@ -1016,6 +1016,33 @@ void or_brute_closest(const float *verts, const int32_t *tris, int64_t nt, const
   });
 }
 
+// Nearest point in double precision, brute force: BvhN<Vector3d, 3, double>::closestElement over
+// points (BvhN.hpp:1050-1063) with MetricL2's point distance (MetricL2.hpp:176-193), as
+// Icp3<double> queries ShapeAlign's target points (Icp3.hpp:282-296, ShapeAlign.cpp:339). d^2 is
+// (dx^2 + dy^2) + dz^2 (Eigen 3.4 reduces a Vector3d with a two-double packet and adds the third
+// term last); a point is admitted only if d^2 < dist_bound^2 (ClosestPairFunctor, BvhN.hpp:441-466);
+// exact ties go to the lowest index.
+void or_brute_closest_points_f64(const double *pts, int64_t n, const double *q, int64_t nq, double dist_bound,
+                                 int nthreads, int32_t *idx, double *d2) {
+  const double bound2 = dist_bound >= 0 ? dist_bound * dist_bound : -1;
+  parallel_for(nq, nthreads, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) {
+      const double qx = q[3 * i], qy = q[3 * i + 1], qz = q[3 * i + 2];
+      int64_t bi = -1;
+      double bd = 0;
+      for (int64_t j = 0; j < n; ++j) {
+        const double dx = pts[3 * j] - qx, dy = pts[3 * j + 1] - qy, dz = pts[3 * j + 2] - qz;
+        const double d = (dx * dx + dy * dy) + dz * dz;
+        if (!(d == d)) continue;
+        if (bound2 >= 0 && !(d < bound2)) continue;
+        if (bi < 0 || d < bd) { bi = j; bd = d; }
+      }
+      idx[i] = (int32_t)bi;
+      if (d2) d2[i] = bi >= 0 ? bd : -1.0;
+    }
+  });
+}
+
 // FP64 brute force closest distance (Ericson) -> exact-ish distance and its argmin.
 void or_brute_closest_f64(const float *verts, const int32_t *tris, int64_t nt, const float *q, int64_t nq, int nthreads,
                           int32_t *idx, double *dist) {
@@ -1221,8 +1248,67 @@ void or_fast_sqrt(const float *x, int64_t n, float *out) {
 // Writes idx/sep (n x max_degree, -1 / -1 padded, sorted), deg[n], avg[0] the initial RMS
 // separation estimate and avg[1] the final average edge length (updateAverageSeparation,
 // PointSet3.hpp:683-697).
+// PointSet3::extractOriginalAdjacencies (PointSet3.cpp:158-178): for each original sample i < S,
+// Dijkstra from i over the directed graph of all n samples (out-edges = a sample's neighbour list,
+// weights = its separations, summed in double; SamplePointerGraph, PointSet3.cpp:91-136), and the
+// callback (DijkstraCallback, :140-156) inserting every finalised original sample other than i into
+// a bounded set of max_degree (float(distance), index) pairs until it is full. Finalisation order is
+// (distance, index) here: Thea's heap decides equal distances, which no test can pin. out: S x md
+// indices (-1 padded), separations (float), degrees.
+void or_extract_original(const int32_t *nbr_idx, const float *nbr_sep, const int32_t *deg, int64_t n, int md,
+                         int64_t S, int nthreads, int32_t *out_idx, float *out_sep, int32_t *out_deg) {
+  parallel_for(S, nthreads, [&](int64_t b, int64_t e) {
+    std::vector<double> dist(n, INFINITY);
+    std::vector<char> done(n, 0);
+    std::vector<int64_t> touched;
+    typedef std::pair<double, int64_t> Item;
+    for (int64_t src = b; src < e; ++src) {
+      std::priority_queue<Item, std::vector<Item>, std::greater<Item>> pq;
+      std::vector<std::pair<float, int64_t>> set;
+      dist[src] = 0;
+      touched.push_back(src);
+      pq.push({0.0, src});
+      while (!pq.empty()) {
+        const Item it = pq.top();
+        pq.pop();
+        const int64_t v = it.second;
+        if (done[v] || it.first > dist[v]) continue;
+        done[v] = 1;
+        if (v != src && v < S) {
+          set.push_back({(float)it.first, v});
+          if ((int)set.size() >= md) break;
+        }
+        for (int j = 0; j < deg[v]; ++j) {
+          const int64_t u = nbr_idx[(size_t)v * md + j];
+          const double nd = it.first + (double)nbr_sep[(size_t)v * md + j];
+          if (!done[u] && nd < dist[u]) {
+            if (dist[u] == INFINITY) touched.push_back(u);
+            dist[u] = nd;
+            pq.push({nd, u});
+          }
+        }
+      }
+      std::sort(set.begin(), set.end());  // the bounded set's order: (separation, sample address = index)
+      for (int j = 0; j < md; ++j) {
+        out_idx[(size_t)src * md + j] = j < (int)set.size() ? (int32_t)set[j].second : -1;
+        out_sep[(size_t)src * md + j] = j < (int)set.size() ? set[j].first : -1.0f;
+      }
+      out_deg[src] = (int32_t)set.size();
+      for (int64_t t : touched) { dist[t] = INFINITY; done[t] = 0; }
+      touched.clear();
+    }
+  });
+}
+
+// surface (nullable, a mesh KdTree): the reachability test of NeighborFunctor::isValidNeighbor
+// (PointSet3.hpp:633-641), applied when normals are given: a candidate is rejected if the ray from
+// the sample lifted by LIFT_FACTOR (5) * separation along normalize(n_i + n_j), in the direction of
+// the candidate (p_j - p_i), hits the surface before time 1 (BvhN::rayIntersects(ray, 1): a hit
+// strictly before max_time, BvhN.hpp:1266-1289,2189-2235).
 int or_proximity_graph(const float *pts, const float *nrm, int64_t n, int max_degree, const int64_t *picks,
-                       int64_t ntrials, int32_t *out_idx, float *out_sep, int32_t *out_deg, float *avg) {
+                       int64_t ntrials, int32_t *out_idx, float *out_sep, int32_t *out_deg, float *avg,
+                       const void *surface) {
+  const KdTree *surf = (const KdTree *)surface;
   if (n <= 1 || max_degree <= 0) return -1;
   std::vector<V3> P(n), N;
   for (int64_t i = 0; i < n; ++i) P[i] = ld3(pts + 3 * i);
@@ -1284,7 +1370,15 @@ int or_proximity_graph(const float *pts, const float *nrm, int64_t n, int max_de
               const float d2 = sqn(d);
               if (!(d2 <= r2)) continue;
               if (nrm && dot(N[i], N[j]) < -0.5f) continue;
-              set.emplace_back(fast_sqrt(d2), j);
+              const float sep = fast_sqrt(d2);
+              if (surf && nrm) {
+                const float LIFT_FACTOR = 5;
+                const V3 lift = normalized(add(N[i], N[j]));
+                RayQuery rq{surf, add(P[i], mul(LIFT_FACTOR * sep, lift)), d, 1.0f, 1, 0.0f};
+                rq.run();
+                if (rq.best_idx >= 0) continue;
+              }
+              set.emplace_back(sep, j);
             }
           }
       std::sort(set.begin(), set.end());

@@ -129,6 +129,20 @@ int main(int argc, char **argv) {
     double err = -1;
     Icp3::Transform tr = icp.align((intx)moved.size(), moved.data(), bvh, &err);
     std::printf("icp %.6f %.6f %.6f %.3g\n", tr[3], tr[7], tr[11], err);
+    // ShapeAlign's form (ShapeAlign.cpp:339,448): a double point BVH of the target samples and
+    // Icp3<double> with Thea's defaults
+    std::vector<double> target(3 * surf.size());
+    for (size_t i = 0; i < surf.size(); ++i)
+      for (int c = 0; c < 3; ++c) target[3 * i + c] = surf[i][c];
+    PointBvhD to_bvh(ctx, target.data(), (intx)surf.size());
+    std::array<double, 3> q0{{0.3, 0.2, 1.5}}, c0{};
+    double d0 = -1;
+    intx i0 = to_bvh.closestElement<MetricL2>(q0, -1, UniversalCompatibility(), &d0, &c0);
+    std::printf("closest64 %lld %.17g %.17g %.17g %.17g\n", (long long)i0, d0, c0[0], c0[1], c0[2]);
+    Icp3 icp_d;
+    double err_d = -1;
+    Icp3::Transform tr_d = icp_d.align((intx)moved.size(), moved.data(), to_bvh, &err_d);
+    std::printf("icp64 %.17g %.17g %.17g %.17g\n", tr_d[3], tr_d[7], tr_d[11], err_d);
     // Shape diameter at the first samples (ShapeDiameter over a precomputed BVH, default scale)
     ShapeDiameter sdf(&bvh);
     std::printf("scale %.9g\n", sdf.getScale());

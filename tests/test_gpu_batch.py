@@ -102,13 +102,14 @@ def test_c4_full_size_oracle(oracle):
 
     import torch
     from thea_amd import workloads as W
-    from thea_amd.batch import BatchPlan, autotune_leaf, concat
+    from thea_amd.batch import BatchPlan, autotune_plan, concat
     meshes, queries = W.many_meshes_c4()
     nv, nt, nq, verts, tris, qs = concat(meshes, queries)
     print(f"case C4 full size: {len(nt)} meshes, {int(nt.sum())} tris, {int(nq.sum())} queries", flush=True)
     V, T, Q = (torch.from_numpy(x).cuda() for x in (verts, tris, qs))
-    leaf = autotune_leaf(nv, nt, nq, V, T, Q)
-    plan = BatchPlan(nv, nt, nq, leaf_sizes=leaf)
+    leaf, order, launch = autotune_plan(nv, nt, nq, V, T, Q)
+    assert set(np.unique(order)) <= {0, 1, 2} and launch[0] in (1, 2, 4, 8) and launch[1] in (0, 1)
+    plan = BatchPlan(nv, nt, nq, leaf_sizes=leaf, child_orders=order, launch=launch)
     plan.run(V, T, Q)  # captures the graph
     idx, dist = plan.run(V, T, Q)  # replays it
     torch.cuda.synchronize()
@@ -132,7 +133,35 @@ def test_c4_full_size_oracle(oracle):
     with ThreadPoolExecutor(max_workers=8) as ex:
         done = list(ex.map(check, range(len(meshes))))
     assert len(done) == 512
-    print(f"C4: {len(done)} meshes checked; leaf sizes {dict(zip(*np.unique(leaf, return_counts=True)))}")
+    print(f"C4: {len(done)} meshes checked; leaf sizes {dict(zip(*np.unique(leaf, return_counts=True)))}, "
+          f"child orders {dict(zip(*np.unique(order, return_counts=True)))}, launch {launch}")
+
+
+def test_child_orders_and_launch_shapes_change_no_result():
+    """Every child visiting order and query launch shape the tuner may pick gives the same bits."""
+    import torch
+    from thea_amd.batch import BatchPlan, concat
+    sizes = [17, 1000, 5000, 20000, 60000, 300]
+    qsizes = [1000, 5000, 20000, 30000, 40000, 1]
+    print("case batch orders and shapes", flush=True)
+    meshes, queries = _batch(3, sizes, qsizes)
+    nv, nt, nq, verts, tris, qs = concat(meshes, queries)
+    V, T, Q = (torch.from_numpy(x).cuda() for x in (verts, tris, qs))
+    plan = BatchPlan(nv, nt, nq, leaf_sizes=np.full(len(nt), 8, np.int32))
+    ref = None
+    for order in (0, 1, 2):
+        for bw, spill in ((1, 0), (2, 1), (4, 1), (8, 0), (8, 1)):
+            plan.set_launch(np.full(len(nt), order, np.int32), bw, spill)
+            plan.run(V, T, Q)
+            idx, dist = plan.run(V, T, Q)
+            torch.cuda.synchronize()
+            got = (idx.cpu().numpy(), dist.cpu().numpy())
+            if ref is None:
+                ref = got
+                _check(meshes, queries, *got)
+            assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1].view(np.uint32), ref[1].view(np.uint32)), \
+                (order, bw, spill)
+    plan.close()
 
 
 def test_bad_vertex_index_rejected():

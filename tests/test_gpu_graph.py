@@ -60,3 +60,81 @@ def test_graph_c2_samples(oracle):
     got = b.proximityGraph(normals=nrm, max_degree=16, seed=4)
     ref = oracle.proximity_graph(pos, nrm, 16, oracle.graph_picks(len(pos), 100, 4))
     _cmp(got, ref, "C2")
+
+
+def _oversampled_ref(oracle, p, nrm, md, S, seed):
+    gi, gs, gd, gavg = oracle.proximity_graph(p, nrm, md, oracle.graph_picks(len(p), 100, seed))
+    ridx, rsep, rdeg = oracle.extract_original(gi, gs, gd, S)
+    tot = np.float32(0)
+    for i in range(S):  # updateAverageSeparation over the original samples, in order
+        for j in range(rdeg[i]):
+            tot = np.float32(tot + rsep[i, j])
+    e = int(rdeg.sum())
+    return ridx, rsep, rdeg, (gavg[0], float(np.float32(tot / np.float32(e))) if e else 0.0)
+
+
+@pytest.mark.parametrize("S,md,with_normals", [(2000, 8, True), (5000, 8, False), (800, 16, True)])
+def test_graph_oversampled_matches_oracle(oracle, S, md, with_normals):
+    """SampleGraph below min_samples: S samples of a mesh plus dense area samples up to 30,000 points
+    (SampleGraph.cpp:296-334); each sample keeps its max_degree nearest other samples by shortest
+    path through the dense graph (PointSet3.cpp:158-178)."""
+    import thea_amd
+    from thea_amd import workloads as W
+    print(f"case oversampled graph S={S} md={md} normals={with_normals}", flush=True)
+    v, t = W.torus(96, 48, R=1.0, r=0.3, amp=0.03, seed=7)
+    sampler = thea_amd.MeshSampler(thea_amd.Mesh(v, t))
+    sp, sn, _ = sampler.sampleEvenlyByArea(S, seed=5, stream_id=1)
+    dp, dn, _ = sampler.sampleEvenlyByArea(30000 - S, seed=5, stream_id=2)
+    p = np.concatenate([sp, dp])
+    nrm = np.concatenate([sn, dn]) if with_normals else None
+    b = thea_amd.PointBvh(p, leaf_size=8)
+    got = b.proximityGraph(normals=nrm, max_degree=md, seed=11, num_original=S)
+    ref = _oversampled_ref(oracle, p, nrm, md, S, 11)
+    assert got[0].shape == (S, md)
+    _cmp(got, ref, f"oversampled S={S}")
+    assert got[2].min() >= 1
+
+
+def test_graph_oversampled_sparse_originals_use_the_large_tables(oracle):
+    """40 samples among 40,000 points: every search settles thousands of vertices, more than the
+    per-wave LDS tables hold, and is rerun with tables for the whole graph."""
+    import thea_amd
+    print("case oversampled graph, 40 originals", flush=True)
+    g = np.random.default_rng(8)
+    u = g.normal(size=(40000, 3))
+    p = (u / np.linalg.norm(u, axis=1)[:, None]).astype(np.float32)
+    S = 40
+    b = thea_amd.PointBvh(p)
+    got = b.proximityGraph(max_degree=8, seed=2, num_original=S)
+    ref = _oversampled_ref(oracle, p, None, 8, S, 2)
+    _cmp(got, ref, "sparse originals")
+
+
+@pytest.mark.parametrize("S", [None, 1500])
+def test_graph_reachability_matches_oracle(oracle, S):
+    """updateGraph(surface) (SampleGraph -r): the 64 ellipsoids of the k-NN workload, whose surfaces
+    cut each other, so candidates across another ellipsoid's surface are blocked; with and without
+    oversampling. The graph must differ from the one without the surface."""
+    import thea_amd
+    from thea_amd import workloads as W
+    print(f"case graph reachability S={S}", flush=True)
+    v, t = W.ellipsoids_c2()
+    mesh = thea_amd.Mesh(v, t)
+    p, nrm, _ = thea_amd.MeshSampler(mesh).sampleEvenlyByArea(20000, seed=3, stream_id=1)
+    mb = thea_amd.MeshBvh(mesh, leaf_size=1)
+    b = thea_amd.PointBvh(p, leaf_size=8)
+    got = b.proximityGraph(normals=nrm, max_degree=8, seed=6, num_original=S, surface=mb)
+    kd = oracle.KdTree(v, t)
+    gi, gs, gd, gavg = oracle.proximity_graph(p, nrm, 8, oracle.graph_picks(len(p), 100, 6), surface=kd)
+    if S is None:
+        ref = (gi, gs, gd, gavg)
+    else:
+        ridx, rsep, rdeg = oracle.extract_original(gi, gs, gd, S)
+        tot = np.float32(0)
+        for i in range(S):
+            for j in range(rdeg[i]):
+                tot = np.float32(tot + rsep[i, j])
+        ref = (ridx, rsep, rdeg, (gavg[0], float(np.float32(tot / np.float32(rdeg.sum())))))
+    _cmp(got, ref, f"reachability S={S}")
+    plain = b.proximityGraph(normals=nrm, max_degree=8, seed=6, num_original=S)
+    assert not np.array_equal(plain[0], got[0])

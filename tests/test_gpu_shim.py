@@ -64,6 +64,17 @@ def test_cpp_api_matches_python_binding(oracle):
     assert abs(tx - tr[0, 3]) <= 1e-6 and abs(ty - tr[1, 3]) <= 1e-6 and abs(tz - tr[2, 3]) <= 1e-6
     assert abs(cerr - err) <= 1e-3 * err
     assert -0.06 < tx < -0.02  # moved back towards the cube (ICP slides on the faces parallel to x)
+    # double point BVH + Icp3<double> (ShapeAlign's form): the nearest target sample against the
+    # oracle's double brute force, and the alignment against the double-precision Python path
+    target = surf.astype(np.float64)
+    i0, d0 = oracle.brute_closest_points_f64(target, np.array([[0.3, 0.2, 1.5]]))
+    row = got["closest64"][0]
+    assert int(row[0]) == i0[0] and float(row[1]) == np.sqrt(d0[0])
+    assert [float(x) for x in row[2:]] == list(target[i0[0]])
+    tb = thea_amd.PointBvhF64(target)
+    tr_d, err_d = thea_amd.Icp3().align(moved, tb)
+    r = [float(x) for x in got["icp64"][0]]
+    assert r[:3] == [tr_d[0, 3], tr_d[1, 3], tr_d[2, 3]] and r[3] == err_d
     # ShapeDiameter with the default scale: the diagonal of the 1.05-padded unit cube
     scale = float(got["scale"][0][0])
     assert abs(scale - 1.05 * np.sqrt(3.0)) < 1e-6

@@ -73,3 +73,81 @@ def test_graph_oracle_matches_brute_force():
             for j in range(bd[i]):
                 total = np.float32(total + bs[i, j])
         assert np.float32(avg[1]) == np.float32(total / np.float32(bd.sum()))
+
+
+def _scipy_extract(idx, sep, deg, S):
+    """extractOriginalAdjacencies by an independent route: all-source shortest paths from
+    scipy.sparse.csgraph.dijkstra (double sums of the float weights) over the directed graph, then
+    the max_degree smallest (distance, index) other original samples, reported as
+    (float(distance), index) in ascending order."""
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import dijkstra
+    n, md = idx.shape
+    rows = np.repeat(np.arange(n), deg)
+    cols = np.concatenate([idx[i, :deg[i]] for i in range(n)]).astype(np.int64)
+    w = np.concatenate([sep[i, :deg[i]] for i in range(n)]).astype(np.float64)
+    # zero weights must survive the sparse matrix: shift by a tiny tag and subtract it back is not
+    # exact, so the test graphs have positive weights only
+    assert np.all(w > 0)
+    G = csr_matrix((w, (rows, cols)), shape=(n, n))
+    D = dijkstra(G, directed=True, indices=np.arange(S))
+    oi = -np.ones((S, md), np.int32)
+    os_ = -np.ones((S, md), np.float32)
+    od = np.zeros(S, np.int32)
+    for s in range(S):
+        d = D[s, :S].copy()
+        d[s] = np.inf
+        order = np.lexsort((np.arange(S), d))
+        keep = [j for j in order[:md] if np.isfinite(d[j])]
+        pairs = sorted((np.float32(d[j]), j) for j in keep)
+        od[s] = len(pairs)
+        for k, (fd, j) in enumerate(pairs):
+            oi[s, k], os_[s, k] = j, fd
+    return oi, os_, od
+
+
+def test_extract_original_against_scipy_dijkstra():
+    """The oracle's per-sample Dijkstra with early stop equals the k smallest all-pairs shortest
+    paths among the original samples, on a random proximity graph of 3,000 points, 300 originals."""
+    g = np.random.default_rng(21)
+    u = g.normal(size=(3000, 3))
+    p = (u / np.linalg.norm(u, axis=1)[:, None]).astype(np.float32)
+    idx, sep, deg, _ = O.proximity_graph(p, None, 8, O.graph_picks(len(p), 100, 3))
+    S = 300
+    got = O.extract_original(idx, sep, deg, S)
+    ref = _scipy_extract(idx, sep, deg, S)
+    assert np.array_equal(got[2], ref[2])
+    assert np.array_equal(got[0], ref[0])
+    assert np.array_equal(got[1].view(np.int32), ref[1].view(np.int32))
+
+
+def test_extract_original_path_graph_closed_form():
+    """A directed path 0 -> 1 -> ... -> n-1 with unit weights and originals every 3rd vertex: sample
+    3a reaches originals 3(a+1), 3(a+2), ... at distances 3, 6, ...; the last ones see none."""
+    n, md = 30, 4
+    idx = -np.ones((n, md), np.int32)
+    sep = -np.ones((n, md), np.float32)
+    deg = np.zeros(n, np.int32)
+    for i in range(n - 1):
+        idx[i, 0], sep[i, 0], deg[i] = i + 1, 1.0, 1
+    # originals are 0..9 by index: relabel so that original a sits at path position 3a
+    perm = np.empty(n, np.int64)  # new index of path position q
+    orig_pos = np.arange(0, n, 3)
+    others = np.setdiff1d(np.arange(n), orig_pos)
+    perm[orig_pos] = np.arange(len(orig_pos))
+    perm[others] = len(orig_pos) + np.arange(len(others))
+    ridx = -np.ones_like(idx)
+    rsep = -np.ones_like(sep)
+    rdeg = np.zeros_like(deg)
+    for q in range(n):
+        v = perm[q]
+        rdeg[v] = deg[q]
+        if deg[q]:
+            ridx[v, 0], rsep[v, 0] = perm[idx[q, 0]], sep[q, 0]
+    S = len(orig_pos)
+    oi, os_, od = O.extract_original(ridx, rsep, rdeg, S)
+    for a in range(S):
+        want = [b for b in range(a + 1, S)][:md]
+        assert od[a] == len(want)
+        assert list(oi[a, :len(want)]) == want
+        assert list(os_[a, :len(want)]) == [3.0 * (b - a) for b in want]

@@ -106,3 +106,29 @@ def test_loop_converges_on_small_rotation():
     assert err < 0.1 * e0 and np.abs(tr[:, :3] @ R - np.eye(3)).max() < 0.03
     tr, err, its = I.icp_align(nearest, src, max_it=30, frac_thr=1e-9)
     assert its == 4 and err > e0
+
+
+def test_double_nearest_point_oracle_against_kdtree():
+    """The oracle's double brute-force nearest point (BvhN<Vector3d>::closestElement semantics)
+    against scipy's kd-tree, an independent search: same indices and distances to the last bit on
+    random points (exact ties aside), the strict bound, and lowest index on exact duplicates."""
+    from scipy.spatial import cKDTree
+    from oracle import oracle as O
+    g = np.random.default_rng(41)
+    pts = g.normal(size=(4000, 3))
+    q = g.normal(size=(3000, 3)) * 1.5
+    idx, d2 = O.brute_closest_points_f64(pts, q)
+    dd, ii = cKDTree(pts).query(q)
+    assert np.array_equal(idx, ii)
+    d = pts[idx] - q
+    assert np.array_equal(d2, (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2])
+    assert np.allclose(np.sqrt(d2), dd, rtol=1e-15, atol=0)
+    # bound: admitted iff d^2 < bound^2
+    b = float(np.median(np.sqrt(d2)))
+    bidx, bd2 = O.brute_closest_points_f64(pts, q, b)
+    inside = d2 < b * b
+    assert np.array_equal(bidx[inside], idx[inside]) and np.all(bidx[~inside] == -1) and np.all(bd2[~inside] == -1)
+    # duplicates: lowest index
+    dup = np.concatenate([pts, pts])
+    didx, _ = O.brute_closest_points_f64(dup, q)
+    assert np.array_equal(didx, idx)

@@ -5,7 +5,7 @@ include/thea_gpu.h). This package is the Python host side: ctypes bindings (_cap
 classes (bvh, icp), the request server (serve), batched plans (batch), and synthetic workload
 generators (workloads).
 """
-from .bvh import (Context, Mesh, MeshBvh, MeshSampler, PointBvh, default_context, furthestPointSampling,  # noqa: F401
+from .bvh import (Context, Mesh, MeshBvh, MeshSampler, PointBvh, PointBvhF64, default_context, furthestPointSampling,  # noqa: F401
                   version)
 from .icp import Icp3  # noqa: F401
 from ._capi import TheaGpuError  # noqa: F401

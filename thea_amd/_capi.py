@@ -56,7 +56,7 @@ class IcpOpts(C.Structure):
 
 class GraphOpts(C.Structure):
     _fields_ = [("max_degree", C.c_int32), ("separation_trials", C.c_int32), ("seed", C.c_uint64),
-                ("reserved", C.c_int64 * 4)]
+                ("num_original", C.c_int64), ("surface", C.c_void_p), ("reserved", C.c_int64 * 2)]
 
 
 class MeshFile(C.Structure):
@@ -89,12 +89,15 @@ SIGNATURES = {
     "thg_bvh_bounds": (C.c_int, [vp, vp, vp]),
     "thg_closest_point": (C.c_int, [vp, vp, i64, f64, vp, vp, vp, vp, i32, vp]),
     "thg_closest_point_noalloc": (C.c_int, [vp, vp, i64, f64, vp, vp, vp, vp, vp]),
+    "thg_bvh_build_points_f64": (C.c_int, [vp, vp, i64, i32, C.POINTER(BuildOpts), vp, C.POINTER(vp)]),
+    "thg_closest_point_f64": (C.c_int, [vp, vp, i64, f64, vp, vp, vp, vp, i32, vp]),
     "thg_k_closest": (C.c_int, [vp, vp, i64, i32, f64, i32, vp, vp, vp, i32, vp]),
     "thg_ray_first_hit": (C.c_int, [vp, vp, vp, i64, f32, vp, vp, vp, i32, vp]),
     "thg_ray_any_hit": (C.c_int, [vp, vp, vp, i64, f32, vp, i32, vp]),
     "thg_sample_area": (C.c_int, [vp, i64, u64, u64, vp, vp, vp, C.POINTER(i64), i32, vp]),
     "thg_sample_area_approx": (C.c_int, [vp, i64, u64, u64, i64, vp, vp, vp, C.POINTER(i64), i32, vp]),
     "thg_closest_stats": (C.c_int, [vp, vp, i64, vp, C.POINTER(i64)]),
+    "thg_knn_wave_stats": (C.c_int, [vp, vp, i64, i32, vp, vp]),
     "thg_ray_stats": (C.c_int, [vp, vp, vp, i64, vp, vp, vp]),
     "thg_ctx_set_timing": (C.c_int, [vp, i32]),
     "thg_ctx_kernel_time": (C.c_int, [vp, C.c_char_p, C.POINTER(f64), C.POINTER(i64), i32]),
@@ -121,6 +124,8 @@ SIGNATURES = {
     "thg_batch_plan_run": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp]),
     "thg_batch_plan_destroy": (C.c_int, [vp]),
     "thg_autotune_leaf": (C.c_int, [vp, i32, vp, vp, vp, vp, vp, vp, vp]),
+    "thg_autotune_plan": (C.c_int, [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "thg_batch_plan_set_launch": (C.c_int, [vp, vp, i32, i32]),
     "thg_batch_closest": (C.c_int, [vp, i32, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp]),
 }
 

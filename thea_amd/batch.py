@@ -1,4 +1,5 @@
-"""Python binding of the batched many-mesh closest-point plan (thg_batch_plan_*, thg_autotune_leaf)."""
+"""Python binding of the batched many-mesh closest-point plan (thg_batch_plan_*, thg_autotune_leaf,
+thg_autotune_plan)."""
 import ctypes as C
 
 import numpy as np
@@ -26,7 +27,9 @@ def concat(meshes, queries):
 class BatchPlan:
     """A reusable plan for one set of mesh/query sizes; run() takes torch device tensors."""
 
-    def __init__(self, nv, nt, nq, leaf_sizes=None, ctx=None):
+    def __init__(self, nv, nt, nq, leaf_sizes=None, ctx=None, child_orders=None, launch=None):
+        """launch: (waves per workgroup, 8-resident-wave build) of the query launch, as autotune_plan
+        returns it; child_orders: per mesh (0 majority, 1 nearest, 2 build order)."""
         self.ctx = ctx or default_context()
         self.nv, self.nt, self.nq = _i64(nv), _i64(nt), _i64(nq)
         self.nm = len(self.nt)
@@ -36,6 +39,13 @@ class BatchPlan:
             self.ctx.h, self.nm, self.nv.ctypes.data_as(C.c_void_p), self.nt.ctypes.data_as(C.c_void_p),
             self.nq.ctypes.data_as(C.c_void_p), None if ls is None else ls.ctypes.data_as(C.c_void_p), C.byref(h)))
         self.h = h
+        if child_orders is not None or launch is not None:
+            self.set_launch(child_orders, *(launch if launch is not None else (4, 1)))
+
+    def set_launch(self, child_orders=None, block_waves=4, spill_build=1):
+        co = None if child_orders is None else np.ascontiguousarray(child_orders, np.int32)
+        capi.check(capi.lib().thg_batch_plan_set_launch(self.h, None if co is None else co.ctypes.data_as(C.c_void_p),
+                                                        int(block_waves), int(spill_build)))
 
     def close(self):
         if getattr(self, "h", None):
@@ -73,3 +83,19 @@ def autotune_leaf(nv, nt, nq, verts, tris, queries, ctx=None):
                                             C.c_void_p(tris.data_ptr()), C.c_void_p(queries.data_ptr()),
                                             out.ctypes.data_as(C.c_void_p)))
     return out
+
+
+def autotune_plan(nv, nt, nq, verts, tris, queries, ctx=None):
+    """(leaf sizes, child orders, (waves per workgroup, 8-resident-wave build)) chosen by timing the
+    plan's closest-point kernel per size class and over the sampled meshes (thg_autotune_plan)."""
+    ctx = ctx or default_context()
+    nv, nt, nq = _i64(nv), _i64(nt), _i64(nq)
+    leaf = np.zeros(len(nt), np.int32)
+    order = np.zeros(len(nt), np.int32)
+    launch = np.zeros(2, np.int32)
+    capi.check(capi.lib().thg_autotune_plan(ctx.h, len(nt), nv.ctypes.data_as(C.c_void_p), nt.ctypes.data_as(C.c_void_p),
+                                            nq.ctypes.data_as(C.c_void_p), C.c_void_p(verts.data_ptr()),
+                                            C.c_void_p(tris.data_ptr()), C.c_void_p(queries.data_ptr()),
+                                            leaf.ctypes.data_as(C.c_void_p), order.ctypes.data_as(C.c_void_p),
+                                            launch.ctypes.data_as(C.c_void_p)))
+    return leaf, order, (int(launch[0]), int(launch[1]))

@@ -47,7 +47,7 @@ class _Args:
     def inp(self, a, dtype, cols=None):
         if self.device_mode:
             torch = _torch()
-            tdt = {np.float32: torch.float32, np.int32: torch.int32}[dtype]
+            tdt = {np.float32: torch.float32, np.float64: torch.float64, np.int32: torch.int32}[dtype]
             if not _is_torch(a):
                 a = torch.as_tensor(np.asarray(a), device=self.torch_device)
             t = a.to(device=self.torch_device, dtype=tdt).contiguous()
@@ -331,12 +331,18 @@ class PointBvh(_BvhBase):
         capi.check(capi.lib().thg_bvh_build_points(self.ctx.h, pp, n, a.mem, C.byref(opts), a.stream, C.byref(h)))
         self._finish_build(h)
 
-    def proximityGraph(self, normals=None, max_degree=8, separation_trials=100, seed=0, average=True):
+    def proximityGraph(self, normals=None, max_degree=8, separation_trials=100, seed=0, average=True,
+                       num_original=None, surface=None):
         """PointSet3::updateGraph (PointSet3.hpp:324-386) over these points as the samples:
-        (neighbour indices (n, max_degree), separations (n, max_degree), degrees (n,),
-        (initial separation estimate, average edge length)). Outputs are numpy arrays, or torch
-        tensors when the normals are a torch device tensor. average=False skips the averages (the
-        final one is a sequential float sum on the host, as in Thea) and returns None for them."""
+        (neighbour indices (S, max_degree), separations (S, max_degree), degrees (S,),
+        (initial separation estimate, average edge length)). With num_original = S < n the first S
+        points are the samples and the rest their oversampling (PointSet3::addOversampling): the
+        graph is built over all points and each sample keeps its max_degree nearest other samples
+        by shortest-path distance (extractOriginalAdjacencies, PointSet3.cpp:158-178). Outputs are
+        numpy arrays, or torch tensors when the normals are a torch device tensor. average=False
+        skips the averages (the final one is a sequential float sum on the host, as in Thea) and
+        returns None for them. surface: a MeshBvh for the reachability test of
+        updateGraph(surface) (PointSet3.hpp:633-641; needs normals)."""
         n = self.info["elements"]
         dev = _is_torch(normals) and normals.is_cuda
         a = _Args(dev, self.ctx.torch_device() if dev else None)
@@ -346,14 +352,46 @@ class PointBvh(_BvhBase):
                 raise ValueError("one normal per point")
         else:
             np_ = None
-        idx, ip = a.out((n, max_degree), np.int32)
-        sep, sp = a.out((n, max_degree), np.float32)
-        deg, dp = a.out((n,), np.int32)
+        S = n if num_original is None else int(num_original)
+        idx, ip = a.out((S, max_degree), np.int32)
+        sep, sp = a.out((S, max_degree), np.float32)
+        deg, dp = a.out((S,), np.int32)
         avg = (C.c_float * 2)()
-        o = capi.GraphOpts(max_degree=max_degree, separation_trials=separation_trials, seed=seed)
+        o = capi.GraphOpts(max_degree=max_degree, separation_trials=separation_trials, seed=seed,
+                           num_original=0 if num_original is None else S,
+                           surface=None if surface is None else surface.h)
         capi.check(capi.lib().thg_proximity_graph(self.h, np_, C.byref(o), a.mem, a.stream, ip, sp, dp,
                                                   avg if average else None))
         return idx, sep, deg, ((avg[0], avg[1]) if average else None)
+
+
+class PointBvhF64(PointBvh):
+    """BVH over double-precision points (Thea's BvhN<Vector3d, 3, double> over a point set, the
+    target structure of ShapeAlign's ICP, Tools/ShapeAlign/ShapeAlign.cpp:339). closest() searches
+    in double; the float methods inherited from PointBvh search the points rounded to float."""
+
+    def __init__(self, points, leaf_size=4, ctx=None):
+        self.ctx = ctx or default_context()
+        a = self._args(points)
+        _, pp, n = a.inp(points, np.float64, 3)
+        opts = capi.BuildOpts(leaf_size=leaf_size)
+        h = C.c_void_p()
+        capi.check(capi.lib().thg_bvh_build_points_f64(self.ctx.h, pp, n, a.mem, C.byref(opts), a.stream,
+                                                       C.byref(h)))
+        self._finish_build(h)
+
+    def closest(self, queries, dist_bound=-1.0, want_point=True, want_dist=True, want_d2=False):
+        """(idx, dist, closest_point, d2) in double (BvhN<Vector3d>::closestElement, BvhN.hpp:1050-1063);
+        unrequested outputs are None."""
+        a = self._args(queries)
+        _, qp, n = a.inp(queries, np.float64, 3)
+        idx, ip = a.out((n,), np.int32)
+        dist, dp = a.out((n,), np.float64, want_dist)
+        cp, cpp = a.out((n, 3), np.float64, want_point)
+        d2, d2p = a.out((n,), np.float64, want_d2)
+        capi.check(capi.lib().thg_closest_point_f64(self.h, qp, n, float(dist_bound), ip, cpp, dp, d2p, a.mem,
+                                                    a.stream))
+        return idx, dist, cp, d2
 
 
 class MeshSampler:

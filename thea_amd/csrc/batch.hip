@@ -36,6 +36,7 @@ struct MeshInfo {  // per mesh, device side
   uint64_t wave_begin;       // first 64-query wave of the mesh within its traversal group
   int32_t seg_slot;          // row of the segment table (wave-cooperative meshes first)
   int32_t lane;              // > 0: log2 lanes per query (sparse queries), 0: wave-cooperative
+  int32_t child_order;       // visiting order of the closest-point kernel on this mesh (CHILD_*)
 };
 
 struct BatchPlan {
@@ -54,6 +55,8 @@ struct BatchPlan {
   DevBuf center, blo, bhi, keys, vals, tk, tv, hist;
   DevBuf left, right, rfirst, rlast, nparent, lparent, flags, nbox, keep, fidx, stmp;
   DevBuf nodes, recs, seg, out_idx, out_dist, counters, rent, rcount, bad;
+  int block_waves = 4;     // workgroup shape of the query launch (waves per workgroup)
+  bool spill_build = true;  // the 8-resident-wave build of the query kernel
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
   hipStream_t stream = nullptr;  // plan-owned: graphs cannot be captured on the null stream
@@ -319,6 +322,7 @@ __global__ void k_bsegs(const MeshInfo *__restrict__ mi, int nm, const int *__re
   d.root = root;
   d.nelem = (int32_t)x.nt;
   d.lanes_log2 = x.lane;
+  d.child_order = x.child_order;
   seg[x.seg_slot] = d;
 }
 
@@ -395,7 +399,8 @@ static void plan_pipeline(BatchPlan &P, hipStream_t s, hipEvent_t mid = nullptr)
     hipLaunchKernelGGL(k_bempty, dim3(ceil_div(P.NQ, BB)), dim3(BB), 0, s, mi, nm, P.NQ, P.out_idx.as<int32_t>(),
                        P.out_dist.as<float>());
     launch_check("k_bempty");
-    launch_closest_multi(P.seg.as<SegDesc>(), P.nwave_meshes, P.total_waves, false, s);
+    launch_closest_multi(P.seg.as<SegDesc>(), P.nwave_meshes, P.total_waves, false, s, false, P.block_waves,
+                         P.spill_build);
     launch_closest_group_multi(P.seg.as<SegDesc>() + P.nwave_meshes, nm - P.nwave_meshes, P.lane_waves, false, 64, s);
   }
 }
@@ -434,9 +439,42 @@ static void check_batch_indices(const int32_t *tris, int nm, const int64_t *nv, 
   require(hb == 0, "triangle vertex index out of range of its mesh");
 }
 
-// Leaf sizes chosen by measurement per size class (floor(log2(triangles))), cached for the process.
+// Leaf sizes chosen by measurement per size class (floor(log2(triangles))), cached for the process,
+// and the child orders chosen at those leaf sizes.
 static std::mutex g_tune_mu;
-static std::map<int, int> g_tuned_leaf;
+static std::map<int, int> g_tuned_leaf, g_tuned_order;
+
+// The plan's query launch for one mesh alone: its queries spatially sorted, one segment, the given
+// visiting order and launch shape; returns the average time of `reps` launches after one warm-up.
+static double time_plan_kernel(const Bvh &b, const float *q, int64_t n, int child_order, int block_waves, bool spill,
+                               int reps, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+  QueryScratch ws;
+  ws.timed = false;
+  order_queries(b, q, n, ws, s);
+  AsyncBuf oi(n * 4, s), od(n * 4, s), dseg(sizeof(SegDesc), s);
+  SegDesc d{};
+  d.nodes = b.nodes.p;
+  d.elems = b.elems.p;
+  d.q = q;
+  d.order = ws.order;
+  d.idx = oi.as<int32_t>();
+  d.dist = od.as<float>();
+  d.n = n;
+  d.best_init = INFINITY;
+  d.scale = b.scale;
+  d.nelem = (int32_t)b.n;
+  d.child_order = child_order;
+  THG_HIP(hipMemcpyAsync(dseg.p, &d, sizeof(SegDesc), hipMemcpyHostToDevice, s));
+  const uint64_t waves = (uint64_t)((n + SEG_WAVE_QUERIES - 1) / SEG_WAVE_QUERIES);
+  launch_closest_multi(dseg.as<SegDesc>(), 1, waves, false, s, false, block_waves, spill);
+  THG_HIP(hipEventRecord(e0, s));
+  for (int r = 0; r < reps; ++r) launch_closest_multi(dseg.as<SegDesc>(), 1, waves, false, s, false, block_waves, spill);
+  THG_HIP(hipEventRecord(e1, s));
+  THG_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  THG_HIP(hipEventElapsedTime(&ms, e0, e1));
+  return ms / reps;
+}
 
 }  // namespace thg
 
@@ -604,101 +642,203 @@ int thg_batch_plan_run(thg_batch_plan plan, const float *verts, const int32_t *t
   });
 }
 
+}  // extern "C"
+
+namespace thg {
+
+// Per size class (floor(log2 triangles)): the leaf size (2..16), then at that leaf size the child
+// visiting order (CHILD_*), each the fastest for the plan's query kernel on up to two sampled
+// meshes of the class (the largest query batches, up to 262,144 queries each); both cached for the
+// process. Then, if `launch` is given, the launch shape of the whole plan (waves per workgroup 1, 2,
+// 4 or 8; the spill-free or the 8-resident-wave build), the fastest summed over every class's sampled
+// meshes at their tuned leaf sizes and orders. A leaf size whose tree is too deep for the query
+// kernel's 64-entry stack on a sampled mesh is not a candidate.
+static void autotune_plan(Ctx *c, int32_t nm, const int64_t *nv, const int64_t *nt, const int64_t *nq,
+                          const float *verts, const int32_t *tris, const float *queries, int32_t *leaf_sizes,
+                          int32_t *child_orders, int32_t *launch) {
+  THG_HIP(hipSetDevice(c->device));
+  std::vector<int64_t> vo(nm + 1, 0), to(nm + 1, 0), qo(nm + 1, 0);
+  for (int m = 0; m < nm; ++m) { vo[m + 1] = vo[m] + nv[m]; to[m + 1] = to[m] + nt[m]; qo[m + 1] = qo[m] + nq[m]; }
+  std::map<int, std::vector<int>> buckets;
+  for (int m = 0; m < nm; ++m) {
+    if (nt[m] <= 0) continue;
+    int b = 0;
+    while ((int64_t(1) << (b + 1)) <= nt[m]) ++b;
+    buckets[b].push_back(m);
+  }
+  const int leaves[] = {2, 4, 6, 8, 12, 16};
+  const int REPS = 3;
+  struct TuneRes {  // stream and events released on every path, including a failed build
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~TuneRes() {
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+      if (s) (void)hipStreamDestroy(s);
+    }
+  } R;
+  THG_HIP(hipStreamCreateWithFlags(&R.s, hipStreamNonBlocking));
+  THG_HIP(hipEventCreate(&R.e0));
+  THG_HIP(hipEventCreate(&R.e1));
+  hipStream_t s = R.s;
+  check_batch_indices(tris, nm, nv, nt, s);
+  struct Sampled {
+    int m;
+    int64_t n;
+    std::unique_ptr<Bvh> b;  // built at the class's tuned leaf size
+    int order;
+  };
+  std::vector<Sampled> all;
+  auto build = [&](int m, int leaf) {
+    std::unique_ptr<Bvh> b(new Bvh());
+    b->ctx = c;
+    // vertex indices are local to their mesh: offset the vertex pointer
+    build_bvh(*b, verts + 3 * vo[m], tris + 3 * to[m], nt[m], false, leaf, s);
+    return b;
+  };
+  for (auto &kv : buckets) {
+    int best_leaf = 0, best_order = -1;
+    {
+      std::lock_guard<std::mutex> g(g_tune_mu);
+      auto it = g_tuned_leaf.find(kv.first);
+      if (it != g_tuned_leaf.end()) best_leaf = it->second;
+      auto io = g_tuned_order.find(kv.first);
+      if (io != g_tuned_order.end()) best_order = io->second;
+    }
+    std::vector<int> ms = kv.second;
+    ms.erase(std::remove_if(ms.begin(), ms.end(), [&](int m) { return nq[m] <= 0; }), ms.end());
+    std::sort(ms.begin(), ms.end(), [&](int a, int b) { return nq[a] > nq[b]; });
+    if (ms.size() > 2) ms.resize(2);
+    if (best_leaf == 0) {
+      double best_ms = 1e30;
+      for (int leaf : leaves) {
+        // a candidate counts only if it timed every sampled mesh
+        double tot = 0;
+        bool timed_all = !ms.empty();
+        for (int m : ms) {
+          std::unique_ptr<Bvh> b = build(m, leaf);
+          if (b->max_depth >= 64) { timed_all = false; break; }
+          tot += time_plan_kernel(*b, queries + 3 * qo[m], std::min<int64_t>(nq[m], 262144), CHILD_MAJORITY, 4, true,
+                                  REPS, s, R.e0, R.e1);
+        }
+        if (timed_all && tot < best_ms) { best_ms = tot; best_leaf = leaf; }
+      }
+      if (best_leaf == 0) best_leaf = 4;
+      std::lock_guard<std::mutex> g(g_tune_mu);
+      g_tuned_leaf[kv.first] = best_leaf;
+    }
+    std::vector<Sampled> here;
+    if ((child_orders && best_order < 0) || launch) {
+      for (int m : ms) {
+        std::unique_ptr<Bvh> b = build(m, best_leaf);
+        if (b->max_depth >= 64) continue;
+        here.push_back(Sampled{m, std::min<int64_t>(nq[m], 262144), std::move(b), CHILD_MAJORITY});
+      }
+    }
+    if (child_orders && best_order < 0) {
+      double best_ms = 1e30;
+      for (int order : {CHILD_MAJORITY, CHILD_LEADER, CHILD_BUILD}) {
+        double tot = 0;
+        for (auto &x : here)
+          tot += time_plan_kernel(*x.b, queries + 3 * qo[x.m], x.n, order, 4, true, REPS, s, R.e0, R.e1);
+        if (tot < best_ms) { best_ms = tot; best_order = order; }
+      }
+      if (here.empty()) best_order = CHILD_MAJORITY;
+      std::lock_guard<std::mutex> g(g_tune_mu);
+      g_tuned_order[kv.first] = best_order;
+    }
+    if (best_order < 0) best_order = CHILD_MAJORITY;
+    for (auto &x : here) x.order = best_order;
+    for (auto &x : here) all.push_back(std::move(x));
+    for (int m : kv.second) {
+      leaf_sizes[m] = best_leaf;
+      if (child_orders) child_orders[m] = best_order;
+    }
+  }
+  for (int m = 0; m < nm; ++m)
+    if (nt[m] <= 0) {
+      leaf_sizes[m] = 4;
+      if (child_orders) child_orders[m] = CHILD_MAJORITY;
+    }
+  if (launch) {
+    double best_ms = 1e30;
+    launch[0] = 4;
+    launch[1] = 1;
+    for (int spill = 0; spill < 2; ++spill)
+      for (int bw : {1, 2, 4, 8}) {
+        double tot = 0;
+        for (auto &x : all)
+          tot += time_plan_kernel(*x.b, queries + 3 * qo[x.m], x.n, x.order, bw, spill != 0, REPS, s, R.e0, R.e1);
+        if (!all.empty() && tot < best_ms) {
+          best_ms = tot;
+          launch[0] = bw;
+          launch[1] = spill;
+        }
+      }
+  }
+}
+
+}  // namespace thg
+
+extern "C" {
+
 int thg_autotune_leaf(thg_ctx ctx, int32_t nm, const int64_t *nv, const int64_t *nt, const int64_t *nq,
                       const float *verts, const int32_t *tris, const float *queries, int32_t *leaf_sizes) {
   return b_guarded([&] {
     require(ctx && nv && nt && nq && leaf_sizes && nm >= 1, "bad arguments");
-    Ctx *c = reinterpret_cast<Ctx *>(ctx);
-    THG_HIP(hipSetDevice(c->device));
-    std::vector<int64_t> vo(nm + 1, 0), to(nm + 1, 0), qo(nm + 1, 0);
-    for (int m = 0; m < nm; ++m) { vo[m + 1] = vo[m] + nv[m]; to[m + 1] = to[m] + nt[m]; qo[m + 1] = qo[m] + nq[m]; }
-    std::map<int, std::vector<int>> buckets;
-    for (int m = 0; m < nm; ++m) {
-      if (nt[m] <= 0) continue;
-      int b = 0;
-      while ((int64_t(1) << (b + 1)) <= nt[m]) ++b;
-      buckets[b].push_back(m);
-    }
-    const int candidates[] = {2, 4, 6, 8, 12, 16};
-    struct TuneRes {  // stream and events released on every path, including a failed build
-      hipStream_t s = nullptr;
-      hipEvent_t e0 = nullptr, e1 = nullptr;
-      ~TuneRes() {
-        if (e0) (void)hipEventDestroy(e0);
-        if (e1) (void)hipEventDestroy(e1);
-        if (s) (void)hipStreamDestroy(s);
-      }
-    } R;
-    THG_HIP(hipStreamCreateWithFlags(&R.s, hipStreamNonBlocking));
-    THG_HIP(hipEventCreate(&R.e0));
-    THG_HIP(hipEventCreate(&R.e1));
-    hipStream_t s = R.s;
-    check_batch_indices(tris, nm, nv, nt, s);
-    for (auto &kv : buckets) {
-      int best_leaf;
-      {
-        std::lock_guard<std::mutex> g(g_tune_mu);
-        auto it = g_tuned_leaf.find(kv.first);
-        best_leaf = it != g_tuned_leaf.end() ? it->second : 0;
-      }
-      if (best_leaf == 0) {
-        double best_ms = 1e30;
-        // the largest query batch of the bucket, up to two meshes, up to 262144 queries each
-        std::vector<int> ms = kv.second;
-        ms.erase(std::remove_if(ms.begin(), ms.end(), [&](int m) { return nq[m] <= 0; }), ms.end());
-        std::sort(ms.begin(), ms.end(), [&](int a, int b) { return nq[a] > nq[b]; });
-        if (ms.size() > 2) ms.resize(2);
-        for (int leaf : candidates) {
-          // a candidate counts only if it timed every sampled mesh (a tree too deep for the query
-          // kernels' stack, or a mesh without queries, leaves it untimed)
-          double tot = 0;
-          bool timed_all = true;
-          for (int m : ms) {
-            Bvh b;
-            b.ctx = c;
-            // vertex indices are local to their mesh: offset the vertex pointer
-            build_bvh(b, verts + 3 * vo[m], tris + 3 * to[m], nt[m], false, leaf, s);
-            int64_t n = std::min<int64_t>(nq[m], 262144);
-            if (n == 0 || b.max_depth >= 64) {
-              timed_all = false;
-              break;
-            }
-            AsyncBuf oi(n * 4, s);
-            QueryScratch ws;
-            launch_closest(b, queries + 3 * qo[m], n, -1.0, 0, oi.as<int32_t>(), nullptr, nullptr, nullptr, ws, s);
-            THG_HIP(hipEventRecord(R.e0, s));
-            for (int rep = 0; rep < 3; ++rep)
-              launch_closest(b, queries + 3 * qo[m], n, -1.0, 0, oi.as<int32_t>(), nullptr, nullptr, nullptr, ws, s);
-            THG_HIP(hipEventRecord(R.e1, s));
-            THG_HIP(hipEventSynchronize(R.e1));
-            float ms_ = 0;
-            THG_HIP(hipEventElapsedTime(&ms_, R.e0, R.e1));
-            tot += ms_;
-          }
-          if (timed_all && tot < best_ms) { best_ms = tot; best_leaf = leaf; }
-        }
-        if (best_leaf == 0) best_leaf = 4;
-        std::lock_guard<std::mutex> g(g_tune_mu);
-        g_tuned_leaf[kv.first] = best_leaf;
-      }
-      for (int m : kv.second) leaf_sizes[m] = best_leaf;
-    }
-    for (int m = 0; m < nm; ++m)
-      if (nt[m] <= 0) leaf_sizes[m] = 4;
+    autotune_plan(reinterpret_cast<Ctx *>(ctx), nm, nv, nt, nq, verts, tris, queries, leaf_sizes, nullptr, nullptr);
+  });
+}
+
+int thg_autotune_plan(thg_ctx ctx, int32_t nm, const int64_t *nv, const int64_t *nt, const int64_t *nq,
+                      const float *verts, const int32_t *tris, const float *queries, int32_t *leaf_sizes,
+                      int32_t *child_orders, int32_t launch[2]) {
+  return b_guarded([&] {
+    require(ctx && nv && nt && nq && leaf_sizes && child_orders && launch && nm >= 1, "bad arguments");
+    autotune_plan(reinterpret_cast<Ctx *>(ctx), nm, nv, nt, nq, verts, tris, queries, leaf_sizes, child_orders,
+                  launch);
+  });
+}
+
+int thg_batch_plan_set_launch(thg_batch_plan plan, const int32_t *child_orders, int32_t block_waves,
+                              int32_t spill_build) {
+  return b_guarded([&] {
+    BatchPlan *P = reinterpret_cast<BatchPlan *>(plan);
+    require(P != nullptr, "plan is NULL");
+    require(block_waves == 1 || block_waves == 2 || block_waves == 4 || block_waves == 8,
+            "block_waves must be 1, 2, 4 or 8");
+    if (child_orders)
+      for (int m = 0; m < P->nm; ++m)
+        require(child_orders[m] >= CHILD_MAJORITY && child_orders[m] <= CHILD_BUILD, "child order must be 0, 1 or 2");
+    THG_HIP(hipSetDevice(P->ctx->device));
+    THG_HIP(hipStreamSynchronize(P->stream));
+    for (int m = 0; m < P->nm; ++m) P->info[m].child_order = child_orders ? child_orders[m] : CHILD_MAJORITY;
+    THG_HIP(hipMemcpy(P->d_info.p, P->info.data(), sizeof(MeshInfo) * P->nm, hipMemcpyHostToDevice));
+    P->block_waves = block_waves;
+    P->spill_build = spill_build != 0;
+    // the captured graph holds the old launch: capture again at the next run
+    if (P->exec) { (void)hipGraphExecDestroy(P->exec); P->exec = nullptr; }
+    if (P->graph) { (void)hipGraphDestroy(P->graph); P->graph = nullptr; }
   });
 }
 
 int thg_batch_closest(thg_ctx ctx, int32_t nm, const int64_t *nv, const int64_t *nt, const int64_t *nq,
                       const float *verts, const int32_t *tris, const float *queries, int32_t leaf_size, int32_t *idx,
                       float *dist, void *stream, double timings_ms[2]) {
-  std::vector<int32_t> leaves(nm > 0 ? nm : 1, leaf_size > 0 ? leaf_size : 4);
+  std::vector<int32_t> leaves(nm > 0 ? nm : 1, leaf_size > 0 ? leaf_size : 4), orders(nm > 0 ? nm : 1, 0);
+  int32_t launch[2] = {4, 1};
   if (leaf_size == 0 && nm > 0) {
-    int rc = thg_autotune_leaf(ctx, nm, nv, nt, nq, verts, tris, queries, leaves.data());
+    int rc = thg_autotune_plan(ctx, nm, nv, nt, nq, verts, tris, queries, leaves.data(), orders.data(), launch);
     if (rc != THG_OK) return rc;
   }
   thg_batch_plan plan = nullptr;
   int rc = thg_batch_plan_create(ctx, nm, nv, nt, nq, leaves.data(), &plan);
   if (rc != THG_OK) return rc;
+  rc = thg_batch_plan_set_launch(plan, orders.data(), launch[0], launch[1]);
+  if (rc != THG_OK) {
+    thg_batch_plan_destroy(plan);
+    return rc;
+  }
   rc = thg_batch_plan_run(plan, verts, tris, queries, idx, dist, stream, timings_ms);
   thg_batch_plan_destroy(plan);
   return rc;

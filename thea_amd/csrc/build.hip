@@ -435,6 +435,21 @@ __global__ void __launch_bounds__(B_BLOCK) k_point_records(const float *__restri
   out[k] = r;
 }
 
+// Ray records from the triangle records, in the same (leaf) order.
+__global__ void __launch_bounds__(B_BLOCK) k_ray_records(const TriRec *__restrict__ tr, int64_t n,
+                                                         RayTri *__restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * B_BLOCK + threadIdx.x;
+  if (k >= n) return;
+  const TriRec &r = tr[k];
+  const float3 v0 = f3(r.p0.x, r.p0.y, r.p0.z);
+  const float3 e01 = sub3(f3(r.p1.x, r.p1.y, r.p1.z), v0), e02 = sub3(f3(r.p2.x, r.p2.y, r.p2.z), v0);
+  RayTri o;
+  o.a = make_float4(v0.x, v0.y, v0.z, r.pl.w);
+  o.b = make_float4(e01.x, e01.y, e01.z, 0.0f);
+  o.c = make_float4(e02.x, e02.y, e02.z, 0.0f);
+  out[k] = o;
+}
+
 // Single-leaf tree for n <= leaf_size: child 0 holds every element, child 1 is an empty leaf.
 __global__ void k_single_leaf(int64_t n, float3 lo, float3 hi, BNode *out) {
   BNode nd;
@@ -623,6 +638,12 @@ void build_bvh(Bvh &b, const float *verts, const int32_t *tris, int64_t n, bool 
     hipLaunchKernelGGL(k_tri_records, dim3(ceil_div(n, B_BLOCK)), dim3(B_BLOCK), 0, s, verts, tris, sorted, n,
                        b.elems.as<TriRec>());
   launch_check("k_records");
+  if (!points) {
+    b.rays.alloc(n * sizeof(RayTri));
+    hipLaunchKernelGGL(k_ray_records, dim3(ceil_div(n, B_BLOCK)), dim3(B_BLOCK), 0, s, b.elems.as<TriRec>(), n,
+                       b.rays.as<RayTri>());
+    launch_check("k_ray_records");
+  }
 
   if (n <= leaf_size) {
     b.nodes.alloc(sizeof(BNode));

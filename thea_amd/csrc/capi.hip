@@ -6,6 +6,7 @@
 #include "common.hpp"
 #include "kernels_common.hpp"
 #include "query.hpp"
+#include "query64.hpp"
 
 namespace thg {
 
@@ -260,6 +261,31 @@ int thg_bvh_build_points(thg_ctx ctx, const float *xyz, int64_t n, int32_t mem, 
   });
 }
 
+int thg_bvh_build_points_f64(thg_ctx ctx, const double *xyz, int64_t n, int32_t mem, const thg_build_opts *opts,
+                             void *stream, thg_bvh *out) {
+  return guarded([&] {
+    require(ctx && out, "ctx/out is NULL");
+    require(n >= 0 && (n == 0 || xyz), "bad point array");
+    require(n <= (int64_t)0x7ffffff0, "too many points for int32 indices");
+    check_mem(mem);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    set_device(c);
+    int leaf = leaf_from(opts);
+    hipStream_t s = as_stream(stream);
+    std::unique_ptr<Bvh> b(new Bvh());
+    b->ctx = c;
+    DevBuf staged;
+    const double *dp = xyz;
+    if (mem == THG_MEM_HOST && n > 0) {
+      staged.alloc(n * 3 * sizeof(double));
+      THG_HIP(hipMemcpyAsync(staged.p, xyz, n * 3 * sizeof(double), hipMemcpyHostToDevice, s));
+      dp = staged.as<double>();
+    }
+    build_points64(*b, dp, n, leaf, s);
+    *out = reinterpret_cast<thg_bvh>(b.release());
+  });
+}
+
 int thg_bvh_destroy(thg_bvh bvh) {
   return guarded([&] { delete reinterpret_cast<Bvh *>(bvh); });
 }
@@ -350,6 +376,28 @@ int thg_closest_point(thg_bvh bvh, const float *q, int64_t n, double dist_bound,
   });
 }
 
+int thg_closest_point_f64(thg_bvh bvh, const double *q, int64_t n, double dist_bound, int32_t *idx, double *cp,
+                          double *dist, double *d2, int32_t mem, void *stream) {
+  return guarded([&] {
+    require(bvh != nullptr, "bvh is NULL");
+    require(n >= 0, "negative query count");
+    require(n == 0 || (q && idx), "q/idx is NULL");
+    require(!std::isnan(dist_bound), "dist_bound is NaN");
+    check_mem(mem);
+    Bvh *b = reinterpret_cast<Bvh *>(bvh);
+    require(b->f64, "not a double-precision point BVH (build it with thg_bvh_build_points_f64)");
+    set_device(b->ctx);
+    hipStream_t s = as_stream(stream);
+    Stage st(s, mem == THG_MEM_HOST);
+    const double *dq = st.in(q, n * 3);
+    int32_t *didx = st.out(idx, n);
+    double *dcp = st.out(cp, n * 3), *ddist = st.out(dist, n), *dd2 = st.out(d2, n);
+    QueryScratch ws;
+    launch_closest64(*b, dq, n, dist_bound, didx, dcp, ddist, dd2, ws, s);
+    st.finish();
+  });
+}
+
 int thg_closest_point_noalloc(thg_bvh bvh, const float *q, int64_t n, double dist_bound, int32_t *idx, float *cp,
                               float *dist, float *d2, void *stream) {
   return guarded([&] {
@@ -375,6 +423,18 @@ int thg_closest_point_noalloc(thg_bvh bvh, const float *q, int64_t n, double dis
     ws.ext_cap = THG_NOALLOC_MAX_QUERIES;
     ws.timed = false;  // timing events are host objects created per launch; this path creates none
     launch_closest(*b, q, n, dist_bound, 0, idx, cp, dist, d2, ws, s);
+  });
+}
+
+int thg_knn_wave_stats(thg_bvh bvh, const float *q, int64_t n, int32_t exclude_self, void *stream,
+                       int64_t *per_wave) {
+  return guarded([&] {
+    require(bvh && per_wave, "bvh/per_wave is NULL");
+    require(n >= 0 && (n == 0 || q), "bad query array");
+    Bvh *b = reinterpret_cast<Bvh *>(bvh);
+    set_device(b->ctx);
+    QueryScratch ws;
+    knn_wave_stats(*b, q, n, exclude_self, per_wave, ws, as_stream(stream));
   });
 }
 

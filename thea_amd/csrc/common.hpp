@@ -203,11 +203,15 @@ struct Bvh {
   int32_t wave_g = 1;    // lanes per query of the wave-cooperative kernels
   bool wave_pre = true;   // per-element box prefilter in the wave-cooperative kernels
   bool wave_cmp = true;   // compacted (lane, triangle) evaluation queue
+  int32_t child_order = 0;  // child visiting order of the compacted kernel (0: majority of lanes)
   float scale = 0;       // max |coordinate| over the elements
   float3 lo{}, hi{};     // scene bounds (element boxes)
   DevBuf nodes;          // BNode[num_nodes]
   DevBuf elems;          // TriRec[n] or PointRec[n], leaf order
-  size_t device_bytes() const { return nodes.bytes + elems.bytes; }
+  DevBuf rays;           // RayTri[n] (mesh BVHs), leaf order
+  bool f64 = false;      // point BVH built from double coordinates
+  DevBuf elems64;        // PointRecD[n] (f64 only), leaf order
+  size_t device_bytes() const { return nodes.bytes + elems.bytes + rays.bytes + elems64.bytes; }
 };
 
 inline int float_to_ord_host(float f) {

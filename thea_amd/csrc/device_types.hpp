@@ -4,6 +4,7 @@
 //   BNode[num_nodes]  64 B each: both children's boxes + refs, one 64-B segment per visit.
 //   TriRec[n]        128 B each (one L2 line): a triangle with every per-triangle constant the
 //                     closest-point and ray tests need, in leaf order, so a leaf is a contiguous run.
+//   RayTri[n]         48 B each (mesh BVHs): v0 and the edges of each triangle, leaf order, for rays.
 //   PointRec[n]       16 B each: point + original index, in leaf order.
 #pragma once
 
@@ -55,6 +56,17 @@ __host__ __device__ __forceinline__ int trirec_id(const float4 &pl) {
   m.f = pl.w;
   return m.i;
 }
+
+// Ray record of a triangle, in leaf order beside its TriRec: v0 and the two edges Eberly's test
+// starts from (e01 = v1 - v0, e02 = v2 - v0, the same float subtractions as Triangle3.hpp:151-152),
+// and the caller's index. 48 bytes, so a ray's triangle tests read 48 of every 128 bytes a TriRec
+// would take and neighbouring leaves share cache lines.
+struct __align__(16) RayTri {
+  float4 a;  // v0.xyz, id (int bits)
+  float4 b;  // e01.xyz, 0
+  float4 c;  // e02.xyz, 0
+};
+static_assert(sizeof(RayTri) == 48, "RayTri must be 48 bytes");
 
 struct __align__(16) PointRec {
   float4 p;    // xyz, id (int bits)

@@ -11,8 +11,10 @@
 // so each fit uses the correspondences of the positions one iteration earlier, exactly as in Thea.
 //
 // Device side: the nearest points of all active problems' positions come from one closest-point
-// launch (the float BVH; positions are rounded to float for the search, the correspondences are
-// the float closest points). Means, covariance and error are double sums over each problem's points,
+// launch. On a double point BVH (ShapeAlign's BvhN<Vector3d, 3, double>, ShapeAlign.cpp:339) the
+// positions stay double and the search is in double, as in Thea; on a float BVH (a mesh, or float
+// points) the positions are rounded to float and the correspondences are the float closest points.
+// Means, covariance and error are double sums over each problem's points,
 // reduced in a fixed order (per-thread sequential sums of every 256th point, an LDS tree per block,
 // then the blocks in order), so a problem's result does not depend on the other problems in the batch. The 3x3 SVD and
 // the convergence logic run on the host.
@@ -24,6 +26,7 @@
 #include "common.hpp"
 #include "kernels_common.hpp"
 #include "query.hpp"
+#include "query64.hpp"
 
 namespace thg {
 
@@ -61,10 +64,11 @@ __global__ void __launch_bounds__(256) k_icp_init(int64_t n, int K, const double
 }
 
 // Positions of the active problems (slot s is problem prob[s], its points [s n, (s + 1) n)):
-// pos = cur[k] * src[k] in double, rounded to float for the search.
+// pos = cur[k] * src[k] in double, rounded to T (float for a float BVH, double for a double one).
+template <typename T>
 __global__ void __launch_bounds__(256) k_icp_pos(int64_t n, int A, const int32_t *__restrict__ prob,
                                                  const double *__restrict__ src, const double *__restrict__ cur,
-                                                 float *__restrict__ pos) {
+                                                 T *__restrict__ pos) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n * A) return;
   const int s = (int)(i / n);
@@ -73,20 +77,20 @@ __global__ void __launch_bounds__(256) k_icp_pos(int64_t n, int A, const int32_t
   const double *p = src + 3 * ((int64_t)k * n + j);
   double x, y, z;
   xf_apply(cur + 12 * k, p[0], p[1], p[2], x, y, z);
-  pos[3 * i] = (float)x;
-  pos[3 * i + 1] = (float)y;
-  pos[3 * i + 2] = (float)z;
+  pos[3 * i] = (T)x;
+  pos[3 * i + 1] = (T)y;
+  pos[3 * i + 2] = (T)z;
 }
 
 // Sums over the points of each active problem, with q the correspondences by problem (q[k n + j]).
 // MODE 0: wt p (3), wt q (3), wt (1). MODE 1: the cross-covariance sum of (wr dp)(wr dq)^T with the
 // up component removed (9). MODE 2: wr^2 |L dp - dq|^2 (1). aux (12 per problem): up in [0..2] and
 // has_up in [3] (MODE 1), L row-major in [0..8] (MODE 2); sums7: the MODE 0 sums (means).
-template <int MODE>
+template <int MODE, typename QT>
 __global__ void __launch_bounds__(ICP_THREADS) k_icp_sums(int64_t n, const int32_t *__restrict__ prob,
                                                           const double *__restrict__ src,
                                                           const double *__restrict__ cur,
-                                                          const float *__restrict__ q, const double *__restrict__ wt,
+                                                          const QT *__restrict__ q, const double *__restrict__ wt,
                                                           const double *__restrict__ wr,
                                                           const double *__restrict__ sums7,
                                                           const double *__restrict__ aux, int nb,
@@ -116,7 +120,7 @@ __global__ void __launch_bounds__(ICP_THREADS) k_icp_sums(int64_t n, const int32
     const double *sp = src + 3 * ((int64_t)k * n + j);
     double p[3];
     xf_apply(xf, sp[0], sp[1], sp[2], p[0], p[1], p[2]);
-    const float *qq = q + 3 * ((int64_t)k * n + j);
+    const QT *qq = q + 3 * ((int64_t)k * n + j);
     const double qd[3] = {(double)qq[0], (double)qq[1], (double)qq[2]};
     if (MODE == 0) {
       const double w = wt ? wt[j] : 1.0;
@@ -337,7 +341,11 @@ void icp_align(const Bvh &to, const double *from, int64_t n, int K, const double
   AsyncBuf src(n * K * 24, s), cur(K * 96, s), aux(K * 96, s), sums7(K * 56, s), cov(K * 72, s), errs(K * 8, s);
   // qpts: the correspondences of problem k at [k n, (k + 1) n) (kept across iterations whatever the
   // active set); qtmp: the nearest points of one launch, by slot
-  AsyncBuf pos(n * K * 12, s), qpts(n * K * 12, s), qtmp(n * K * 12, s), idx(n * K * 4, s), prob(K * 4, s);
+  // A double point BVH (BvhN<Vector3d>, ShapeAlign.cpp:339) is searched in double with the positions
+  // kept in double; a float BVH with the positions rounded to float.
+  const bool f64 = to.f64;
+  const int64_t pb = f64 ? 24 : 12;  // bytes per position / correspondence
+  AsyncBuf pos(n * K * pb, s), qpts(n * K * pb, s), qtmp(n * K * pb, s), idx(n * K * 4, s), prob(K * 4, s);
   AsyncBuf partial((size_t)K * nb * ICP_MAXC * 8, s);
   hipLaunchKernelGGL(k_icp_init, dim3(ceil_div(n * K, 256)), dim3(256), 0, s, n, K, from, init, src.as<double>());
   launch_check("k_icp_init");
@@ -361,28 +369,36 @@ void icp_align(const Bvh &to, const double *from, int64_t n, int K, const double
   // nearest points of the active problems' current positions
   auto nearest = [&]() {
     const int A = (int)active.size();
-    hipLaunchKernelGGL(k_icp_pos, dim3(ceil_div(n * A, 256)), dim3(256), 0, s, n, A, prob.as<int32_t>(), src.as<double>(),
-                       cur.as<double>(), pos.as<float>());
-    launch_check("k_icp_pos");
     QueryScratch ws;
-    launch_closest(to, pos.as<float>(), n * A, -1.0, 0, idx.as<int32_t>(), qtmp.as<float>(), nullptr, nullptr, ws, s);
+    if (f64) {
+      hipLaunchKernelGGL(k_icp_pos<double>, dim3(ceil_div(n * A, 256)), dim3(256), 0, s, n, A, prob.as<int32_t>(),
+                         src.as<double>(), cur.as<double>(), pos.as<double>());
+      launch_check("k_icp_pos");
+      launch_closest64(to, pos.as<double>(), n * A, -1.0, idx.as<int32_t>(), qtmp.as<double>(), nullptr, nullptr, ws, s);
+    } else {
+      hipLaunchKernelGGL(k_icp_pos<float>, dim3(ceil_div(n * A, 256)), dim3(256), 0, s, n, A, prob.as<int32_t>(),
+                         src.as<double>(), cur.as<double>(), pos.as<float>());
+      launch_check("k_icp_pos");
+      launch_closest(to, pos.as<float>(), n * A, -1.0, 0, idx.as<int32_t>(), qtmp.as<float>(), nullptr, nullptr, ws, s);
+    }
     for (int sl = 0; sl < A; ++sl)
-      THG_HIP(hipMemcpyAsync(qpts.as<float>() + 3 * (int64_t)active[sl] * n, qtmp.as<float>() + 3 * (int64_t)sl * n, n * 12,
+      THG_HIP(hipMemcpyAsync((char *)qpts.p + pb * (int64_t)active[sl] * n, (char *)qtmp.p + pb * (int64_t)sl * n, n * pb,
                              hipMemcpyDeviceToDevice, s));
   };
   auto sums = [&](int mode, double *dst, int stride) {
     const int A = (int)active.size();
     const int nc = mode == 0 ? 7 : (mode == 1 ? 9 : 1);
     const dim3 g((unsigned)(A * nb));
-    if (mode == 0)
-      hipLaunchKernelGGL(k_icp_sums<0>, g, dim3(ICP_THREADS), 0, s, n, prob.as<int32_t>(), src.as<double>(), cur.as<double>(),
-                         qpts.as<float>(), wt, wr, sums7.as<double>(), aux.as<double>(), nb, partial.as<double>());
-    else if (mode == 1)
-      hipLaunchKernelGGL(k_icp_sums<1>, g, dim3(ICP_THREADS), 0, s, n, prob.as<int32_t>(), src.as<double>(), cur.as<double>(),
-                         qpts.as<float>(), wt, wr, sums7.as<double>(), aux.as<double>(), nb, partial.as<double>());
-    else
-      hipLaunchKernelGGL(k_icp_sums<2>, g, dim3(ICP_THREADS), 0, s, n, prob.as<int32_t>(), src.as<double>(), cur.as<double>(),
-                         qpts.as<float>(), wt, wr, sums7.as<double>(), aux.as<double>(), nb, partial.as<double>());
+#define THG_SUMS(M, QT)                                                                                            \
+  hipLaunchKernelGGL((k_icp_sums<M, QT>), g, dim3(ICP_THREADS), 0, s, n, prob.as<int32_t>(), src.as<double>(),      \
+                     cur.as<double>(), qpts.as<QT>(), wt, wr, sums7.as<double>(), aux.as<double>(), nb,              \
+                     partial.as<double>())
+    if (f64) {
+      if (mode == 0) THG_SUMS(0, double); else if (mode == 1) THG_SUMS(1, double); else THG_SUMS(2, double);
+    } else {
+      if (mode == 0) THG_SUMS(0, float); else if (mode == 1) THG_SUMS(1, float); else THG_SUMS(2, float);
+    }
+#undef THG_SUMS
     launch_check("k_icp_sums");
     hipLaunchKernelGGL(k_icp_final, dim3(ceil_div((int64_t)A * nc, 64)), dim3(64), 0, s, A, nb, nc, prob.as<int32_t>(),
                        partial.as<double>(), stride, dst);

@@ -226,32 +226,6 @@ __device__ __forceinline__ float lane_readf(float x, int32_t l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
 }
 
-// Per-lane seed of the wave traversals: descend from `root` to the leaf whose box is nearest the
-// query (no backtracking) and return it as [first, first + count). The elements found there give
-// each lane a real upper bound before the shared walk starts, so a wave whose queries are far apart
-// does not admit every node some other lane has not bounded yet. Depth is below 64 for the wave
-// kernels (use_wave_traversal), which bounds the loop.
-__device__ __forceinline__ void seed_leaf(const BNode *__restrict__ nodes, int32_t root, float3 qlo, float3 qhi,
-                                          int32_t &first, int32_t &count) {
-  int32_t nd_i = root;
-  first = 0;
-  count = 0;
-  for (int it = 0; it < 64; ++it) {
-    const BNode nd = nodes[nd_i];
-    const float l0 = box_lb(qlo, qhi, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y);
-    const float l1 = box_lb(qlo, qhi, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w);
-    const bool one = l1 < l0;
-    const int32_t r = one ? nd.d.y : nd.d.x;
-    if (r >= 0) {
-      nd_i = r;
-      continue;
-    }
-    first = ~r;
-    count = one ? nd.d.w : nd.d.z;
-    return;
-  }
-}
-
 // G lanes share one query (64 / G queries per wave): lane g of a group tests elements g, g + G, ...
 // of each admitted leaf; the group's best distance (a min over its lanes) drives pruning, and the
 // group's lexicographic minimum is reduced at the end. With G = 1 element records are wave-uniform
@@ -548,7 +522,8 @@ __device__ __forceinline__ void closest_cmp_body(CmpWave &W, const BNode *__rest
                                                  bool wave_ok, float best_init, float scale, int exclude_self,
                                                  int32_t *__restrict__ out_idx, float *__restrict__ out_cp,
                                                  float *__restrict__ out_dist, float *__restrict__ out_d2,
-                                                 unsigned long long *__restrict__ stats, int32_t root = 0) {
+                                                 unsigned long long *__restrict__ stats, int32_t root = 0,
+                                                 int child_order = 0) {
   uint32_t st_nodes = 0, st_boxes = 0, st_loaded = 0, st_flush = 0, st_pairs = 0;
   const int lane = lane_id();
   const int64_t t = wave * WAVE + lane;
@@ -609,7 +584,12 @@ __device__ __forceinline__ void closest_cmp_body(CmpWave &W, const BNode *__rest
     const float lb0 = box_lb(qlo, qhi, a0, a1, a2, a3, b0, b1);
     const float lb1 = box_lb(qlo, qhi, b2, b3, c0, c1, c2, c3);
     const uint64_t m0 = __ballot(lb0 <= best), m1 = __ballot(lb1 <= best);
-    const bool first1 = 2 * __popcll(__ballot(lb1 < lb0) & amask) > nact;
+    // visiting order (CHILD_*): the child most lanes find nearer, the child the wave's first active
+    // query finds nearer, or the build's order (ballots only, so the choice costs no registers)
+    const uint64_t near1 = __ballot(lb1 < lb0) & amask;
+    const bool first1 = child_order == CHILD_MAJORITY ? 2 * __popcll(near1) > nact
+                        : child_order == CHILD_LEADER ? (near1 & (amask & (~amask + 1))) != 0
+                                                      : false;
     int32_t next = -1;
 #pragma unroll
     for (int cc = 0; cc < 2; ++cc) {
@@ -719,12 +699,13 @@ __global__ void __launch_bounds__(W_BLOCK, WAVES) k_closest_cmp(const BNode *__r
                                                          int exclude_self, int32_t *__restrict__ out_idx,
                                                          float *__restrict__ out_cp, float *__restrict__ out_dist,
                                                          float *__restrict__ out_d2,
-                                                         unsigned long long *__restrict__ stats = nullptr) {
+                                                         unsigned long long *__restrict__ stats = nullptr,
+                                                         int child_order = CHILD_MAJORITY) {
   __shared__ CmpWave s_w[W_WAVES];
   const uint32_t lb_blk = xcd_remap(blockIdx.x, gridDim.x);
   const int64_t wave = (int64_t)lb_blk * W_WAVES + (threadIdx.x / WAVE);
   closest_cmp_body<STATS>(s_w[threadIdx.x / WAVE], nodes, tr, q, order, n, wave, lb_blk < nb_real, best_init, scale,
-                          exclude_self, out_idx, out_cp, out_dist, out_d2, stats);
+                          exclude_self, out_idx, out_cp, out_dist, out_d2, stats, 0, child_order);
 }
 
 template <bool POINTS, int G, bool PRE, bool STATS = false>
@@ -756,11 +737,12 @@ __device__ __forceinline__ int find_segment(const SegDesc *__restrict__ seg, int
 // WAVES: resident waves per SIMD to compile for. The request server, whose four streams run these
 // launches concurrently, keeps the spill-free 6 (the 8-wave variant, which spills 32 bytes, cut its
 // saturated throughput 7%); the batched plan's single stream takes the 8-wave variant.
-template <bool POINTS, int WAVES>
-__global__ void __launch_bounds__(W_BLOCK, WAVES) k_closest_multi(const SegDesc *__restrict__ seg, int nseg,
-                                                           uint64_t total_waves, uint32_t nb_real) {
+// WPB: waves per workgroup (the launch's block shape), one wave's work per 64 lanes either way.
+template <bool POINTS, int WAVES, int WPB = W_WAVES>
+__global__ void __launch_bounds__(WPB * WAVE, WAVES) k_closest_multi(const SegDesc *__restrict__ seg, int nseg,
+                                                                uint64_t total_waves, uint32_t nb_real) {
   const uint32_t lb_blk = xcd_remap(blockIdx.x, gridDim.x);
-  const int64_t gw = (int64_t)lb_blk * W_WAVES + (threadIdx.x / WAVE);
+  const int64_t gw = (int64_t)lb_blk * WPB + (threadIdx.x / WAVE);
   if (lb_blk >= nb_real || gw >= (int64_t)total_waves) return;
   const int si = uni(find_segment(seg, nseg, gw));
   const SegDesc &d = seg[si];
@@ -769,10 +751,10 @@ __global__ void __launch_bounds__(W_BLOCK, WAVES) k_closest_multi(const SegDesc 
                                             gw - (int64_t)d.wave_begin, true, d.best_init, d.scale, d.exclude_self,
                                             d.idx, d.cp, d.dist, d.d2, nullptr, d.root);
   } else {
-    __shared__ CmpWave s_w[W_WAVES];
+    __shared__ CmpWave s_w[WPB];
     closest_cmp_body<false>(s_w[threadIdx.x / WAVE], (const BNode *)d.nodes, (const TriRec *)d.elems, d.q, d.order,
                             d.n, gw - (int64_t)d.wave_begin, true, d.best_init, d.scale, d.exclude_self, d.idx, d.cp,
-                            d.dist, d.d2, nullptr, d.root);
+                            d.dist, d.d2, nullptr, d.root, d.child_order);
   }
 }
 
@@ -1341,13 +1323,16 @@ __device__ __forceinline__ void knn_wave_body(const BNode *__restrict__ nodes, c
   const int32_t self = exclude_self ? (int32_t)qi : -1;
   const TriRec *__restrict__ tr = (const TriRec *)elems;
   const PointRec *__restrict__ pr = (const PointRec *)elems;
-  // Seed: the k + 1 elements around the nearest leaf in leaf (Morton) order, which are spatially
-  // close; they bound the k-th distance from the start and are skipped by the walk below.
+  // Seed: the elements around the nearest leaf in leaf (Morton) order, which are spatially close;
+  // they bound the k-th distance from the start and are skipped by the walk below. A window of
+  // k + 1 can lie across a jump of the Morton curve (see knn_pts_body), so points take 4k + 1 and
+  // triangles, whose tests cost more, 2k + 1.
   int32_t s0 = 0, s1 = 0;
   if (active && nelem > 0) {
     int32_t f, cnt;
     seed_leaf(nodes, root, qlo, qhi, f, cnt);
-    const int32_t w = nelem < k + 1 ? nelem : k + 1;
+    const int32_t want = POINTS ? 4 * k + 1 : 2 * k + 1;
+    const int32_t w = nelem < want ? nelem : want;
     s0 = f + cnt / 2 - w / 2;
     s0 = s0 < 0 ? 0 : (s0 > nelem - w ? nelem - w : s0);
     s1 = s0 + w;
@@ -1481,6 +1466,220 @@ __device__ __forceinline__ void knn_wave_body(const BNode *__restrict__ nodes, c
   }
 }
 
+//--------------------------------------------------------------------------------------------------
+// k nearest over a point BVH, wave-cooperative, with batched leaves and deferred insertion.
+//
+// Candidates are 64-bit keys (float bits of d^2 << 32 | index). d^2 is never negative, so the
+// unsigned order of the keys is the lexicographic (d^2, index) order of the results; a lane's list
+// is K ascending keys and its admission threshold the k-th. A node's admitted leaves (at most two,
+// 16 points each) are read with one vector load, lane j holding point j of the batch, which is
+// issued together with the scalar load of the next node. Every lane then tests every point of the
+// batch against its k-th key as the batch began (pass 1, a bit per admitted point), and afterwards
+// inserts only its own admitted points, gathering their coordinates from the loading lanes (pass 2).
+// The wave runs the insertion as often as its busiest lane needs it, instead of once for every
+// point some lane admits. A stale threshold admits a superset of what immediate insertion would;
+// pass 2 re-tests each against the current k-th key, and the list is the k smallest keys of all
+// admitted points in any order, so results are those of knn_wave_body.
+//--------------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t knn_key(float d, int32_t id) {
+  return ((uint64_t)(uint32_t)__float_as_int(d) << 32) | (uint32_t)id;
+}
+
+template <int K> __device__ __forceinline__ void key_insert(uint64_t (&kk)[K], uint64_t key) {
+  bool b[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) b[j] = key < kk[j];
+#pragma unroll
+  for (int j = K - 1; j > 0; --j) kk[j] = b[j] ? (b[j - 1] ? kk[j - 1] : key) : kk[j];
+  kk[0] = b[0] ? key : kk[0];
+}
+
+// Key of slot k - 1 (1 <= k <= K); k is uniform.
+template <int K> __device__ __forceinline__ uint64_t key_kth(const uint64_t (&kk)[K], int k) {
+  if (k == K) return kk[K - 1];
+  uint64_t r = kk[K - 1];
+#pragma unroll
+  for (int j = 0; j < K - 1; ++j)
+    if (j == k - 1) r = kk[j];
+  return r;
+}
+
+// STATS: per wave, wstats[4 w ..] = elapsed wall clock (10 ns ticks), nodes entered, points tested,
+// insertion rounds (diagnostics).
+template <int K, bool STATS = false>
+__device__ __forceinline__ void knn_pts_body(const BNode *__restrict__ nodes, const PointRec *__restrict__ pr,
+                                             const float *__restrict__ q, const uint32_t *__restrict__ order,
+                                             int64_t n, int64_t wave, bool wave_ok, int k, float best_init, float scale,
+                                             int exclude_self, int32_t *__restrict__ out_idx,
+                                             float *__restrict__ out_dist, float *__restrict__ out_d2, int32_t nelem,
+                                             int32_t root = 0, int64_t *__restrict__ wstats = nullptr) {
+  const uint64_t t_start = STATS ? wall_clock64() : 0;
+  int64_t st_nodes = 0, st_pts = 0, st_rounds = 0;
+  const int lane = lane_id();
+  const int64_t t = wave * WAVE + lane;
+  const bool inrange = wave_ok && t < n;
+  bool active = inrange;
+  int64_t qi = 0;
+  float3 p = f3(0, 0, 0);
+  if (active) {
+    qi = order ? (int64_t)order[t] : t;
+    p = load_q(q, qi);
+    active = !(isnan(p.x) || isnan(p.y) || isnan(p.z));
+  }
+  uint64_t amask = __ballot(active);
+  const int nact = __popcll(amask);
+  const float m = fmaxf(scale, fmaxf(fabsf(p.x), fmaxf(fabsf(p.y), fabsf(p.z))));
+  const float sl = m * SLACK_REL;
+  const float3 qlo = f3(p.x - sl, p.y - sl, p.z - sl), qhi = f3(p.x + sl, p.y + sl, p.z + sl);
+  // Empty slots hold (best_init, INT_MAX): a point is admitted iff d^2 < bound^2, as for the float
+  // lists. Inactive lanes and a negative bound admit nothing (every key is >= 0).
+  const uint64_t init = (active && best_init >= 0.0f) ? knn_key(best_init, 0x7fffffff) : 0ull;
+  uint64_t kk[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) kk[j] = init;
+  uint64_t kth = init;
+  const int32_t self = exclude_self ? (int32_t)qi : -1;
+  // Seed: the 4k + 1 points around the nearest leaf in leaf (Morton) order (skipped by the walk).
+  // k + 1 points can lie across a jump of the Morton curve for a query near one, and one such loose
+  // bound makes the whole wave walk towards the far side: on the 1M C2 samples the slowest waves
+  // started from k-th bounds up to 200x the true 16-NN distance (2.5 against 0.0146) and took 15x
+  // the median wave's time; 4k + 1 points keep every bound within a few times the true one.
+  int32_t s0 = 0, s1 = 0;
+  if (active && nelem > 0) {
+    int32_t f, cnt;
+    seed_leaf(nodes, root, qlo, qhi, f, cnt);
+    const int32_t w = nelem < 4 * k + 1 ? nelem : 4 * k + 1;
+    s0 = f + cnt / 2 - w / 2;
+    s0 = s0 < 0 ? 0 : (s0 > nelem - w ? nelem - w : s0);
+    s1 = s0 + w;
+    for (int32_t e = s0; e < s1; ++e) {
+      const float4 pe = pr[e].p;
+      const int32_t id = __float_as_int(pe.w);
+      const uint64_t key = knn_key(sqn3(sub3(f3(pe.x, pe.y, pe.z), p)), id);
+      if (id != self && key < kth) {
+        key_insert<K>(kk, key);
+        kth = key_kth<K>(kk, k);
+      }
+    }
+  }
+  int32_t s_ref = 0;
+  float s_lx = 0, s_ly = 0, s_lz = 0, s_hx = 0, s_hy = 0, s_hz = 0;
+  int32_t sp = 0;
+  int32_t node = root;
+  BNode nd = nodes[uni(node)];
+  while (amask != 0) {
+    node = uni(node);
+    if (STATS) ++st_nodes;
+    const float a0 = unif(nd.a.x), a1 = unif(nd.a.y), a2 = unif(nd.a.z), a3 = unif(nd.a.w);
+    const float b0 = unif(nd.b.x), b1 = unif(nd.b.y), b2 = unif(nd.b.z), b3 = unif(nd.b.w);
+    const float c0 = unif(nd.c.x), c1 = unif(nd.c.y), c2 = unif(nd.c.z), c3 = unif(nd.c.w);
+    const int32_t r0 = uni(nd.d.x), r1 = uni(nd.d.y), n0 = uni(nd.d.z), n1 = uni(nd.d.w);
+    // the k-th distance bounds the search ball (pruning compares floats, as in knn_wave_body)
+    const float kb = active ? __int_as_float((int32_t)(kth >> 32)) : -1.0f;
+    const float lb0 = box_lb(qlo, qhi, a0, a1, a2, a3, b0, b1);
+    const float lb1 = box_lb(qlo, qhi, b2, b3, c0, c1, c2, c3);
+    const uint64_t m0 = __ballot(lb0 <= kb), m1 = __ballot(lb1 <= kb);
+    const bool first1 = 2 * __popcll(__ballot(lb1 < lb0) & amask) > nact;
+    int32_t next = -1;
+    {
+      const int cf = first1 ? 1 : 0;
+      const uint64_t mf = cf == 0 ? m0 : m1, ms = cf == 0 ? m1 : m0;
+      const int32_t rf = cf == 0 ? r0 : r1, rs = cf == 0 ? r1 : r0;
+      next = (mf != 0 && rf >= 0) ? rf : ((ms != 0 && rs >= 0) ? rs : -1);
+    }
+    // admitted leaves of this node: [lf0, lf0 + lc0) then [lf1, lf1 + lc1)
+    int32_t lf0 = 0, lc0 = 0, lf1 = 0, lc1 = 0;
+    if (m0 != 0 && r0 < 0) { lf0 = ~r0; lc0 = n0; }
+    if (m1 != 0 && r1 < 0) {
+      if (lc0 == 0) { lf0 = ~r1; lc0 = n1; }
+      else { lf1 = ~r1; lc1 = n1; }
+    }
+    const int32_t total = lc0 + lc1;  // <= 2 x 16 (leaf_size <= 16)
+    float4 pe = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (lane < total) pe = pr[lane < lc0 ? lf0 + lane : lf1 + (lane - lc0)].p;
+    BNode nd_next;
+    if (next >= 0) nd_next = nodes[next];
+    bool entered = false;
+#pragma unroll
+    for (int cc = 0; cc < 2; ++cc) {
+      const int c = first1 ? 1 - cc : cc;
+      const uint64_t mc = c == 0 ? m0 : m1;
+      const int32_t r = c == 0 ? r0 : r1;
+      if (mc == 0 || r < 0) continue;
+      if (!entered && r == next) {
+        entered = true;
+      } else if (sp < W_STACK) {  // never full when depth < 64 (checked by every caller)
+        if (lane == sp) {
+          s_ref = r;
+          if (c == 0) { s_lx = a0; s_ly = a1; s_lz = a2; s_hx = a3; s_hy = b0; s_hz = b1; }
+          else { s_lx = b2; s_ly = b3; s_lz = c0; s_hx = c1; s_hy = c2; s_hz = c3; }
+        }
+        ++sp;
+      }
+    }
+    if (total > 0) {
+      if (STATS) st_pts += total;
+      // pass 1: bit j of `take` = point j of the batch beats this lane's k-th key
+      uint32_t take = 0;
+      for (int32_t j = 0; j < total; ++j) {
+        const float x = lane_readf(pe.x, j), y = lane_readf(pe.y, j), z = lane_readf(pe.z, j);
+        const int32_t id = lane_read(__float_as_int(pe.w), j);
+        const int32_t e = j < lc0 ? lf0 + j : lf1 + (j - lc0);
+        const uint64_t key = knn_key(sqn3(sub3(f3(x, y, z), p)), id);
+        const bool tk = id != self && (e < s0 || e >= s1) && key < kth;
+        take |= tk ? (1u << j) : 0u;
+      }
+      // pass 2: each lane inserts its own admitted points
+      while (__ballot(take != 0)) {
+        if (STATS) ++st_rounds;
+        const bool has = take != 0;
+        const int j = has ? __builtin_ctz(take) : 0;
+        take &= take - 1u;
+        const float x = __shfl(pe.x, j), y = __shfl(pe.y, j), z = __shfl(pe.z, j);
+        const int32_t id = __shfl(__float_as_int(pe.w), j);
+        const uint64_t key = knn_key(sqn3(sub3(f3(x, y, z), p)), id);
+        if (has && key < kth) {
+          key_insert<K>(kk, key);
+          kth = key_kth<K>(kk, k);
+        }
+      }
+    }
+    if (next >= 0) { node = next; nd = nd_next; continue; }
+    bool found = false;
+    const float kbn = active ? __int_as_float((int32_t)(kth >> 32)) : -1.0f;
+    while (sp > 0) {
+      --sp;
+      const float lx = lane_readf(s_lx, sp), ly = lane_readf(s_ly, sp), lz = lane_readf(s_lz, sp);
+      const float hx = lane_readf(s_hx, sp), hy = lane_readf(s_hy, sp), hz = lane_readf(s_hz, sp);
+      if (__ballot(box_lb(qlo, qhi, lx, ly, lz, hx, hy, hz) <= kbn)) {
+        node = lane_read(s_ref, sp);
+        found = true;
+        break;
+      }
+    }
+    if (!found) break;
+    nd = nodes[uni(node)];
+  }
+  if (STATS && lane == 0 && wave_ok) {
+    wstats[4 * wave] = (int64_t)(wall_clock64() - t_start);
+    wstats[4 * wave + 1] = st_nodes;
+    wstats[4 * wave + 2] = st_pts;
+    wstats[4 * wave + 3] = st_rounds;
+  }
+  if (!inrange) return;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    if (j < k) {
+      const int32_t id = (int32_t)(uint32_t)kk[j];
+      const float d = __int_as_float((int32_t)(kk[j] >> 32));
+      const bool hit = active && id != 0x7fffffff;
+      out_idx[qi * k + j] = hit ? id : -1;
+      if (out_dist) out_dist[qi * k + j] = hit ? sqrtf(d) : -1.0f;
+      if (out_d2) out_d2[qi * k + j] = hit ? d : -1.0f;
+    }
+  }
+}
+
 template <bool POINTS, int K>
 __global__ void __launch_bounds__(W_BLOCK) k_knn_wave(const BNode *__restrict__ nodes, const void *__restrict__ elems,
                                                       const float *__restrict__ q, const uint32_t *__restrict__ order,
@@ -1492,6 +1691,19 @@ __global__ void __launch_bounds__(W_BLOCK) k_knn_wave(const BNode *__restrict__ 
   const int64_t wave = (int64_t)lb_blk * W_WAVES + (threadIdx.x / WAVE);
   knn_wave_body<POINTS, K>(nodes, elems, q, order, n, wave, lb_blk < nb_real, k, best_init, scale, exclude_self,
                            out_idx, out_dist, out_d2, nelem);
+}
+
+template <int K, bool STATS = false>
+__global__ void __launch_bounds__(W_BLOCK) k_knn_pts(const BNode *__restrict__ nodes, const PointRec *__restrict__ pr,
+                                                     const float *__restrict__ q, const uint32_t *__restrict__ order,
+                                                     int64_t n, uint32_t nb_real, int k, float best_init, float scale,
+                                                     int exclude_self, int32_t *__restrict__ out_idx,
+                                                     float *__restrict__ out_dist, float *__restrict__ out_d2,
+                                                     int32_t nelem, int64_t *__restrict__ wstats = nullptr) {
+  const uint32_t lb_blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t wave = (int64_t)lb_blk * W_WAVES + (threadIdx.x / WAVE);
+  knn_pts_body<K, STATS>(nodes, pr, q, order, n, wave, lb_blk < nb_real, k, best_init, scale, exclude_self, out_idx,
+                         out_dist, out_d2, nelem, 0, wstats);
 }
 
 template <bool POINTS, int K>
@@ -1526,7 +1738,8 @@ __device__ __forceinline__ float2 slab(float3 o, float3 inv, float w, float lx, 
 // and more waves fit on a CU.
 template <int STACK, bool ANY, bool STATS = false, bool TS = true, int REG = 0>
 __device__ __forceinline__ void ray_body(int32_t *__restrict__ s_ref, float *__restrict__ s_t,
-                                         const BNode *__restrict__ nodes, const TriRec *__restrict__ tr,
+                                         const BNode *__restrict__ nodes, const RayTri *__restrict__ rt,
+                                         const TriRec *__restrict__ tr,
                                          const float *__restrict__ orig, const float *__restrict__ dirs, int64_t ri,
                                          float best_init, float max_time, float scale, int32_t *__restrict__ out_idx,
                                          float *__restrict__ out_t, float *__restrict__ out_n,
@@ -1573,12 +1786,12 @@ __device__ __forceinline__ void ray_body(int32_t *__restrict__ s_ref, float *__r
       if (r < 0) {
         int32_t f = ~r;
         for (int32_t k = f; k < f + cnt; ++k) {
-          const TriRec *t = tr + k;
+          const RayTri *t = rt + k;
           if (STATS) c_tris++;
-          float4 a = t->p0, b = t->p1, cc = t->p2;
-          float th = tri_ray(f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), f3(cc.x, cc.y, cc.z), o, d, max_time);
+          const float4 a = t->a, b = t->b, cc = t->c;
+          float th = tri_ray_e(f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), f3(cc.x, cc.y, cc.z), o, d, max_time);
           if (th >= 0) {
-            int32_t id = trirec_id(t->pl);
+            int32_t id = __float_as_int(a.w);
             if (better(th, id, best, bidx)) {
               best = th;
               bidx = id;
@@ -1645,7 +1858,8 @@ __device__ __forceinline__ void ray_body(int32_t *__restrict__ s_ref, float *__r
 }
 
 template <int STACK, bool ANY, bool TS = true, int REG = 0>
-__global__ void __launch_bounds__(Q_BLOCK) k_ray(const BNode *__restrict__ nodes, const TriRec *__restrict__ tr,
+__global__ void __launch_bounds__(Q_BLOCK) k_ray(const BNode *__restrict__ nodes, const RayTri *__restrict__ rt,
+                                                 const TriRec *__restrict__ tr,
                                                  const float *__restrict__ orig, const float *__restrict__ dirs,
                                                  const uint32_t *__restrict__ order, int64_t n, uint32_t nb_real,
                                                  float best_init, float max_time, float scale,
@@ -1656,13 +1870,14 @@ __global__ void __launch_bounds__(Q_BLOCK) k_ray(const BNode *__restrict__ nodes
   bool active;
   int64_t ri = query_slot(nb_real, n, order, active);
   if (!active) return;
-  ray_body<STACK, ANY, false, TS, TS ? 0 : REG>(s_ref, s_t, nodes, tr, orig, dirs, ri, best_init, max_time, scale,
+  ray_body<STACK, ANY, false, TS, TS ? 0 : REG>(s_ref, s_t, nodes, rt, tr, orig, dirs, ri, best_init, max_time, scale,
                                                 out_idx, out_t, out_n, out_any);
 }
 
 // Per-ray traversal counts (diagnostics): nodes visited, triangles tested, deepest stack, pops.
 template <int STACK>
-__global__ void __launch_bounds__(Q_BLOCK) k_ray_stats(const BNode *__restrict__ nodes, const TriRec *__restrict__ tr,
+__global__ void __launch_bounds__(Q_BLOCK) k_ray_stats(const BNode *__restrict__ nodes, const RayTri *__restrict__ rt,
+                                                       const TriRec *__restrict__ tr,
                                                        const float *__restrict__ orig, const float *__restrict__ dirs,
                                                        int64_t n, float scale, int32_t *__restrict__ out_idx,
                                                        int32_t *__restrict__ counts) {
@@ -1670,7 +1885,7 @@ __global__ void __launch_bounds__(Q_BLOCK) k_ray_stats(const BNode *__restrict__
   __shared__ float s_t[STACK * Q_BLOCK];
   const int64_t ri = (int64_t)blockIdx.x * Q_BLOCK + threadIdx.x;
   if (ri >= n) return;
-  ray_body<STACK, false, true>(s_ref, s_t, nodes, tr, orig, dirs, ri, INFINITY, -1.0f, scale, out_idx, nullptr,
+  ray_body<STACK, false, true>(s_ref, s_t, nodes, rt, tr, orig, dirs, ri, INFINITY, -1.0f, scale, out_idx, nullptr,
                                nullptr, nullptr, 0, counts);
 }
 
@@ -1687,7 +1902,8 @@ __global__ void __launch_bounds__(Q_BLOCK) k_ray_multi(const SegDesc *__restrict
   const int64_t t = ((int64_t)lb_blk - (int64_t)d.wave_begin) * Q_BLOCK + threadIdx.x;
   if (t >= d.n) return;
   const int64_t ri = d.order ? (int64_t)d.order[t] : t;
-  ray_body<STACK, false, false, false, LANE_REG>(s_ref, s_t, (const BNode *)d.nodes, (const TriRec *)d.elems, d.q,
+  ray_body<STACK, false, false, false, LANE_REG>(s_ref, s_t, (const BNode *)d.nodes, (const RayTri *)d.rays,
+                                                 (const TriRec *)d.elems, d.q,
                                                  d.q2, ri, d.best_init, d.max_time, d.scale, d.idx, d.dist, d.cp,
                                                  nullptr, d.root);
 }
@@ -1885,11 +2101,11 @@ void launch_closest(const Bvh &b, const float *q, int64_t n, double dist_bound, 
       if (one_round)
         hipLaunchKernelGGL((k_closest_cmp<false, 1>), dim3(cgrid), dim3(W_BLOCK), 0, s, b.nodes.as<BNode>(),
                            b.elems.as<TriRec>(), q, ws.order, n, cnb, best, b.scale, exclude_self, idx, cp, dist, d2,
-                           nullptr);
+                           nullptr, b.child_order);
       else
         hipLaunchKernelGGL((k_closest_cmp<false, CMP_WAVES>), dim3(cgrid), dim3(W_BLOCK), 0, s, b.nodes.as<BNode>(),
                            b.elems.as<TriRec>(), q, ws.order, n, cnb, best, b.scale, exclude_self, idx, cp, dist, d2,
-                           nullptr);
+                           nullptr, b.child_order);
       launch_check("k_closest_cmp");
       kt.stop();
       return;
@@ -1967,17 +2183,31 @@ static void launch_knn_group_k(const Bvh &b, bool deep, uint32_t gnb, int lg, co
 }
 
 void launch_closest_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, bool points, hipStream_t s,
-                          bool concurrent) {
+                          bool concurrent, int block_waves, bool spill_build) {
   if (nseg <= 0 || total_waves == 0) return;
-  uint32_t nb = ceil_div(total_waves, W_WAVES);
-  uint32_t grid = round_up8(nb);
-  if (points)
-    hipLaunchKernelGGL((k_closest_multi<true, 1>), dim3(grid), dim3(W_BLOCK), 0, s, seg_dev, nseg, total_waves, nb);
-  else if (concurrent)
-    hipLaunchKernelGGL((k_closest_multi<false, 1>), dim3(grid), dim3(W_BLOCK), 0, s, seg_dev, nseg, total_waves, nb);
-  else
-    hipLaunchKernelGGL((k_closest_multi<false, CMP_WAVES>), dim3(grid), dim3(W_BLOCK), 0, s, seg_dev, nseg, total_waves,
+  if (points) {
+    const uint32_t nb = ceil_div(total_waves, W_WAVES);
+    hipLaunchKernelGGL((k_closest_multi<true, 1>), dim3(round_up8(nb)), dim3(W_BLOCK), 0, s, seg_dev, nseg, total_waves,
                        nb);
+    launch_check("k_closest_multi");
+    return;
+  }
+  // block_waves: waves per workgroup (1, 2, 4 or 8); spill_build: the build for 8 resident waves per
+  // SIMD (a 32-byte spill) instead of the compiler's register allocation. Concurrent launches (the
+  // request server's streams) keep the spill-free build.
+  const bool w8 = spill_build && !concurrent;
+  const uint32_t nb = ceil_div(total_waves, (uint64_t)block_waves);
+  const uint32_t grid = round_up8(nb);
+#define THG_CM(W, B)                                                                                                   \
+  hipLaunchKernelGGL((k_closest_multi<false, W, B>), dim3(grid), dim3(B * WAVE), 0, s, seg_dev, nseg, total_waves, nb)
+  if (w8) {
+    if (block_waves == 1) THG_CM(CMP_WAVES, 1); else if (block_waves == 2) THG_CM(CMP_WAVES, 2);
+    else if (block_waves == 8) THG_CM(CMP_WAVES, 8); else THG_CM(CMP_WAVES, 4);
+  } else {
+    if (block_waves == 1) THG_CM(1, 1); else if (block_waves == 2) THG_CM(1, 2);
+    else if (block_waves == 8) THG_CM(1, 8); else THG_CM(1, 4);
+  }
+#undef THG_CM
   launch_check("k_closest_multi");
 }
 
@@ -2067,7 +2297,7 @@ void ray_stats(const Bvh &b, const float *o, const float *d, int64_t n, int32_t 
   const uint32_t nb = ceil_div(n, Q_BLOCK);
   const int st = pick_stack(b.max_depth);
 #define THG_RS(S) hipLaunchKernelGGL(k_ray_stats<S>, dim3(nb), dim3(Q_BLOCK), 0, s, b.nodes.as<BNode>(), \
-                                     b.elems.as<TriRec>(), o, d, n, b.scale, idx, counts)
+                                     b.rays.as<RayTri>(), b.elems.as<TriRec>(), o, d, n, b.scale, idx, counts)
   if (st <= 32) THG_RS(32); else if (st == 64) THG_RS(64); else THG_RS(128);
 #undef THG_RS
   launch_check("k_ray_stats");
@@ -2093,6 +2323,19 @@ void closest_wave_stats(const Bvh &b, const float *q, int64_t n, unsigned long l
   launch_check("k_closest_wave(stats)");
 }
 
+void knn_wave_stats(const Bvh &b, const float *q, int64_t n, int exclude_self, int64_t *wstats, QueryScratch &ws,
+                    hipStream_t s) {
+  require(b.points && b.leaf_size <= 16, "k-NN wave statistics need a point BVH with leaves of at most 16");
+  if (n <= 0 || b.n == 0) return;
+  order_queries(b, q, n, ws, s);
+  AsyncBuf idx(n * 16 * 4, s);
+  const uint32_t wnb = ceil_div(n, W_BLOCK);
+  hipLaunchKernelGGL((k_knn_pts<16, true>), dim3(round_up8(wnb)), dim3(W_BLOCK), 0, s, b.nodes.as<BNode>(),
+                     b.elems.as<PointRec>(), q, ws.order, n, wnb, 16, INFINITY, b.scale, exclude_self, idx.as<int32_t>(),
+                     nullptr, nullptr, (int32_t)b.n, wstats);
+  launch_check("k_knn_pts(stats)");
+}
+
 void launch_knn(const Bvh &b, const float *q, int64_t n, int k, double dist_bound, int exclude_self, int32_t *idx,
                 float *dist, float *d2, QueryScratch &ws, hipStream_t s) {
   if (n <= 0) return;
@@ -2104,15 +2347,23 @@ void launch_knn(const Bvh &b, const float *q, int64_t n, int k, double dist_boun
     uint32_t wnb = ceil_div(n, W_BLOCK);
     uint32_t wgrid = round_up8(wnb);
     KernelTimer kt(b.ctx, s, "knn");
+    // Point BVHs take the batched-leaf kernel (THG_KNN_BATCH=0 selects the one-point-at-a-time walk).
+    static const bool batch = !(getenv("THG_KNN_BATCH") && atoi(getenv("THG_KNN_BATCH")) == 0);
+#define THG_KP(K)                                                                                                     \
+  hipLaunchKernelGGL((k_knn_pts<K>), dim3(wgrid), dim3(W_BLOCK), 0, s, b.nodes.as<BNode>(), b.elems.as<PointRec>(), q, \
+                     ws.order, n, wnb, k, best, b.scale, exclude_self, idx, dist, d2, (int32_t)b.n)
 #define THG_KW(P, K)                                                                                                 \
   hipLaunchKernelGGL((k_knn_wave<P, K>), dim3(wgrid), dim3(W_BLOCK), 0, s, b.nodes.as<BNode>(), b.elems.p, q, ws.order, \
                      n, wnb, k, best, b.scale, exclude_self, idx, dist, d2, (int32_t)b.n)
-    if (b.points) {
+    if (b.points && batch && b.leaf_size <= 16 && k <= 16) {  // K = 32 would spill its 64-bit keys
+      if (k <= 4) THG_KP(4); else if (k <= 8) THG_KP(8); else THG_KP(16);
+    } else if (b.points) {
       if (k <= 4) THG_KW(true, 4); else if (k <= 8) THG_KW(true, 8); else if (k <= 16) THG_KW(true, 16); else THG_KW(true, 32);
     } else {
       if (k <= 4) THG_KW(false, 4); else if (k <= 8) THG_KW(false, 8); else if (k <= 16) THG_KW(false, 16); else THG_KW(false, 32);
     }
 #undef THG_KW
+#undef THG_KP
     launch_check("k_knn_wave");
     kt.stop();
     return;
@@ -2143,6 +2394,7 @@ void launch_ray(const Bvh &b, const float *o, const float *d, int64_t n, float m
                 float *t, float *nrm, uint8_t *hit, QueryScratch &ws, hipStream_t s, bool sort_rays,
                 const uint32_t *given_order) {
   if (n <= 0) return;
+  require(!b.points && (b.n == 0 || b.rays.p != nullptr), "ray queries need a triangle BVH");
   if (sort_rays) order_rays(b, o, d, n, ws, s);
   else ws.order = given_order;
   uint32_t nb = ceil_div(n, Q_BLOCK);
@@ -2159,11 +2411,12 @@ void launch_ray(const Bvh &b, const float *o, const float *d, int64_t n, float m
   }();
 #define THG_RAY(S, A)                                                                                                \
   if (tstack)                                                                                                        \
-    hipLaunchKernelGGL((k_ray<S, A, true>), dim3(grid), dim3(Q_BLOCK), 0, s, b.nodes.as<BNode>(), b.elems.as<TriRec>(), \
-                       o, d, ws.order, n, nb, best, max_time, b.scale, idx, t, nrm, hit);                           \
+    hipLaunchKernelGGL((k_ray<S, A, true>), dim3(grid), dim3(Q_BLOCK), 0, s, b.nodes.as<BNode>(), b.rays.as<RayTri>(), \
+                       b.elems.as<TriRec>(), o, d, ws.order, n, nb, best, max_time, b.scale, idx, t, nrm, hit);      \
   else                                                                                                               \
     hipLaunchKernelGGL((k_ray<S, A, false, LANE_REG>), dim3(grid), dim3(Q_BLOCK), 0, s, b.nodes.as<BNode>(),         \
-                       b.elems.as<TriRec>(), o, d, ws.order, n, nb, best, max_time, b.scale, idx, t, nrm, hit)
+                       b.rays.as<RayTri>(), b.elems.as<TriRec>(), o, d, ws.order, n, nb, best, max_time, b.scale, idx, \
+                       t, nrm, hit)
   if (any) {
     if (st <= 32) THG_RAY(32, true); else if (st == 64) THG_RAY(64, true); else THG_RAY(128, true);
   } else {

@@ -8,11 +8,18 @@ namespace thg {
 // Batches smaller than this are processed in input order (sorting would cost more than it saves).
 constexpr int64_t ORDER_MIN_QUERIES = 4096;
 
+// Child visiting orders of the compacted closest-point kernel (Bvh::child_order, SegDesc::child_order):
+// results never depend on the order, only the work does.
+constexpr int CHILD_MAJORITY = 0;  // the child most of the wave's lanes find nearer
+constexpr int CHILD_LEADER = 1;    // the child the wave's first active query finds nearer
+constexpr int CHILD_BUILD = 2;     // child 0 first
+
 // One batch of a multi-batch launch (k_closest_multi / k_knn_multi / k_ray_multi): the BVH it
 // queries, its inputs and outputs, and the first wave of the launch that works on it.
 struct SegDesc {
   const void *nodes;
   const void *elems;
+  const void *rays;      // RayTri records (ray segments)
   const float *q;        // queries (closest, k-NN) or ray origins
   const float *q2;       // ray directions
   const uint32_t *order; // optional spatial order of the batch
@@ -31,6 +38,7 @@ struct SegDesc {
   int32_t root;          // node the traversal starts from
   int32_t nelem;         // elements of the segment's tree (k-NN seed window)
   int32_t lanes_log2;    // lanes per query (log2) of the grouped sparse kernels
+  int32_t child_order;   // visiting order of the compacted closest-point kernel (0: majority)
 };
 
 // Per-call temporaries of a query batch (spatial order of the queries). By default the sort memory
@@ -81,8 +89,9 @@ void scan_excl_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *tmp, 
 // descriptor table with wave_begin filled in. Requires every segment's BVH to fit the wave stack.
 // concurrent: the launch shares the device with other streams' launches (request server), which
 // selects the variant that needs no scratch memory.
+// block_waves: waves per workgroup (1, 2, 4, 8); spill_build: the 8-resident-wave build.
 void launch_closest_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, bool points, hipStream_t s,
-                          bool concurrent = false);
+                          bool concurrent = false, int block_waves = 4, bool spill_build = true);
 void launch_knn_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, int k, bool points, hipStream_t s);
 // One query per lane over a segment table (stack >= the deepest tree of the segments, at most 64).
 void launch_closest_lane_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, bool points, int stack,
@@ -106,6 +115,9 @@ void closest_wave_stats(const Bvh &b, const float *q, int64_t n, unsigned long l
                         hipStream_t s);
 void launch_knn(const Bvh &b, const float *q, int64_t n, int k, double dist_bound, int exclude_self, int32_t *idx,
                 float *dist, float *d2, QueryScratch &ws, hipStream_t s);
+// Per-wave statistics of the 16-nearest wave kernel on a point BVH (4 int64 per 64-query wave).
+void knn_wave_stats(const Bvh &b, const float *q, int64_t n, int exclude_self, int64_t *wstats, QueryScratch &ws,
+                    hipStream_t s);
 // sort_rays = false traces the rays in the order given by given_order (ray given_order[j] in slot j;
 // nullptr: input order), for callers that make the rays coherent themselves. Results are always
 // written at the rays' input indices.
@@ -118,7 +130,7 @@ void mesh_cdf(Mesh &m, hipStream_t s);
 int64_t launch_sample_approx(Mesh &m, int64_t n, uint64_t seed, uint64_t stream_id, int64_t capacity, float *pos,
                              float *nrm, int32_t *tri, hipStream_t s);
 float graph_build(Bvh *b, const float *d_nrm, int md, int64_t trials, uint64_t seed, hipStream_t s, int32_t *d_idx,
-                  float *d_sep, int32_t *d_deg);
+                  float *d_sep, int32_t *d_deg, const Bvh *surface = nullptr);
 void launch_sample(Mesh &m, int64_t n, uint64_t seed, uint64_t stream_id, float *pos, float *nrm, int32_t *tri,
                    hipStream_t s);
 

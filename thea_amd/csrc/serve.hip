@@ -171,6 +171,7 @@ static void worker_loop(Server *S, int wi) {
             SegDesc d{};
             d.nodes = b.nodes.p;
             d.elems = b.elems.p;
+            d.rays = b.rays.p;
             d.q = q.q;
             d.q2 = q.dirs;
             d.idx = q.idx;
@@ -335,8 +336,8 @@ int thg_server_add_mesh(thg_server srv, const float *verts, int64_t nv, const in
     require(nt > 0, "a served mesh needs at least one triangle");
     require(leaf_size >= 0 && leaf_size <= 16, "leaf_size must be in 0..16");
     size_t mesh_bytes = sm.mesh->verts.bytes + sm.mesh->tris.bytes;
-    // records (128 B) + nodes (at most n - 1 of 64 B) bound the BVH's size before it is built
-    size_t est = mesh_bytes + (size_t)nt * (sizeof(TriRec) + sizeof(BNode));
+    // records (128 + 48 B) + nodes (at most n - 1 of 64 B) bound the BVH's size before it is built
+    size_t est = mesh_bytes + (size_t)nt * (sizeof(TriRec) + sizeof(RayTri) + sizeof(BNode));
     {
       std::lock_guard<std::mutex> g(S->mu);
       if (S->resident + est > S->budget)

@@ -22,6 +22,32 @@ __device__ __forceinline__ float box_lb(float3 qlo, float3 qhi, float lx, float 
   return dx * dx + (dy * dy + dz * dz);
 }
 
+// Per-lane seed of the wave traversals: descend from `root` to the leaf whose box is nearest the
+// query (no backtracking) and return it as [first, first + count). The elements found there give
+// each lane a real upper bound before the shared walk starts, so a wave whose queries are far apart
+// does not admit every node some other lane has not bounded yet. Depth is below 64 for the wave
+// kernels (use_wave_traversal), which bounds the loop.
+__device__ __forceinline__ void seed_leaf(const BNode *__restrict__ nodes, int32_t root, float3 qlo, float3 qhi,
+                                          int32_t &first, int32_t &count) {
+  int32_t nd_i = root;
+  first = 0;
+  count = 0;
+  for (int it = 0; it < 64; ++it) {
+    const BNode nd = nodes[nd_i];
+    const float l0 = box_lb(qlo, qhi, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y);
+    const float l1 = box_lb(qlo, qhi, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w);
+    const bool one = l1 < l0;
+    const int32_t r = one ? nd.d.y : nd.d.x;
+    if (r >= 0) {
+      nd_i = r;
+      continue;
+    }
+    first = ~r;
+    count = one ? nd.d.w : nd.d.z;
+    return;
+  }
+}
+
 // Lexicographic (value, index) order: exact ties go to the lower index.
 __device__ __forceinline__ bool better(float d, int32_t i, float bd, int32_t bi) {
   return d < bd || (d == bd && i < bi);

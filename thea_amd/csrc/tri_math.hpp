@@ -33,9 +33,8 @@ THG_HD float3 cross3(float3 a, float3 b) {
 THG_HD float comp(float3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
 THG_HD float fabs_(float x) { return x < 0 ? -x : (x == 0 ? 0.0f : x); }
 
-// Hyperplane3.hpp:34-41 fromThreePoints normal, with Eigen normalized().
-THG_HD float3 plane_normal(float3 v0, float3 v1, float3 v2) {
-  float3 c = cross3(sub3(v1, v0), sub3(v2, v0));
+// Eigen normalized(): each component divided by sqrt(squaredNorm()), unchanged if that is 0.
+THG_HD float3 normalize3(float3 c) {
   float nn = sqn3(c);
   if (nn > 0) {
     float s = sqrtf(nn);
@@ -43,6 +42,9 @@ THG_HD float3 plane_normal(float3 v0, float3 v1, float3 v2) {
   }
   return c;
 }
+
+// Hyperplane3.hpp:34-41 fromThreePoints normal, with Eigen normalized().
+THG_HD float3 plane_normal(float3 v0, float3 v1, float3 v2) { return normalize3(cross3(sub3(v1, v0), sub3(v2, v0))); }
 
 // MatrixUtil.hpp:108-115 maxAbsAxis (first maximum wins, as Eigen maxCoeff).
 THG_HD int max_abs_axis(float3 n) {
@@ -213,9 +215,10 @@ THG_HD float tri_closest(const TriRec &r, float3 p, float3 &cp) {
 // Triangle3.hpp:1464-1524 rayTriangleIntersectionTime (Eberly) with edges e01 = v1 - v0 and
 // e02 = v2 - v0 (Triangle3.hpp:151-152), then TriangleN::rayIntersection's max_time rule
 // (Triangle3.hpp:542-552). Returns t >= 0 on a hit, -1 otherwise.
-THG_HD float tri_ray(float3 v0, float3 v1, float3 v2, float3 o, float3 dir, float max_time) {
+// Same test from v0 and the edges (RayTri): e01 and e02 computed beforehand with the same
+// subtractions give the same bits.
+THG_HD float tri_ray_e(float3 v0, float3 e01, float3 e02, float3 o, float3 dir, float max_time) {
   const float EPS = 1.19209290e-07f;  // FLT_EPSILON, Math::eps<float>()
-  float3 e01 = sub3(v1, v0), e02 = sub3(v2, v0);
   float3 diff = sub3(o, v0);
   float3 normal = cross3(e01, e02);
   float DdN = dot3(dir, normal);
@@ -237,6 +240,10 @@ THG_HD float tri_ray(float3 v0, float3 v1, float3 v2, float3 o, float3 dir, floa
     }
   }
   return -1.0f;
+}
+
+THG_HD float tri_ray(float3 v0, float3 v1, float3 v2, float3 o, float3 dir, float max_time) {
+  return tri_ray_e(v0, sub3(v1, v0), sub3(v2, v0), o, dir, max_time);
 }
 
 }  // namespace thg

@@ -47,7 +47,9 @@ class Icp3:
 
     def alignBatch(self, from_points, to, initial=None, translation_weights=None, rotation_weights=None):
         """(transforms (K, 3, 4), errors (K,), iterations (K,)) for the source points moved by each of
-        the K initial (3, 4) transforms first (identity when None, K = 1); `to` is a MeshBvh or PointBvh."""
+        the K initial (3, 4) transforms first (identity when None, K = 1); `to` is a MeshBvh or PointBvh
+        (positions rounded to float for the search), or a PointBvhF64 (searched in double, as
+        ShapeAlign's BvhN<Vector3d, 3, double> target, Tools/ShapeAlign/ShapeAlign.cpp:339)."""
         f = np.ascontiguousarray(from_points, np.float64).reshape(-1, 3)
         init = None if initial is None else np.ascontiguousarray(initial, np.float64).reshape(-1, 12)
         K = 1 if init is None else len(init)
