@@ -6,10 +6,14 @@ closest-point pass over the whole 10M-query batch (spatial ordering of the queri
 writing triangle index, closest point and distance per query) with the mesh and its BVH resident
 in HBM and the queries already in device memory. BVH build time is reported separately.
 
-Prints one JSON line. `roofline` uses the algorithmic bytes of Thea's own traversal on the same
-inputs (tests/golden/workload_counts.json) over the traversal kernel's average duration, measured
-with HIP events on the kernel's stream. `cpu_baseline` times the oracle (a C++ restatement of
-Thea's kd-tree and traversal) on a bounded sample of the same queries on this host.
+Prints one JSON line. `roofline` prices the traversal kernel against the ceiling that binds it
+(vector ALU issue or HBM bandwidth, from rocprofv3 counters of the same kernel under profiles/)
+over its average duration, measured with HIP events on the kernel's stream; the bytes Thea's own
+traversal reads on the same inputs (tests/golden/workload_counts.json) ride along as a labelled
+diagnostic. `cpu_baseline` times the oracle (a C++ restatement of Thea's kd-tree and traversal) on
+a bounded sample of the same queries on this host. `served` is the second half of the headline
+metric: the request stream over 64 resident meshes (served requests/s, p50/p99 latency and
+requests per launch, against plain one-call-per-request dispatch), as `--workload c5` runs it.
 """
 import argparse
 import ctypes as C
@@ -22,9 +26,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
-# BVH leaf size per workload, the fastest of 1, 2, 4, 6, 8, 12 measured on one MI355X for that query type
-# (closest point prefers 8, k-NN over points 12, incoherent rays 1); the library default is 4.
-LEAF_DEFAULT = {"c3": 8, "c1": 8, "c3ray": 1, "c2knn": 12, "c2sample": 4, "icp": 8, "graph": 12, "sdf": 1,
+# BVH leaf size per workload, the fastest of 1, 2, 4, 6, 8, 12, 16 measured on one MI355X for that query
+# type (closest point prefers 8, k-NN over points 16, incoherent rays 1); the library default is 4.
+LEAF_DEFAULT = {"c3": 8, "c1": 8, "c3ray": 1, "c2knn": 16, "c2sample": 4, "icp": 8, "graph": 12, "sdf": 1,
                 "vis": 1}
 
 
@@ -40,6 +44,7 @@ def parse():
     p.add_argument("--leaf", type=int, default=None,
                    help="BVH leaf size (default: the measured best per workload, LEAF_DEFAULT)")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-served", action="store_true", help="c3: leave out the served-stream half of the line")
     p.add_argument("--kabsch", action="store_true", help="icp: least-squares fit instead of Thea's")
     p.add_argument("--cpu-threads", type=int, default=None)
     return p.parse_args()
@@ -57,6 +62,78 @@ def load_traffic(name):
         return None
     with open(path) as f:
         return json.load(f).get("hbm_bytes_per_launch")
+
+
+def load_counters(name):
+    path = os.path.join(ROOT, "profiles", f"{name}_counters.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+# Vector ALU ceiling: each of a CU's 4 SIMDs executes 16 lanes of a 32-bit VALU operation per cycle
+# (a wave64 instruction every 4 cycles) at MI355X's 2.4 GHz peak engine clock: 256 CUs x 4 x 16 x
+# 2.4 GHz = 39.3 T lane-operations/s (the published 157.3 TFLOP/s FP32 counts an FMA as 2 and packed
+# FP32 as 2 more; these kernels do neither).
+ENGINE_GHZ = 2.4
+SIMDS_PER_CU = 4
+LANES_PER_SIMD = 16
+CYCLES_PER_VALU = 4
+
+
+def measured_roofline(kernel, wl, kernel_avg_ms, alg_bytes_per_launch, num_cus, model_note):
+    """Roofline object of a traversal kernel against the resource that binds it.
+
+    Two ceilings are priced with rocprofv3 counters of the same kernel on the same workload
+    (profiles/<kernel>_<wl>_counters.json and _traffic.json; the counters are fixed per launch for a
+    fixed input) over the kernel's average duration measured here with HIP events on its stream:
+      - vector ALU: lane operations per launch = SQ_INSTS_VALU wave instructions x 64 x the active-lane
+        fraction (VALUUtilization), against CUs x 4 SIMDs x 16 lanes x 2.4 GHz. Beside it, the issue
+        slots the kernel fills: SQ_INSTS_VALU x 4 cycles / (CUs x 4 SIMDs x kernel cycles) (this can
+        pass 1 where transcendental instructions issue alongside other vector work);
+      - HBM: measured bytes per launch (FETCH_SIZE and WRITE_SIZE scaled by a plain copy of known
+        size) against 8 TB/s.
+    `bound` names the larger fraction. Thea's own traversal bytes (what its kd-tree walk tests on the
+    same inputs) are kept as a labelled diagnostic: most of them are L2/MALL re-reads, not HBM."""
+    sec = kernel_avg_ms * 1e-3
+    traffic = load_traffic(f"{kernel}_{wl}")
+    ctr = load_counters(f"{kernel}_{wl}") or {}
+    valu = ctr.get("SQ_INSTS_VALU_per_dispatch")
+    out = {"kernel": ctr.get("kernel", kernel), "kernel_avg_ms": kernel_avg_ms}
+    hbm = None
+    if traffic:
+        gbps = traffic / sec / 1e9
+        hbm = {"achieved": gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": gbps / HBM_PEAK_GBPS,
+               "bytes_per_launch": traffic, "source": f"profiles/{kernel}_{wl}_traffic.json"}
+    issue = None
+    util = ctr.get("VALUUtilization")
+    if valu and util:
+        rate = valu * 64 * (util / 100.0) / sec / 1e12
+        peak = num_cus * SIMDS_PER_CU * LANES_PER_SIMD * ENGINE_GHZ / 1e3
+        issue = {"achieved": rate, "peak": peak, "unit": "T VALU lane-ops/s", "frac": rate / peak,
+                 "issue_slot_fraction": valu * CYCLES_PER_VALU / (num_cus * SIMDS_PER_CU * sec * ENGINE_GHZ * 1e9),
+                 "active_lane_fraction": util / 100.0, "valu_instructions_per_launch": valu,
+                 "counter_run_kernel_ms": ctr.get("counter_run_kernel_ms"), "l2_hit_rate": ctr.get("l2_hit_rate"),
+                 "source": f"profiles/{kernel}_{wl}_counters.json"}
+    if issue and (not hbm or issue["frac"] >= hbm["frac"]):
+        bound, main = "valu", issue
+    elif hbm:
+        bound, main = "hbm", hbm
+    else:
+        bound, main = None, None
+    if main:
+        out.update({"bound": bound, "achieved": main["achieved"], "peak": main["peak"], "unit": main["unit"],
+                    "frac": main["frac"]})
+    else:
+        out.update({"bound": None, "achieved": None, "peak": None, "unit": None, "frac": None})
+    out["traffic"] = traffic
+    out["valu_issue"] = issue
+    out["hbm"] = hbm
+    alg_gbps = alg_bytes_per_launch / sec / 1e9
+    out["thea_bytes_model"] = {"bytes_per_launch": alg_bytes_per_launch, "GB/s": alg_gbps,
+                               "ratio_to_hbm_peak": alg_gbps / HBM_PEAK_GBPS, "note": model_note}
+    return out
 
 
 def log(msg):
@@ -222,8 +299,13 @@ def main():
     ms_per_step = elapsed * 1e3 / args.steps
     value = nq * args.steps / elapsed
     k_avg = k_ms / max(k_n, 1)
-    achieved = bytes_per_query * nq / (k_avg * 1e-3) / 1e9
-    traffic = load_traffic(f"{kernel}_{wl}")
+    roofline = measured_roofline(
+        kernel, wl, k_avg, bytes_per_query * nq, torch.cuda.get_device_properties(dev).multi_processor_count,
+        "bytes Thea's own traversal reads per query (the boxes and elements its kd-tree tests) on the same "
+        f"inputs: {bytes_per_query:.1f} B/query; most are re-read from L2/MALL, so this is no HBM figure")
+    roofline["bytes_per_query"] = bytes_per_query
+    roofline["thea_traversal_counts"] = metric_counts
+    roofline["kernel_launches"] = k_n
 
     result = {
         "metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -232,13 +314,7 @@ def main():
         "config": {"workload": workload, "triangles": int(len(t)), "queries_per_step": int(nq),
                    "leaf_size": bvh.info["leaf_size"], "bvh_nodes": int(bvh.info["nodes"]),
                    "bvh_max_depth": int(bvh.info["max_depth"])},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
-                     "kernel": f"{kernel} traversal", "kernel_avg_ms": k_avg, "kernel_launches": k_n,
-                     "bytes_per_query": bytes_per_query, "thea_traversal_counts": metric_counts,
-                     "model": "algorithmic bytes = what Thea's own traversal reads per query (boxes and elements "
-                              "it tests) on the same inputs; they exceed the HBM traffic when re-read from L2/MALL, "
-                              "which `traffic` (measured HBM bytes per launch) shows"},
+        "roofline": roofline,
         "build_ms": build_ms,
         "order_kernels_avg_ms": o_ms / max(o_n, 1) if o_n else 0.0,
         "input_generation_s": gen_s,
@@ -259,6 +335,16 @@ def main():
     if not args.no_cpu_baseline and wl in ("c3", "c1", "c3ray", "c2knn", "c2sample"):
         result["cpu_baseline"] = cpu_baseline(args, wl, v, t, q if wl in ("c3", "c1") else None,
                                               (ro, rd) if wl == "c3ray" else None, unit)
+    if wl == "c3" and not args.no_served:
+        del qd, idx, dist, cp
+        bvh.close()
+        mesh.close()
+        torch.cuda.empty_cache()
+        log("served stream (64 resident meshes)")
+        c5 = c5_measure(args)
+        result["metric"] = "closest-pt queries/s (1M tri, 10M q); served p50/p99 + qps vs plain dispatch"
+        result["served"] = {k: c5[k] for k in ("metric", "value", "unit", "config", "saturation", "poisson",
+                                               "requests_per_launch")}
     print(json.dumps(result), flush=True)
 
 
@@ -615,7 +701,7 @@ def run_c4(args, counts):
     import thea_amd
     from thea_amd import _capi as capi
     from thea_amd import workloads as W
-    from thea_amd.batch import BatchPlan, autotune_leaf, concat
+    from thea_amd.batch import BatchPlan, autotune_plan, concat
 
     dev = torch.device("cuda", 0)
     t0 = time.time()
@@ -627,11 +713,11 @@ def run_c4(args, counts):
     V, T, Q = (torch.from_numpy(x).to(dev) for x in (verts, tris, qs))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    leaf = autotune_leaf(nv, nt, nq, V, T, Q)
+    leaf, order, launch = autotune_plan(nv, nt, nq, V, T, Q)
     tune_s = time.perf_counter() - t0
-    log(f"autotuned leaf sizes in {tune_s:.1f} s")
+    log(f"autotuned leaf sizes, child orders and launch shape {launch} in {tune_s:.1f} s")
     t0 = time.perf_counter()
-    plan = BatchPlan(nv, nt, nq, leaf_sizes=leaf)
+    plan = BatchPlan(nv, nt, nq, leaf_sizes=leaf, child_orders=order, launch=launch)
     plan_s = time.perf_counter() - t0
     phases = []
     for _ in range(3):  # eager runs with per-phase HIP events on the plan's stream
@@ -695,7 +781,10 @@ def run_c4(args, counts):
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "batched build + closest point over 512 meshes in one captured graph",
                    "meshes": int(len(nt)), "triangles_total": NT, "queries_total": NQ,
-                   "leaf_sizes": {str(k): int(v) for k, v in zip(*np.unique(leaf, return_counts=True))}},
+                   "leaf_sizes": {str(k): int(v) for k, v in zip(*np.unique(leaf, return_counts=True))},
+                   "child_orders": {["majority", "nearest", "build"][k]: int(v)
+                                    for k, v in zip(*np.unique(order, return_counts=True))},
+                   "launch": {"waves_per_workgroup": launch[0], "eight_wave_build": bool(launch[1])}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
                      "kernel": "build phase" if dom_build else "query phase",
@@ -712,6 +801,10 @@ def run_c4(args, counts):
 
 
 def run_c5(args):
+    print(json.dumps(c5_measure(args)), flush=True)
+
+
+def c5_measure(args):
     """Served request stream: 64 resident meshes (log-uniform 10k-4M triangles), requests with a
     uniform mesh, a log-uniform batch of 256-1M points and a 60/25/15 mix of closest point, k-NN
     (k = 8) and first-hit rays. Compares the server (continuous batching, multi-batch launches on 4
@@ -788,7 +881,7 @@ def run_c5(args):
             lat.append(time.perf_counter() - t_start - a)
         return time.perf_counter() - t_start, np.array(lat) * 1e6
 
-    def server_run(arrivals):
+    def server_run(arrivals, stats=None):
         srv.stats(reset=True)
         rids = [None] * len(arrivals)
         t_start = time.perf_counter()
@@ -806,7 +899,10 @@ def run_c5(args):
             rc, l_us = srv.wait(rid)
             capi.check(rc)
             lat.append(l_us)
-        return time.perf_counter() - t_start, np.array(lat)
+        elapsed = time.perf_counter() - t_start
+        if stats is not None:
+            stats.update(srv.stats())
+        return elapsed, np.array(lat)
 
     log(f"{len(meshes)} meshes resident, {R} requests ({total_q} points) prepared")
     for r in reqs[:8]:  # warm up both paths
@@ -815,16 +911,17 @@ def run_c5(args):
     zero = np.zeros(R)
     t_plain, _ = plain_run(zero)
     log(f"plain dispatch, all queued: {t_plain:.2f} s")
-    t_srv, _ = server_run(zero)
+    st_sat = {}
+    t_srv, _ = server_run(zero, st_sat)
     log(f"server, all queued: {t_srv:.2f} s")
     cap_plain = R / t_plain
     lam = args.load * cap_plain
     arrivals = np.cumsum(g.exponential(1.0 / lam, size=R))
     tp, lat_p = plain_run(arrivals)
     log(f"plain dispatch, Poisson arrivals: {tp:.2f} s")
-    ts, lat_s = server_run(arrivals)
+    st = {}
+    ts, lat_s = server_run(arrivals, st)
     log(f"server, Poisson arrivals: {ts:.2f} s")
-    st = srv.stats()
 
     def summ(t_total, lat):
         return {"requests_per_s": R / t_total, "queries_per_s": total_q / t_total,
@@ -843,10 +940,15 @@ def run_c5(args):
                        "speedup": t_plain / t_srv},
         "poisson": {"rate_requests_per_s": lam, "load_vs_plain_capacity": args.load,
                     "server": summ(ts, lat_s), "plain_dispatch": summ(tp, lat_p)},
-        "server_stats": st, "mesh_generation_s": gen_s, "server_build_s": srv_build_s,
+        "requests_per_launch": {"saturation": st_sat.get("requests_per_launch"),
+                                "poisson": st.get("requests_per_launch"),
+                                "note": "requests a worker takes from the queue and serves with one multi-batch "
+                                        "launch per (type, traversal, k) group"},
+        "server_stats": {"saturation": st_sat, "poisson": st}, "mesh_generation_s": gen_s,
+        "server_build_s": srv_build_s,
     }
-    print(json.dumps(result), flush=True)
     srv.close()
+    return result
 
 
 def cpu_baseline(args, wl, v, t, q, rays, unit):
