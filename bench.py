@@ -28,7 +28,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBPS = 8000.0
 # BVH leaf size per workload, the fastest of 1, 2, 4, 6, 8, 12, 16 measured on one MI355X for that query
 # type (closest point prefers 8, k-NN over points 16, incoherent rays 1); the library default is 4.
-LEAF_DEFAULT = {"c3": 8, "c1": 8, "c3ray": 1, "c2knn": 16, "c2sample": 4, "icp": 8, "graph": 12, "sdf": 1,
+LEAF_DEFAULT = {"c3": 8, "c1": 8, "c3ray": 2, "c2knn": 16, "c2sample": 4, "icp": 8, "graph": 12, "sdf": 1,
                 "vis": 1}
 
 
@@ -93,8 +93,11 @@ def measured_roofline(kernel, wl, kernel_avg_ms, alg_bytes_per_launch, num_cus, 
         slots the kernel fills: SQ_INSTS_VALU x 4 cycles / (CUs x 4 SIMDs x kernel cycles) (this can
         pass 1 where transcendental instructions issue alongside other vector work);
       - HBM: measured bytes per launch (FETCH_SIZE and WRITE_SIZE scaled by a plain copy of known
-        size) against 8 TB/s.
-    `bound` names the larger fraction. Thea's own traversal bytes (what its kd-tree walk tests on the
+        size) against 8 TB/s;
+      - vector memory data path: the fraction of GPU-active cycles the texture data units (TD, one
+        per CU) are busy returning load data (TD_TD_BUSY over GRBM_GUI_ACTIVE), the ceiling of
+        gather-heavy traversals whose loads mostly hit in cache.
+    `bound` names the largest fraction. Thea's own traversal bytes (what its kd-tree walk tests on the
     same inputs) are kept as a labelled diagnostic: most of them are L2/MALL re-reads, not HBM."""
     sec = kernel_avg_ms * 1e-3
     traffic = load_traffic(f"{kernel}_{wl}")
@@ -116,10 +119,14 @@ def measured_roofline(kernel, wl, kernel_avg_ms, alg_bytes_per_launch, num_cus, 
                  "active_lane_fraction": util / 100.0, "valu_instructions_per_launch": valu,
                  "counter_run_kernel_ms": ctr.get("counter_run_kernel_ms"), "l2_hit_rate": ctr.get("l2_hit_rate"),
                  "source": f"profiles/{kernel}_{wl}_counters.json"}
-    if issue and (not hbm or issue["frac"] >= hbm["frac"]):
-        bound, main = "valu", issue
-    elif hbm:
-        bound, main = "hbm", hbm
+    vmem = None
+    if ctr.get("TD_busy_fraction") is not None:
+        vmem = {"achieved": ctr["TD_busy_fraction"], "peak": 1.0, "unit": "TD busy fraction",
+                "frac": ctr["TD_busy_fraction"], "ta_busy_fraction": ctr.get("TA_busy_fraction"),
+                "source": f"profiles/{kernel}_{wl}_counters.json"}
+    cands = [(c["frac"], name, c) for name, c in (("valu", issue), ("hbm", hbm), ("vmem", vmem)) if c]
+    if cands:
+        _, bound, main = max(cands, key=lambda x: x[0])
     else:
         bound, main = None, None
     if main:
@@ -130,6 +137,7 @@ def measured_roofline(kernel, wl, kernel_avg_ms, alg_bytes_per_launch, num_cus, 
     out["traffic"] = traffic
     out["valu_issue"] = issue
     out["hbm"] = hbm
+    out["vmem"] = vmem
     alg_gbps = alg_bytes_per_launch / sec / 1e9
     out["thea_bytes_model"] = {"bytes_per_launch": alg_bytes_per_launch, "GB/s": alg_gbps,
                                "ratio_to_hbm_peak": alg_gbps / HBM_PEAK_GBPS, "note": model_note}

@@ -15,6 +15,7 @@
 // Boxes are exact float min/max of element coordinates; the query kernels add their own rounding
 // slack, so no padding is applied here.
 #include "kernels_common.hpp"
+#include "query.hpp"
 #include "tri_math.hpp"
 
 namespace thg {
@@ -450,6 +451,17 @@ __global__ void __launch_bounds__(B_BLOCK) k_ray_records(const TriRec *__restric
   out[k] = o;
 }
 
+// Flags the last ray record of every non-empty leaf (RayTri::b.w int bits = 1), so ray traversal can
+// run a leaf from its first record without its count.
+__global__ void __launch_bounds__(B_BLOCK) k_leaf_ends(const BNode *__restrict__ nodes, int64_t num_nodes,
+                                                       RayTri *__restrict__ rays) {
+  const int64_t i = (int64_t)blockIdx.x * B_BLOCK + threadIdx.x;
+  if (i >= num_nodes) return;
+  const int4 d = nodes[i].d;
+  if (d.x < 0 && d.z > 0) rays[(int64_t)~d.x + d.z - 1].b.w = __int_as_float(1);
+  if (d.y < 0 && d.w > 0) rays[(int64_t)~d.y + d.w - 1].b.w = __int_as_float(1);
+}
+
 // Single-leaf tree for n <= leaf_size: child 0 holds every element, child 1 is an empty leaf.
 __global__ void k_single_leaf(int64_t n, float3 lo, float3 hi, BNode *out) {
   BNode nd;
@@ -649,9 +661,15 @@ void build_bvh(Bvh &b, const float *verts, const int32_t *tris, int64_t n, bool 
     b.nodes.alloc(sizeof(BNode));
     hipLaunchKernelGGL(k_single_leaf, dim3(1), dim3(1), 0, s, n, b.lo, b.hi, b.nodes.as<BNode>());
     launch_check("k_single_leaf");
+    if (!points) {
+      hipLaunchKernelGGL(k_leaf_ends, dim3(1), dim3(B_BLOCK), 0, s, b.nodes.as<BNode>(), (int64_t)1,
+                         b.rays.as<RayTri>());
+      launch_check("k_leaf_ends");
+    }
     b.num_nodes = 1;
     b.num_leaves = 1;
     b.max_depth = 1;
+    if (!points) build_qbvh4(b, s);
     THG_HIP(hipStreamSynchronize(s));
     return;
   }
@@ -694,6 +712,12 @@ void build_bvh(Bvh &b, const float *verts, const int32_t *tris, int64_t n, bool 
   b.num_nodes = kept;
   b.num_leaves = cnt[0];
   b.max_depth = (int32_t)cnt[1];
+  if (!points) {
+    hipLaunchKernelGGL(k_leaf_ends, dim3(ceil_div(kept, B_BLOCK)), dim3(B_BLOCK), 0, s, b.nodes.as<BNode>(),
+                       (int64_t)kept, b.rays.as<RayTri>());
+    launch_check("k_leaf_ends");
+    build_qbvh4(b, s);
+  }
 }
 
 }  // namespace thg

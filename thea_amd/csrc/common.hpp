@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -35,6 +36,12 @@ inline void hip_check(hipError_t e, const char *what) {
 }
 
 #define THG_HIP(call) ::thg::hip_check((call), #call)
+
+// Integer tuning override from the environment (unset or empty: the default).
+inline int env_int(const char *name, int dflt) {
+  const char *e = std::getenv(name);
+  return e && *e ? std::atoi(e) : dflt;
+}
 
 inline void require(bool ok, const std::string &msg) {
   if (!ok) fail(THG_E_INVALID, msg);
@@ -209,9 +216,12 @@ struct Bvh {
   DevBuf nodes;          // BNode[num_nodes]
   DevBuf elems;          // TriRec[n] or PointRec[n], leaf order
   DevBuf rays;           // RayTri[n] (mesh BVHs), leaf order
+  DevBuf qnodes;         // QNode4[num_qnodes] (mesh BVHs): the 4-wide quantized tree for rays
+  int64_t num_qnodes = 0;
+  int32_t q_depth = 0;   // QNode4 levels
   bool f64 = false;      // point BVH built from double coordinates
   DevBuf elems64;        // PointRecD[n] (f64 only), leaf order
-  size_t device_bytes() const { return nodes.bytes + elems.bytes + rays.bytes + elems64.bytes; }
+  size_t device_bytes() const { return nodes.bytes + elems.bytes + rays.bytes + qnodes.bytes + elems64.bytes; }
 };
 
 inline int float_to_ord_host(float f) {

@@ -5,6 +5,8 @@
 //   TriRec[n]        128 B each (one L2 line): a triangle with every per-triangle constant the
 //                     closest-point and ray tests need, in leaf order, so a leaf is a contiguous run.
 //   RayTri[n]         48 B each (mesh BVHs): v0 and the edges of each triangle, leaf order, for rays.
+//   QNode4[nq]        64 B each (mesh BVHs): the same tree with every other level folded in and the
+//                     child boxes quantized to bytes, for rays.
 //   PointRec[n]       16 B each: point + original index, in leaf order.
 #pragma once
 
@@ -60,13 +62,28 @@ __host__ __device__ __forceinline__ int trirec_id(const float4 &pl) {
 // Ray record of a triangle, in leaf order beside its TriRec: v0 and the two edges Eberly's test
 // starts from (e01 = v1 - v0, e02 = v2 - v0, the same float subtractions as Triangle3.hpp:151-152),
 // and the caller's index. 48 bytes, so a ray's triangle tests read 48 of every 128 bytes a TriRec
-// would take and neighbouring leaves share cache lines.
+// would take and neighbouring leaves share cache lines. b.w flags the last record of a leaf, so a
+// traversal holding only a leaf's first index knows where it ends.
 struct __align__(16) RayTri {
   float4 a;  // v0.xyz, id (int bits)
-  float4 b;  // e01.xyz, 0
+  float4 b;  // e01.xyz, last-of-leaf flag (int bits: 1 or 0)
   float4 c;  // e02.xyz, 0
 };
 static_assert(sizeof(RayTri) == 48, "RayTri must be 48 bytes");
+
+// Quantized 4-wide node of the ray BVH (64 bytes, one per visit as for BNode, but four child boxes
+// instead of two). A child box is stored as 8-bit offsets on a per-node, per-axis power-of-two grid:
+// lo = fmaf(qlo, 2^e, origin), hi = fmaf(qhi, 2^e, origin), rounded outward at build so the decoded
+// box (with exactly that fmaf) contains the child's exact box. Children 0..nchild-1 are valid;
+// refs follow BNode (>= 0 inner QNode4, < 0 leaf ~first in RayTri order, ended by its last flag).
+// The leaf counts sit in the last 8 bytes, which ray traversal does not load (56 of 64 bytes).
+struct __align__(64) QNode4 {
+  float4 a;  // origin xyz, meta (int bits): biased exponents ex | ey << 8 | ez << 16 | nchild << 24
+  uint4 b;   // qlo_x, qlo_y, qlo_z, qhi_x (byte k: child k)
+  int4 c;    // qhi_y, qhi_z (uint bits), ref0, ref1
+  int4 d;    // ref2, ref3, cnt0 | cnt1 << 16, cnt2 | cnt3 << 16 (leaf element counts, 0 for inner)
+};
+static_assert(sizeof(QNode4) == 64, "QNode4 must be 64 bytes");
 
 struct __align__(16) PointRec {
   float4 p;    // xyz, id (int bits)

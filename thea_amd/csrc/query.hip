@@ -1908,6 +1908,245 @@ __global__ void __launch_bounds__(Q_BLOCK) k_ray_multi(const SegDesc *__restrict
                                                  nullptr, d.root);
 }
 
+// Per-lane state of a ray (one ray per lane) and its set-up and output.
+struct RayLane {
+  float3 o, d, inv;
+  float3 olo, ohi;  // o + w and o - w: the widened slab planes are (lo - olo) * inv, (hi - ohi) * inv
+  float best;
+  int32_t bidx, bslot;
+  int64_t ri;
+};
+
+__device__ __forceinline__ void ray_lane_init(RayLane &L, const float *__restrict__ orig,
+                                              const float *__restrict__ dirs, int64_t ri, float best_init,
+                                              float scale) {
+  L.ri = ri;
+  L.o = load_q(orig, ri);
+  L.d = load_q(dirs, ri);
+  L.inv = f3(1.0f / L.d.x, 1.0f / L.d.y, 1.0f / L.d.z);
+  const float m = fmaxf(scale, fmaxf(fabsf(L.o.x), fmaxf(fabsf(L.o.y), fabsf(L.o.z))));
+  const float w = m * SLACK_REL;
+  L.olo = f3(L.o.x + w, L.o.y + w, L.o.z + w);
+  L.ohi = f3(L.o.x - w, L.o.y - w, L.o.z - w);
+  L.best = best_init;
+  L.bidx = 0x7fffffff;
+  L.bslot = -1;
+}
+
+template <bool ANY>
+__device__ __forceinline__ void ray_lane_finish(const RayLane &L, const TriRec *__restrict__ tr,
+                                                int32_t *__restrict__ out_idx, float *__restrict__ out_t,
+                                                float *__restrict__ out_n, uint8_t *__restrict__ out_any) {
+  const bool hit = L.bidx != 0x7fffffff;
+  if (ANY) {
+    out_any[L.ri] = hit ? 1 : 0;
+    return;
+  }
+  out_idx[L.ri] = hit ? L.bidx : -1;
+  if (out_t) out_t[L.ri] = hit ? L.best : -1.0f;
+  if (out_n) {
+    float nx = 0, ny = 0, nz = 0;
+    if (hit) { nx = tr[L.bslot].p0.w; ny = tr[L.bslot].p1.w; nz = tr[L.bslot].p2.w; }
+    out_n[3 * L.ri] = nx; out_n[3 * L.ri + 1] = ny; out_n[3 * L.ri + 2] = nz;
+  }
+}
+
+// Leaf end flag of a RayTri (int bits of b.w).
+__device__ __forceinline__ bool raytri_last(const float4 &b) { return __float_as_int(b.w) != 0; }
+
+// Node reference that ends a lane's traversal (~RAY_DONE would be leaf INT32_MAX: never a real one).
+constexpr int32_t RAY_DONE = INT32_MIN;
+
+// Rays over the 4-wide quantized tree (QNode4), one per lane, in while-while form (Aila and Laine,
+// "Understanding the efficiency of ray traversal on GPUs", HPG 2009): a lane walks inner nodes,
+// postponing the first leaf it meets, and the wave runs its leaf tests together once every lane
+// still walking holds one. Leaves are node references like inner nodes (on the stack too), and a
+// leaf's triangles run up to the record flagged as its last (RayTri::b.w). Results do not depend on
+// the visiting order: a ray's answer is the least (t, index) over the triangles it tests, and a box
+// is skipped only when it cannot hold that.
+// A node's four child boxes are decoded only on the ray's near and far planes per axis (the
+// direction's sign picks qlo or qhi). Fast form, when every direction component of the wave's rays
+// has |1/d| <= 2^40 and the scene is below 2^60: t = fmaf(q, 2^e / d, (origin - o') / d), whose
+// error is below 2^-19 of max(scale, |o|), inside the 2^-16 widening. Otherwise the wave decodes
+// each plane with the build's own fmaf and intersects it as k_ray does, so rays parallel to an axis
+// keep IEEE infinities. The traversal stack holds LST entries per lane in LDS and OVF more in
+// private memory; the host picks OVF so that LST + OVF covers three entries per QNode4 level.
+__device__ __forceinline__ float q_byte(uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xffu); }
+
+// Scenes at or beyond 2^60 take the decoding form (the fast form's terms could overflow).
+__host__ __device__ __forceinline__ int q4_exact_only(float scale) { return scale < 1.152921504606846976e18f ? 0 : 1; }
+
+template <bool ANY, int LST, int OVF>
+__device__ __forceinline__ void ray_q4_body(int32_t *__restrict__ s_ref, const QNode4 *__restrict__ qn,
+                                            const RayTri *__restrict__ rt, const TriRec *__restrict__ tr,
+                                            const float *__restrict__ orig, const float *__restrict__ dirs,
+                                            int64_t ri, float best_init, float max_time, float scale,
+                                            int exact_only, int32_t *__restrict__ out_idx,
+                                            float *__restrict__ out_t, float *__restrict__ out_n,
+                                            uint8_t *__restrict__ out_any) {
+  int32_t ovf[OVF];
+  const int tid = threadIdx.x;
+  const float REL = 1.0f + 1.0f / 4096.0f;
+  RayLane L;
+  ray_lane_init(L, orig, dirs, ri, best_init, scale);
+  const bool px = L.inv.x >= 0.0f, py = L.inv.y >= 0.0f, pz = L.inv.z >= 0.0f;
+  const float3 on = f3(px ? L.olo.x : L.ohi.x, py ? L.olo.y : L.ohi.y, pz ? L.olo.z : L.ohi.z);
+  const float3 of = f3(px ? L.ohi.x : L.olo.x, py ? L.ohi.y : L.olo.y, pz ? L.ohi.z : L.olo.z);
+  const float INV_MAX = 1099511627776.0f;  // 2^40
+  const bool lane_fast = fabsf(L.inv.x) <= INV_MAX && fabsf(L.inv.y) <= INV_MAX && fabsf(L.inv.z) <= INV_MAX;
+  const bool fast = !exact_only && __ballot(!lane_fast) == 0;
+  int sp = 0;
+  auto push = [&](int32_t v) {
+    if (sp < LST) s_ref[sp * Q_BLOCK + tid] = v;
+    else if (sp < LST + OVF) ovf[sp - LST] = v;
+    else return;  // never taken: LST + OVF covers the deepest path
+    ++sp;
+  };
+  auto pop = [&]() -> int32_t {
+    if (sp == 0) return RAY_DONE;
+    --sp;
+    return sp < LST ? s_ref[sp * Q_BLOCK + tid] : ovf[sp - LST];
+  };
+  int32_t node = 0;
+  int32_t leaf = 0;
+  while (node != RAY_DONE || leaf < 0) {
+    while (node >= 0) {
+      const QNode4 *np = qn + node;
+      const float4 na = np->a;
+      const uint4 nb4 = np->b;
+      const int4 nc4 = np->c;
+      const int2 nd2 = *reinterpret_cast<const int2 *>(&np->d);
+      const uint32_t meta = __float_as_uint(na.w);
+      const int nc = (int)(meta >> 24);
+      const float3 sc = f3(__uint_as_float((meta & 0xffu) << 23), __uint_as_float(((meta >> 8) & 0xffu) << 23),
+                           __uint_as_float(((meta >> 16) & 0xffu) << 23));
+      const uint32_t qhy = (uint32_t)nc4.x, qhz = (uint32_t)nc4.y;
+      const uint32_t qnx = px ? nb4.x : nb4.w, qfx = px ? nb4.w : nb4.x;
+      const uint32_t qny = py ? nb4.y : qhy, qfy = py ? qhy : nb4.y;
+      const uint32_t qnz = pz ? nb4.z : qhz, qfz = pz ? qhz : nb4.z;
+      const float lim = L.best * REL;
+      float key[4];
+      int32_t r[4] = {nc4.z, nc4.w, nd2.x, nd2.y};
+      int h = 0;
+      if (fast) {
+        const float3 A = f3(sc.x * L.inv.x, sc.y * L.inv.y, sc.z * L.inv.z);
+        const float3 Bn = f3((na.x - on.x) * L.inv.x, (na.y - on.y) * L.inv.y, (na.z - on.z) * L.inv.z);
+        const float3 Bf = f3((na.x - of.x) * L.inv.x, (na.y - of.y) * L.inv.y, (na.z - of.z) * L.inv.z);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const float t0 = fmaxf(fmaxf(fmaf(q_byte(qnx, k), A.x, Bn.x), fmaf(q_byte(qny, k), A.y, Bn.y)),
+                                 fmaxf(fmaf(q_byte(qnz, k), A.z, Bn.z), 0.0f));
+          const float t1 = fminf(fminf(fmaf(q_byte(qfx, k), A.x, Bf.x), fmaf(q_byte(qfy, k), A.y, Bf.y)),
+                                 fmaf(q_byte(qfz, k), A.z, Bf.z));
+          const bool hit = k < nc && t0 <= t1 * REL && t0 <= lim;
+          key[k] = hit ? fminf(t0, 3.40282347e+38f) : INFINITY;
+          h += hit ? 1 : 0;
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const float t0 =
+              fmaxf(fmaxf((fmaf(q_byte(qnx, k), sc.x, na.x) - on.x) * L.inv.x,
+                          (fmaf(q_byte(qny, k), sc.y, na.y) - on.y) * L.inv.y),
+                    fmaxf((fmaf(q_byte(qnz, k), sc.z, na.z) - on.z) * L.inv.z, 0.0f));
+          const float t1 = fminf(fminf((fmaf(q_byte(qfx, k), sc.x, na.x) - of.x) * L.inv.x,
+                                       (fmaf(q_byte(qfy, k), sc.y, na.y) - of.y) * L.inv.y),
+                                 (fmaf(q_byte(qfz, k), sc.z, na.z) - of.z) * L.inv.z);
+          const bool hit = k < nc && t0 <= t1 * REL && t0 <= lim;
+          key[k] = hit ? fminf(t0, 3.40282347e+38f) : INFINITY;
+          h += hit ? 1 : 0;
+        }
+      }
+      // Nearest first (misses, keyed +inf, sort last).
+#define THG_CSWAP(i, j)                                              \
+  if (key[j] < key[i]) {                                             \
+    const float tk = key[i]; key[i] = key[j]; key[j] = tk;           \
+    const int32_t tr_ = r[i]; r[i] = r[j]; r[j] = tr_;               \
+  }
+      THG_CSWAP(0, 1) THG_CSWAP(2, 3) THG_CSWAP(0, 2) THG_CSWAP(1, 3) THG_CSWAP(1, 2)
+#undef THG_CSWAP
+      if (h == 0) {
+        node = pop();
+      } else {
+        node = r[0];
+        if (h > 3) push(r[3]);
+        if (h > 2) push(r[2]);
+        if (h > 1) push(r[1]);
+      }
+      if (node < 0 && node != RAY_DONE && leaf == 0) {
+        leaf = node;
+        node = pop();
+      }
+      if (__ballot(leaf == 0) == 0) break;
+    }
+    while (leaf < 0) {
+      int32_t k = ~leaf;
+      bool last;
+      do {
+        const RayTri *t = rt + k;
+        const float4 a = t->a, b = t->b, c = t->c;
+        last = raytri_last(b);
+        const float th = tri_ray_e(f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), f3(c.x, c.y, c.z), L.o, L.d, max_time);
+        if (th >= 0) {
+          const int32_t id = __float_as_int(a.w);
+          if (better(th, id, L.best, L.bidx)) {
+            L.best = th;
+            L.bidx = id;
+            L.bslot = k;
+            if (ANY) break;
+          }
+        }
+        ++k;
+      } while (!last);
+      if (ANY && L.bidx != 0x7fffffff) {
+        node = RAY_DONE;
+        leaf = 0;
+        break;
+      }
+      leaf = 0;
+      if (node < 0 && node != RAY_DONE) {
+        leaf = node;
+        node = pop();
+      }
+    }
+  }
+  ray_lane_finish<ANY>(L, tr, out_idx, out_t, out_n, out_any);
+}
+
+template <bool ANY, int LST, int OVF>
+__global__ void __launch_bounds__(Q_BLOCK) k_ray_q4(const QNode4 *__restrict__ qn, const RayTri *__restrict__ rt,
+                                                    const TriRec *__restrict__ tr, const float *__restrict__ orig,
+                                                    const float *__restrict__ dirs, const uint32_t *__restrict__ order,
+                                                    int64_t n, uint32_t nb_real, float best_init, float max_time,
+                                                    float scale, int exact_only, int32_t *__restrict__ out_idx,
+                                                    float *__restrict__ out_t, float *__restrict__ out_n,
+                                                    uint8_t *__restrict__ out_any) {
+  __shared__ int32_t s_ref[LST * Q_BLOCK];
+  bool active;
+  const int64_t ri = query_slot(nb_real, n, order, active);
+  if (!active) return;
+  ray_q4_body<ANY, LST, OVF>(s_ref, qn, rt, tr, orig, dirs, ri, best_init, max_time, scale, exact_only, out_idx,
+                             out_t, out_n, out_any);
+}
+
+// Many first-hit ray batches in one launch over their meshes' QNode4 trees; one 64-lane block per
+// wave of a segment (SEG_WAVE_QUERIES rays).
+template <int LST, int OVF>
+__global__ void __launch_bounds__(Q_BLOCK) k_ray_q4_multi(const SegDesc *__restrict__ seg, int nseg,
+                                                          uint64_t total_waves, uint32_t nb_real) {
+  __shared__ int32_t s_ref[LST * Q_BLOCK];
+  const uint32_t lb_blk = xcd_remap(blockIdx.x, gridDim.x);
+  if (lb_blk >= nb_real || (uint64_t)lb_blk >= total_waves) return;
+  const int si = uni(find_segment(seg, nseg, (int64_t)lb_blk));
+  const SegDesc &d = seg[si];
+  const int64_t t = ((int64_t)lb_blk - (int64_t)d.wave_begin) * Q_BLOCK + threadIdx.x;
+  if (t >= d.n) return;
+  const int64_t ri = d.order ? (int64_t)d.order[t] : t;
+  ray_q4_body<false, LST, OVF>(s_ref, (const QNode4 *)d.qnodes, (const RayTri *)d.rays, (const TriRec *)d.elems, d.q,
+                               d.q2, ri, d.best_init, d.max_time, d.scale, q4_exact_only(d.scale), d.idx, d.dist,
+                               d.cp, nullptr);
+}
+
 //--------------------------------------------------------------------------------------------------
 // Spatial ordering of a query batch
 //--------------------------------------------------------------------------------------------------
@@ -2283,10 +2522,20 @@ void launch_knn_group_multi(const SegDesc *seg_dev, int nseg, uint64_t total_wav
   launch_check("k_knn_group_multi");
 }
 
-void launch_ray_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, int stack, hipStream_t s) {
+void launch_ray_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, int stack, hipStream_t s,
+                      int q_stack) {
   if (nseg <= 0 || total_waves == 0) return;
   uint32_t nb = (uint32_t)total_waves;
   uint32_t grid = round_up8(nb);
+  if (q_stack > 0) {
+    require(q_stack <= Q4_MAX_STACK, "ray tree too deep for the 4-wide traversal stack");
+    if (q_stack <= 16 + 32)
+      hipLaunchKernelGGL((k_ray_q4_multi<16, 32>), dim3(grid), dim3(Q_BLOCK), 0, s, seg_dev, nseg, total_waves, nb);
+    else
+      hipLaunchKernelGGL((k_ray_q4_multi<16, 96>), dim3(grid), dim3(Q_BLOCK), 0, s, seg_dev, nseg, total_waves, nb);
+    launch_check("k_ray_q4_multi");
+    return;
+  }
   if (stack <= 32) hipLaunchKernelGGL(k_ray_multi<32>, dim3(grid), dim3(Q_BLOCK), 0, s, seg_dev, nseg, total_waves, nb);
   else hipLaunchKernelGGL(k_ray_multi<64>, dim3(grid), dim3(Q_BLOCK), 0, s, seg_dev, nseg, total_waves, nb);
   launch_check("k_ray_multi");
@@ -2403,6 +2652,25 @@ void launch_ray(const Bvh &b, const float *o, const float *d, int64_t n, float m
   float best = max_time >= 0 ? prev_float(max_time) : INFINITY;
   int st = pick_stack(b.max_depth);
   KernelTimer kt(b.ctx, s, any ? "any_hit" : "ray");
+  // Quantized 4-wide tree (k_ray_q4; THG_RAY_Q4=0 selects the binary kernels below) when its stack
+  // bound (three entries per level) fits 16 LDS + 96 private entries.
+  static const bool q4 = env_int("THG_RAY_Q4", 1) != 0;
+  if (q4 && b.qnodes.p && q4_stack(b) <= Q4_MAX_STACK) {
+    const int exact_only = q4_exact_only(b.scale);
+    const bool small = q4_stack(b) <= 16 + 32;
+#define THG_RQ(A, O)                                                                                                 \
+  hipLaunchKernelGGL((k_ray_q4<A, 16, O>), dim3(grid), dim3(Q_BLOCK), 0, s, b.qnodes.as<QNode4>(), b.rays.as<RayTri>(), \
+                     b.elems.as<TriRec>(), o, d, ws.order, n, nb, best, max_time, b.scale, exact_only, idx, t, nrm, hit)
+    if (any) {
+      if (small) { THG_RQ(true, 32); } else { THG_RQ(true, 96); }
+    } else {
+      if (small) { THG_RQ(false, 32); } else { THG_RQ(false, 96); }
+    }
+#undef THG_RQ
+    launch_check("k_ray_q4");
+    kt.stop();
+    return;
+  }
   // Entry times on the stack (THG_RAY_TSTACK=1) cost half the LDS and with it half the resident
   // waves; on C3 rays the ref-only stack is 1.7x faster.
   static const bool tstack = [] {

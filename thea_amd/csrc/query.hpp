@@ -20,6 +20,7 @@ struct SegDesc {
   const void *nodes;
   const void *elems;
   const void *rays;      // RayTri records (ray segments)
+  const void *qnodes;    // QNode4 tree (ray segments)
   const float *q;        // queries (closest, k-NN) or ray origins
   const float *q2;       // ray directions
   const uint32_t *order; // optional spatial order of the batch
@@ -64,6 +65,8 @@ size_t sort_scratch_words(int64_t cap);
 // [0, nv); synchronises `s`.
 void check_tri_indices(const int32_t *tris, int64_t nt, int64_t nv, hipStream_t s);
 void build_bvh(Bvh &b, const float *verts, const int32_t *tris, int64_t n, bool points, int leaf_size, hipStream_t s);
+// The 4-wide quantized ray tree (Bvh::qnodes) of a built mesh BVH; synchronises `s`.
+void build_qbvh4(Bvh &b, hipStream_t s);
 void order_queries(const Bvh &b, const float *q, int64_t n, QueryScratch &ws, hipStream_t s);
 void order_rays(const Bvh &b, const float *o, const float *d, int64_t n, QueryScratch &ws, hipStream_t s);
 float bound_to_best(double dist_bound);
@@ -107,8 +110,13 @@ void launch_knn_group_multi(const SegDesc *seg_dev, int nseg, uint64_t total_wav
 // log2 of the lanes per query the sparse kernels give a batch of n queries on a tree of this depth.
 int group_lanes_log2(int64_t n, int depth);
 bool use_wave_traversal(const Bvh &b, int64_t n);
-// stack: the deepest tree among the segments plus one (32 or 64 entries).
-void launch_ray_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, int stack, hipStream_t s);
+// stack: the deepest tree among the segments plus one (32 or 64 entries). q_stack > 0 traverses
+// the segments' QNode4 trees instead; it must be at least q4_stack() of every segment's BVH.
+void launch_ray_multi(const SegDesc *seg_dev, int nseg, uint64_t total_waves, int stack, hipStream_t s,
+                      int q_stack = 0);
+// Stack entries a QNode4 traversal of b may need (three per level) and the most the kernels hold.
+inline int q4_stack(const Bvh &b) { return 3 * b.q_depth + 1; }
+constexpr int Q4_MAX_STACK = 16 + 96;
 constexpr int SEG_WAVE_QUERIES = 64;  // queries per wave in the multi-batch kernels
 void ray_stats(const Bvh &b, const float *o, const float *d, int64_t n, int32_t *idx, int32_t *counts, hipStream_t s);
 void closest_wave_stats(const Bvh &b, const float *q, int64_t n, unsigned long long *stats, QueryScratch &ws,

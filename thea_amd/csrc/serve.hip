@@ -172,6 +172,7 @@ static void worker_loop(Server *S, int wi) {
             d.nodes = b.nodes.p;
             d.elems = b.elems.p;
             d.rays = b.rays.p;
+            d.qnodes = b.qnodes.p;
             d.q = q.q;
             d.q2 = q.dirs;
             d.idx = q.idx;
@@ -211,7 +212,13 @@ static void worker_loop(Server *S, int wi) {
             else if (lane) launch_knn_lane_multi(dd, nseg, waves, kk.first, false, stack, s);
             else launch_knn_multi(dd, nseg, waves, kk.first, false, s);
           } else {
-            launch_ray_multi(dd, nseg, waves, stack, s);
+            int q_stack = 0;
+            bool q4 = true;
+            for (SrvRequest *r : list) {
+              q4 = q4 && r->bvh->qnodes.p != nullptr && q4_stack(*r->bvh) <= Q4_MAX_STACK;
+              q_stack = std::max(q_stack, q4_stack(*r->bvh));
+            }
+            launch_ray_multi(dd, nseg, waves, stack, s, q4 ? q_stack : 0);
           }
         }
       }
@@ -336,8 +343,10 @@ int thg_server_add_mesh(thg_server srv, const float *verts, int64_t nv, const in
     require(nt > 0, "a served mesh needs at least one triangle");
     require(leaf_size >= 0 && leaf_size <= 16, "leaf_size must be in 0..16");
     size_t mesh_bytes = sm.mesh->verts.bytes + sm.mesh->tris.bytes;
-    // records (128 + 48 B) + nodes (at most n - 1 of 64 B) bound the BVH's size before it is built
-    size_t est = mesh_bytes + (size_t)nt * (sizeof(TriRec) + sizeof(RayTri) + sizeof(BNode));
+    // records (128 + 48 B) + binary nodes (at most n - 1 of 64 B) + 4-wide ray nodes (the binary
+    // nodes at even depth, at most about two thirds of them) bound the BVH's size before it is built
+    size_t est = mesh_bytes + (size_t)nt * (sizeof(TriRec) + sizeof(RayTri) + sizeof(BNode)) +
+                 (size_t)nt * sizeof(QNode4) * 2 / 3;
     {
       std::lock_guard<std::mutex> g(S->mu);
       if (S->resident + est > S->budget)
