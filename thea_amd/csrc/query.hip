@@ -1976,7 +1976,7 @@ __device__ __forceinline__ float q_byte(uint32_t w, int k) { return (float)((w >
 // Scenes at or beyond 2^60 take the decoding form (the fast form's terms could overflow).
 __host__ __device__ __forceinline__ int q4_exact_only(float scale) { return scale < 1.152921504606846976e18f ? 0 : 1; }
 
-template <bool ANY, int LST, int OVF>
+template <bool ANY, int LST, int OVF, bool POST>
 __device__ __forceinline__ void ray_q4_body(int32_t *__restrict__ s_ref, const QNode4 *__restrict__ qn,
                                             const RayTri *__restrict__ rt, const TriRec *__restrict__ tr,
                                             const float *__restrict__ orig, const float *__restrict__ dirs,
@@ -2006,6 +2006,28 @@ __device__ __forceinline__ void ray_q4_body(int32_t *__restrict__ s_ref, const Q
     if (sp == 0) return RAY_DONE;
     --sp;
     return sp < LST ? s_ref[sp * Q_BLOCK + tid] : ovf[sp - LST];
+  };
+  // The triangles of leaf lf; true when an any-hit query has its answer.
+  auto run_leaf = [&](int32_t lf) -> bool {
+    int32_t k = ~lf;
+    bool last;
+    do {
+      const RayTri *t = rt + k;
+      const float4 a = t->a, b = t->b, c = t->c;
+      last = raytri_last(b);
+      const float th = tri_ray_e(f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), f3(c.x, c.y, c.z), L.o, L.d, max_time);
+      if (th >= 0) {
+        const int32_t id = __float_as_int(a.w);
+        if (better(th, id, L.best, L.bidx)) {
+          L.best = th;
+          L.bidx = id;
+          L.bslot = k;
+          if (ANY) return true;
+        }
+      }
+      ++k;
+    } while (!last);
+    return false;
   };
   int32_t node = 0;
   int32_t leaf = 0;
@@ -2073,6 +2095,11 @@ __device__ __forceinline__ void ray_q4_body(int32_t *__restrict__ s_ref, const Q
         if (h > 2) push(r[2]);
         if (h > 1) push(r[1]);
       }
+      if (!POST) {
+        // Leaves are tested as soon as they are met (any hit: the first hit ends the ray).
+        while (node < 0 && node != RAY_DONE) node = run_leaf(node) ? RAY_DONE : pop();
+        continue;
+      }
       if (node < 0 && node != RAY_DONE && leaf == 0) {
         leaf = node;
         node = pop();
@@ -2080,25 +2107,7 @@ __device__ __forceinline__ void ray_q4_body(int32_t *__restrict__ s_ref, const Q
       if (__ballot(leaf == 0) == 0) break;
     }
     while (leaf < 0) {
-      int32_t k = ~leaf;
-      bool last;
-      do {
-        const RayTri *t = rt + k;
-        const float4 a = t->a, b = t->b, c = t->c;
-        last = raytri_last(b);
-        const float th = tri_ray_e(f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), f3(c.x, c.y, c.z), L.o, L.d, max_time);
-        if (th >= 0) {
-          const int32_t id = __float_as_int(a.w);
-          if (better(th, id, L.best, L.bidx)) {
-            L.best = th;
-            L.bidx = id;
-            L.bslot = k;
-            if (ANY) break;
-          }
-        }
-        ++k;
-      } while (!last);
-      if (ANY && L.bidx != 0x7fffffff) {
+      if (run_leaf(leaf)) {
         node = RAY_DONE;
         leaf = 0;
         break;
@@ -2113,7 +2122,7 @@ __device__ __forceinline__ void ray_q4_body(int32_t *__restrict__ s_ref, const Q
   ray_lane_finish<ANY>(L, tr, out_idx, out_t, out_n, out_any);
 }
 
-template <bool ANY, int LST, int OVF>
+template <bool ANY, int LST, int OVF, bool POST>
 __global__ void __launch_bounds__(Q_BLOCK) k_ray_q4(const QNode4 *__restrict__ qn, const RayTri *__restrict__ rt,
                                                     const TriRec *__restrict__ tr, const float *__restrict__ orig,
                                                     const float *__restrict__ dirs, const uint32_t *__restrict__ order,
@@ -2125,8 +2134,8 @@ __global__ void __launch_bounds__(Q_BLOCK) k_ray_q4(const QNode4 *__restrict__ q
   bool active;
   const int64_t ri = query_slot(nb_real, n, order, active);
   if (!active) return;
-  ray_q4_body<ANY, LST, OVF>(s_ref, qn, rt, tr, orig, dirs, ri, best_init, max_time, scale, exact_only, out_idx,
-                             out_t, out_n, out_any);
+  ray_q4_body<ANY, LST, OVF, POST>(s_ref, qn, rt, tr, orig, dirs, ri, best_init, max_time, scale, exact_only,
+                                   out_idx, out_t, out_n, out_any);
 }
 
 // Many first-hit ray batches in one launch over their meshes' QNode4 trees; one 64-lane block per
@@ -2142,7 +2151,7 @@ __global__ void __launch_bounds__(Q_BLOCK) k_ray_q4_multi(const SegDesc *__restr
   const int64_t t = ((int64_t)lb_blk - (int64_t)d.wave_begin) * Q_BLOCK + threadIdx.x;
   if (t >= d.n) return;
   const int64_t ri = d.order ? (int64_t)d.order[t] : t;
-  ray_q4_body<false, LST, OVF>(s_ref, (const QNode4 *)d.qnodes, (const RayTri *)d.rays, (const TriRec *)d.elems, d.q,
+  ray_q4_body<false, LST, OVF, true>(s_ref, (const QNode4 *)d.qnodes, (const RayTri *)d.rays, (const TriRec *)d.elems, d.q,
                                d.q2, ri, d.best_init, d.max_time, d.scale, q4_exact_only(d.scale), d.idx, d.dist,
                                d.cp, nullptr);
 }
@@ -2658,14 +2667,21 @@ void launch_ray(const Bvh &b, const float *o, const float *d, int64_t n, float m
   if (q4 && b.qnodes.p && q4_stack(b) <= Q4_MAX_STACK) {
     const int exact_only = q4_exact_only(b.scale);
     const bool small = q4_stack(b) <= 16 + 32;
-#define THG_RQ(A, O)                                                                                                 \
-  hipLaunchKernelGGL((k_ray_q4<A, 16, O>), dim3(grid), dim3(Q_BLOCK), 0, s, b.qnodes.as<QNode4>(), b.rays.as<RayTri>(), \
-                     b.elems.as<TriRec>(), o, d, ws.order, n, nb, best, max_time, b.scale, exact_only, idx, t, nrm, hit)
+    // Postponed leaves (while-while) for first hits, leaves tested when met for any hit
+    // (THG_RAY_POST / THG_RAY_ANY_POST override).
+    static const bool post = env_int("THG_RAY_POST", 1) != 0, any_post = env_int("THG_RAY_ANY_POST", 0) != 0;
+#define THG_RQ(A, O, P)                                                                                               \
+  hipLaunchKernelGGL((k_ray_q4<A, 16, O, P>), dim3(grid), dim3(Q_BLOCK), 0, s, b.qnodes.as<QNode4>(),               \
+                     b.rays.as<RayTri>(), b.elems.as<TriRec>(), o, d, ws.order, n, nb, best, max_time, b.scale,     \
+                     exact_only, idx, t, nrm, hit)
+#define THG_RQP(A, O, P) \
+  if (P) { THG_RQ(A, O, true); } else { THG_RQ(A, O, false); }
     if (any) {
-      if (small) { THG_RQ(true, 32); } else { THG_RQ(true, 96); }
+      if (small) { THG_RQP(true, 32, any_post); } else { THG_RQP(true, 96, any_post); }
     } else {
-      if (small) { THG_RQ(false, 32); } else { THG_RQ(false, 96); }
+      if (small) { THG_RQP(false, 32, post); } else { THG_RQP(false, 96, post); }
     }
+#undef THG_RQP
 #undef THG_RQ
     launch_check("k_ray_q4");
     kt.stop();
