@@ -441,7 +441,8 @@ def run_sdf(args):
         "data": "synthetic",
         "config": {"workload": "ShapeDiameter at C3 area samples with face normals", "points": n, "rays": 30 * n,
                    "leaf_size": args.leaf, "scale": scale},
-        "roofline": ray_roofline("c3_sdf_ray", 30 * n, k_ms / max(k_n, 1), "first-hit traversal"),
+        "roofline": ray_roofline("c3_sdf_ray", 30 * n, k_ms / max(args.steps, 1), k_n / max(args.steps, 1), "ray",
+                                 "sdf"),
         "ray_kernel_avg_ms": k_ms / max(k_n, 1), "rays_per_s_kernel": 30 * n / (k_ms / max(k_n, 1) * 1e-3),
         "median_sdf": float(sdf[sdf >= 0].median()),
     }
@@ -459,19 +460,25 @@ def run_sdf(args):
     print(json.dumps(result), flush=True)
 
 
-def ray_roofline(counts_key, rays, kernel_ms, kernel):
-    """Roofline object of a ray feature's traversal: Thea's per-ray box and triangle tests on the same
-    rays (tests/golden/workload_counts.json) priced at 32 + 24 V_box + 36 V_tri bytes per ray, over
-    the traversal kernels' time for the batch (HIP events on their stream)."""
+def ray_roofline(counts_key, rays, kernel_ms, launches, kernel, wl):
+    """Roofline object of a ray feature's traversal kernel (kernel_ms: its total time for the batch,
+    over `launches` launches, HIP events on its stream), against the measured ceilings of
+    measured_roofline (profiles/<kernel>_<wl>_*.json). Thea's per-ray box and triangle tests on the same
+    rays (tests/golden/workload_counts.json) priced at 32 + 24 V_box + 36 V_tri bytes per ray are
+    kept as a labelled diagnostic."""
+    import torch
     c = load_counts().get(counts_key)
-    if not c or kernel_ms <= 0:
+    if not c or kernel_ms <= 0 or launches <= 0:
         return None
     bpr = 32 + 24 * c["V_box"] + 36 * c["V_tri"]
-    achieved = bpr * rays / (kernel_ms * 1e-3) / 1e9
-    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBPS, "traffic": None, "kernel": kernel, "kernel_ms_per_step": kernel_ms,
-            "bytes_per_ray": bpr, "thea_traversal_counts": c,
-            "model": "algorithmic bytes of Thea's own first-hit traversal of the same rays (sample of the batch)"}
+    out = measured_roofline(kernel, wl, kernel_ms / launches, bpr * rays / launches,
+                            torch.cuda.get_device_properties(0).multi_processor_count,
+                            f"bytes Thea's own traversal reads per ray (the boxes and triangles its kd-tree tests, "
+                            f"sample of the batch): {bpr:.1f} B/ray; most are re-read from L2/MALL")
+    out["bytes_per_ray"] = bpr
+    out["thea_traversal_counts"] = c
+    out["kernel_launches_per_batch"] = launches
+    return out
 
 
 def run_vis(args):
@@ -507,7 +514,8 @@ def run_vis(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 rays, f64 fractions",
         "data": "synthetic",
         "config": {"workload": "Visibility at C3 area samples", "points": n, "rays": 100 * n, "leaf_size": args.leaf},
-        "roofline": ray_roofline("c3_vis_ray", 100 * n, k_ms / max(args.steps, 1), "any-hit traversal"),
+        "roofline": ray_roofline("c3_vis_ray", 100 * n, k_ms / max(args.steps, 1), k_n / max(args.steps, 1),
+                                 "any_hit", "vis"),
         "any_hit_ms_per_batch": k_ms / max(args.steps, 1), "any_hit_launches_per_batch": k_n / max(args.steps, 1),
         "rays_per_s_kernel": 100 * n * args.steps / (k_ms * 1e-3) if k_ms > 0 else None,
         "mean_visibility": float(vis.mean()),
@@ -779,9 +787,14 @@ def run_c4(args, counts):
     c = counts["c4_closest"]
     bq = 32 + 24 * c["V_box"] + 36 * c["V_tri"]
     build_gbps = 244.0 * NT / (build_ms * 1e-3) / 1e9
-    query_gbps = bq * NQ / (query_ms * 1e-3) / 1e9
-    dom_build = build_ms >= query_ms
-    achieved = build_gbps if dom_build else query_gbps
+    # The query phase (query sort + one multi-segment closest-point launch) against the measured
+    # ceilings of its kernel; the build phase's bytes and Thea's traversal bytes are diagnostics.
+    roof = measured_roofline("closest_multi", "c4", query_ms, bq * NQ, torch.cuda.get_device_properties(dev).multi_processor_count,
+                             f"bytes Thea's own traversal reads per query: {bq:.1f} B/query (per-mesh means "
+                             f"weighted by query count); most are re-read from L2/MALL")
+    roof["kernel_avg_ms_note"] = "query phase: query-key sort and the multi-segment launch, HIP events"
+    roof["build_phase"] = {"ms": build_ms, "bytes_per_triangle": 244.0, "GB/s": build_gbps}
+    roof["query_phase"] = {"ms": query_ms, "bytes_per_query": bq, "thea_traversal_counts": c}
     result = {
         "metric": "batched many-mesh build+query queries/s (512 meshes, 1k-500k tris, 1k-200k queries each)",
         "value": NQ / (ms_per_step * 1e-3), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
@@ -793,12 +806,7 @@ def run_c4(args, counts):
                    "child_orders": {["majority", "nearest", "build"][k]: int(v)
                                     for k, v in zip(*np.unique(order, return_counts=True))},
                    "launch": {"waves_per_workgroup": launch[0], "eight_wave_build": bool(launch[1])}},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
-                     "kernel": "build phase" if dom_build else "query phase",
-                     "build_phase": {"ms": build_ms, "bytes_per_triangle": 244.0, "GB/s": build_gbps},
-                     "query_phase": {"ms": query_ms, "bytes_per_query": bq, "GB/s": query_gbps,
-                                     "thea_traversal_counts": c}},
+        "roofline": roof,
         "graph_replay_ms": ms_per_step, "eager_ms": build_ms + query_ms,
         "plain_dispatch": {"seconds": plain_s, "queries_per_s": NQ / plain_s, "speedup": plain_s / (ms_per_step * 1e-3),
                            "mismatches_vs_batched": mism},
