@@ -398,8 +398,9 @@ int thg_shape_diameter(thg_bvh mesh_bvh, const float *positions, const float *no
  * Request server (Thea has no analogue: its queries run synchronously on the caller's thread).
  * Keeps meshes and their BVHs resident on one GPU under a memory budget and serves closest-point,
  * k-NN and ray requests with continuous batching: whenever one of `num_streams` stream workers is
- * free it takes every queued request that the memory budget admits (FIFO, up to the batch limits)
- * and runs them as one multi-batch launch per query type (each request a segment against its own
+ * free it takes the queued requests that the memory budget admits (FIFO, up to the batch limits and
+ * to its share of the queue: the queued points over `num_streams`, at least 256k points, so a deep
+ * queue is spread over the streams) and runs them as one multi-batch launch per query type (each request a segment against its own
  * mesh), so requests that arrive while others run are admitted into the next launch and overlap
  * on the device. Results are bit-identical to the single-call entry points. All request arrays
  * are device pointers owned by the caller and must stay valid until the request completes.

@@ -73,6 +73,24 @@ def test_knn_on_triangles(oracle):
     assert np.array_equal(idx[:, 0], b.closestElement(q))
 
 
+@pytest.mark.parametrize("k,bound", [(1, -1.0), (4, -1.0), (8, -1.0), (8, 0.05), (16, -1.0), (24, -1.0)])
+def test_knn_triangles_dense_vs_oracle(oracle, k, bound):
+    """Dense triangle batches (the wave-cooperative walk, K = 4 to 32) against the oracle kd-tree
+    for every query, with and without a bound."""
+    import thea_amd
+    g = np.random.default_rng(300 + k)
+    v, t = random_mesh(g, 6000, size=0.03)
+    q = g.uniform(-1.1, 1.1, size=(20_000, 3)).astype(np.float32)
+    q[:8] = v[t[:8, 0]]  # queries on triangle vertices (zero distances)
+    print(f"case knn dense triangles k={k} bound={bound}", flush=True)
+    b = thea_amd.MeshBvh(thea_amd.Mesh(v, t))
+    idx, dist, d2 = b.kClosestElements(q, k, dist_bound=bound, want_d2=True)
+    ridx, rd2, _ = oracle.KdTree(v, t).knn(q, k, mode=0, dist_bound=bound)
+    assert np.array_equal(idx, ridx), f"{np.count_nonzero(idx != ridx)} index mismatches"
+    assert np.array_equal(d2.view(np.uint32), rd2.view(np.uint32))
+    assert np.all(dist[idx < 0] == -1)
+
+
 def test_c2_knn_graph(oracle):
     """C2: 64 ellipsoids, 1M area samples (GPU sampler), 16-NN graph without self."""
     import torch

@@ -55,6 +55,7 @@ struct Server {
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<int64_t> queue;
+  int64_t queued_points = 0;  // points of the queued requests
   std::unordered_map<int64_t, SrvRequest> reqs;
   int64_t next_id = 1;
   bool stopping = false;
@@ -73,6 +74,8 @@ static size_t request_bytes(const thg_request &r) {
   if (r.n >= ORDER_MIN_QUERIES) b += (size_t)r.n * 16 + radix_hist_words(r.n) * 4;
   return b;
 }
+
+constexpr int64_t FAIR_MIN_POINTS = (int64_t)1 << 18;
 
 static void worker_loop(Server *S, int wi) {
   hipStream_t s = S->streams[wi];
@@ -97,14 +100,18 @@ static void worker_loop(Server *S, int wi) {
       if (S->stopping && S->queue.empty()) break;
       int64_t nq = 0;
       size_t avail = S->budget - S->resident;
+      // A worker takes its share of the queued points (all of them split over the streams, at least
+      // FAIR_MIN_POINTS), so a deep queue keeps every stream busy instead of landing on one.
+      const int64_t share = std::max<int64_t>(S->queued_points / (int64_t)S->streams.size(), FAIR_MIN_POINTS);
       while (!S->queue.empty() && (int)batch.size() < (S->opts.coalesce < 0 ? 1 : maxreq)) {
         SrvRequest &r = S->reqs[S->queue.front()];
-        if (!batch.empty() && nq + r.r.n > S->opts.max_batch_queries) break;
+        if (!batch.empty() && (nq + r.r.n > S->opts.max_batch_queries || nq >= share)) break;
         if (S->inflight + r.bytes > avail) break;  // admission by memory budget
         S->inflight += r.bytes;
         S->peak_inflight = std::max(S->peak_inflight, S->inflight);
         r.state = RUNNING;
         nq += r.r.n;
+        S->queued_points -= r.r.n;
         batch.push_back(r.id);
         S->queue.pop_front();
       }
@@ -113,6 +120,7 @@ static void worker_loop(Server *S, int wi) {
           // cannot fit even with nothing in flight (meshes added after it was queued): fail it
           SrvRequest &r = S->reqs[S->queue.front()];
           S->queue.pop_front();
+          S->queued_points -= r.r.n;
           r.state = DONE;
           r.status = THG_E_BUDGET;
           r.t_done = Clock::now();
@@ -407,6 +415,7 @@ int thg_server_submit(thg_server srv, const thg_request *req, int64_t *req_id) {
       *req_id = sr.id;
       S->reqs[sr.id] = sr;
       S->queue.push_back(sr.id);
+      S->queued_points += sr.r.n;
     }
     S->cv_work.notify_one();
   });
@@ -449,6 +458,7 @@ int thg_server_cancel(thg_server srv, int64_t req_id) {
     }
     S->queue.erase(it);
     SrvRequest &r = S->reqs[req_id];
+    S->queued_points -= r.r.n;
     r.state = DONE;
     r.status = THG_E_CANCELLED;
     r.t_done = Clock::now();
