@@ -138,6 +138,11 @@ def measured_roofline(kernel, wl, kernel_avg_ms, alg_bytes_per_launch, num_cus, 
     out["valu_issue"] = issue
     out["hbm"] = hbm
     out["vmem"] = vmem
+    # the other two figures rocprof reports for the kernel: L2 hit rate and wave occupancy
+    out["l2_hit_rate"] = ctr.get("l2_hit_rate")
+    out["occupancy"] = None if ctr.get("occupancy_percent") is None else {
+        "percent_of_max": ctr["occupancy_percent"], "mean_waves_per_simd": ctr.get("mean_waves_per_simd"),
+        "note": ctr.get("occupancy_note"), "source": f"profiles/{kernel}_{wl}_counters.json"}
     alg_gbps = alg_bytes_per_launch / sec / 1e9
     out["thea_bytes_model"] = {"bytes_per_launch": alg_bytes_per_launch, "GB/s": alg_gbps,
                                "ratio_to_hbm_peak": alg_gbps / HBM_PEAK_GBPS, "note": model_note}
