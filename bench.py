@@ -46,6 +46,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-served", action="store_true", help="c3: leave out the served-stream half of the line")
     p.add_argument("--kabsch", action="store_true", help="icp: least-squares fit instead of Thea's")
+    p.add_argument("--f64", action="store_true",
+                   help="icp: double-precision target (ShapeAlign's BvhN<Vector3d>): double positions and search")
     p.add_argument("--cpu-threads", type=int, default=None)
     return p.parse_args()
 
@@ -636,7 +638,8 @@ def run_icp(args):
     src = frm.astype(np.float64) @ R.T + np.array([0.05, -0.03, 0.02])
     c = src.mean(0)
     inits = np.stack([np.hstack([m, (c - m @ c).reshape(3, 1)]) for m in cube_rotations()])
-    bvh = thea_amd.PointBvh(to, leaf_size=args.leaf)
+    bvh = (thea_amd.PointBvhF64(to.astype(np.float64), leaf_size=args.leaf) if args.f64
+           else thea_amd.PointBvh(to, leaf_size=args.leaf))
     icp = thea_amd.Icp3(kabsch=args.kabsch)
     for i in range(args.warmup):
         icp.alignBatch(src, bvh, initial=inits)
@@ -654,11 +657,11 @@ def run_icp(args):
     result = {
         "metric": "ShapeAlign 24-rotation ICP search, 1M source onto 1M target points (seconds per search)",
         "value": sec, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": sec * 1e3,
-        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64 fit, f32 nearest",
-        "data": "synthetic",
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 fit, f64 nearest" if args.f64 else "f64 fit, f32 nearest", "data": "synthetic",
         "config": {"workload": "icp rotation search on C3 torus samples", "source_points": n, "target_points": n,
                    "starting_rotations": len(inits), "fit": "least squares" if args.kabsch else "Thea (U V^T)",
-                   "leaf_size": args.leaf},
+                   "target": "double point BVH" if args.f64 else "float point BVH", "leaf_size": args.leaf},
         "nearest_queries_per_s": queries / sec,
         "iterations": its.tolist(), "errors": err.tolist(), "best_rotation": best,
         "best_transform": (tr[best][:, :3] @ inits[best][:, :3]).tolist(),
