@@ -100,6 +100,14 @@ int thg_bvh_bounds(thg_bvh bvh, float lo[3], float hi[3]);
  * [~ref, ~ref + count) of the leaf order. elem_ids (nullable): info[3] caller indices in leaf order.
  * Node 0 is the root. */
 int thg_bvh_export(thg_bvh bvh, void *nodes, int32_t *elem_ids);
+/* Copy a mesh BVH's ray tree to host memory (diagnostics and tests; synchronous; no counterpart in
+ * Thea). *num_qnodes receives the node count; qnodes (nullable) receives that many 64-byte nodes,
+ * each {origin xyz (float), meta (uint32: biased exponents ex | ey << 8 | ez << 16 | children << 24),
+ * qlo_x, qlo_y, qlo_z, qhi_x, qhi_y, qhi_z (uint32, byte k for child k), ref0..ref3 (int32, >= 0 a
+ * node of this tree, < 0 a leaf ~first in leaf order), counts (uint16 x 4)}. Child k's box decodes
+ * as fmaf(q, 2^e, origin) per axis and contains the binary tree's box for it. Node 0 is the root;
+ * point BVHs have none (*num_qnodes = 0). */
+int thg_bvh_export_ray_tree(thg_bvh bvh, void *qnodes, int64_t *num_qnodes);
 
 /* Closest element and closest point for n query points `q` (float3).
  * Replaces BvhN::closestElement<MetricL2>(query, dist_bound, compat, &dist, &closest_point)

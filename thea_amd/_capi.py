@@ -86,6 +86,7 @@ SIGNATURES = {
     "thg_bvh_destroy": (C.c_int, [vp]),
     "thg_bvh_info": (C.c_int, [vp, C.POINTER(i64)]),
     "thg_bvh_export": (C.c_int, [vp, vp, vp]),
+    "thg_bvh_export_ray_tree": (C.c_int, [vp, vp, vp]),
     "thg_bvh_bounds": (C.c_int, [vp, vp, vp]),
     "thg_closest_point": (C.c_int, [vp, vp, i64, f64, vp, vp, vp, vp, i32, vp]),
     "thg_closest_point_noalloc": (C.c_int, [vp, vp, i64, f64, vp, vp, vp, vp, vp]),

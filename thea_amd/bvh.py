@@ -170,6 +170,16 @@ class _BvhBase:
         capi.check(capi.lib().thg_bvh_export(self.h, raw.ctypes.data_as(C.c_void_p), ids.ctypes.data_as(C.c_void_p)))
         return raw.view(np.float32), raw, ids
 
+    def exportRayTree(self):
+        """The mesh BVH's 4-wide quantized ray tree as an int32 array (nodes, 16) (layout in
+        thg_bvh_export_ray_tree); empty for point BVHs."""
+        cnt = C.c_int64()
+        capi.check(capi.lib().thg_bvh_export_ray_tree(self.h, None, C.byref(cnt)))
+        raw = np.zeros((cnt.value, 16), np.int32)
+        if cnt.value:
+            capi.check(capi.lib().thg_bvh_export_ray_tree(self.h, raw.ctypes.data_as(C.c_void_p), C.byref(cnt)))
+        return raw
+
     def getBounds(self):
         """(lo, hi) float32: BvhN::getBounds (BvhN.hpp:991), the union of the element boxes scaled by
         1.05 about its centre in float arithmetic (AxisAlignedBoxN.hpp:274,316-324; BvhN.hpp:1437)."""

@@ -335,6 +335,18 @@ int thg_bvh_export(thg_bvh bvh, void *nodes, int32_t *elem_ids) {
   });
 }
 
+int thg_bvh_export_ray_tree(thg_bvh bvh, void *qnodes, int64_t *num_qnodes) {
+  return guarded([&] {
+    require(bvh != nullptr, "bvh is NULL");
+    require(num_qnodes != nullptr, "num_qnodes is NULL");
+    Bvh *b = reinterpret_cast<Bvh *>(bvh);
+    set_device(b->ctx);
+    *num_qnodes = b->qnodes.p ? b->num_qnodes : 0;
+    if (qnodes && *num_qnodes > 0)
+      THG_HIP(hipMemcpy(qnodes, b->qnodes.p, (size_t)*num_qnodes * sizeof(QNode4), hipMemcpyDeviceToHost));
+  });
+}
+
 // Fill per-query "not found" outputs for an empty BVH.
 __global__ void k_fill_missing(int64_t n, int k, int32_t *idx, float *a, float *b, float *cp3) {
   int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
