@@ -605,14 +605,15 @@ __device__ __forceinline__ void closest_cmp_body(CmpWave &W, const BNode *__rest
           const TriRec *tk = tr + k;
           if (STATS) st_loaded++;
           const float4 sbv = make_float4(unif(tk->sb.x), unif(tk->sb.y), unif(tk->sb.z), unif(tk->sb.w));
+          // conservative rejects (fused multiply-adds: the margins above dwarf their rounding)
           const float dx = p.x - sbv.x, dy = p.y - sbv.y, dz = p.z - sbv.z;
-          const float dc2 = dx * dx + dy * dy + dz * dz;
-          const float T = sbest * REJ_REL + sbv.w + 2.0f * sl;
+          const float dc2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+          const float T = fmaf(2.0f, sl, fmaf(sbest, REJ_REL, sbv.w));
           bool need = in_c && !(dc2 > T * T);
           if (__ballot(need) == 0) continue;
           const float nx = unif(tk->p0.w), ny = unif(tk->p1.w), nz = unif(tk->p2.w), pd = unif(tk->pl.x);
-          const float sd = dot3(p, f3(nx, ny, nz)) - pd;
-          need = need && !(fabsf(sd) - 2.0f * sl > sbest * REJ_REL);
+          const float sd = fmaf(p.x, nx, fmaf(p.y, ny, fmaf(p.z, nz, -pd)));
+          need = need && !(fmaf(-2.0f, sl, fabsf(sd)) > sbest * REJ_REL);
           const uint64_t nb = __ballot(need);
           if (nb == 0) continue;
           if (need) {

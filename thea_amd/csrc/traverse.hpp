@@ -13,13 +13,14 @@ namespace thg {
 constexpr float SLACK_REL = 1.0f / 65536.0f;
 
 // Squared distance between the query box [qlo, qhi] and a node box: a lower bound of the squared
-// distance from the query to anything in the node.
+// distance from the query to anything in the node. Only pruning reads it, so it may fuse its
+// multiply-adds: the query box's slack (2^-16 of the scale per axis) is far wider than their rounding.
 __device__ __forceinline__ float box_lb(float3 qlo, float3 qhi, float lx, float ly, float lz, float hx, float hy,
                                         float hz) {
   float dx = fmaxf(fmaxf(lx - qhi.x, qlo.x - hx), 0.0f);
   float dy = fmaxf(fmaxf(ly - qhi.y, qlo.y - hy), 0.0f);
   float dz = fmaxf(fmaxf(lz - qhi.z, qlo.z - hz), 0.0f);
-  return dx * dx + (dy * dy + dz * dz);
+  return fmaf(dx, dx, fmaf(dy, dy, dz * dz));
 }
 
 // Per-lane seed of the wave traversals: descend from `root` to the leaf whose box is nearest the
