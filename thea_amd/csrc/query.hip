@@ -515,7 +515,11 @@ __device__ __forceinline__ void cmp_flush(CmpWave &W, const TriRec *__restrict__
   __builtin_amdgcn_wave_barrier();
 }
 
-template <bool STATS>
+// PF (batches too small to fill the GPU): a leaf's screening data (bounding spheres, normals and
+// plane offsets) arrives with one vector load per lane and is read per triangle with v_readlane, so
+// a wave waits once per leaf instead of once per triangle; large batches hide that latency with
+// other waves and keep the scalar loads, which cost no vector instructions.
+template <bool STATS, bool PF = false>
 __device__ __forceinline__ void closest_cmp_body(CmpWave &W, const BNode *__restrict__ nodes,
                                                  const TriRec *__restrict__ tr, const float *__restrict__ q,
                                                  const uint32_t *__restrict__ order, int64_t n, int64_t wave,
@@ -601,17 +605,27 @@ __device__ __forceinline__ void closest_cmp_body(CmpWave &W, const BNode *__rest
       if (r < 0) {
         const bool in_c = (mc >> lane) & 1;
         const int32_t f = ~r;
+        float4 lsb = make_float4(0.0f, 0.0f, 0.0f, 0.0f), lpl = lsb;  // PF: lane j holds triangle f + j's
+        if (PF && lane < cnt) {
+          const TriRec *tj = tr + f + lane;
+          lsb = tj->sb;
+          lpl = make_float4(tj->p0.w, tj->p1.w, tj->p2.w, tj->pl.x);
+        }
         for (int32_t k = f; k < f + cnt; ++k) {
           const TriRec *tk = tr + k;
           if (STATS) st_loaded++;
-          const float4 sbv = make_float4(unif(tk->sb.x), unif(tk->sb.y), unif(tk->sb.z), unif(tk->sb.w));
+          const int jl = k - f;
+          const float4 sbv = PF ? make_float4(lane_readf(lsb.x, jl), lane_readf(lsb.y, jl), lane_readf(lsb.z, jl),
+                                              lane_readf(lsb.w, jl))
+                                : make_float4(unif(tk->sb.x), unif(tk->sb.y), unif(tk->sb.z), unif(tk->sb.w));
           // conservative rejects (fused multiply-adds: the margins above dwarf their rounding)
           const float dx = p.x - sbv.x, dy = p.y - sbv.y, dz = p.z - sbv.z;
           const float dc2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
           const float T = fmaf(2.0f, sl, fmaf(sbest, REJ_REL, sbv.w));
           bool need = in_c && !(dc2 > T * T);
           if (__ballot(need) == 0) continue;
-          const float nx = unif(tk->p0.w), ny = unif(tk->p1.w), nz = unif(tk->p2.w), pd = unif(tk->pl.x);
+          const float nx = PF ? lane_readf(lpl.x, jl) : unif(tk->p0.w), ny = PF ? lane_readf(lpl.y, jl) : unif(tk->p1.w);
+          const float nz = PF ? lane_readf(lpl.z, jl) : unif(tk->p2.w), pd = PF ? lane_readf(lpl.w, jl) : unif(tk->pl.x);
           const float sd = fmaf(p.x, nx, fmaf(p.y, ny, fmaf(p.z, nz, -pd)));
           need = need && !(fmaf(-2.0f, sl, fabsf(sd)) > sbest * REJ_REL);
           const uint64_t nb = __ballot(need);
@@ -693,7 +707,7 @@ __device__ __forceinline__ void closest_cmp_body(CmpWave &W, const BNode *__rest
 }
 
 // WAVES: resident waves per SIMD to compile for (1: the compiler's choice, 6 here, no spill).
-template <bool STATS = false, int WAVES = 1>
+template <bool STATS = false, int WAVES = 1, bool PF = false>
 __global__ void __launch_bounds__(W_BLOCK, WAVES) k_closest_cmp(const BNode *__restrict__ nodes, const TriRec *__restrict__ tr,
                                                          const float *__restrict__ q, const uint32_t *__restrict__ order,
                                                          int64_t n, uint32_t nb_real, float best_init, float scale,
@@ -705,8 +719,8 @@ __global__ void __launch_bounds__(W_BLOCK, WAVES) k_closest_cmp(const BNode *__r
   __shared__ CmpWave s_w[W_WAVES];
   const uint32_t lb_blk = xcd_remap(blockIdx.x, gridDim.x);
   const int64_t wave = (int64_t)lb_blk * W_WAVES + (threadIdx.x / WAVE);
-  closest_cmp_body<STATS>(s_w[threadIdx.x / WAVE], nodes, tr, q, order, n, wave, lb_blk < nb_real, best_init, scale,
-                          exclude_self, out_idx, out_cp, out_dist, out_d2, stats, 0, child_order);
+  closest_cmp_body<STATS, PF>(s_w[threadIdx.x / WAVE], nodes, tr, q, order, n, wave, lb_blk < nb_real, best_init,
+                              scale, exclude_self, out_idx, out_cp, out_dist, out_d2, stats, 0, child_order);
 }
 
 template <bool POINTS, int G, bool PRE, bool STATS = false>
@@ -2347,7 +2361,20 @@ void launch_closest(const Bvh &b, const float *q, int64_t n, double dist_bound, 
       // would only pay the 8-wave build's spill (C1, 100k queries: 1.56 against 1.64 ms).
       const bool one_round = ceil_div(n, (uint64_t)WAVE) <= (uint64_t)b.ctx->num_cus * 4 * 6;
       const uint32_t cgrid = round_up8(ceil_div(n, (uint64_t)W_BLOCK)), cnb = (uint32_t)ceil_div(n, (uint64_t)W_BLOCK);
-      if (one_round)
+      // When every wave is resident at once, the walk's load latency is exposed: prefetch leaves
+      // (PF; C1, 100k queries: 1.55 -> 1.34 ms; 300k: 1.61 -> 1.36 ms). THG_LEAF_PREFETCH=0 or 1
+      // overrides, for either build.
+      static const int pf_env = env_int("THG_LEAF_PREFETCH", -1);
+      const bool pf = pf_env >= 0 ? pf_env != 0 : one_round;
+      if (!one_round && pf)
+        hipLaunchKernelGGL((k_closest_cmp<false, CMP_WAVES, true>), dim3(cgrid), dim3(W_BLOCK), 0, s,
+                           b.nodes.as<BNode>(), b.elems.as<TriRec>(), q, ws.order, n, cnb, best, b.scale, exclude_self,
+                           idx, cp, dist, d2, nullptr, b.child_order);
+      else if (one_round && pf)
+        hipLaunchKernelGGL((k_closest_cmp<false, 1, true>), dim3(cgrid), dim3(W_BLOCK), 0, s, b.nodes.as<BNode>(),
+                           b.elems.as<TriRec>(), q, ws.order, n, cnb, best, b.scale, exclude_self, idx, cp, dist, d2,
+                           nullptr, b.child_order);
+      else if (one_round)
         hipLaunchKernelGGL((k_closest_cmp<false, 1>), dim3(cgrid), dim3(W_BLOCK), 0, s, b.nodes.as<BNode>(),
                            b.elems.as<TriRec>(), q, ws.order, n, cnb, best, b.scale, exclude_self, idx, cp, dist, d2,
                            nullptr, b.child_order);
