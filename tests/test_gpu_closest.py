@@ -113,6 +113,40 @@ def test_degenerate_triangles(oracle):
     _assert_same(idx, cp, d2, ridx, rcp, rd2, "degenerate")
 
 
+@pytest.mark.parametrize("nq", [30_000, 450_000])
+def test_slivers_and_near_collinear(oracle, nq):
+    """Triangles whose float normal is not perpendicular to their edges (collinear in real numbers,
+    perturbed by float rounding), thin slivers and ordinary ones, far from the origin as well, with
+    queries on and just off them: the plane-and-disc pruning bound must fall back to the bounding
+    sphere wherever the stored plane is not tight. 450k queries run the many-waves build of the
+    closest-point kernel, 30k the prefetching one."""
+    g = np.random.default_rng(19)
+    n = 6000
+    off = np.repeat(g.choice([0.0, 1.0, 300.0], size=(n // 3, 1)), 3, axis=0) * g.normal(size=(n, 3))
+    size = 10 ** g.uniform(-3, -1, size=(n, 1))
+    v0 = off + g.uniform(-1, 1, size=(n, 3))
+    d = size * g.normal(size=(n, 3))
+    kind = np.arange(n) % 3
+    v1 = v0 + d
+    v2 = np.where(kind[:, None] == 0, v0 + size * g.normal(size=(n, 3)),
+                  np.where(kind[:, None] == 1, v0 + 2 * d, v0 + d * 0.5 + size * 1e-4 * g.normal(size=(n, 3))))
+    v = np.stack([v0, v1, v2], axis=1).reshape(-1, 3).astype(np.float32)
+    t = np.arange(3 * n, dtype=np.int32).reshape(-1, 3)
+    w = g.dirichlet([1, 1, 1], size=nq)
+    k = g.integers(0, n, size=nq)
+    V = v.reshape(n, 3, 3).astype(np.float64)
+    foot = np.einsum("ij,ijk->ik", w, V[k])
+    h = 10 ** g.uniform(-7, -1, size=(nq, 1)) * size[k]
+    q = (foot + h * g.normal(size=(nq, 3))).astype(np.float32)
+    print(f"case slivers: {n} tris x {nq} queries", flush=True)
+    idx, dist, cp, d2, _ = _gpu_closest(v, t, q, leaf=8)
+    kd = oracle.KdTree(v, t)
+    ridx, rcp, rd2, _ = kd.closest(q, mode=0)
+    _assert_same(idx, cp, d2, ridx, rcp, rd2, "slivers")
+    bidx, bcp, bd2 = oracle.brute_closest(v, t, q[:3000])
+    _assert_same(idx[:3000], cp[:3000], d2[:3000], bidx, bcp, bd2, "slivers brute")
+
+
 @pytest.mark.parametrize("bound", [0.0, 0.05, 0.1, 0.3, 1e-3])
 def test_dist_bound(oracle, bound):
     g = np.random.default_rng(5)

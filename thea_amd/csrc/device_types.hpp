@@ -39,7 +39,7 @@ struct __align__(128) TriRec {
   float4 e0;   // dir0.xyz, len0   (edge v0 -> v1)
   float4 e1;   // dir1.xyz, len1   (edge v1 -> v2)
   float4 e2;   // dir2.xyz, len2   (edge v2 -> v0)
-  float4 sb;   // bounding sphere centre xyz, radius
+  float4 sb;   // bounding sphere: centre xyz (in the triangle's plane), radius
 };
 static_assert(sizeof(TriRec) == 128, "TriRec must be 128 bytes");
 

@@ -329,10 +329,11 @@ __device__ __forceinline__ void closest_wave_body(const BNode *__restrict__ node
             if (STATS) st_loaded++;
             const float4 sbv = make_float4(unif(tk->sb.x), unif(tk->sb.y), unif(tk->sb.z), unif(tk->sb.w));
             bool sph = active;
+            float dc2 = 0.0f;
             if (PRE) {
               const float dx = p.x - sbv.x, dy = p.y - sbv.y, dz = p.z - sbv.z;
-              const float dc2 = dx * dx + dy * dy + dz * dz;
-              const float T = sbest * REJ_REL + sbv.w + 2.0f * sl;
+              dc2 = dx * dx + dy * dy + dz * dz;
+              const float T = sbest * REJ_REL + fabsf(sbv.w) + 2.0f * sl;
               sph = active && !(dc2 > T * T);
               if (__ballot(sph) == 0) continue;
             }
@@ -342,10 +343,11 @@ __device__ __forceinline__ void closest_wave_body(const BNode *__restrict__ node
             const float4 pl = make_float4(unif(tk->pl.x), unif(tk->pl.y), unif(tk->pl.z), unif(tk->pl.w));
             // HyperplaneN.hpp:145-160 signedDistance, shared with the exact test below
             const float sd = dot3(p, f3(p0.w, p1.w, p2.w)) - pl.x;
-            if (PRE && __ballot(active && !(fabsf(sd) - 2.0f * sl > sbest * REJ_REL)) == 0) continue;
+            const float Tp = sbest * REJ_REL + 2.0f * sl;
+            if (PRE && __ballot(sph && !(tri_lb2(dc2, sd, sbv.w, sl) > Tp * Tp)) == 0) continue;
             if (STATS) {
               st_tested += WAVE;
-              st_plane += __popcll(__ballot(sph && !(fabsf(sd) - 2.0f * sl > sbest * REJ_REL)));
+              st_plane += __popcll(__ballot(sph && !(tri_lb2(dc2, sd, sbv.w, sl) > Tp * Tp)));
             }
             const float4 e0 = make_float4(unif(tk->e0.x), unif(tk->e0.y), unif(tk->e0.z), unif(tk->e0.w));
             const float4 e1 = make_float4(unif(tk->e1.x), unif(tk->e1.y), unif(tk->e1.z), unif(tk->e1.w));
@@ -459,7 +461,8 @@ __device__ __forceinline__ void closest_wave_body(const BNode *__restrict__ node
 //--------------------------------------------------------------------------------------------------
 // Compacted wave traversal (triangles). The wave walks one tree for its 64 queries as above, but a
 // triangle of an admitted leaf is not evaluated by the whole wave: every lane applies the cheap
-// conservative rejects (bounding sphere, plane distance) against its own best, and each surviving
+// conservative rejects (bounding sphere, then plane and in-plane disc distance, tri_lb2) against its
+// own best, and each surviving
 // (lane, triangle) pair is appended to a per-wave LDS queue. Whenever 64 pairs are queued the wave
 // evaluates them one per lane with Thea's exact arithmetic and merges each result into its owner
 // lane's best through an LDS 64-bit atomic min on (float bits of d^2) << 32 | triangle id, which
@@ -621,13 +624,14 @@ __device__ __forceinline__ void closest_cmp_body(CmpWave &W, const BNode *__rest
           // conservative rejects (fused multiply-adds: the margins above dwarf their rounding)
           const float dx = p.x - sbv.x, dy = p.y - sbv.y, dz = p.z - sbv.z;
           const float dc2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-          const float T = fmaf(2.0f, sl, fmaf(sbest, REJ_REL, sbv.w));
+          const float T = fmaf(2.0f, sl, fmaf(sbest, REJ_REL, fabsf(sbv.w)));
           bool need = in_c && !(dc2 > T * T);
           if (__ballot(need) == 0) continue;
           const float nx = PF ? lane_readf(lpl.x, jl) : unif(tk->p0.w), ny = PF ? lane_readf(lpl.y, jl) : unif(tk->p1.w);
           const float nz = PF ? lane_readf(lpl.z, jl) : unif(tk->p2.w), pd = PF ? lane_readf(lpl.w, jl) : unif(tk->pl.x);
           const float sd = fmaf(p.x, nx, fmaf(p.y, ny, fmaf(p.z, nz, -pd)));
-          need = need && !(fmaf(-2.0f, sl, fabsf(sd)) > sbest * REJ_REL);
+          const float Tp = fmaf(2.0f, sl, sbest * REJ_REL);
+          need = need && !(tri_lb2(dc2, sd, sbv.w, sl) > Tp * Tp);
           const uint64_t nb = __ballot(need);
           if (nb == 0) continue;
           if (need) {

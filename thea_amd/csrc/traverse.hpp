@@ -23,6 +23,25 @@ __device__ __forceinline__ float box_lb(float3 qlo, float3 qhi, float lx, float 
   return fmaf(dx, dx, fmaf(dy, dy, dz * dz));
 }
 
+// Lower bound on the squared distance from a query p to a triangle, from the triangle's plane and a
+// disc holding it: sd = n.p - pd (unit normal n) is p's distance across the plane, and p's foot on
+// the plane lies sqrt(|p - c|^2 - sd^2) from the disc's centre c (a point of the plane), so at least
+// that less the radius r from every point of the triangle; the two distances are orthogonal and
+// their squares add. Each is lowered by the slack sl, which covers the rounding of the stored plane
+// and centre and of this arithmetic, and dc2 = |p - c|^2 by a relative 2^-20, so the hardware square
+// root (1 ulp) is enough. A negative r marks a triangle whose plane is not tight (degenerate or
+// near-collinear; c is then its box centre): the bound is the bounding sphere's, |p - c| - |r|. A
+// NaN sd gives 0, which prunes nothing.
+__device__ __forceinline__ float tri_lb2(float dc2, float sd, float r, float sl) {
+  const bool tight = __float_as_int(r) >= 0;
+  const float asd = fabsf(sd);
+  const float a = tight ? fmaxf(asd - sl, 0.0f) : 0.0f;
+  const float u = tight ? asd + sl : 0.0f;
+  const float ip2 = fmaf(-u, u, dc2 * (1.0f - 1.0f / 1048576.0f));
+  const float b = fmaxf(__builtin_amdgcn_sqrtf(fmaxf(ip2, 0.0f)) - fabsf(r) - sl, 0.0f);
+  return fmaf(a, a, b * b);
+}
+
 // Per-lane seed of the wave traversals: descend from `root` to the leaf whose box is nearest the
 // query (no backtracking) and return it as [first, first + count). The elements found there give
 // each lane a real upper bound before the shared walk starts, so a wave whose queries are far apart

@@ -96,19 +96,40 @@ THG_HD TriRec make_trirec(float3 v0, float3 v1, float3 v2, int32_t id) {
   r.e0 = make_float4(d0.x, d0.y, d0.z, l0);
   r.e1 = make_float4(d1.x, d1.y, d1.z, l1);
   r.e2 = make_float4(d2.x, d2.y, d2.z, l2);
-  // Bounding sphere: box centre, radius = largest vertex distance (double), rounded up.
+  // Bounding sphere, centred in the triangle's plane when that plane is tight: the box centre is
+  // moved along the normal onto n.x = pd (double) and the radius is the largest vertex distance
+  // (double), rounded up, so the triangle lies in the disc of that radius about the centre. The
+  // plane is tight when |n|^2 is within 2^-18 of 1 and every vertex is within 2^-20 of the largest
+  // vertex coordinate of it; a degenerate or near-collinear triangle's float normal can be far from
+  // perpendicular to its edges, and then the centre stays at the box centre and the radius is
+  // stored negated, which tells the pruning tests to use the sphere alone.
   double cx = 0.5 * ((double)fminf(v0.x, fminf(v1.x, v2.x)) + (double)fmaxf(v0.x, fmaxf(v1.x, v2.x)));
   double cy = 0.5 * ((double)fminf(v0.y, fminf(v1.y, v2.y)) + (double)fmaxf(v0.y, fmaxf(v1.y, v2.y)));
   double cz = 0.5 * ((double)fminf(v0.z, fminf(v1.z, v2.z)) + (double)fmaxf(v0.z, fmaxf(v1.z, v2.z)));
+  const float3 vv[3] = {v0, v1, v2};
+  const double nn = (double)n.x * n.x + (double)n.y * n.y + (double)n.z * n.z;
+  bool tight = nn > 1.0 - 1.0 / 262144.0 && nn < 1.0 + 1.0 / 262144.0;
+  double vmax = 0, off = 0;
+  for (int k = 0; k < 3; ++k) {
+    vmax = fmax(vmax, fmax(fabs((double)vv[k].x), fmax(fabs((double)vv[k].y), fabs((double)vv[k].z))));
+    off = fmax(off, fabs((double)n.x * vv[k].x + (double)n.y * vv[k].y + (double)n.z * vv[k].z - (double)pd));
+  }
+  tight = tight && off <= vmax * (1.0 / 1048576.0);
+  if (tight) {
+    const double s = ((double)n.x * cx + (double)n.y * cy + (double)n.z * cz - (double)pd) / nn;
+    cx -= s * n.x;
+    cy -= s * n.y;
+    cz -= s * n.z;
+  }
   float fcx = (float)cx, fcy = (float)cy, fcz = (float)cz;
   double rr = 0;
-  const float3 vv[3] = {v0, v1, v2};
   for (int k = 0; k < 3; ++k) {
     double dx = (double)vv[k].x - fcx, dy = (double)vv[k].y - fcy, dz = (double)vv[k].z - fcz;
     double dd = dx * dx + dy * dy + dz * dz;
     rr = dd > rr ? dd : rr;
   }
   float rad = (float)(sqrt(rr) * (1.0 + 1e-6));
+  if (!tight) rad = -rad;
   r.sb = make_float4(fcx, fcy, fcz, rad);
   return r;
 }
