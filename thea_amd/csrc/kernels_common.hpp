@@ -12,6 +12,8 @@ namespace thg {
 constexpr int WAVE = 64;  // CDNA wavefront width
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
+// Wave-wide ballot of a lane predicate straight from the compare mask (no 0/1 round trip).
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
   int l = lane_id();

@@ -543,13 +543,18 @@ __device__ __forceinline__ void closest_cmp_body(CmpWave &W, const BNode *__rest
     p = load_q(q, qi);
     active = !(isnan(p.x) || isnan(p.y) || isnan(p.z));
   }
-  const uint64_t amask = __ballot(active);
+  const uint64_t amask = ballot(active);
   const int nact = __popcll(amask);
   const float m = fmaxf(scale, fmaxf(fabsf(p.x), fmaxf(fabsf(p.y), fabsf(p.z))));
   const float sl = m * SLACK_REL;
   const float3 qlo = f3(p.x - sl, p.y - sl, p.z - sl), qhi = f3(p.x + sl, p.y + sl, p.z + sl);
   float best = active ? best_init : -1.0f;  // inactive lanes admit nothing
-  float sbest = active ? sqrtf(best_init) : 0.0f;
+  // squared reject threshold (sqrt(best) REJ_REL + 2 sl)^2, refreshed whenever best changes
+  auto rej_of = [sl](float bst) {
+    const float t = fmaf(2.0f, sl, sqrtf(bst) * REJ_REL);
+    return t * t;
+  };
+  float rej2 = active ? rej_of(best_init) : 0.0f;
   const int32_t self = exclude_self ? (int32_t)qi : -1;
   unsigned long long key0 = cmp_key(active ? best_init : 0.0f, 0x7fffffff);
   float3 cp0 = f3(0, 0, 0);
@@ -569,7 +574,7 @@ __device__ __forceinline__ void closest_cmp_body(CmpWave &W, const BNode *__rest
       }
     }
     best = cmp_key_d(key0);
-    sbest = sqrtf(best);
+    rej2 = rej_of(best);
   }
   W.qp[lane] = make_float4(p.x, p.y, p.z, __int_as_float(self));
   W.key[lane] = key0;
@@ -590,10 +595,10 @@ __device__ __forceinline__ void closest_cmp_body(CmpWave &W, const BNode *__rest
     const int32_t r0 = uni(nd.d.x), r1 = uni(nd.d.y), n0 = uni(nd.d.z), n1 = uni(nd.d.w);
     const float lb0 = box_lb(qlo, qhi, a0, a1, a2, a3, b0, b1);
     const float lb1 = box_lb(qlo, qhi, b2, b3, c0, c1, c2, c3);
-    const uint64_t m0 = __ballot(lb0 <= best), m1 = __ballot(lb1 <= best);
+    const uint64_t m0 = ballot(lb0 <= best), m1 = ballot(lb1 <= best);
     // visiting order (CHILD_*): the child most lanes find nearer, the child the wave's first active
     // query finds nearer, or the build's order (ballots only, so the choice costs no registers)
-    const uint64_t near1 = __ballot(lb1 < lb0) & amask;
+    const uint64_t near1 = ballot(lb1 < lb0) & amask;
     const bool first1 = child_order == CHILD_MAJORITY ? 2 * __popcll(near1) > nact
                         : child_order == CHILD_LEADER ? (near1 & (amask & (~amask + 1))) != 0
                                                       : false;
@@ -606,7 +611,6 @@ __device__ __forceinline__ void closest_cmp_body(CmpWave &W, const BNode *__rest
       const int32_t r = c == 0 ? r0 : r1;
       const int32_t cnt = c == 0 ? n0 : n1;
       if (r < 0) {
-        const bool in_c = (mc >> lane) & 1;
         const int32_t f = ~r;
         float4 lsb = make_float4(0.0f, 0.0f, 0.0f, 0.0f), lpl = lsb;  // PF: lane j holds triangle f + j's
         if (PF && lane < cnt) {
@@ -621,18 +625,26 @@ __device__ __forceinline__ void closest_cmp_body(CmpWave &W, const BNode *__rest
           const float4 sbv = PF ? make_float4(lane_readf(lsb.x, jl), lane_readf(lsb.y, jl), lane_readf(lsb.z, jl),
                                               lane_readf(lsb.w, jl))
                                 : make_float4(unif(tk->sb.x), unif(tk->sb.y), unif(tk->sb.z), unif(tk->sb.w));
-          // conservative reject (fused multiply-adds: the slack dwarfs their rounding). The plane
-          // and disc bound alone: a bounding-sphere test first, to skip the plane's loads, cost
-          // more than it saved (C3: 18.1 against 19.4 ms per launch).
+          // conservative reject (fused multiply-adds: the slack dwarfs their rounding): the plane
+          // and disc bound. Large batches apply it alone (a bounding-sphere test first, to skip the
+          // plane's scalar loads, cost more than it saved: C3 18.1 against 19.4 ms per launch);
+          // prefetched leaves test the sphere first, which skips the plane's four lane reads when
+          // no lane passes (C1: 1.45 against 1.57 ms). sqrt(rej2) widened by 2^-20 covers the
+          // hardware square root's rounding.
           const float dx = p.x - sbv.x, dy = p.y - sbv.y, dz = p.z - sbv.z;
           const float dc2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+          if (PF) {
+            const float T = fmaf(__builtin_amdgcn_sqrtf(rej2), 1.0f + 1.0f / 1048576.0f, fabsf(sbv.w));
+            if ((ballot(!(dc2 > T * T)) & mc) == 0) continue;
+          }
           const float nx = PF ? lane_readf(lpl.x, jl) : unif(tk->p0.w), ny = PF ? lane_readf(lpl.y, jl) : unif(tk->p1.w);
           const float nz = PF ? lane_readf(lpl.z, jl) : unif(tk->p2.w), pd = PF ? lane_readf(lpl.w, jl) : unif(tk->pl.x);
           const float sd = fmaf(p.x, nx, fmaf(p.y, ny, fmaf(p.z, nz, -pd)));
-          const float Tp = fmaf(2.0f, sl, sbest * REJ_REL);
-          const bool need = in_c && !(tri_lb2(dc2, sd, sbv.w, sl) > Tp * Tp);
-          const uint64_t nb = __ballot(need);
+          // every lane computes the bound (no divergence); lanes that did not admit the leaf are
+          // masked out of the ballot
+          const uint64_t nb = ballot(!(tri_lb2(dc2, sd, sbv.w, sl) > rej2)) & mc;
           if (nb == 0) continue;
+          const bool need = (nb >> lane) & 1;
           if (need) {
             const int pos = qn + __popcll(nb & lanemask_lt());
             W.qe[pos] = k;
@@ -647,7 +659,7 @@ __device__ __forceinline__ void closest_cmp_body(CmpWave &W, const BNode *__rest
             if (STATS) st_flush++;
             if (active) {
               best = cmp_key_d(W.key[lane]);
-              sbest = sqrtf(best);
+              rej2 = rej_of(best);
             }
           }
         }
@@ -669,7 +681,7 @@ __device__ __forceinline__ void closest_cmp_body(CmpWave &W, const BNode *__rest
       const float lx = lane_readf(s_lx, sp), ly = lane_readf(s_ly, sp), lz = lane_readf(s_lz, sp);
       const float hx = lane_readf(s_hx, sp), hy = lane_readf(s_hy, sp), hz = lane_readf(s_hz, sp);
       if (STATS) st_boxes++;
-      if (__ballot(box_lb(qlo, qhi, lx, ly, lz, hx, hy, hz) <= best)) {
+      if (ballot(box_lb(qlo, qhi, lx, ly, lz, hx, hy, hz) <= best)) {
         node = lane_read(s_ref, sp);
         found = true;
         break;

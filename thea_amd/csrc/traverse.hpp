@@ -28,16 +28,18 @@ __device__ __forceinline__ float box_lb(float3 qlo, float3 qhi, float lx, float 
 // the plane lies sqrt(|p - c|^2 - sd^2) from the disc's centre c (a point of the plane), so at least
 // that less the radius r from every point of the triangle; the two distances are orthogonal and
 // their squares add. Each is lowered by the slack sl, which covers the rounding of the stored plane
-// and centre and of this arithmetic, and dc2 = |p - c|^2 by a relative 2^-20, so the hardware square
-// root (1 ulp) is enough. A negative r marks a triangle whose plane is not tight (degenerate or
-// near-collinear; c is then its box centre): the bound is the bounding sphere's, |p - c| - |r|. A
-// NaN sd gives 0, which prunes nothing.
+// and centre and of this arithmetic: dc2 = |p - c|^2 may round up by a relative 4 2^-24, which the
+// relative 2^-20 added to |sd| (in the term subtracted from it) more than covers across the plane
+// and which moves the in-plane distance by far less than sl, as does the hardware square root
+// (1 ulp). A negative r marks a triangle whose plane is not tight (degenerate or near-collinear; c
+// is then its box centre): the bound is the bounding sphere's, |p - c| - |r|. A NaN sd gives 0,
+// which prunes nothing.
 __device__ __forceinline__ float tri_lb2(float dc2, float sd, float r, float sl) {
   const bool tight = __float_as_int(r) >= 0;
   const float asd = fabsf(sd);
   const float a = tight ? fmaxf(asd - sl, 0.0f) : 0.0f;
-  const float u = tight ? asd + sl : 0.0f;
-  const float ip2 = fmaf(-u, u, dc2 * (1.0f - 1.0f / 1048576.0f));
+  const float u = tight ? fmaf(asd, 1.0f + 1.0f / 1048576.0f, sl) : 0.0f;
+  const float ip2 = fmaf(-u, u, dc2);
   const float b = fmaxf(__builtin_amdgcn_sqrtf(fmaxf(ip2, 0.0f)) - fabsf(r) - sl, 0.0f);
   return fmaf(a, a, b * b);
 }
