@@ -145,6 +145,14 @@ def test_slivers_and_near_collinear(oracle, nq):
     _assert_same(idx, cp, d2, ridx, rcp, rd2, "slivers")
     bidx, bcp, bd2 = oracle.brute_closest(v, t, q[:3000])
     _assert_same(idx[:3000], cp[:3000], d2[:3000], bidx, bcp, bd2, "slivers brute")
+    if nq <= 100_000:
+        # the k-NN walk over triangles prunes with the same bound against each lane's k-th distance
+        import thea_amd
+        b = thea_amd.MeshBvh(thea_amd.Mesh(v, t), leaf_size=8)
+        kidx, _, kd2 = b.kClosestElements(q, 6, want_d2=True)
+        ridx, rd2, _ = kd.knn(q, 6, mode=0)
+        assert np.array_equal(kidx, ridx), f"k-NN: {np.count_nonzero(kidx != ridx)} index mismatches"
+        assert np.array_equal(kd2.view(np.uint32), rd2.view(np.uint32))
 
 
 @pytest.mark.parametrize("bound", [0.0, 0.05, 0.1, 0.3, 1e-3])

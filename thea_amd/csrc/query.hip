@@ -22,6 +22,13 @@ constexpr int Q_BLOCK = 64;
 // Relative margin of the element reject tests: a bound is compared with sqrt(best) * REJ_REL.
 constexpr float REJ_REL = 1.0f + 1.0f / 4096.0f;
 
+// Squared threshold of the triangle reject (tri_lb2) for a lane whose current best (or k-th) squared
+// distance is `best`: (sqrt(best) REJ_REL + 2 sl)^2; +inf while the lane has no bound.
+__device__ __forceinline__ float rej_sq(float best, float sl) {
+  const float t = fmaf(2.0f, sl, sqrtf(best) * REJ_REL);
+  return t * t;
+}
+
 __device__ __forceinline__ float3 load_q(const float *__restrict__ q, int64_t i) {
   return f3(q[3 * i], q[3 * i + 1], q[3 * i + 2]);
 }
@@ -549,12 +556,7 @@ __device__ __forceinline__ void closest_cmp_body(CmpWave &W, const BNode *__rest
   const float sl = m * SLACK_REL;
   const float3 qlo = f3(p.x - sl, p.y - sl, p.z - sl), qhi = f3(p.x + sl, p.y + sl, p.z + sl);
   float best = active ? best_init : -1.0f;  // inactive lanes admit nothing
-  // squared reject threshold (sqrt(best) REJ_REL + 2 sl)^2, refreshed whenever best changes
-  auto rej_of = [sl](float bst) {
-    const float t = fmaf(2.0f, sl, sqrtf(bst) * REJ_REL);
-    return t * t;
-  };
-  float rej2 = active ? rej_of(best_init) : 0.0f;
+  float rej2 = active ? rej_sq(best_init, sl) : 0.0f;  // refreshed whenever best changes
   const int32_t self = exclude_self ? (int32_t)qi : -1;
   unsigned long long key0 = cmp_key(active ? best_init : 0.0f, 0x7fffffff);
   float3 cp0 = f3(0, 0, 0);
@@ -574,7 +576,7 @@ __device__ __forceinline__ void closest_cmp_body(CmpWave &W, const BNode *__rest
       }
     }
     best = cmp_key_d(key0);
-    rej2 = rej_of(best);
+    rej2 = rej_sq(best, sl);
   }
   W.qp[lane] = make_float4(p.x, p.y, p.z, __int_as_float(self));
   W.key[lane] = key0;
@@ -659,7 +661,7 @@ __device__ __forceinline__ void closest_cmp_body(CmpWave &W, const BNode *__rest
             if (STATS) st_flush++;
             if (active) {
               best = cmp_key_d(W.key[lane]);
-              rej2 = rej_of(best);
+              rej2 = rej_sq(best, sl);
             }
           }
         }
@@ -1389,6 +1391,8 @@ __device__ __forceinline__ void knn_wave_body(const BNode *__restrict__ nodes, c
       }
     }
   }
+  // triangles: the reject threshold from the k-th distance, refreshed after every insertion
+  float rej2 = (!POINTS && active) ? rej_sq(kb, sl) : 0.0f;
   int32_t s_ref = 0;
   float s_lx = 0, s_ly = 0, s_lz = 0, s_hx = 0, s_hy = 0, s_hz = 0;
   int32_t sp = 0;
@@ -1436,6 +1440,15 @@ __device__ __forceinline__ void knn_wave_body(const BNode *__restrict__ nodes, c
             d = sqn3(sub3(pt, p));
           } else {
             const TriRec *tk = tr + e;
+            // conservative reject of the whole wave before Thea's exact arithmetic: the plane and
+            // disc bound against each admitting lane's k-th distance
+            {
+              const float4 sbv = make_float4(unif(tk->sb.x), unif(tk->sb.y), unif(tk->sb.z), unif(tk->sb.w));
+              const float dx = p.x - sbv.x, dy = p.y - sbv.y, dz = p.z - sbv.z;
+              const float dc2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+              const float sd = fmaf(p.x, unif(tk->p0.w), fmaf(p.y, unif(tk->p1.w), fmaf(p.z, unif(tk->p2.w), -unif(tk->pl.x))));
+              if ((ballot(!(tri_lb2(dc2, sd, sbv.w, sl) > rej2)) & mc) == 0) continue;
+            }
             float3 cp;
             id = uni(trirec_id(tk->pl));
             d = tri_closest(make_float4(unif(tk->p0.x), unif(tk->p0.y), unif(tk->p0.z), unif(tk->p0.w)),
@@ -1455,6 +1468,7 @@ __device__ __forceinline__ void knn_wave_body(const BNode *__restrict__ nodes, c
 #pragma unroll
               for (int j = 0; j < K - 1; ++j)
                 if (j == k - 1) { kb = kd[j]; kbi = ki[j]; }
+              if (!POINTS) rej2 = rej_sq(kb, sl);
             }
           }
         }
