@@ -41,6 +41,7 @@ def _check_tree(bvh, elo, ehi):
     st, ct, bx = st[o], ct[o], bx[o]
     assert st[0] == 0 and np.array_equal(st[1:], (st + ct)[:-1]) and st[-1] + ct[-1] == n, "leaves do not tile"
     assert np.all(ct >= 1) and np.all(ct <= bvh.info["leaf_size"])
+    assert bvh.info["leaves"] == len(st), (bvh.info["leaves"], len(st))
     assert np.array_equal(bx[:, 0:3], np.minimum.reduceat(elo, st, axis=0))
     assert np.array_equal(bx[:, 3:6], np.maximum.reduceat(ehi, st, axis=0))
     # every internal node except the root is referenced exactly once

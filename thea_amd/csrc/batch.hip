@@ -376,7 +376,7 @@ static void plan_pipeline(BatchPlan &P, hipStream_t s, hipEvent_t mid = nullptr)
       scan_excl_u32(P.keep.as<uint32_t>(), P.fidx.as<uint32_t>(), P.NI, P.stmp.as<uint32_t>(), s);
       launch_emit(P.NI, P.nsmall, P.keep.as<uint32_t>(), P.fidx.as<uint32_t>(), P.left.as<int32_t>(), P.right.as<int32_t>(),
                   P.rfirst.as<int32_t>(), P.rlast.as<int32_t>(), P.vals.as<uint32_t>(), P.blo.as<float4>(),
-                  P.bhi.as<float4>(), P.nbox.as<float>(), P.nodes.as<BNode>(), P.counters.as<uint32_t>(), s);
+                  P.bhi.as<float4>(), P.nbox.as<float>(), P.nodes.as<BNode>(), nullptr, s);
       launch_depth(P.NT, P.lparent.as<int32_t>(), P.nparent.as<int32_t>(), P.keep.as<uint32_t>(),
                    P.rfirst.as<int32_t>(), P.counters.as<int32_t>() + 1, s);
     }

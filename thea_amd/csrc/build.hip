@@ -348,7 +348,10 @@ __device__ __forceinline__ void child_box(int32_t c, const uint32_t *sorted, con
 }
 
 // Emit the collapsed tree. `fidx` = exclusive prefix of keep (final node index of kept nodes, plus
-// `base`); a child that is not kept becomes a leaf over its whole element range.
+// `base`); a child that is not kept becomes a leaf over its whole element range. `leaf_count`, when
+// given, receives the number of leaves (a diagnostic: a full binary tree with `kept` internal nodes
+// has kept + 1 leaves, which the builders use instead, since one counter that every wave adds to
+// serialises the launch: 7.6 ms of the 512-mesh batched build).
 __global__ void __launch_bounds__(B_BLOCK) k_emit(int64_t n, int32_t base, const uint32_t *__restrict__ keep,
                                                   const uint32_t *__restrict__ fidx, const int32_t *__restrict__ left,
                                                   const int32_t *__restrict__ right, const int32_t *__restrict__ rfirst,
@@ -385,9 +388,11 @@ __global__ void __launch_bounds__(B_BLOCK) k_emit(int64_t n, int32_t base, const
     nd.d = make_int4(ref[0], ref[1], cnt[0], cnt[1]);
     out[base + fidx[i]] = nd;
   }
-  // one atomic per wave on the shared counter (every lane of the wave takes part in the sum)
-  leaves = wave_sum(leaves);
-  if (lane_id() == 0 && leaves) atomicAdd(leaf_count, leaves);
+  if (leaf_count) {
+    // one atomic per wave on the shared counter (every lane of the wave takes part in the sum)
+    leaves = wave_sum(leaves);
+    if (lane_id() == 0 && leaves) atomicAdd(leaf_count, leaves);
+  }
 }
 
 // Maximum leaf depth of the collapsed tree (number of kept ancestors), via atomicMax. The elements
@@ -696,12 +701,11 @@ void build_bvh(Bvh &b, const float *verts, const int32_t *tris, int64_t n, bool 
   THG_HIP(hipMemcpyAsync(&kept, fidx.as<uint32_t>() + ni, 4, hipMemcpyDeviceToHost, s));
   THG_HIP(hipStreamSynchronize(s));
   b.nodes.alloc((size_t)kept * sizeof(BNode));
-  uint32_t *leaf_count = counters.as<uint32_t>();
   int32_t *maxd = reinterpret_cast<int32_t *>(counters.as<uint32_t>() + 1);
   hipLaunchKernelGGL(k_emit, dim3(ceil_div(ni, B_BLOCK)), dim3(B_BLOCK), 0, s, n, 0, keep.as<uint32_t>(),
                      fidx.as<uint32_t>(), left.as<int32_t>(), right.as<int32_t>(), rfirst.as<int32_t>(),
                      rlast.as<int32_t>(), sorted, blo.as<float4>(), bhi.as<float4>(), nbox.as<float>(),
-                     b.nodes.as<BNode>(), leaf_count);
+                     b.nodes.as<BNode>(), nullptr);
   launch_check("k_emit");
   hipLaunchKernelGGL(k_depth, dim3(ceil_div(n, B_BLOCK)), dim3(B_BLOCK), 0, s, n, lparent.as<int32_t>(),
                      nparent.as<int32_t>(), keep.as<uint32_t>(), rfirst.as<int32_t>(), maxd);
@@ -710,7 +714,7 @@ void build_bvh(Bvh &b, const float *verts, const int32_t *tris, int64_t n, bool 
   THG_HIP(hipMemcpyAsync(cnt, counters.p, 8, hipMemcpyDeviceToHost, s));
   THG_HIP(hipStreamSynchronize(s));
   b.num_nodes = kept;
-  b.num_leaves = cnt[0];
+  b.num_leaves = (int64_t)kept + 1;  // a full binary tree
   b.max_depth = (int32_t)cnt[1];
   if (!points) {
     hipLaunchKernelGGL(k_leaf_ends, dim3(ceil_div(kept, B_BLOCK)), dim3(B_BLOCK), 0, s, b.nodes.as<BNode>(),
