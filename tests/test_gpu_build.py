@@ -106,3 +106,17 @@ def test_c3_tree_exact_and_refits_agree():
     other = np.load(io.BytesIO(out), allow_pickle=False)
     f, i, _ = b.export()
     assert np.array_equal(other, i)
+
+
+def test_level_refit_large_point_tree():
+    """Over 4M elements (4096 spans of 1024) the refit finishes the top tree level by level, one
+    launch per level; clustered and duplicated points make the tree unbalanced and deep."""
+    import thea_amd
+    print("case build 4.6M points (level refit)", flush=True)
+    g = np.random.default_rng(23)
+    n = 4_600_000
+    p = g.random((n, 3)).astype(np.float32)
+    p[: n // 4] = (p[: n // 4] * 1e-3 + 0.5).astype(np.float32)  # a dense cluster
+    p[n // 4 : n // 4 + 200_000] = p[0]  # one point repeated: equal Morton codes
+    b = thea_amd.PointBvh(p, leaf_size=4)
+    _check_tree(b, p, p)

@@ -54,7 +54,7 @@ struct BatchPlan {
   DevBuf d_info, d_tchunks, d_qchunks, mbounds;
   DevBuf center, blo, bhi, keys, vals, tk, tv, hist;
   DevBuf left, right, rfirst, rlast, nparent, lparent, flags, nbox, keep, fidx, stmp;
-  DevBuf nodes, recs, seg, out_idx, out_dist, counters, rent, rcount, bad;
+  DevBuf nodes, recs, seg, out_idx, out_dist, counters, rent, rcount, rlist, bad;
   int block_waves = 4;     // workgroup shape of the query launch (waves per workgroup)
   bool spill_build = true;  // the 8-resident-wave build of the query kernel
   hipGraphExec_t exec = nullptr;
@@ -368,7 +368,7 @@ static void plan_pipeline(BatchPlan &P, hipStream_t s, hipEvent_t mid = nullptr)
     launch_refit(P.NT, P.vals.as<uint32_t>(), P.blo.as<float4>(), P.bhi.as<float4>(), P.left.as<int32_t>(),
                  P.right.as<int32_t>(), P.rfirst.as<int32_t>(), P.rlast.as<int32_t>(), P.nparent.as<int32_t>(),
                  P.lparent.as<int32_t>(), P.flags.as<uint32_t>(), P.nbox.as<float>(), P.rent.as<int32_t>(),
-                 P.rcount.as<uint32_t>(), s);
+                 P.rcount.as<uint32_t>(), P.rlist.as<int32_t>(), s);
     if (P.NI > 0) {
       hipLaunchKernelGGL(k_bkeep, dim3(ceil_div(P.NI, BB)), dim3(BB), 0, s, P.rfirst.as<int32_t>(), P.rlast.as<int32_t>(),
                          mi, nm, P.NI, P.keep.as<uint32_t>());
@@ -561,7 +561,8 @@ int thg_batch_plan_create(thg_ctx ctx, int32_t nm, const int64_t *nv, const int6
     P->out_dist.alloc(std::max<int64_t>(qo, 1) * 4);
     P->counters.alloc(16);
     P->rent.alloc(T1 * 4);
-    P->rcount.alloc(4);
+    P->rcount.alloc(refit_count_words(T1) * 4);
+    P->rlist.alloc(refit_needs_list(T1) ? T1 * 4 : 4);
     P->bad.alloc(4);
     THG_HIP(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
     *out = reinterpret_cast<thg_batch_plan>(P.release());

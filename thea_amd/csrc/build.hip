@@ -220,11 +220,16 @@ __global__ void __launch_bounds__(B_BLOCK) k_refit(int64_t n, const uint32_t *__
 // (left child = [first, g], right child = [g + 1, last]) is unique, so LDS slot g - span start
 // holds the node's arrival counter and one box per side; the second child to arrive merges, with
 // workgroup-scope ordering only. Node boxes go to global memory as they are computed. A child whose
-// parent straddles the span appends the parent to `entries`. Pass 2 (k_refit_top) finishes the
-// straddling nodes with the device-wide arrival protocol of k_refit; there are O(depth) of them per
-// span boundary, so the costly device-scope fences run for a small fraction of the nodes.
+// parent straddles the span appends the parent to the span's own list, entries[span start ...]
+// with an LDS counter, and the block stores the list's length in nentries[span]. (One device-wide
+// counter for all spans serialised the pass: a few hundred thousand appends on one address.) The
+// maximal subtrees inside a span have disjoint ranges in it, so a list never outgrows the span.
+// Pass 2 (k_refit_top) finishes the straddling nodes with the device-wide arrival protocol of
+// k_refit; there are O(depth) of them per span boundary, so the costly device-scope fences run for
+// a small fraction of the nodes.
 constexpr int REFIT_SPAN = 1024;
 constexpr int REFIT_THREADS = 256;
+constexpr int REFIT_TOP_LANES = 16;
 
 __device__ __forceinline__ void elem_box(const uint32_t *__restrict__ sorted, const float4 *__restrict__ blo,
                                          const float4 *__restrict__ bhi, int64_t pos, float b[6]) {
@@ -244,9 +249,11 @@ __global__ void __launch_bounds__(REFIT_THREADS) k_refit_span(int64_t n, const u
                                                               uint32_t *__restrict__ nentries) {
   __shared__ float s_l[REFIT_SPAN * 6], s_r[REFIT_SPAN * 6];
   __shared__ uint32_t s_flag[REFIT_SPAN];
+  __shared__ uint32_t s_nent;
   const int64_t lo = (int64_t)blockIdx.x * REFIT_SPAN;
   const int64_t hi = lo + REFIT_SPAN < n ? lo + REFIT_SPAN : n;
   for (int i = threadIdx.x; i < REFIT_SPAN; i += REFIT_THREADS) s_flag[i] = 0;
+  if (threadIdx.x == 0) s_nent = 0;
   __syncthreads();
   for (int64_t k = lo + threadIdx.x; k < hi; k += REFIT_THREADS) {
     float b[6];
@@ -256,7 +263,7 @@ __global__ void __launch_bounds__(REFIT_THREADS) k_refit_span(int64_t n, const u
     while (p >= 0) {
       const int64_t fp = rfirst[p], lp = rlast[p];
       if (fp < lo || lp >= hi) {
-        entries[wave_append(nentries)] = p;
+        entries[lo + atomicAdd(&s_nent, 1u)] = p;
         break;
       }
       const bool is_left = cf == fp;
@@ -281,50 +288,123 @@ __global__ void __launch_bounds__(REFIT_THREADS) k_refit_span(int64_t n, const u
       p = node_parent[p];
     }
   }
+  __syncthreads();
+  if (threadIdx.x == 0) nentries[blockIdx.x] = s_nent;
 }
 
+// Box of internal node `cur` from its two children: element boxes, or node boxes written before
+// (read at L2 with agent-scope atomics, so a concurrent writer's box is never a stale L1 copy).
+__device__ __forceinline__ void refit_node(int32_t cur, const uint32_t *__restrict__ sorted,
+                                           const float4 *__restrict__ blo, const float4 *__restrict__ bhi,
+                                           const int32_t *__restrict__ left, const int32_t *__restrict__ right,
+                                           float *nbox) {
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  const int32_t ch[2] = {left[cur], right[cur]};
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    float b[6];
+    if (ch[c] < 0) {
+      elem_box(sorted, blo, bhi, ch[c] & 0x7fffffff, b);
+    } else {
+      const float *q = nbox + 6 * (int64_t)ch[c];
+#pragma unroll
+      for (int a = 0; a < 6; ++a) b[a] = ld_coherent(q + a);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = fminf(lo[a], b[a]);
+      hi[a] = fmaxf(hi[a], b[3 + a]);
+    }
+  }
+  float *o = nbox + 6 * (int64_t)cur;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) { st_coherent(o + a, lo[a]); st_coherent(o + 3 + a, hi[a]); }
+}
+
+// Fenced walk from `cur` to the root with the device-wide arrival protocol of k_refit.
+__device__ __forceinline__ void refit_walk(int32_t cur, const uint32_t *__restrict__ sorted,
+                                           const float4 *__restrict__ blo, const float4 *__restrict__ bhi,
+                                           const int32_t *__restrict__ left, const int32_t *__restrict__ right,
+                                           const int32_t *__restrict__ node_parent, uint32_t *flags, float *nbox) {
+  while (cur >= 0) {
+    __threadfence();
+    if (atomicAdd(&flags[cur], 1u) == 0) break;  // first arrival; the other child's thread continues
+    __threadfence();
+    refit_node(cur, sorted, blo, bhi, left, right, nbox);
+    cur = node_parent[cur];
+  }
+}
+
+// Pass 2 for small trees: REFIT_TOP_LANES threads per span list walk up with fences.
 __global__ void __launch_bounds__(B_BLOCK) k_refit_top(const uint32_t *__restrict__ sorted,
                                                        const float4 *__restrict__ blo, const float4 *__restrict__ bhi,
                                                        const int32_t *__restrict__ left,
                                                        const int32_t *__restrict__ right,
                                                        const int32_t *__restrict__ node_parent,
                                                        const int32_t *__restrict__ entries,
-                                                       const uint32_t *__restrict__ nentries, uint32_t *flags,
-                                                       float *nbox) {
-  const uint32_t count = *nentries;
-  for (uint32_t e = blockIdx.x * B_BLOCK + threadIdx.x; e < count; e += gridDim.x * B_BLOCK) {
-    int32_t cur = entries[e];
-    while (cur >= 0) {
-      __threadfence();
-      if (atomicAdd(&flags[cur], 1u) == 0) break;  // first arrival; the other child's thread continues
-      __threadfence();
-      float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-      const int32_t ch[2] = {left[cur], right[cur]};
-#pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        float b[6];
-        if (ch[c] < 0) {
-          elem_box(sorted, blo, bhi, ch[c] & 0x7fffffff, b);
-        } else {
-          const float *q = nbox + 6 * (int64_t)ch[c];
-#pragma unroll
-          for (int a = 0; a < 6; ++a) b[a] = ld_coherent(q + a);
-        }
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          lo[a] = fminf(lo[a], b[a]);
-          hi[a] = fmaxf(hi[a], b[3 + a]);
-        }
-      }
-      float *o = nbox + 6 * (int64_t)cur;
-#pragma unroll
-      for (int a = 0; a < 3; ++a) { st_coherent(o + a, lo[a]); st_coherent(o + 3 + a, hi[a]); }
-      cur = node_parent[cur];
-    }
+                                                       const uint32_t *__restrict__ nentries, uint32_t spans,
+                                                       uint32_t *flags, float *nbox) {
+  const uint64_t t = (uint64_t)blockIdx.x * B_BLOCK + threadIdx.x;
+  const uint32_t sp = (uint32_t)(t / REFIT_TOP_LANES);
+  if (sp >= spans) return;
+  const uint32_t count = nentries[sp];
+  for (uint32_t e = (uint32_t)(t % REFIT_TOP_LANES); e < count; e += REFIT_TOP_LANES)
+    refit_walk(entries[(int64_t)sp * REFIT_SPAN + e], sorted, blo, bhi, left, right, node_parent, flags, nbox);
+}
+
+// Pass 2 for large trees, level by level. A device-scope fence writes back the issuing XCD's L2 on
+// this GPU, and the fenced walk pays two per node: 5-7 ms on the 512-mesh batched build, where
+// the top tree above 41.7k spans has some 10^5 nodes. Instead, k_refit_arrive counts one arrival
+// per span-list entry, and a node's second arrival queues it as ready (both children's boxes are
+// written); each k_refit_level launch then writes the boxes of the ready nodes and counts one
+// arrival at each parent, queueing the parents that became ready for the next launch. Kernel
+// boundaries order every box write before its reads, so no fence is needed. After REFIT_LEVELS
+// launches (more than a tree's usual height), the last one finishes any remaining nodes with the
+// fenced walk, so the result never depends on the height.
+constexpr int REFIT_LEVELS = 64;
+constexpr uint32_t REFIT_LEVEL_MIN_SPANS = 4096;
+constexpr uint32_t REFIT_LEVEL_GRID = 512;
+
+__global__ void k_zero_words(uint32_t *p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+__global__ void __launch_bounds__(B_BLOCK) k_refit_arrive(const int32_t *__restrict__ entries,
+                                                          const uint32_t *__restrict__ nentries, uint32_t spans,
+                                                          uint32_t *flags, int32_t *__restrict__ out,
+                                                          uint32_t *out_cnt) {
+  const uint64_t t = (uint64_t)blockIdx.x * B_BLOCK + threadIdx.x;
+  const uint32_t sp = (uint32_t)(t / REFIT_TOP_LANES);
+  if (sp >= spans) return;
+  const uint32_t count = nentries[sp];
+  for (uint32_t e = (uint32_t)(t % REFIT_TOP_LANES); e < count; e += REFIT_TOP_LANES) {
+    const int32_t p = entries[(int64_t)sp * REFIT_SPAN + e];
+    if (atomicAdd(&flags[p], 1u) == 1u) out[wave_append(out_cnt)] = p;
   }
 }
 
-__global__ void k_zero_u32(uint32_t *p) { *p = 0; }
+__global__ void __launch_bounds__(B_BLOCK) k_refit_level(const uint32_t *__restrict__ sorted,
+                                                         const float4 *__restrict__ blo,
+                                                         const float4 *__restrict__ bhi,
+                                                         const int32_t *__restrict__ left,
+                                                         const int32_t *__restrict__ right,
+                                                         const int32_t *__restrict__ node_parent,
+                                                         const int32_t *__restrict__ in,
+                                                         const uint32_t *__restrict__ in_cnt,
+                                                         int32_t *__restrict__ out, uint32_t *out_cnt,
+                                                         uint32_t *flags, float *nbox, int last) {
+  const uint32_t count = *in_cnt;
+  for (uint32_t i = blockIdx.x * B_BLOCK + threadIdx.x; i < count; i += gridDim.x * B_BLOCK) {
+    const int32_t p = in[i];
+    refit_node(p, sorted, blo, bhi, left, right, nbox);
+    const int32_t q = node_parent[p];
+    if (last) {
+      refit_walk(q, sorted, blo, bhi, left, right, node_parent, flags, nbox);
+    } else if (q >= 0 && atomicAdd(&flags[q], 1u) == 1u) {
+      out[wave_append(out_cnt)] = q;
+    }
+  }
+}
 
 // keep[i] = 1 if internal node i stays an internal node of the collapsed tree (range > leaf_size).
 __global__ void __launch_bounds__(B_BLOCK) k_keep(int64_t n, const int32_t *__restrict__ rfirst,
@@ -562,11 +642,17 @@ static bool refit_atomic_only() {
   return v;
 }
 
-// `flags` (one per internal node) must be zero; `entries` holds n int32, `nentries` one uint32.
+size_t refit_count_words(int64_t n) {
+  return (size_t)std::max<int64_t>(ceil_div(n, (int64_t)REFIT_SPAN), 1) + REFIT_LEVELS + 2;
+}
+bool refit_needs_list(int64_t n) { return ceil_div(n, (int64_t)REFIT_SPAN) >= REFIT_LEVEL_MIN_SPANS; }
+
+// `flags` (one per internal node) must be zero; `entries` holds n int32, `nentries`
+// refit_count_words(n) uint32, and `list` n int32 when refit_needs_list(n) (else it may be null).
 void launch_refit(int64_t n, const uint32_t *sorted, const float4 *blo, const float4 *bhi, const int32_t *left,
                   const int32_t *right, const int32_t *rfirst, const int32_t *rlast, const int32_t *nparent,
                   const int32_t *lparent, uint32_t *flags, float *nbox, int32_t *entries, uint32_t *nentries,
-                  hipStream_t s) {
+                  int32_t *list, hipStream_t s) {
   if (n <= 0) return;
   if (refit_atomic_only()) {
     hipLaunchKernelGGL(k_refit, dim3(ceil_div(n, B_BLOCK)), dim3(B_BLOCK), 0, s, n, sorted, blo, bhi, left, right,
@@ -574,16 +660,30 @@ void launch_refit(int64_t n, const uint32_t *sorted, const float4 *blo, const fl
     launch_check("k_refit");
     return;
   }
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, nentries);
-  launch_check("k_zero_u32");
   const uint32_t spans = ceil_div(n, (int64_t)REFIT_SPAN);
   hipLaunchKernelGGL(k_refit_span, dim3(spans), dim3(REFIT_THREADS), 0, s, n, sorted, blo, bhi, rfirst, rlast, nparent,
                      lparent, nbox, entries, nentries);
   launch_check("k_refit_span");
-  const uint32_t grid = std::min<uint32_t>(std::max<uint32_t>(spans / 2, 1), 4096);
-  hipLaunchKernelGGL(k_refit_top, dim3(grid), dim3(B_BLOCK), 0, s, sorted, blo, bhi, left, right, nparent, entries,
-                     nentries, flags, nbox);
-  launch_check("k_refit_top");
+  const uint32_t grid = (uint32_t)ceil_div((int64_t)spans * REFIT_TOP_LANES, (int64_t)B_BLOCK);
+  if (!refit_needs_list(n)) {
+    hipLaunchKernelGGL(k_refit_top, dim3(grid), dim3(B_BLOCK), 0, s, sorted, blo, bhi, left, right, nparent, entries,
+                       nentries, spans, flags, nbox);
+    launch_check("k_refit_top");
+    return;
+  }
+  require(list != nullptr, "refit: level list missing");
+  uint32_t *cnt = nentries + spans;  // REFIT_LEVELS + 1 list lengths
+  hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(128), 0, s, cnt, REFIT_LEVELS + 1);
+  launch_check("k_zero_words");
+  hipLaunchKernelGGL(k_refit_arrive, dim3(grid), dim3(B_BLOCK), 0, s, entries, nentries, spans, flags, list, cnt);
+  launch_check("k_refit_arrive");
+  // level L reads list L (even: `list`, odd: `entries`, whose span lists are consumed) and writes L + 1
+  for (int L = 0; L < REFIT_LEVELS; ++L) {
+    int32_t *in = (L & 1) ? entries : list, *out = (L & 1) ? list : entries;
+    hipLaunchKernelGGL(k_refit_level, dim3(REFIT_LEVEL_GRID), dim3(B_BLOCK), 0, s, sorted, blo, bhi, left, right,
+                       nparent, in, cnt + L, out, cnt + L + 1, flags, nbox, L == REFIT_LEVELS - 1 ? 1 : 0);
+    launch_check("k_refit_level");
+  }
 }
 
 void launch_emit(int64_t ni, int32_t base, const uint32_t *keep, const uint32_t *fidx, const int32_t *left,
@@ -682,7 +782,7 @@ void build_bvh(Bvh &b, const float *verts, const int32_t *tris, int64_t n, bool 
   int64_t ni = n - 1;
   DevBuf left(ni * 4), right(ni * 4), rfirst(ni * 4), rlast(ni * 4), nparent(ni * 4), lparent(n * 4);
   DevBuf flags(ni * 4), nbox(ni * 6 * sizeof(float)), keep(ni * 4), fidx((ni + 1) * 4), counters(16);
-  DevBuf entries(n * 4), nentries(4);
+  DevBuf entries(n * 4), nentries(refit_count_words(n) * 4), rlist(refit_needs_list(n) ? n * 4 : 4);
   THG_HIP(hipMemsetAsync(flags.p, 0, ni * 4, s));
   THG_HIP(hipMemsetAsync(counters.p, 0, 16, s));
   hipLaunchKernelGGL(k_karras, dim3(ceil_div(ni, B_BLOCK)), dim3(B_BLOCK), 0, s, keys.as<uint32_t>(), n,
@@ -691,7 +791,7 @@ void build_bvh(Bvh &b, const float *verts, const int32_t *tris, int64_t n, bool 
   launch_check("k_karras");
   launch_refit(n, sorted, blo.as<float4>(), bhi.as<float4>(), left.as<int32_t>(), right.as<int32_t>(),
                rfirst.as<int32_t>(), rlast.as<int32_t>(), nparent.as<int32_t>(), lparent.as<int32_t>(),
-               flags.as<uint32_t>(), nbox.as<float>(), entries.as<int32_t>(), nentries.as<uint32_t>(), s);
+               flags.as<uint32_t>(), nbox.as<float>(), entries.as<int32_t>(), nentries.as<uint32_t>(), rlist.as<int32_t>(), s);
   hipLaunchKernelGGL(k_keep, dim3(ceil_div(ni, B_BLOCK)), dim3(B_BLOCK), 0, s, n, rfirst.as<int32_t>(),
                      rlast.as<int32_t>(), leaf_size, keep.as<uint32_t>());
   launch_check("k_keep");

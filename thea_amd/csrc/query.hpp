@@ -73,10 +73,14 @@ float bound_to_best(double dist_bound);
 void launch_closest(const Bvh &b, const float *q, int64_t n, double dist_bound, int exclude_self, int32_t *idx,
                     float *cp, float *dist, float *d2, QueryScratch &ws, hipStream_t s);
 // Build-pipeline pieces reused by the batched many-mesh plan (batch.hip).
+// Words of the refit's per-span entry counts (and level-list lengths) for n elements, and whether
+// the refit of n elements needs its n-word level list.
+size_t refit_count_words(int64_t n);
+bool refit_needs_list(int64_t n);
 void launch_refit(int64_t n, const uint32_t *sorted, const float4 *blo, const float4 *bhi, const int32_t *left,
                   const int32_t *right, const int32_t *rfirst, const int32_t *rlast, const int32_t *nparent,
                   const int32_t *lparent, uint32_t *flags, float *nbox, int32_t *entries, uint32_t *nentries,
-                  hipStream_t s);
+                  int32_t *list, hipStream_t s);
 void launch_emit(int64_t ni, int32_t base, const uint32_t *keep, const uint32_t *fidx, const int32_t *left,
                  const int32_t *right, const int32_t *rfirst, const int32_t *rlast, const uint32_t *sorted,
                  const float4 *blo, const float4 *bhi, const float *nbox, BNode *out, uint32_t *leaf_count,
