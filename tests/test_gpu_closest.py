@@ -280,3 +280,25 @@ def test_teapot_closest_and_rays(oracle):
     assert np.array_equal(ridx, oidx)
     assert np.array_equal(rt.view(np.uint32), ot.view(np.uint32))
     print(f"teapot rays: {np.sum(ridx >= 0)} of {len(o)} hit")
+
+
+@pytest.mark.parametrize("leaf", [1, 4, 8, 16])
+def test_points_dense_vs_oracle(oracle, leaf):
+    """Nearest point over a point BVH for a dense batch (the batched-leaf wave kernel), with exact
+    duplicates (lowest index wins) and a bound, against the oracle kd-tree."""
+    import thea_amd
+    g = np.random.default_rng(60 + leaf)
+    pts = g.normal(size=(200_000, 3)).astype(np.float32)
+    pts[150_000:160_000] = pts[:10_000]  # duplicates
+    q = (g.normal(size=(120_000, 3)) * 1.3).astype(np.float32)
+    q[:5000] = pts[:5000]  # queries on points (zero distance, duplicate ties)
+    print(f"case points dense leaf={leaf}", flush=True)
+    pb = thea_amd.PointBvh(pts, leaf_size=leaf)
+    kd = oracle.KdTree(points=pts)
+    idx, dist, cp, d2 = pb.closest(q, want_d2=True)
+    ridx, rcp, rd2, _ = kd.closest(q, mode=0)
+    _assert_same(idx, cp, d2, ridx, rcp, rd2, f"points leaf {leaf}")
+    _dist_rule(dist, d2, idx)
+    idx, dist, cp, d2 = pb.closest(q, dist_bound=0.05, want_d2=True)
+    ridx, rcp, rd2, _ = kd.closest(q, dist_bound=0.05, mode=0)
+    _assert_same(idx, cp, d2, ridx, rcp, rd2, f"points leaf {leaf} bounded")
