@@ -547,6 +547,62 @@ __global__ void __launch_bounds__(B_BLOCK) k_leaf_ends(const BNode *__restrict__
   if (d.y < 0 && d.w > 0) rays[(int64_t)~d.y + d.w - 1].b.w = __int_as_float(1);
 }
 
+// Leaf bounds of the mesh BVH's leaves (one thread per node; nodes [0, nsmall + *kept), kept may
+// be null for 0): the normal is the normalised sum of the leaf's triangle normals and the centre the
+// leaf box's centre, both rounded to float; the half-thickness t and the radius r are then the
+// largest vertex offsets from them, in double and rounded up, so the leaf's triangles (convex hulls
+// of their vertices) lie inside the stored slab and cylinder whatever the normal is.
+__global__ void __launch_bounds__(B_BLOCK) k_leaf_bounds(const BNode *__restrict__ nodes, int64_t nsmall,
+                                                         const uint32_t *__restrict__ kept, int64_t max_nodes,
+                                                         const TriRec *__restrict__ tr, LeafBound *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * B_BLOCK + threadIdx.x;
+  const int64_t count = nsmall + (kept ? (int64_t)*kept : 0);
+  if (i >= count || i >= max_nodes) return;
+  const BNode nd = nodes[i];
+  const float lo6[2][3] = {{nd.a.x, nd.a.y, nd.a.z}, {nd.b.z, nd.b.w, nd.c.x}};
+  const float hi6[2][3] = {{nd.a.w, nd.b.x, nd.b.y}, {nd.c.y, nd.c.z, nd.c.w}};
+  const int32_t ref[2] = {nd.d.x, nd.d.y}, cnt[2] = {nd.d.z, nd.d.w};
+  for (int c = 0; c < 2; ++c) {
+    if (ref[c] >= 0 || cnt[c] <= 0) continue;
+    const int64_t f = ~(int64_t)ref[c];
+    double sx = 0, sy = 0, sz = 0;
+    for (int64_t e = f; e < f + cnt[c]; ++e) {
+      const float4 p0 = tr[e].p0, p1 = tr[e].p1, p2 = tr[e].p2;
+      if (isfinite(p0.w) && isfinite(p1.w) && isfinite(p2.w)) { sx += p0.w; sy += p1.w; sz += p2.w; }
+    }
+    double nn = sqrt(sx * sx + sy * sy + sz * sz);
+    if (!(nn > 1e-30)) { sx = 0; sy = 0; sz = 1; nn = 1; }
+    const float nx = (float)(sx / nn), ny = (float)(sy / nn), nz = (float)(sz / nn);
+    const float cx = (float)(0.5 * ((double)lo6[c][0] + (double)hi6[c][0]));
+    const float cy = (float)(0.5 * ((double)lo6[c][1] + (double)hi6[c][1]));
+    const float cz = (float)(0.5 * ((double)lo6[c][2] + (double)hi6[c][2]));
+    const double len = sqrt((double)nx * nx + (double)ny * ny + (double)nz * nz);
+    double t = 0, r2 = 0;
+    for (int64_t e = f; e < f + cnt[c]; ++e) {
+      const float4 pv[3] = {tr[e].p0, tr[e].p1, tr[e].p2};
+      for (int k = 0; k < 3; ++k) {
+        const double dx = (double)pv[k].x - cx, dy = (double)pv[k].y - cy, dz = (double)pv[k].z - cz;
+        const double s = nx * dx + ny * dy + nz * dz;
+        t = fmax(t, fabs(s));
+        const double sn = s / len;
+        r2 = fmax(r2, dx * dx + dy * dy + dz * dz - sn * sn);
+      }
+    }
+    LeafBound lb;
+    lb.a = make_float4(cx, cy, cz, (float)(sqrt(fmax(r2, 0.0)) * (1.0 + 1e-6)));
+    lb.b = make_float4(nx, ny, nz, (float)(t * (1.0 + 1e-6)));
+    out[f] = lb;
+  }
+}
+
+void launch_leaf_bounds(const BNode *nodes, int64_t nsmall, const uint32_t *kept, int64_t max_nodes, const TriRec *tr,
+                        LeafBound *out, hipStream_t s) {
+  if (max_nodes <= 0) return;
+  hipLaunchKernelGGL(k_leaf_bounds, dim3(ceil_div(max_nodes, B_BLOCK)), dim3(B_BLOCK), 0, s, nodes, nsmall, kept,
+                     max_nodes, tr, out);
+  launch_check("k_leaf_bounds");
+}
+
 // Single-leaf tree for n <= leaf_size: child 0 holds every element, child 1 is an empty leaf.
 __global__ void k_single_leaf(int64_t n, float3 lo, float3 hi, BNode *out) {
   BNode nd;
@@ -770,6 +826,8 @@ void build_bvh(Bvh &b, const float *verts, const int32_t *tris, int64_t n, bool 
       hipLaunchKernelGGL(k_leaf_ends, dim3(1), dim3(B_BLOCK), 0, s, b.nodes.as<BNode>(), (int64_t)1,
                          b.rays.as<RayTri>());
       launch_check("k_leaf_ends");
+      b.lbounds.alloc(n * sizeof(LeafBound));
+      launch_leaf_bounds(b.nodes.as<BNode>(), 1, nullptr, 1, b.elems.as<TriRec>(), b.lbounds.as<LeafBound>(), s);
     }
     b.num_nodes = 1;
     b.num_leaves = 1;
@@ -820,6 +878,9 @@ void build_bvh(Bvh &b, const float *verts, const int32_t *tris, int64_t n, bool 
     hipLaunchKernelGGL(k_leaf_ends, dim3(ceil_div(kept, B_BLOCK)), dim3(B_BLOCK), 0, s, b.nodes.as<BNode>(),
                        (int64_t)kept, b.rays.as<RayTri>());
     launch_check("k_leaf_ends");
+    b.lbounds.alloc(n * sizeof(LeafBound));
+    launch_leaf_bounds(b.nodes.as<BNode>(), (int64_t)kept, nullptr, (int64_t)kept, b.elems.as<TriRec>(),
+                       b.lbounds.as<LeafBound>(), s);
     build_qbvh4(b, s);
   }
 }

@@ -85,6 +85,14 @@ struct __align__(64) QNode4 {
 };
 static_assert(sizeof(QNode4) == 64, "QNode4 must be 64 bytes");
 
+// Leaf bound (mesh BVHs), stored at the position of the leaf's first triangle record: every point
+// of the leaf's triangles lies within `t` of the plane through c with normal n (n.(x - c), n as
+// stored) and within `r` of the axis through c along n. Only pruning reads it.
+struct __align__(16) LeafBound {
+  float4 a;    // c.xyz, r
+  float4 b;    // n.xyz, t
+};
+
 struct __align__(16) PointRec {
   float4 p;    // xyz, id (int bits)
 };

@@ -529,7 +529,7 @@ __device__ __forceinline__ void cmp_flush(CmpWave &W, const TriRec *__restrict__
 // plane offsets) arrives with one vector load per lane and is read per triangle with v_readlane, so
 // a wave waits once per leaf instead of once per triangle; large batches hide that latency with
 // other waves and keep the scalar loads, which cost no vector instructions.
-template <bool STATS, bool PF = false>
+template <bool STATS, bool PF = false, bool LB = false>
 __device__ __forceinline__ void closest_cmp_body(CmpWave &W, const BNode *__restrict__ nodes,
                                                  const TriRec *__restrict__ tr, const float *__restrict__ q,
                                                  const uint32_t *__restrict__ order, int64_t n, int64_t wave,
@@ -537,7 +537,8 @@ __device__ __forceinline__ void closest_cmp_body(CmpWave &W, const BNode *__rest
                                                  int32_t *__restrict__ out_idx, float *__restrict__ out_cp,
                                                  float *__restrict__ out_dist, float *__restrict__ out_d2,
                                                  unsigned long long *__restrict__ stats, int32_t root = 0,
-                                                 int child_order = 0) {
+                                                 int child_order = 0,
+                                                 const LeafBound *__restrict__ lbnd = nullptr) {
   uint32_t st_nodes = 0, st_boxes = 0, st_loaded = 0, st_flush = 0, st_pairs = 0;
   const int lane = lane_id();
   const int64_t t = wave * WAVE + lane;
@@ -614,6 +615,19 @@ __device__ __forceinline__ void closest_cmp_body(CmpWave &W, const BNode *__rest
       const int32_t cnt = c == 0 ? n0 : n1;
       if (r < 0) {
         const int32_t f = ~r;
+        // leaf-level reject: lanes whose bound to the whole leaf exceeds their threshold drop out,
+        // and a leaf no lane keeps is skipped without screening its triangles
+        uint64_t ml = mc;
+        if (LB && lbnd) {
+          const LeafBound *L = lbnd + f;
+          const float lx = unif(L->a.x), ly = unif(L->a.y), lz = unif(L->a.z), lr = unif(L->a.w);
+          const float nx = unif(L->b.x), ny = unif(L->b.y), nz = unif(L->b.z), lt = unif(L->b.w);
+          const float dx = p.x - lx, dy = p.y - ly, dz = p.z - lz;
+          const float dc2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+          const float sd = fmaf(dx, nx, fmaf(dy, ny, dz * nz));
+          ml &= ballot(!(leaf_lb2(dc2, sd, lt, lr, sl) > rej2));
+          if (ml == 0) continue;
+        }
         float4 lsb = make_float4(0.0f, 0.0f, 0.0f, 0.0f), lpl = lsb;  // PF: lane j holds triangle f + j's
         if (PF && lane < cnt) {
           const TriRec *tj = tr + f + lane;
@@ -637,14 +651,14 @@ __device__ __forceinline__ void closest_cmp_body(CmpWave &W, const BNode *__rest
           const float dc2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
           if (PF) {
             const float T = fmaf(__builtin_amdgcn_sqrtf(rej2), 1.0f + 1.0f / 1048576.0f, fabsf(sbv.w));
-            if ((ballot(!(dc2 > T * T)) & mc) == 0) continue;
+            if ((ballot(!(dc2 > T * T)) & ml) == 0) continue;
           }
           const float nx = PF ? lane_readf(lpl.x, jl) : unif(tk->p0.w), ny = PF ? lane_readf(lpl.y, jl) : unif(tk->p1.w);
           const float nz = PF ? lane_readf(lpl.z, jl) : unif(tk->p2.w), pd = PF ? lane_readf(lpl.w, jl) : unif(tk->pl.x);
           const float sd = fmaf(p.x, nx, fmaf(p.y, ny, fmaf(p.z, nz, -pd)));
           // every lane computes the bound (no divergence); lanes that did not admit the leaf are
           // masked out of the ballot
-          const uint64_t nb = ballot(!(tri_lb2(dc2, sd, sbv.w, sl) > rej2)) & mc;
+          const uint64_t nb = ballot(!(tri_lb2(dc2, sd, sbv.w, sl) > rej2)) & ml;
           if (nb == 0) continue;
           const bool need = (nb >> lane) & 1;
           if (need) {
@@ -732,12 +746,14 @@ __global__ void __launch_bounds__(W_BLOCK, WAVES) k_closest_cmp(const BNode *__r
                                                          float *__restrict__ out_cp, float *__restrict__ out_dist,
                                                          float *__restrict__ out_d2,
                                                          unsigned long long *__restrict__ stats = nullptr,
-                                                         int child_order = CHILD_MAJORITY) {
+                                                         int child_order = CHILD_MAJORITY,
+                                                         const LeafBound *__restrict__ lbnd = nullptr) {
   __shared__ CmpWave s_w[W_WAVES];
   const uint32_t lb_blk = xcd_remap(blockIdx.x, gridDim.x);
   const int64_t wave = (int64_t)lb_blk * W_WAVES + (threadIdx.x / WAVE);
-  closest_cmp_body<STATS, PF>(s_w[threadIdx.x / WAVE], nodes, tr, q, order, n, wave, lb_blk < nb_real, best_init,
-                              scale, exclude_self, out_idx, out_cp, out_dist, out_d2, stats, 0, child_order);
+  closest_cmp_body<STATS, PF, true>(s_w[threadIdx.x / WAVE], nodes, tr, q, order, n, wave, lb_blk < nb_real,
+                                    best_init, scale, exclude_self, out_idx, out_cp, out_dist, out_d2, stats, 0,
+                                    child_order, lbnd);
 }
 
 template <bool POINTS, int G, bool PRE, bool STATS = false>
@@ -2398,19 +2414,19 @@ void launch_closest(const Bvh &b, const float *q, int64_t n, double dist_bound, 
       if (!one_round && pf)
         hipLaunchKernelGGL((k_closest_cmp<false, CMP_WAVES, true>), dim3(cgrid), dim3(W_BLOCK), 0, s,
                            b.nodes.as<BNode>(), b.elems.as<TriRec>(), q, ws.order, n, cnb, best, b.scale, exclude_self,
-                           idx, cp, dist, d2, nullptr, b.child_order);
+                           idx, cp, dist, d2, nullptr, b.child_order, b.lbounds.as<LeafBound>());
       else if (one_round && pf)
         hipLaunchKernelGGL((k_closest_cmp<false, 1, true>), dim3(cgrid), dim3(W_BLOCK), 0, s, b.nodes.as<BNode>(),
                            b.elems.as<TriRec>(), q, ws.order, n, cnb, best, b.scale, exclude_self, idx, cp, dist, d2,
-                           nullptr, b.child_order);
+                           nullptr, b.child_order, b.lbounds.as<LeafBound>());
       else if (one_round)
         hipLaunchKernelGGL((k_closest_cmp<false, 1>), dim3(cgrid), dim3(W_BLOCK), 0, s, b.nodes.as<BNode>(),
                            b.elems.as<TriRec>(), q, ws.order, n, cnb, best, b.scale, exclude_self, idx, cp, dist, d2,
-                           nullptr, b.child_order);
+                           nullptr, b.child_order, b.lbounds.as<LeafBound>());
       else
         hipLaunchKernelGGL((k_closest_cmp<false, CMP_WAVES>), dim3(cgrid), dim3(W_BLOCK), 0, s, b.nodes.as<BNode>(),
                            b.elems.as<TriRec>(), q, ws.order, n, cnb, best, b.scale, exclude_self, idx, cp, dist, d2,
-                           nullptr, b.child_order);
+                           nullptr, b.child_order, b.lbounds.as<LeafBound>());
       launch_check("k_closest_cmp");
       kt.stop();
       return;
@@ -2628,7 +2644,8 @@ void closest_wave_stats(const Bvh &b, const float *q, int64_t n, unsigned long l
   float best = INFINITY;
   if (!b.points && wave_config(b).cmp)
     hipLaunchKernelGGL(k_closest_cmp<true>, dim3(wgrid), dim3(W_BLOCK), 0, s, b.nodes.as<BNode>(), b.elems.as<TriRec>(),
-                       q, ws.order, n, wnb, best, b.scale, 0, idx.as<int32_t>(), nullptr, nullptr, nullptr, stats);
+                       q, ws.order, n, wnb, best, b.scale, 0, idx.as<int32_t>(), nullptr, nullptr, nullptr, stats,
+                       (int)b.child_order, b.lbounds.as<LeafBound>());
   else if (b.points)
     hipLaunchKernelGGL((k_closest_wave<true, 1, false, true>), dim3(wgrid), dim3(W_BLOCK), 0, s, b.nodes.as<BNode>(),
                        b.elems.p, q, ws.order, n, wnb, best, b.scale, 0, idx.as<int32_t>(), nullptr, nullptr, nullptr, stats);

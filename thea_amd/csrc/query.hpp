@@ -73,6 +73,10 @@ float bound_to_best(double dist_bound);
 void launch_closest(const Bvh &b, const float *q, int64_t n, double dist_bound, int exclude_self, int32_t *idx,
                     float *cp, float *dist, float *d2, QueryScratch &ws, hipStream_t s);
 // Build-pipeline pieces reused by the batched many-mesh plan (batch.hip).
+// Leaf bounds of a mesh BVH's leaves into out[first record of each leaf]: nodes [0, nsmall + *kept)
+// (kept null: none beyond nsmall), at most max_nodes.
+void launch_leaf_bounds(const BNode *nodes, int64_t nsmall, const uint32_t *kept, int64_t max_nodes, const TriRec *tr,
+                        LeafBound *out, hipStream_t s);
 // Words of the refit's per-span entry counts (and level-list lengths) for n elements, and whether
 // the refit of n elements needs its n-word level list.
 size_t refit_count_words(int64_t n);

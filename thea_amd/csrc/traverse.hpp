@@ -44,6 +44,20 @@ __device__ __forceinline__ float tri_lb2(float dc2, float sd, float r, float sl)
   return fmaf(a, a, b * b);
 }
 
+// Lower bound on the squared distance from a query to a leaf's triangles from its LeafBound: with
+// d = p - c, dc2 = |d|^2 and sd = n.d, the leaf lies within t of the plane and within r of the
+// axis, so the distance across is at least |sd| - t and the distance along the plane at least
+// sqrt(dc2 - sd^2) - r; the same slack and margins as tri_lb2 cover the rounding (the stored normal
+// is a unit vector to float precision).
+__device__ __forceinline__ float leaf_lb2(float dc2, float sd, float t, float r, float sl) {
+  const float asd = fabsf(sd);
+  const float a = fmaxf(asd - t - sl, 0.0f);
+  const float u = fmaf(asd, 1.0f + 1.0f / 1048576.0f, sl);
+  const float ip2 = fmaf(-u, u, dc2);
+  const float b = fmaxf(__builtin_amdgcn_sqrtf(fmaxf(ip2, 0.0f)) - r - sl, 0.0f);
+  return fmaf(a, a, b * b);
+}
+
 // Per-lane seed of the wave traversals: descend from `root` to the leaf whose box is nearest the
 // query (no backtracking) and return it as [first, first + count). The elements found there give
 // each lane a real upper bound before the shared walk starts, so a wave whose queries are far apart
