@@ -1338,13 +1338,14 @@ __global__ void __launch_bounds__(Q_BLOCK) k_knn_group_multi(const SegDesc *__re
 // (d^2, id) pairs in registers; a subtree or element is skipped when every lane's k-th best beats
 // its lower bound. Element records are wave-uniform (scalar loads).
 //--------------------------------------------------------------------------------------------------
-template <bool POINTS, int K>
+template <bool POINTS, int K, bool LB = false>
 __device__ __forceinline__ void knn_wave_body(const BNode *__restrict__ nodes, const void *__restrict__ elems,
                                               const float *__restrict__ q, const uint32_t *__restrict__ order,
                                               int64_t n, int64_t wave, bool wave_ok, int k, float best_init,
                                               float scale, int exclude_self, int32_t *__restrict__ out_idx,
                                               float *__restrict__ out_dist, float *__restrict__ out_d2,
-                                              int32_t nelem, int32_t root = 0) {
+                                              int32_t nelem, int32_t root = 0,
+                                              const LeafBound *__restrict__ lbnd = nullptr) {
   const int lane = lane_id();
   const int64_t t = wave * WAVE + lane;
   const bool inrange = wave_ok && t < n;
@@ -1446,6 +1447,18 @@ __device__ __forceinline__ void knn_wave_body(const BNode *__restrict__ nodes, c
       const int32_t cnt = c == 0 ? n0 : n1;
       if (r < 0) {
         const int32_t f = ~r;
+        // triangles: leaf-level reject against each lane's k-th distance (see closest_cmp_body)
+        uint64_t ml = mc;
+        if (!POINTS && LB && lbnd) {
+          const LeafBound *L = lbnd + f;
+          const float lx = unif(L->a.x), ly = unif(L->a.y), lz = unif(L->a.z), lr = unif(L->a.w);
+          const float nx = unif(L->b.x), ny = unif(L->b.y), nz = unif(L->b.z), lt = unif(L->b.w);
+          const float dx = p.x - lx, dy = p.y - ly, dz = p.z - lz;
+          const float dc2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+          const float sd = fmaf(dx, nx, fmaf(dy, ny, dz * nz));
+          ml &= ballot(!(leaf_lb2(dc2, sd, lt, lr, sl) > rej2));
+          if (ml == 0) continue;
+        }
         for (int32_t e = f; e < f + cnt; ++e) {
           float d;
           int32_t id;
@@ -1463,7 +1476,7 @@ __device__ __forceinline__ void knn_wave_body(const BNode *__restrict__ nodes, c
               const float dx = p.x - sbv.x, dy = p.y - sbv.y, dz = p.z - sbv.z;
               const float dc2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
               const float sd = fmaf(p.x, unif(tk->p0.w), fmaf(p.y, unif(tk->p1.w), fmaf(p.z, unif(tk->p2.w), -unif(tk->pl.x))));
-              if ((ballot(!(tri_lb2(dc2, sd, sbv.w, sl) > rej2)) & mc) == 0) continue;
+              if ((ballot(!(tri_lb2(dc2, sd, sbv.w, sl) > rej2)) & ml) == 0) continue;
             }
             float3 cp;
             id = uni(trirec_id(tk->pl));
@@ -1746,11 +1759,11 @@ __global__ void __launch_bounds__(W_BLOCK) k_knn_wave(const BNode *__restrict__ 
                                                       int64_t n, uint32_t nb_real, int k, float best_init, float scale,
                                                       int exclude_self, int32_t *__restrict__ out_idx,
                                                       float *__restrict__ out_dist, float *__restrict__ out_d2,
-                                                      int32_t nelem) {
+                                                      int32_t nelem, const LeafBound *__restrict__ lbnd = nullptr) {
   const uint32_t lb_blk = xcd_remap(blockIdx.x, gridDim.x);
   const int64_t wave = (int64_t)lb_blk * W_WAVES + (threadIdx.x / WAVE);
-  knn_wave_body<POINTS, K>(nodes, elems, q, order, n, wave, lb_blk < nb_real, k, best_init, scale, exclude_self,
-                           out_idx, out_dist, out_d2, nelem);
+  knn_wave_body<POINTS, K, true>(nodes, elems, q, order, n, wave, lb_blk < nb_real, k, best_init, scale,
+                                 exclude_self, out_idx, out_dist, out_d2, nelem, 0, lbnd);
 }
 
 template <int K, bool STATS = false>
@@ -2686,7 +2699,7 @@ void launch_knn(const Bvh &b, const float *q, int64_t n, int k, double dist_boun
                      ws.order, n, wnb, k, best, b.scale, exclude_self, idx, dist, d2, (int32_t)b.n)
 #define THG_KW(P, K)                                                                                                 \
   hipLaunchKernelGGL((k_knn_wave<P, K>), dim3(wgrid), dim3(W_BLOCK), 0, s, b.nodes.as<BNode>(), b.elems.p, q, ws.order, \
-                     n, wnb, k, best, b.scale, exclude_self, idx, dist, d2, (int32_t)b.n)
+                     n, wnb, k, best, b.scale, exclude_self, idx, dist, d2, (int32_t)b.n, b.lbounds.as<LeafBound>())
     if (b.points && batch && b.leaf_size <= 16 && k <= 16) {  // K = 32 would spill its 64-bit keys
       if (k <= 4) THG_KP(4); else if (k <= 8) THG_KP(8); else THG_KP(16);
     } else if (b.points) {
