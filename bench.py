@@ -27,8 +27,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
 # BVH leaf size per workload, the fastest of 1, 2, 4, 6, 8, 12, 16 measured on one MI355X for that query
-# type (closest point prefers 8, k-NN over points 16, incoherent rays 1); the library default is 4.
-LEAF_DEFAULT = {"c3": 8, "c1": 8, "c3ray": 2, "c2knn": 16, "c2sample": 4, "icp": 8, "graph": 12, "sdf": 1,
+# type (closest point prefers 12 since whole leaves are rejected at once, k-NN over points 16,
+# incoherent rays 1); the library default is 4.
+LEAF_DEFAULT = {"c3": 12, "c1": 16, "c3ray": 2, "c2knn": 16, "c2sample": 4, "icp": 8, "graph": 12, "sdf": 1,
                 "vis": 1}
 
 
