@@ -206,9 +206,10 @@ int thg_ctx_kernel_time(thg_ctx ctx, const char *kernel, double *total_ms, int64
 
 /* Traversal statistics of the closest-point kernel (diagnostics): runs the wave-cooperative
  * traversal over n queries (device pointer) and returns sums over waves of out[0] waves,
- * [1] queries, [2] nodes loaded, [3] child-box tests, [4] elements loaded, [5] lane slots spent on
- * exact element evaluations (64 per element the whole wave evaluates, or 64 per flush of the
- * compacted queue), [6] (lane, element) pairs that passed the per-lane sphere and plane rejects,
+ * [1] queries, [2] nodes loaded, [3] child-box tests, [4] elements screened (those of leaves that
+ * some lane keeps after the leaf-level reject), [5] lane slots spent on exact element evaluations
+ * (64 per element the whole wave evaluates, or 64 per flush of the compacted queue), [6] (lane,
+ * element) pairs that passed the per-lane element rejects,
  * [7] whole-wave exact evaluations whose result came within 2^-8 of some lane's best (0 for the
  * compacted kernel). */
 int thg_closest_stats(thg_bvh bvh, const float *q, int64_t n, void *stream, int64_t out[8]);
