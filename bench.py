@@ -44,6 +44,8 @@ def parse():
     p.add_argument("--queries", type=int, default=None, help="override the query count")
     p.add_argument("--leaf", type=int, default=None,
                    help="BVH leaf size (default: the measured best per workload, LEAF_DEFAULT)")
+    p.add_argument("--c5-leaf", type=int, default=4, help="c5 (and the served half of the default line): leaf size "
+                   "of the resident meshes, for the server and plain dispatch alike")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-served", action="store_true", help="c3: leave out the served-stream half of the line")
     p.add_argument("--kabsch", action="store_true", help="icp: least-squares fit instead of Thea's")
@@ -854,9 +856,9 @@ def c5_measure(args):
     gen_s = time.time() - t0
     srv = Server(num_streams=4)
     t0 = time.perf_counter()
-    ids = [srv.add_mesh(v, t) for v, t in meshes]
+    ids = [srv.add_mesh(v, t, leaf_size=args.c5_leaf) for v, t in meshes]
     srv_build_s = time.perf_counter() - t0
-    bvhs = [thea_amd.MeshBvh(thea_amd.Mesh(v, t)) for v, t in meshes]
+    bvhs = [thea_amd.MeshBvh(thea_amd.Mesh(v, t), leaf_size=args.c5_leaf) for v, t in meshes]
     g = W.rng(5, W.STREAM_TRACE)
     R = args.requests
     nmax = 1 << 20
