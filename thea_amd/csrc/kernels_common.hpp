@@ -82,6 +82,19 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
 
 inline uint32_t round_up8(uint64_t nb) { return (uint32_t)((nb + 7) / 8 * 8); }
 
+// A pointer read from a descriptor in device memory reaches the compiler as a generic (flat)
+// pointer, and loads through it go to the vector memory path even when every lane reads the same
+// address. The trees, records and queries a traversal reads are device memory that no wave writes
+// while the kernel runs: reading the descriptor's field as a pointer to the constant address space
+// (device memory that is read-only for the kernel's duration) lets the compiler carry that into the
+// inlined traversal, so its wave-uniform node and record loads become scalar loads, as they are when
+// the same pointers arrive as const __restrict__ kernel arguments. Loads that differ across lanes
+// stay vector loads.
+template <typename T, typename F> __device__ __forceinline__ const T *readonly_ptr(F *const &field) {
+  typedef const __attribute__((address_space(4))) T *cptr;
+  return (const T *)*reinterpret_cast<const cptr *>(reinterpret_cast<const void *>(&field));
+}
+
 inline void launch_check(const char *name) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) fail(THG_E_DEVICE, std::string("kernel launch ") + name + ": " + hipGetErrorString(e));

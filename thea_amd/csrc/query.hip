@@ -795,12 +795,14 @@ __global__ void __launch_bounds__(WPB * WAVE, WAVES) k_closest_multi(const SegDe
   const int si = uni(find_segment(seg, nseg, gw));
   const SegDesc &d = seg[si];
   if constexpr (POINTS) {
-    closest_wave_body<true, 1, true, false>((const BNode *)d.nodes, d.elems, d.q, d.order, d.n,
+    closest_wave_body<true, 1, true, false>(readonly_ptr<BNode>(d.nodes), readonly_ptr<float>(d.elems), readonly_ptr<float>(d.q),
+                                            readonly_ptr<uint32_t>(d.order), d.n,
                                             gw - (int64_t)d.wave_begin, true, d.best_init, d.scale, d.exclude_self,
                                             d.idx, d.cp, d.dist, d.d2, nullptr, d.root);
   } else {
     __shared__ CmpWave s_w[WPB];
-    closest_cmp_body<false>(s_w[threadIdx.x / WAVE], (const BNode *)d.nodes, (const TriRec *)d.elems, d.q, d.order,
+    closest_cmp_body<false>(s_w[threadIdx.x / WAVE], readonly_ptr<BNode>(d.nodes), readonly_ptr<TriRec>(d.elems),
+                            readonly_ptr<float>(d.q), readonly_ptr<uint32_t>(d.order),
                             d.n, gw - (int64_t)d.wave_begin, true, d.best_init, d.scale, d.exclude_self, d.idx, d.cp,
                             d.dist, d.d2, nullptr, d.root, d.child_order);
   }
@@ -939,7 +941,8 @@ __global__ void __launch_bounds__(Q_BLOCK) k_closest_lane_multi(const SegDesc *_
   const int64_t t = ((int64_t)lb_blk - (int64_t)d.wave_begin) * Q_BLOCK + threadIdx.x;
   if (t >= d.n) return;
   const int64_t qi = d.order ? (int64_t)d.order[t] : t;
-  closest_lane_body<STACK, POINTS>(s_ref, s_lb, (const BNode *)d.nodes, d.elems, d.q, qi, d.best_init, d.scale,
+  closest_lane_body<STACK, POINTS>(s_ref, s_lb, readonly_ptr<BNode>(d.nodes), readonly_ptr<float>(d.elems),
+                                   readonly_ptr<float>(d.q), qi, d.best_init, d.scale,
                                    d.exclude_self, d.idx, d.cp, d.dist, d.d2, d.root);
 }
 
@@ -955,7 +958,8 @@ __global__ void __launch_bounds__(Q_BLOCK) k_knn_lane_multi(const SegDesc *__res
   const int64_t t = ((int64_t)lb_blk - (int64_t)d.wave_begin) * Q_BLOCK + threadIdx.x;
   if (t >= d.n) return;
   const int64_t qi = d.order ? (int64_t)d.order[t] : t;
-  knn_lane_body<STACK, POINTS, K>(s_ref, s_lb, (const BNode *)d.nodes, d.elems, d.q, qi, d.k, d.best_init, d.scale,
+  knn_lane_body<STACK, POINTS, K>(s_ref, s_lb, readonly_ptr<BNode>(d.nodes), readonly_ptr<float>(d.elems),
+                                  readonly_ptr<float>(d.q), qi, d.k, d.best_init, d.scale,
                                   d.exclude_self, d.idx, d.dist, d.d2, d.root);
 }
 
@@ -1312,7 +1316,8 @@ __global__ void __launch_bounds__(Q_BLOCK, STACK <= 32 ? 5 : 1) k_closest_group_
   const int64_t s = (((int64_t)lb_blk - (int64_t)d.wave_begin) * Q_BLOCK + threadIdx.x) >> lg;
   const bool live = s < d.n;
   const int64_t qi = live ? (d.order ? (int64_t)d.order[s] : s) : 0;
-  closest_group_body<STACK, POINTS>(s_ref, s_lb, (const BNode *)d.nodes, d.elems, d.q, qi, live, lg, d.best_init,
+  closest_group_body<STACK, POINTS>(s_ref, s_lb, readonly_ptr<BNode>(d.nodes), readonly_ptr<float>(d.elems),
+                                    readonly_ptr<float>(d.q), qi, live, lg, d.best_init,
                                     d.scale, d.exclude_self, d.idx, d.cp, d.dist, d.d2, d.root);
 }
 
@@ -1329,7 +1334,8 @@ __global__ void __launch_bounds__(Q_BLOCK) k_knn_group_multi(const SegDesc *__re
   const int64_t s = (((int64_t)lb_blk - (int64_t)d.wave_begin) * Q_BLOCK + threadIdx.x) >> lg;
   const bool live = s < d.n;
   const int64_t qi = live ? (d.order ? (int64_t)d.order[s] : s) : 0;
-  knn_group_body<STACK, POINTS, K>(s_ref, s_lb, (const BNode *)d.nodes, d.elems, d.q, qi, live, lg, d.k,
+  knn_group_body<STACK, POINTS, K>(s_ref, s_lb, readonly_ptr<BNode>(d.nodes), readonly_ptr<float>(d.elems),
+                                   readonly_ptr<float>(d.q), qi, live, lg, d.k,
                                    d.best_init, d.scale, d.exclude_self, d.idx, d.dist, d.d2, d.root);
 }
 
@@ -1787,7 +1793,8 @@ __global__ void __launch_bounds__(W_BLOCK) k_knn_multi(const SegDesc *__restrict
   if (lb_blk >= nb_real || gw >= (int64_t)total_waves) return;
   const int si = uni(find_segment(seg, nseg, gw));
   const SegDesc &d = seg[si];
-  knn_wave_body<POINTS, K>((const BNode *)d.nodes, d.elems, d.q, d.order, d.n, gw - (int64_t)d.wave_begin, true, d.k,
+  knn_wave_body<POINTS, K>(readonly_ptr<BNode>(d.nodes), readonly_ptr<float>(d.elems), readonly_ptr<float>(d.q),
+                           readonly_ptr<uint32_t>(d.order), d.n, gw - (int64_t)d.wave_begin, true, d.k,
                            d.best_init, d.scale, d.exclude_self, d.idx, d.dist, d.d2, d.nelem, d.root);
 }
 
@@ -1975,9 +1982,9 @@ __global__ void __launch_bounds__(Q_BLOCK) k_ray_multi(const SegDesc *__restrict
   const int64_t t = ((int64_t)lb_blk - (int64_t)d.wave_begin) * Q_BLOCK + threadIdx.x;
   if (t >= d.n) return;
   const int64_t ri = d.order ? (int64_t)d.order[t] : t;
-  ray_body<STACK, false, false, false, LANE_REG>(s_ref, s_t, (const BNode *)d.nodes, (const RayTri *)d.rays,
-                                                 (const TriRec *)d.elems, d.q,
-                                                 d.q2, ri, d.best_init, d.max_time, d.scale, d.idx, d.dist, d.cp,
+  ray_body<STACK, false, false, false, LANE_REG>(s_ref, s_t, readonly_ptr<BNode>(d.nodes), readonly_ptr<RayTri>(d.rays),
+                                                 readonly_ptr<TriRec>(d.elems), readonly_ptr<float>(d.q),
+                                                 readonly_ptr<float>(d.q2), ri, d.best_init, d.max_time, d.scale, d.idx, d.dist, d.cp,
                                                  nullptr, d.root);
 }
 
@@ -2224,8 +2231,8 @@ __global__ void __launch_bounds__(Q_BLOCK) k_ray_q4_multi(const SegDesc *__restr
   const int64_t t = ((int64_t)lb_blk - (int64_t)d.wave_begin) * Q_BLOCK + threadIdx.x;
   if (t >= d.n) return;
   const int64_t ri = d.order ? (int64_t)d.order[t] : t;
-  ray_q4_body<false, LST, OVF, true>(s_ref, (const QNode4 *)d.qnodes, (const RayTri *)d.rays, (const TriRec *)d.elems, d.q,
-                               d.q2, ri, d.best_init, d.max_time, d.scale, q4_exact_only(d.scale), d.idx, d.dist,
+  ray_q4_body<false, LST, OVF, true>(s_ref, readonly_ptr<QNode4>(d.qnodes), readonly_ptr<RayTri>(d.rays),
+                                     readonly_ptr<TriRec>(d.elems), readonly_ptr<float>(d.q), readonly_ptr<float>(d.q2), ri, d.best_init, d.max_time, d.scale, q4_exact_only(d.scale), d.idx, d.dist,
                                d.cp, nullptr);
 }
 
