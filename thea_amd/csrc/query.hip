@@ -632,7 +632,7 @@ __device__ __forceinline__ void closest_cmp_body(CmpWave &W, const BNode *__rest
         if (PF && lane < cnt) {
           const TriRec *tj = tr + f + lane;
           lsb = tj->sb;
-          lpl = make_float4(tj->p0.w, tj->p1.w, tj->p2.w, tj->pl.x);
+          lpl = make_float4(tj->p0.w, tj->p1.w, tj->p2.w, 0.0f);
         }
         for (int32_t k = f; k < f + cnt; ++k) {
           const TriRec *tk = tr + k;
@@ -654,8 +654,8 @@ __device__ __forceinline__ void closest_cmp_body(CmpWave &W, const BNode *__rest
             if ((ballot(!(dc2 > T * T)) & ml) == 0) continue;
           }
           const float nx = PF ? lane_readf(lpl.x, jl) : unif(tk->p0.w), ny = PF ? lane_readf(lpl.y, jl) : unif(tk->p1.w);
-          const float nz = PF ? lane_readf(lpl.z, jl) : unif(tk->p2.w), pd = PF ? lane_readf(lpl.w, jl) : unif(tk->pl.x);
-          const float sd = fmaf(p.x, nx, fmaf(p.y, ny, fmaf(p.z, nz, -pd)));
+          const float nz = PF ? lane_readf(lpl.z, jl) : unif(tk->p2.w);
+          const float sd = fmaf(dx, nx, fmaf(dy, ny, dz * nz));
           // every lane computes the bound (no divergence); lanes that did not admit the leaf are
           // masked out of the ballot
           const uint64_t nb = ballot(!(tri_lb2(dc2, sd, sbv.w, sl) > rej2)) & ml;
@@ -1481,7 +1481,7 @@ __device__ __forceinline__ void knn_wave_body(const BNode *__restrict__ nodes, c
               const float4 sbv = make_float4(unif(tk->sb.x), unif(tk->sb.y), unif(tk->sb.z), unif(tk->sb.w));
               const float dx = p.x - sbv.x, dy = p.y - sbv.y, dz = p.z - sbv.z;
               const float dc2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-              const float sd = fmaf(p.x, unif(tk->p0.w), fmaf(p.y, unif(tk->p1.w), fmaf(p.z, unif(tk->p2.w), -unif(tk->pl.x))));
+              const float sd = fmaf(dx, unif(tk->p0.w), fmaf(dy, unif(tk->p1.w), dz * unif(tk->p2.w)));
               if ((ballot(!(tri_lb2(dc2, sd, sbv.w, sl) > rej2)) & ml) == 0) continue;
             }
             float3 cp;

@@ -24,11 +24,12 @@ __device__ __forceinline__ float box_lb(float3 qlo, float3 qhi, float lx, float 
 }
 
 // Lower bound on the squared distance from a query p to a triangle, from the triangle's plane and a
-// disc holding it: sd = n.p - pd (unit normal n) is p's distance across the plane, and p's foot on
-// the plane lies sqrt(|p - c|^2 - sd^2) from the disc's centre c (a point of the plane), so at least
-// that less the radius r from every point of the triangle; the two distances are orthogonal and
-// their squares add. Each is lowered by the slack sl, which covers the rounding of the stored plane
-// and centre and of this arithmetic: dc2 = |p - c|^2 may round up by a relative 4 2^-24, which the
+// disc holding it: with c the disc's centre (a point of the plane), sd = n.(p - c) (unit normal n)
+// is p's distance across the plane, and p's foot on the plane lies sqrt(|p - c|^2 - sd^2) from c,
+// so at least that less the radius r from every point of the triangle; the two distances are
+// orthogonal and their squares add. Each is lowered by the slack sl, which covers the rounding of
+// the stored plane and centre (c is within 2^-23 |c| of the plane, the triangle within 2^-20 of its
+// largest coordinate) and of this arithmetic: dc2 = |p - c|^2 may round up by a relative 4 2^-24, which the
 // relative 2^-20 added to |sd| (in the term subtracted from it) more than covers across the plane
 // and which moves the in-plane distance by far less than sl, as does the hardware square root
 // (1 ulp). A negative r marks a triangle whose plane is not tight (degenerate or near-collinear; c
