@@ -54,7 +54,7 @@ struct BatchPlan {
   DevBuf d_info, d_tchunks, d_qchunks, mbounds;
   DevBuf center, blo, bhi, keys, vals, tk, tv, hist;
   DevBuf left, right, rfirst, rlast, nparent, lparent, flags, nbox, keep, fidx, stmp;
-  DevBuf nodes, recs, seg, out_idx, out_dist, counters, rent, rcount, rlist, bad;
+  DevBuf nodes, recs, screen, seg, out_idx, out_dist, counters, rent, rcount, rlist, bad;
   int block_waves = 4;     // workgroup shape of the query launch (waves per workgroup)
   bool spill_build = true;  // the 8-resident-wave build of the query kernel
   hipGraphExec_t exec = nullptr;
@@ -285,7 +285,8 @@ __global__ void __launch_bounds__(BB) k_brecords(const float *__restrict__ verts
 // Single-leaf roots of small meshes and the segment table of the query launch.
 __global__ void k_bsegs(const MeshInfo *__restrict__ mi, int nm, const int *__restrict__ mb,
                         const uint32_t *__restrict__ fidx, int32_t nsmall, BNode *__restrict__ nodes,
-                        const TriRec *__restrict__ recs, const float *__restrict__ q, const uint32_t *__restrict__ qorder,
+                        const TriRec *__restrict__ recs, const ScreenRec *__restrict__ scr,
+                        const float *__restrict__ q, const uint32_t *__restrict__ qorder,
                         int32_t *__restrict__ out_idx, float *__restrict__ out_dist, SegDesc *__restrict__ seg) {
   int m = blockIdx.x * BB + threadIdx.x;
   if (m >= nm) return;
@@ -311,6 +312,7 @@ __global__ void k_bsegs(const MeshInfo *__restrict__ mi, int nm, const int *__re
   SegDesc d{};
   d.nodes = nodes;
   d.elems = recs;
+  d.screen = scr;
   d.q = q;
   d.order = qorder + x.qoff;
   d.idx = out_idx;
@@ -383,6 +385,7 @@ static void plan_pipeline(BatchPlan &P, hipStream_t s, hipEvent_t mid = nullptr)
     hipLaunchKernelGGL(k_brecords, dim3(ceil_div(P.NT, BB)), dim3(BB), 0, s, P.verts.as<float>(), P.tris.as<int32_t>(),
                        P.vals.as<uint32_t>(), mi, nm, P.NT, P.recs.as<TriRec>());
     launch_check("k_brecords");
+    launch_screen_records(P.recs.as<TriRec>(), P.NT, P.screen.as<ScreenRec>(), s);
   }
   if (mid) THG_HIP(hipEventRecord(mid, s));
   if (P.NQ > 0) {
@@ -393,7 +396,8 @@ static void plan_pipeline(BatchPlan &P, hipStream_t s, hipEvent_t mid = nullptr)
     radix_sort_pairs(P.keys.as<uint32_t>(), P.vals.as<uint32_t>(), P.tk.as<uint32_t>(), P.tv.as<uint32_t>(),
                      P.hist.as<uint32_t>(), P.NQ, 0, 32, s);
     hipLaunchKernelGGL(k_bsegs, dim3(ceil_div(nm, BB)), dim3(BB), 0, s, mi, nm, mb, P.fidx.as<uint32_t>(), P.nsmall,
-                       P.nodes.as<BNode>(), P.recs.as<TriRec>(), P.queries.as<float>(), P.vals.as<uint32_t>(),
+                       P.nodes.as<BNode>(), P.recs.as<TriRec>(), P.screen.as<ScreenRec>(), P.queries.as<float>(),
+                       P.vals.as<uint32_t>(),
                        P.out_idx.as<int32_t>(), P.out_dist.as<float>(), P.seg.as<SegDesc>());
     launch_check("k_bsegs");
     hipLaunchKernelGGL(k_bempty, dim3(ceil_div(P.NQ, BB)), dim3(BB), 0, s, mi, nm, P.NQ, P.out_idx.as<int32_t>(),
@@ -455,6 +459,7 @@ static double time_plan_kernel(const Bvh &b, const float *q, int64_t n, int chil
   SegDesc d{};
   d.nodes = b.nodes.p;
   d.elems = b.elems.p;
+  d.screen = b.screen.p;
   d.q = q;
   d.order = ws.order;
   d.idx = oi.as<int32_t>();
@@ -556,6 +561,7 @@ int thg_batch_plan_create(thg_ctx ctx, int32_t nm, const int64_t *nv, const int6
     P->keep.alloc(I1 * 4); P->fidx.alloc((I1 + 1) * 4); P->stmp.alloc(scan_tmp_words(I1) * 4);
     P->nodes.alloc(sizeof(BNode) * (size_t)(P->nsmall + io + 1));
     P->recs.alloc(sizeof(TriRec) * T1);
+    P->screen.alloc(sizeof(ScreenRec) * T1);
     P->seg.alloc(sizeof(SegDesc) * nm);
     P->out_idx.alloc(std::max<int64_t>(qo, 1) * 4);
     P->out_dist.alloc(std::max<int64_t>(qo, 1) * 4);

@@ -521,6 +521,23 @@ __global__ void __launch_bounds__(B_BLOCK) k_point_records(const float *__restri
   out[k] = r;
 }
 
+// Screening records from the triangle records, in the same (leaf) order.
+__global__ void __launch_bounds__(B_BLOCK) k_screen_records(const TriRec *__restrict__ tr, int64_t n,
+                                                            ScreenRec *__restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * B_BLOCK + threadIdx.x;
+  if (k >= n) return;
+  ScreenRec r;
+  r.c = tr[k].sb;
+  r.n = make_float4(tr[k].p0.w, tr[k].p1.w, tr[k].p2.w, 0.0f);
+  out[k] = r;
+}
+
+void launch_screen_records(const TriRec *tr, int64_t n, ScreenRec *out, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_screen_records, dim3(ceil_div(n, B_BLOCK)), dim3(B_BLOCK), 0, s, tr, n, out);
+  launch_check("k_screen_records");
+}
+
 // Ray records from the triangle records, in the same (leaf) order.
 __global__ void __launch_bounds__(B_BLOCK) k_ray_records(const TriRec *__restrict__ tr, int64_t n,
                                                          RayTri *__restrict__ out) {
@@ -816,6 +833,8 @@ void build_bvh(Bvh &b, const float *verts, const int32_t *tris, int64_t n, bool 
     hipLaunchKernelGGL(k_ray_records, dim3(ceil_div(n, B_BLOCK)), dim3(B_BLOCK), 0, s, b.elems.as<TriRec>(), n,
                        b.rays.as<RayTri>());
     launch_check("k_ray_records");
+    b.screen.alloc(n * sizeof(ScreenRec));
+    launch_screen_records(b.elems.as<TriRec>(), n, b.screen.as<ScreenRec>(), s);
   }
 
   if (n <= leaf_size) {

@@ -218,12 +218,13 @@ struct Bvh {
   DevBuf rays;           // RayTri[n] (mesh BVHs), leaf order
   DevBuf qnodes;         // QNode4[num_qnodes] (mesh BVHs): the 4-wide quantized tree for rays
   DevBuf lbounds;        // LeafBound[n] (mesh BVHs), at each leaf's first record position
+  DevBuf screen;         // ScreenRec[n] (mesh BVHs), leaf order
   int64_t num_qnodes = 0;
   int32_t q_depth = 0;   // QNode4 levels
   bool f64 = false;      // point BVH built from double coordinates
   DevBuf elems64;        // PointRecD[n] (f64 only), leaf order
   size_t device_bytes() const {
-    return nodes.bytes + elems.bytes + rays.bytes + qnodes.bytes + lbounds.bytes + elems64.bytes;
+    return nodes.bytes + elems.bytes + rays.bytes + qnodes.bytes + lbounds.bytes + screen.bytes + elems64.bytes;
   }
 };
 

@@ -59,6 +59,14 @@ __host__ __device__ __forceinline__ int trirec_id(const float4 &pl) {
   return m.i;
 }
 
+// Screening record of a triangle, in leaf order beside its TriRec: the bounding disc (TriRec.sb) and
+// the unit normal, the only fields the closest-point walk's per-triangle reject reads. 32 bytes, so
+// one 128-byte cache line holds four consecutive triangles of a leaf instead of one.
+struct __align__(32) ScreenRec {
+  float4 c;  // disc centre xyz, radius (negated when the plane is not tight)
+  float4 n;  // unit normal xyz, 0
+};
+
 // Ray record of a triangle, in leaf order beside its TriRec: v0 and the two edges Eberly's test
 // starts from (e01 = v1 - v0, e02 = v2 - v0, the same float subtractions as Triangle3.hpp:151-152),
 // and the caller's index. 48 bytes, so a ray's triangle tests read 48 of every 128 bytes a TriRec

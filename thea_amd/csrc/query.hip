@@ -538,7 +538,8 @@ __device__ __forceinline__ void closest_cmp_body(CmpWave &W, const BNode *__rest
                                                  float *__restrict__ out_dist, float *__restrict__ out_d2,
                                                  unsigned long long *__restrict__ stats, int32_t root = 0,
                                                  int child_order = 0,
-                                                 const LeafBound *__restrict__ lbnd = nullptr) {
+                                                 const LeafBound *__restrict__ lbnd = nullptr,
+                                                 const ScreenRec *__restrict__ scr = nullptr) {
   uint32_t st_nodes = 0, st_boxes = 0, st_loaded = 0, st_flush = 0, st_pairs = 0;
   const int lane = lane_id();
   const int64_t t = wave * WAVE + lane;
@@ -630,17 +631,17 @@ __device__ __forceinline__ void closest_cmp_body(CmpWave &W, const BNode *__rest
         }
         float4 lsb = make_float4(0.0f, 0.0f, 0.0f, 0.0f), lpl = lsb;  // PF: lane j holds triangle f + j's
         if (PF && lane < cnt) {
-          const TriRec *tj = tr + f + lane;
-          lsb = tj->sb;
-          lpl = make_float4(tj->p0.w, tj->p1.w, tj->p2.w, 0.0f);
+          const ScreenRec *sj = scr + f + lane;
+          lsb = sj->c;
+          lpl = sj->n;
         }
         for (int32_t k = f; k < f + cnt; ++k) {
-          const TriRec *tk = tr + k;
+          const ScreenRec *sk = scr + k;
           if (STATS) st_loaded++;
           const int jl = k - f;
           const float4 sbv = PF ? make_float4(lane_readf(lsb.x, jl), lane_readf(lsb.y, jl), lane_readf(lsb.z, jl),
                                               lane_readf(lsb.w, jl))
-                                : make_float4(unif(tk->sb.x), unif(tk->sb.y), unif(tk->sb.z), unif(tk->sb.w));
+                                : make_float4(unif(sk->c.x), unif(sk->c.y), unif(sk->c.z), unif(sk->c.w));
           // conservative reject (fused multiply-adds: the slack dwarfs their rounding): the plane
           // and disc bound. Large batches apply it alone (a bounding-sphere test first, to skip the
           // plane's scalar loads, cost more than it saved: C3 18.1 against 19.4 ms per launch);
@@ -653,8 +654,8 @@ __device__ __forceinline__ void closest_cmp_body(CmpWave &W, const BNode *__rest
             const float T = fmaf(__builtin_amdgcn_sqrtf(rej2), 1.0f + 1.0f / 1048576.0f, fabsf(sbv.w));
             if ((ballot(!(dc2 > T * T)) & ml) == 0) continue;
           }
-          const float nx = PF ? lane_readf(lpl.x, jl) : unif(tk->p0.w), ny = PF ? lane_readf(lpl.y, jl) : unif(tk->p1.w);
-          const float nz = PF ? lane_readf(lpl.z, jl) : unif(tk->p2.w);
+          const float nx = PF ? lane_readf(lpl.x, jl) : unif(sk->n.x), ny = PF ? lane_readf(lpl.y, jl) : unif(sk->n.y);
+          const float nz = PF ? lane_readf(lpl.z, jl) : unif(sk->n.z);
           const float sd = fmaf(dx, nx, fmaf(dy, ny, dz * nz));
           // every lane computes the bound (no divergence); lanes that did not admit the leaf are
           // masked out of the ballot
@@ -747,13 +748,14 @@ __global__ void __launch_bounds__(W_BLOCK, WAVES) k_closest_cmp(const BNode *__r
                                                          float *__restrict__ out_d2,
                                                          unsigned long long *__restrict__ stats = nullptr,
                                                          int child_order = CHILD_MAJORITY,
-                                                         const LeafBound *__restrict__ lbnd = nullptr) {
+                                                         const LeafBound *__restrict__ lbnd = nullptr,
+                                                         const ScreenRec *__restrict__ scr = nullptr) {
   __shared__ CmpWave s_w[W_WAVES];
   const uint32_t lb_blk = xcd_remap(blockIdx.x, gridDim.x);
   const int64_t wave = (int64_t)lb_blk * W_WAVES + (threadIdx.x / WAVE);
   closest_cmp_body<STATS, PF, true>(s_w[threadIdx.x / WAVE], nodes, tr, q, order, n, wave, lb_blk < nb_real,
                                     best_init, scale, exclude_self, out_idx, out_cp, out_dist, out_d2, stats, 0,
-                                    child_order, lbnd);
+                                    child_order, lbnd, scr);
 }
 
 template <bool POINTS, int G, bool PRE, bool STATS = false>
@@ -804,7 +806,8 @@ __global__ void __launch_bounds__(WPB * WAVE, WAVES) k_closest_multi(const SegDe
     closest_cmp_body<false>(s_w[threadIdx.x / WAVE], readonly_ptr<BNode>(d.nodes), readonly_ptr<TriRec>(d.elems),
                             readonly_ptr<float>(d.q), readonly_ptr<uint32_t>(d.order),
                             d.n, gw - (int64_t)d.wave_begin, true, d.best_init, d.scale, d.exclude_self, d.idx, d.cp,
-                            d.dist, d.d2, nullptr, d.root, d.child_order);
+                            d.dist, d.d2, nullptr, d.root, d.child_order, nullptr,
+                            readonly_ptr<ScreenRec>(d.screen));
   }
 }
 
@@ -2434,19 +2437,23 @@ void launch_closest(const Bvh &b, const float *q, int64_t n, double dist_bound, 
       if (!one_round && pf)
         hipLaunchKernelGGL((k_closest_cmp<false, CMP_WAVES, true>), dim3(cgrid), dim3(W_BLOCK), 0, s,
                            b.nodes.as<BNode>(), b.elems.as<TriRec>(), q, ws.order, n, cnb, best, b.scale, exclude_self,
-                           idx, cp, dist, d2, nullptr, b.child_order, b.lbounds.as<LeafBound>());
+                           idx, cp, dist, d2, nullptr, b.child_order, b.lbounds.as<LeafBound>(),
+                           b.screen.as<ScreenRec>());
       else if (one_round && pf)
         hipLaunchKernelGGL((k_closest_cmp<false, 1, true>), dim3(cgrid), dim3(W_BLOCK), 0, s, b.nodes.as<BNode>(),
                            b.elems.as<TriRec>(), q, ws.order, n, cnb, best, b.scale, exclude_self, idx, cp, dist, d2,
-                           nullptr, b.child_order, b.lbounds.as<LeafBound>());
+                           nullptr, b.child_order, b.lbounds.as<LeafBound>(),
+                           b.screen.as<ScreenRec>());
       else if (one_round)
         hipLaunchKernelGGL((k_closest_cmp<false, 1>), dim3(cgrid), dim3(W_BLOCK), 0, s, b.nodes.as<BNode>(),
                            b.elems.as<TriRec>(), q, ws.order, n, cnb, best, b.scale, exclude_self, idx, cp, dist, d2,
-                           nullptr, b.child_order, b.lbounds.as<LeafBound>());
+                           nullptr, b.child_order, b.lbounds.as<LeafBound>(),
+                           b.screen.as<ScreenRec>());
       else
         hipLaunchKernelGGL((k_closest_cmp<false, CMP_WAVES>), dim3(cgrid), dim3(W_BLOCK), 0, s, b.nodes.as<BNode>(),
                            b.elems.as<TriRec>(), q, ws.order, n, cnb, best, b.scale, exclude_self, idx, cp, dist, d2,
-                           nullptr, b.child_order, b.lbounds.as<LeafBound>());
+                           nullptr, b.child_order, b.lbounds.as<LeafBound>(),
+                           b.screen.as<ScreenRec>());
       launch_check("k_closest_cmp");
       kt.stop();
       return;
@@ -2665,7 +2672,8 @@ void closest_wave_stats(const Bvh &b, const float *q, int64_t n, unsigned long l
   if (!b.points && wave_config(b).cmp)
     hipLaunchKernelGGL(k_closest_cmp<true>, dim3(wgrid), dim3(W_BLOCK), 0, s, b.nodes.as<BNode>(), b.elems.as<TriRec>(),
                        q, ws.order, n, wnb, best, b.scale, 0, idx.as<int32_t>(), nullptr, nullptr, nullptr, stats,
-                       (int)b.child_order, b.lbounds.as<LeafBound>());
+                       (int)b.child_order, b.lbounds.as<LeafBound>(),
+                           b.screen.as<ScreenRec>());
   else if (b.points)
     hipLaunchKernelGGL((k_closest_wave<true, 1, false, true>), dim3(wgrid), dim3(W_BLOCK), 0, s, b.nodes.as<BNode>(),
                        b.elems.p, q, ws.order, n, wnb, best, b.scale, 0, idx.as<int32_t>(), nullptr, nullptr, nullptr, stats);

@@ -24,6 +24,7 @@ struct SegDesc {
   const float *q;        // queries (closest, k-NN) or ray origins
   const float *q2;       // ray directions
   const uint32_t *order; // optional spatial order of the batch
+  const void *screen;    // ScreenRec records (mesh closest-point segments)
   int32_t *idx;
   float *cp;             // closest points (closest) / normals (rays)
   float *dist;           // distances (closest, k-NN) / hit times (rays)
@@ -75,6 +76,7 @@ void launch_closest(const Bvh &b, const float *q, int64_t n, double dist_bound, 
 // Build-pipeline pieces reused by the batched many-mesh plan (batch.hip).
 // Leaf bounds of a mesh BVH's leaves into out[first record of each leaf]: nodes [0, nsmall + *kept)
 // (kept null: none beyond nsmall), at most max_nodes.
+void launch_screen_records(const TriRec *tr, int64_t n, ScreenRec *out, hipStream_t s);
 void launch_leaf_bounds(const BNode *nodes, int64_t nsmall, const uint32_t *kept, int64_t max_nodes, const TriRec *tr,
                         LeafBound *out, hipStream_t s);
 // Words of the refit's per-span entry counts (and level-list lengths) for n elements, and whether

@@ -181,6 +181,7 @@ static void worker_loop(Server *S, int wi) {
             d.elems = b.elems.p;
             d.rays = b.rays.p;
             d.qnodes = b.qnodes.p;
+            d.screen = b.screen.p;
             d.q = q.q;
             d.q2 = q.dirs;
             d.idx = q.idx;
@@ -351,10 +352,10 @@ int thg_server_add_mesh(thg_server srv, const float *verts, int64_t nv, const in
     require(nt > 0, "a served mesh needs at least one triangle");
     require(leaf_size >= 0 && leaf_size <= 16, "leaf_size must be in 0..16");
     size_t mesh_bytes = sm.mesh->verts.bytes + sm.mesh->tris.bytes;
-    // records (128 + 48 B) + binary nodes (at most n - 1 of 64 B) + leaf bounds (32 B per record
+    // records (128 + 48 + 32 B) + binary nodes (at most n - 1 of 64 B) + leaf bounds (32 B per record
     // position) + 4-wide ray nodes (the binary nodes at even depth, at most about two thirds of them)
     // bound the BVH's size before it is built
-    size_t est = mesh_bytes + (size_t)nt * (sizeof(TriRec) + sizeof(RayTri) + sizeof(BNode) + sizeof(LeafBound)) +
+    size_t est = mesh_bytes + (size_t)nt * (sizeof(TriRec) + sizeof(RayTri) + sizeof(ScreenRec) + sizeof(BNode) + sizeof(LeafBound)) +
                  (size_t)nt * sizeof(QNode4) * 2 / 3;
     {
       std::lock_guard<std::mutex> g(S->mu);
