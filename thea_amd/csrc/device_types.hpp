@@ -4,6 +4,10 @@
 //   BNode[num_nodes]  64 B each: both children's boxes + refs, one 64-B segment per visit.
 //   TriRec[n]        128 B each (one L2 line): a triangle with every per-triangle constant the
 //                     closest-point and ray tests need, in leaf order, so a leaf is a contiguous run.
+//   ScreenRec[n]      32 B each (mesh BVHs): bounding disc and normal, leaf order, for the
+//                     closest-point walk's per-triangle reject.
+//   LeafBound[n]      32 B per record position (mesh BVHs): the leaf-level reject, at each leaf's
+//                     first record.
 //   RayTri[n]         48 B each (mesh BVHs): v0 and the edges of each triangle, leaf order, for rays.
 //   QNode4[nq]        64 B each (mesh BVHs): the same tree with every other level folded in and the
 //                     child boxes quantized to bytes, for rays.
